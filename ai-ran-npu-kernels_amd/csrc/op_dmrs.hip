@@ -1,0 +1,341 @@
+// op_dmrs.hip -- type-1 DMRS generation (Gold -> QPSK -> FD-OCC) and DMRS
+// least-squares channel observation with per-Rx noise estimation.
+#include "common.h"
+#include "gold.h"
+
+using airan::DevGeom;
+using airan::Geom;
+
+namespace {
+
+constexpr uint32_t kDmrsGenMagic = 0x4d444731u;  // "MDG1"
+constexpr uint32_t kDmrsLsMagic = 0x4d4c5331u;   // "MLS1"
+
+// mimo/mimo_dmrs_gen/mimo_dmrs_gen.cpp:38-45
+uint32_t DmrsCinit(const PuschMimoConfig &c, uint32_t symbol)
+{
+    const uint64_t term = (1ull << 17) * (14ull * c.slot_number + symbol + 1ull) *
+                          (2ull * c.dmrs_scrambling_id + 1ull);
+    const uint64_t tail = 2ull * c.dmrs_scrambling_id + c.n_scid;
+    return static_cast<uint32_t>((term + tail) & 0x7fffffffu);
+}
+
+struct DmrsGenDev {
+    uint32_t nl, ndmrs, n_dmrs_re, ref_pad, off;
+    uint32_t cinit[4];
+    uint32_t wf_odd[4];
+};
+
+// r(m) = ((1-2c(2m)) + j(1-2c(2m+1)))/sqrt(2) (mimo_dmrs_gen_kernel.cpp:163),
+// times w_f = (-1)^m on ports 1001/1003.  A thread owns 16 consecutive m.
+__global__ void __launch_bounds__(256) DmrsGenKernel(uint16_t *__restrict__ re, uint16_t *__restrict__ im,
+                                                     DmrsGenDev d)
+{
+    const uint32_t m0 = (blockIdx.x * blockDim.x + threadIdx.x) * 16;
+    if (m0 >= d.ref_pad) return;
+    const uint32_t dd = blockIdx.y, l = blockIdx.z;
+    const uint16_t pos = 0x39a8, neg = 0xb9a8;  // fp16(+-0.70710678)
+    const uint32_t bits = airan::GoldWordAt(d.cinit[dd], 2ull * (m0 + d.off));
+    const size_t base = (static_cast<size_t>(l) * d.ndmrs + dd) * d.ref_pad;
+#pragma unroll
+    for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t m = m0 + j;
+        if (m >= d.ref_pad) break;
+        uint16_t vr = 0, vi = 0;
+        if (m < d.n_dmrs_re) {
+            vr = ((bits >> (2 * j)) & 1u) ? neg : pos;
+            vi = ((bits >> (2 * j + 1)) & 1u) ? neg : pos;
+            if (d.wf_odd[l] && (m & 1u)) {
+                vr ^= 0x8000u;
+                vi ^= 0x8000u;
+            }
+        }
+        re[base + m] = vr;
+        im[base + m] = vi;
+    }
+}
+
+__device__ inline __half HMul(__half a, __half b) { return __float2half_rn(__half2float(a) * __half2float(b)); }
+__device__ inline __half HAdd(__half a, __half b) { return __float2half_rn(__half2float(a) + __half2float(b)); }
+__device__ inline __half HSub(__half a, __half b) { return __float2half_rn(__half2float(a) - __half2float(b)); }
+
+// channel_est/mimo_dmrs_ls/mimo_dmrs_ls_kernel.cpp:146-177,301-331: h = y x*
+// in the reference fp16 operation order, FD-OCC2 pair mean when two layers
+// share the comb.  Grid (pilot blocks, D, NR * L).
+__global__ void __launch_bounds__(256) DmrsLsKernel(const __half *__restrict__ yr,
+                                                    const __half *__restrict__ yi,
+                                                    const __half *__restrict__ xr,
+                                                    const __half *__restrict__ xi,
+                                                    __half *__restrict__ hr, __half *__restrict__ hi,
+                                                    DevGeom g)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.pilot_pad) return;
+    const uint32_t d = blockIdx.y;
+    const uint32_t rx = blockIdx.z / g.nl, l = blockIdx.z - rx * g.nl;
+    const size_t out = ((static_cast<size_t>(rx) * g.nl + l) * g.ndmrs + d) * g.pilot_pad + p;
+    const __half *y_r = yr + (static_cast<size_t>(rx) * g.nsym + g.dmrs_sym[d]) * g.pad;
+    const __half *y_i = yi + (static_cast<size_t>(rx) * g.nsym + g.dmrs_sym[d]) * g.pad;
+    const __half *x_r = xr + (static_cast<size_t>(l) * g.ndmrs + d) * g.ref_pad;
+    const __half *x_i = xi + (static_cast<size_t>(l) * g.ndmrs + d) * g.ref_pad;
+    const uint32_t comb = g.comb[l];
+    auto ls = [&](uint32_t m, __half *or_, __half *oi) {
+        const uint32_t sc = comb + 2 * m;
+        const __half a = y_r[sc], b = y_i[sc], c = x_r[m], e = x_i[m];
+        *or_ = HAdd(HMul(a, c), HMul(b, e));
+        *oi = HSub(HMul(b, c), HMul(a, e));
+    };
+    __half vr = __ushort_as_half(0), vi = __ushort_as_half(0);
+    if (g.occ[l]) {
+        if (p < g.n_occ) {
+            __half r0, i0, r1, i1;
+            ls(2 * p, &r0, &i0);
+            ls(2 * p + 1, &r1, &i1);
+            const __half half_one = __float2half_rn(0.5f);
+            vr = HMul(HAdd(r0, r1), half_one);
+            vi = HMul(HAdd(i0, i1), half_one);
+        }
+    } else if (p < g.n_dmrs_re) {
+        ls(p, &vr, &vi);
+    }
+    hr[out] = vr;
+    hi[out] = vi;
+}
+
+// Noise per Rx from layer 0 (mimo_dmrs_ls_kernel.cpp:179-196,334-337):
+// sum_d sum_p |h_{p+1} - h_p|^2 / (D (cnt - 1)), halved for the comb model.
+// One workgroup per Rx, fp32 accumulation, fixed reduction tree.
+__global__ void __launch_bounds__(256) DmrsNoiseKernel(const __half *__restrict__ hr,
+                                                       const __half *__restrict__ hi,
+                                                       __half *__restrict__ noise, DevGeom g)
+{
+    const uint32_t rx = blockIdx.x;
+    const uint32_t cnt = g.occ[0] ? g.n_occ : g.n_dmrs_re;
+    float acc = 0.0f;
+    for (uint32_t d = 0; d < g.ndmrs; ++d) {
+        const size_t base = (static_cast<size_t>(rx) * g.nl * g.ndmrs + d) * g.pilot_pad;
+        for (uint32_t p = threadIdx.x; p + 1 < cnt; p += blockDim.x) {
+            const float dr = __half2float(hr[base + p + 1]) - __half2float(hr[base + p]);
+            const float di = __half2float(hi[base + p + 1]) - __half2float(hi[base + p]);
+            acc += dr * dr + di * di;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    __shared__ float part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float s = (part[0] + part[1]) + (part[2] + part[3]);
+        float v = s / (static_cast<float>(g.ndmrs) * static_cast<float>(cnt - 1));
+        if (!g.occ[0]) v *= 0.5f;
+        noise[rx] = __float2half_rn(v);
+    }
+}
+
+// Pilot description (mimo_dmrs_ls_kernel.cpp:209-240): observation p of layer
+// l sits at comb + 1 + 4p (FD-OCC2) or comb + 2p; pilot_count[l*D + d].
+__global__ void __launch_bounds__(256) PilotDescKernel(uint16_t *__restrict__ sc,
+                                                       uint16_t *__restrict__ count, DevGeom g)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t d = blockIdx.y, l = blockIdx.z;
+    const uint32_t cnt = g.occ[l] ? g.n_occ : g.n_dmrs_re;
+    if (p < g.pilot_pad)
+        sc[(static_cast<size_t>(l) * g.ndmrs + d) * g.pilot_pad + p] =
+            p < cnt ? static_cast<uint16_t>(g.comb[l] + (g.occ[l] ? 1u + 4u * p : 2u * p)) : 0;
+    if (blockIdx.x == 0 && d == 0 && l == 0 && threadIdx.x < 16) {
+        const uint32_t t = threadIdx.x;
+        const uint32_t tl = t / g.ndmrs;
+        count[t] = tl < g.nl ? static_cast<uint16_t>(g.occ[tl] ? g.n_occ : g.n_dmrs_re) : 0;
+    }
+}
+
+}  // namespace
+
+struct MimoDmrsGenRuntimeV1 {
+    int device;
+};
+
+extern "C" {
+
+// mimo_dmrs_gen.cpp:49-64 (TS 38.211 Table 6.4.1.1.3-1, ports 1000..1003)
+int32_t airan_mimo_dmrs_gen_GetCurrentPortOccSemantics(uint16_t port, MimoDmrsPortOccSemantics *s)
+{
+    if (s == nullptr) return AIRAN_INVALID_ARGUMENT;
+    if (port < 1000 || port > 1003) return AIRAN_UNSUPPORTED_PROFILE;
+    const uint32_t idx = port - 1000u;
+    s->port_index = idx;
+    s->comb_delta = idx / 2u;
+    s->wf_odd_negative = idx & 1u;
+    s->wt_negative[0] = 0;
+    s->wt_negative[1] = 0;
+    return AIRAN_OK;
+}
+
+// mimo_dmrs_gen.cpp:66-132
+int32_t airan_mimo_dmrs_gen_BuildCurrentProfile(const PuschMimoConfig *config, PuschMimoLayout *layout,
+                                                MimoDmrsGenKernelMetadata *m, int32_t *cinit)
+{
+    if (config == nullptr || layout == nullptr || m == nullptr || cinit == nullptr)
+        return AIRAN_INVALID_ARGUMENT;
+    Geom g;
+    int32_t st = airan::DeriveGeom(*config, &g);
+    if (st != AIRAN_OK) return st;
+    if (config->flags != 0 || config->codeword_index != 0 ||
+        (config->num_tx_ports != 1 && config->num_tx_ports != 2 && config->num_tx_ports != 4))
+        return AIRAN_UNSUPPORTED_PROFILE;
+    std::memset(m, 0, sizeof(*m));
+    for (int i = 0; i < 16; ++i) cinit[i] = 0;
+    m->magic = kDmrsGenMagic;
+    m->num_layers = g.nl;
+    m->num_dmrs_symbols = g.ndmrs;
+    m->output_layer_stride = g.ndmrs * g.ref_pad;
+    m->output_dmrs_stride = g.ref_pad;
+    for (uint32_t d = 0; d < g.ndmrs; ++d) {
+        m->dmrs_symbols[d] = g.dmrs_sym[d];
+        cinit[d] = static_cast<int32_t>(DmrsCinit(*config, g.dmrs_sym[d]));
+    }
+    for (uint32_t l = 0; l < g.nl; ++l) {
+        MimoDmrsPortOccSemantics s{};
+        airan_mimo_dmrs_gen_GetCurrentPortOccSemantics(config->dmrs_ports[l], &s);
+        m->port_indices[l] = s.port_index;
+        m->comb_delta[l] = s.comb_delta;
+        m->wf_odd_negative[l] = s.wf_odd_negative;
+    }
+    airan::FillLayout(g, layout);
+    return AIRAN_OK;
+}
+
+int32_t airan_mimo_dmrs_gen_ValidateOpArgs(const MimoDmrsGenOpArgsV1 *a)
+{
+    if (!airan::HeaderOk(a, sizeof(MimoDmrsGenOpArgsV1)) || a->dmrs_re == nullptr ||
+        a->dmrs_im == nullptr || a->config == nullptr || a->layout == nullptr ||
+        a->stream == nullptr || a->dmrs_re == a->dmrs_im)
+        return AIRAN_INVALID_ARGUMENT;
+    PuschMimoLayout expected{};
+    MimoDmrsGenKernelMetadata m{};
+    int32_t cinit[16];
+    int32_t st = airan_mimo_dmrs_gen_BuildCurrentProfile(a->config, &expected, &m, cinit);
+    if (st != AIRAN_OK) return st;
+    return airan::SameLayout(expected, *a->layout) ? AIRAN_OK : AIRAN_INVALID_ARGUMENT;
+}
+
+int32_t airan_mimo_dmrs_gen_CreateRuntime(MimoDmrsGenRuntimeV1 **runtime)
+{
+    if (runtime == nullptr) return AIRAN_INVALID_ARGUMENT;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return AIRAN_LAUNCH_FAILED;
+    if (airan::InitGoldTables() != AIRAN_OK) return AIRAN_LAUNCH_FAILED;
+    *runtime = new MimoDmrsGenRuntimeV1{dev};
+    return AIRAN_OK;
+}
+
+int32_t airan_mimo_dmrs_gen_Enqueue(MimoDmrsGenRuntimeV1 *runtime, const MimoDmrsGenOpArgsV1 *a)
+{
+    if (runtime == nullptr) return AIRAN_INVALID_ARGUMENT;
+    int32_t st = airan_mimo_dmrs_gen_ValidateOpArgs(a);
+    if (st != AIRAN_OK) return st;
+    Geom g;
+    airan::DeriveGeom(*a->config, &g);
+    DmrsGenDev d{};
+    d.nl = g.nl;
+    d.ndmrs = g.ndmrs;
+    d.n_dmrs_re = g.n_dmrs_re;
+    d.ref_pad = g.ref_pad;
+    d.off = 6u * a->config->rb_start;
+    for (uint32_t i = 0; i < g.ndmrs; ++i) d.cinit[i] = DmrsCinit(*a->config, g.dmrs_sym[i]);
+    for (uint32_t l = 0; l < g.nl; ++l) d.wf_odd[l] = (a->config->dmrs_ports[l] - 1000u) & 1u;
+    dim3 grid(airan::CeilDiv(airan::CeilDiv(g.ref_pad, 16), 256), g.ndmrs, g.nl);
+    hipLaunchKernelGGL(DmrsGenKernel, grid, dim3(256), 0, static_cast<hipStream_t>(a->stream),
+                       static_cast<uint16_t *>(a->dmrs_re), static_cast<uint16_t *>(a->dmrs_im), d);
+    return airan::LaunchStatus();
+}
+
+void airan_mimo_dmrs_gen_DestroyRuntime(MimoDmrsGenRuntimeV1 *runtime) { delete runtime; }
+
+// mimo_dmrs_ls.cpp:29-108
+int32_t airan_mimo_dmrs_ls_BuildCurrentProfile(const PuschMimoConfig *config, PuschMimoLayout *layout,
+                                               MimoDmrsLsKernelMetadata *m, uint16_t *pilot_count,
+                                               uint16_t *pilot_sc)
+{
+    if (config == nullptr || layout == nullptr || m == nullptr || pilot_count == nullptr ||
+        pilot_sc == nullptr)
+        return AIRAN_INVALID_ARGUMENT;
+    Geom g;
+    int32_t st = airan::DeriveGeom(*config, &g);
+    if (st != AIRAN_OK) return st;
+    std::memset(m, 0, sizeof(*m));
+    m->magic = kDmrsLsMagic;
+    m->num_rx = g.nr;
+    m->num_layers = g.nl;
+    m->num_dmrs_symbols = g.ndmrs;
+    m->num_symbols = g.nsym;
+    m->used_subcarriers = g.used;
+    m->padded_subcarriers = g.pad;
+    m->dmrs_ref_stride = g.ref_pad;
+    m->pilot_stride = g.pilot_pad;
+    m->block_dim = 256;
+    for (uint32_t d = 0; d < g.ndmrs; ++d) m->dmrs_symbols[d] = g.dmrs_sym[d];
+    for (int i = 0; i < 16; ++i) pilot_count[i] = 0;
+    std::memset(pilot_sc, 0, sizeof(uint16_t) * g.nl * g.ndmrs * g.pilot_pad);
+    for (uint32_t l = 0; l < g.nl; ++l) {
+        m->ports[l] = config->dmrs_ports[l];
+        m->comb_delta[l] = g.comb[l];
+        m->observation_model[l] = g.occ[l];
+        const uint32_t cnt = g.occ[l] ? g.n_occ : g.n_dmrs_re;
+        for (uint32_t d = 0; d < g.ndmrs; ++d) {
+            pilot_count[l * g.ndmrs + d] = static_cast<uint16_t>(cnt);
+            uint16_t *dst = pilot_sc + (static_cast<size_t>(l) * g.ndmrs + d) * g.pilot_pad;
+            for (uint32_t p = 0; p < cnt; ++p)
+                dst[p] = static_cast<uint16_t>(g.comb[l] + (g.occ[l] ? 1u + 4u * p : 2u * p));
+        }
+    }
+    airan::FillLayout(g, layout);
+    return AIRAN_OK;
+}
+
+int32_t airan_mimo_dmrs_ls_ValidateOpArgs(const MimoDmrsLsOpArgsV1 *a)
+{
+    if (!airan::HeaderOk(a, sizeof(MimoDmrsLsOpArgsV1)) || a->rx_grid_re == nullptr ||
+        a->rx_grid_im == nullptr || a->dmrs_ref_re == nullptr || a->dmrs_ref_im == nullptr ||
+        a->h_ls_re == nullptr || a->h_ls_im == nullptr || a->pilot_sc == nullptr ||
+        a->pilot_count == nullptr || a->noise_var_rx == nullptr || a->config == nullptr ||
+        a->layout == nullptr || a->stream == nullptr)
+        return AIRAN_INVALID_ARGUMENT;
+    const void *outs[5] = {a->h_ls_re, a->h_ls_im, a->pilot_sc, a->pilot_count, a->noise_var_rx};
+    const void *ins[4] = {a->rx_grid_re, a->rx_grid_im, a->dmrs_ref_re, a->dmrs_ref_im};
+    for (int i = 0; i < 5; ++i) {
+        for (int j = 0; j < 4; ++j)
+            if (outs[i] == ins[j]) return AIRAN_INVALID_ARGUMENT;
+        for (int j = i + 1; j < 5; ++j)
+            if (outs[i] == outs[j]) return AIRAN_INVALID_ARGUMENT;
+    }
+    Geom g;
+    int32_t st = airan::DeriveGeom(*a->config, &g);
+    if (st != AIRAN_OK) return st;
+    PuschMimoLayout expected{};
+    airan::FillLayout(g, &expected);
+    return airan::SameLayout(expected, *a->layout) ? AIRAN_OK : AIRAN_INVALID_ARGUMENT;
+}
+
+int32_t airan_mimo_dmrs_ls_Enqueue(const MimoDmrsLsOpArgsV1 *a)
+{
+    int32_t st = airan_mimo_dmrs_ls_ValidateOpArgs(a);
+    if (st != AIRAN_OK) return st;
+    Geom g;
+    airan::DeriveGeom(*a->config, &g);
+    hipStream_t s = static_cast<hipStream_t>(a->stream);
+    const DevGeom dg = airan::ToDev(g);
+    hipLaunchKernelGGL(PilotDescKernel, dim3(airan::CeilDiv(g.pilot_pad, 256), g.ndmrs, g.nl), dim3(256), 0, s,
+                       static_cast<uint16_t *>(a->pilot_sc), static_cast<uint16_t *>(a->pilot_count), dg);
+    dim3 grid(airan::CeilDiv(g.pilot_pad, 256), g.ndmrs, g.nr * g.nl);
+    hipLaunchKernelGGL(DmrsLsKernel, grid, dim3(256), 0, s, static_cast<const __half *>(a->rx_grid_re),
+                       static_cast<const __half *>(a->rx_grid_im), static_cast<const __half *>(a->dmrs_ref_re),
+                       static_cast<const __half *>(a->dmrs_ref_im), static_cast<__half *>(a->h_ls_re),
+                       static_cast<__half *>(a->h_ls_im), dg);
+    hipLaunchKernelGGL(DmrsNoiseKernel, dim3(g.nr), dim3(256), 0, s, static_cast<const __half *>(a->h_ls_re),
+                       static_cast<const __half *>(a->h_ls_im), static_cast<__half *>(a->noise_var_rx), dg);
+    return airan::LaunchStatus();
+}
+
+}  // extern "C"

@@ -1,0 +1,292 @@
+// op_ldpc.hip -- BG1 layered normalized min-sum LDPC decoder for gfx950.
+//
+// Schedule and arithmetic follow fec/ldpc_decode/ldpc_decode_kernel.cpp:
+// rows in order (:533-713), per row gather -> subtract old message ->
+// min1/min2/sign product -> alpha = 0.75 (:248-300) -> posterior update and
+// scatter; final clamp +-5120 and bit = (lam < 0) (:719-780).
+//
+// Mapping: one workgroup per code block; the posterior [68][Z] lives in LDS
+// as fp16 (52 KB at Z = 384); thread i owns check i of every row, so within a row the
+// Z threads touch a permutation of each column (no LDS write conflicts) and a
+// barrier separates rows.  A check's messages are stored compressed
+// (min1, min2, index of min1, sign bits): 8 bytes per (row, check) in
+// registers when the active rows fit REG_ROWS, otherwise in a global workspace.
+#include "common.h"
+#include "airan_ldpc_bg1.h"
+
+// Posterior of the code block owned by the workgroup ([68][Z] x 16 bit).
+extern __shared__ uint16_t airan_ldpc_lam[];
+
+namespace {
+
+struct Bg1Dev {
+    uint16_t row_start[AIRAN_BG1_ROWS + 1];
+    uint8_t col[AIRAN_BG1_EDGES];
+    uint16_t shift[AIRAN_BG1_EDGES];
+};
+
+__constant__ Bg1Dev kBg1;
+
+bool g_table_loaded[64] = {};
+
+int32_t LoadTable()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return AIRAN_LAUNCH_FAILED;
+    if (g_table_loaded[dev]) return AIRAN_OK;
+    Bg1Dev t{};
+    for (int i = 0; i <= AIRAN_BG1_ROWS; ++i) t.row_start[i] = AIRAN_BG1_ROW_START[i];
+    for (int e = 0; e < AIRAN_BG1_EDGES; ++e) {
+        t.col[e] = AIRAN_BG1_COL[e];
+        t.shift[e] = AIRAN_BG1_SHIFT[e];
+    }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(kBg1), &t, sizeof(t)) != hipSuccess) return AIRAN_LAUNCH_FAILED;
+    g_table_loaded[dev] = true;
+    return AIRAN_OK;
+}
+
+constexpr int kMaxDeg = AIRAN_BG1_MAX_DEG;
+
+__device__ inline float H16(float v) { return __half2float(__float2half_rn(v)); }
+
+// Message of edge k reconstructed from the compressed state
+// (w0 = fp16 min1 | fp16 min2 << 16, w1 = sign bits | index of min1 << 24):
+// alpha * sign * magnitude rounded to fp16, then (MODE_REF) stored as int8.
+template <int MODE>
+__device__ inline float EdgeMsg(uint32_t w0, uint32_t w1, int k)
+{
+    const uint32_t idx = w1 >> 24;
+    const uint32_t signs = w1 & 0x7ffffu;
+    const uint16_t mh = static_cast<uint16_t>(static_cast<uint32_t>(k) == idx ? (w0 >> 16) : (w0 & 0xffffu));
+    const float mag = __half2float(__ushort_as_half(mh));
+    const uint32_t neg = (__popc(signs) ^ (signs >> k)) & 1u;
+    const float nm = H16((neg ? -0.75f : 0.75f) * mag);
+    if (MODE == AIRAN_LDPC_MODE_REF) {
+        float q = rintf(nm);
+        return q > 127.0f ? 127.0f : (q < -127.0f ? -127.0f : q);
+    }
+    return nm;
+}
+
+// Check i of row r: gather, subtract old message, min1/min2/signs, scatter
+// the updated posterior (fp16 LDS).  Updates the compressed state in place.
+template <int MODE>
+__device__ inline void ProcessRow(uint16_t *lam, int r, uint32_t i, uint32_t Z, uint32_t &w0,
+                                  uint32_t &w1)
+{
+    const int e0 = kBg1.row_start[r];
+    const int deg = kBg1.row_start[r + 1] - e0;
+    uint32_t addr[kMaxDeg];
+    float ext[kMaxDeg];
+    float m1 = INFINITY, m2 = INFINITY;
+    uint32_t idx = 0, signs = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxDeg; ++k) {
+        if (k < deg) {
+            const uint32_t z = static_cast<uint32_t>(kBg1.shift[e0 + k]) % Z;
+            uint32_t p = i + z;
+            p = p >= Z ? p - Z : p;
+            addr[k] = static_cast<uint32_t>(kBg1.col[e0 + k]) * Z + p;
+            const float v = __half2float(__ushort_as_half(lam[addr[k]]));
+            ext[k] = H16(v - EdgeMsg<MODE>(w0, w1, k));
+            const float a = fabsf(ext[k]);
+            signs |= static_cast<uint32_t>(!(ext[k] >= 0.0f)) << k;
+            if (a < m1) {
+                m2 = m1;
+                m1 = a;
+                idx = static_cast<uint32_t>(k);
+            } else if (a < m2) {
+                m2 = a;
+            }
+        }
+    }
+    w0 = static_cast<uint32_t>(__half_as_ushort(__float2half_rn(m1))) |
+         (static_cast<uint32_t>(__half_as_ushort(__float2half_rn(m2))) << 16);
+    w1 = signs | (idx << 24);
+#pragma unroll
+    for (int k = 0; k < kMaxDeg; ++k) {
+        if (k < deg) {
+            const uint32_t neg = (__popc(signs) ^ (signs >> k)) & 1u;
+            const float mag = (static_cast<uint32_t>(k) == idx) ? m2 : m1;
+            const float nm = H16((neg ? -0.75f : 0.75f) * mag);
+            lam[addr[k]] = __half_as_ushort(__float2half_rn(ext[k] + nm));
+        }
+    }
+}
+
+template <int MODE>
+__device__ inline bool HardBit(const uint16_t *lam, uint32_t a)
+{
+    return __half2float(__ushort_as_half(lam[a])) < 0.0f;
+}
+
+// True when check i of every active row is satisfied by the hard decisions.
+template <int MODE>
+__device__ inline bool ChecksOk(const uint16_t *lam, int rows, uint32_t i, uint32_t Z)
+{
+    bool ok = true;
+    for (int r = 0; r < rows; ++r) {
+        const int e0 = kBg1.row_start[r], e1 = kBg1.row_start[r + 1];
+        uint32_t par = 0;
+        for (int e = e0; e < e1; ++e) {
+            uint32_t p = i + static_cast<uint32_t>(kBg1.shift[e]) % Z;
+            p = p >= Z ? p - Z : p;
+            par ^= HardBit<MODE>(lam, static_cast<uint32_t>(kBg1.col[e]) * Z + p) ? 1u : 0u;
+        }
+        ok &= par == 0;
+    }
+    return ok;
+}
+
+struct LdpcDev {
+    uint32_t z, k, iters, rows, early_stop;
+};
+
+template <int MODE, int REG_ROWS>
+__global__ void __launch_bounds__(384) LdpcKernel(const int16_t *__restrict__ lam_in,
+                                                  int8_t *__restrict__ bits,
+                                                  int16_t *__restrict__ lam_out,
+                                                  uint8_t *__restrict__ iters_out,
+                                                  uint2 *__restrict__ msg_ws, LdpcDev d)
+{
+    uint16_t *lam = airan_ldpc_lam;
+    const uint32_t Z = d.z, N = 68u * Z;
+    const uint32_t cb = blockIdx.x;
+    const uint32_t i = threadIdx.x;
+    const int16_t *src = lam_in + static_cast<size_t>(cb) * N;
+    for (uint32_t t = i; t < N; t += blockDim.x) {
+        const float scale = MODE == AIRAN_LDPC_MODE_REF ? 1.0f : 1.0f / 256.0f;
+        lam[t] = __half_as_ushort(__float2half_rn(static_cast<float>(src[t]) * scale));
+    }
+    uint32_t rw0[REG_ROWS > 0 ? REG_ROWS : 1], rw1[REG_ROWS > 0 ? REG_ROWS : 1];
+#pragma unroll
+    for (int r = 0; r < (REG_ROWS > 0 ? REG_ROWS : 1); ++r) {
+        rw0[r] = 0;
+        rw1[r] = 0;
+    }
+    uint2 *ws = msg_ws + static_cast<size_t>(cb) * d.rows * Z;
+    if (REG_ROWS == 0 && i < Z)
+        for (uint32_t r = 0; r < d.rows; ++r) ws[r * Z + i] = make_uint2(0u, 0u);
+    __syncthreads();
+    const int rows = static_cast<int>(d.rows);
+    uint32_t it = 0;
+    for (; it < d.iters; ++it) {
+        if (REG_ROWS > 0) {
+#pragma unroll
+            for (int r = 0; r < REG_ROWS; ++r) {
+                if (r < rows) {
+                    if (i < Z) ProcessRow<MODE>(lam, r, i, Z, rw0[r], rw1[r]);
+                    __syncthreads();
+                }
+            }
+        } else {
+            for (int r = 0; r < rows; ++r) {
+                if (i < Z) {
+                    uint2 s = ws[r * Z + i];
+                    ProcessRow<MODE>(lam, r, i, Z, s.x, s.y);
+                    ws[r * Z + i] = s;
+                }
+                __syncthreads();
+            }
+        }
+        if (d.early_stop) {
+            const bool ok = i < Z ? ChecksOk<MODE>(lam, rows, i, Z) : true;
+            if (__syncthreads_and(ok ? 1 : 0)) {
+                ++it;
+                break;
+            }
+        }
+    }
+    if (iters_out != nullptr && i == 0) iters_out[cb] = static_cast<uint8_t>(it);
+    int8_t *bo = bits + static_cast<size_t>(cb) * d.k;
+    for (uint32_t t = i; t < d.k; t += blockDim.x) bo[t] = HardBit<MODE>(lam, t) ? 1 : 0;
+    if (lam_out != nullptr) {
+        int16_t *lo = lam_out + static_cast<size_t>(cb) * N;
+        const float clip = MODE == AIRAN_LDPC_MODE_REF ? 5120.0f : 20.0f;
+        const float out_scale = MODE == AIRAN_LDPC_MODE_REF ? 1.0f : 256.0f;
+        for (uint32_t t = i; t < N; t += blockDim.x) {
+            float v = __half2float(__ushort_as_half(lam[t]));
+            v = v > clip ? clip : (v < -clip ? -clip : v);
+            lo[t] = static_cast<int16_t>(rintf(v * out_scale));
+        }
+    }
+}
+
+constexpr int kRegRows = 8;
+
+}  // namespace
+
+extern "C" {
+
+int32_t airan_ldpc_decode_ValidateOpArgs(const LdpcDecodeOpArgsV1 *a)
+{
+    if (!airan::HeaderOk(a, sizeof(LdpcDecodeOpArgsV1)) || a->lam_in == nullptr ||
+        a->bits == nullptr || a->stream == nullptr || a->num_cb == 0)
+        return AIRAN_INVALID_ARGUMENT;
+    if (a->lam_in == a->bits || a->lam_in == a->lam_out || a->bits == a->lam_out)
+        return AIRAN_INVALID_ARGUMENT;
+    bool z_ok = false;
+    for (uint32_t z = 3; z <= 384; z *= 2) z_ok |= a->z == z;
+    if (!z_ok || a->z < 24) return AIRAN_UNSUPPORTED_PROFILE;
+    if (a->k == 0 || a->k > 22 * a->z || a->max_iterations == 0 || a->max_iterations > 100 ||
+        a->active_rows < 4 || a->active_rows > AIRAN_BG1_ROWS ||
+        (a->mode != AIRAN_LDPC_MODE_REF && a->mode != AIRAN_LDPC_MODE_FP16) || a->num_cb > 1u << 20)
+        return AIRAN_INVALID_ARGUMENT;
+    return AIRAN_OK;
+}
+
+size_t airan_ldpc_decode_WorkspaceBytes(uint32_t num_cb, uint32_t z, uint32_t active_rows)
+{
+    if (active_rows <= static_cast<uint32_t>(kRegRows)) return 0;
+    return static_cast<size_t>(num_cb) * active_rows * z * sizeof(uint2);
+}
+
+int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, size_t workspace_bytes)
+{
+    int32_t st = airan_ldpc_decode_ValidateOpArgs(a);
+    if (st != AIRAN_OK) return st;
+    if ((st = LoadTable()) != AIRAN_OK) return st;
+    const size_t need = airan_ldpc_decode_WorkspaceBytes(a->num_cb, a->z, a->active_rows);
+    if (need > 0 && (workspace == nullptr || workspace_bytes < need)) return AIRAN_RESOURCE_TOO_SMALL;
+    LdpcDev d{a->z, a->k, a->max_iterations, a->active_rows, a->early_stop ? 1u : 0u};
+    const uint32_t threads = airan::AlignUp(a->z, 64);
+    const size_t lds = static_cast<size_t>(68) * a->z * sizeof(uint16_t);
+    hipStream_t s = static_cast<hipStream_t>(a->stream);
+    const auto *in = static_cast<const int16_t *>(a->lam_in);
+    auto *bits = static_cast<int8_t *>(a->bits);
+    auto *lo = static_cast<int16_t *>(a->lam_out);
+    auto *io = static_cast<uint8_t *>(a->iterations_out);
+    auto *ws = static_cast<uint2 *>(workspace);
+    const bool reg = a->active_rows <= static_cast<uint32_t>(kRegRows);
+    if (a->mode == AIRAN_LDPC_MODE_FP16) {
+        if (reg)
+            hipLaunchKernelGGL((LdpcKernel<AIRAN_LDPC_MODE_FP16, kRegRows>), dim3(a->num_cb), dim3(threads), lds, s, in, bits, lo, io, ws, d);
+        else
+            hipLaunchKernelGGL((LdpcKernel<AIRAN_LDPC_MODE_FP16, 0>), dim3(a->num_cb), dim3(threads), lds, s, in, bits, lo, io, ws, d);
+    } else {
+        if (reg)
+            hipLaunchKernelGGL((LdpcKernel<AIRAN_LDPC_MODE_REF, kRegRows>), dim3(a->num_cb), dim3(threads), lds, s, in, bits, lo, io, ws, d);
+        else
+            hipLaunchKernelGGL((LdpcKernel<AIRAN_LDPC_MODE_REF, 0>), dim3(a->num_cb), dim3(threads), lds, s, in, bits, lo, io, ws, d);
+    }
+    return airan::LaunchStatus();
+}
+
+// Rows 0..3 always; extension row r >= 4 only when its parity column 22 + r
+// holds a transmitted circular-buffer position.
+uint32_t airan_ldpc_ActiveRows(const AiranTbPlanV1 *p)
+{
+    if (p == nullptr || p->Z == 0) return AIRAN_BG1_ROWS;
+    const uint32_t nf = p->Ncb - p->filler;
+    const uint32_t e_max = p->E_high > p->E_low ? p->E_high : p->E_low;
+    if (p->k0 != 0 || e_max >= nf) return AIRAN_BG1_ROWS;
+    const uint32_t fstart = p->Kprime - 2 * p->Z;
+    const uint32_t last = e_max - 1;
+    const uint32_t pos = last < fstart ? last : last + p->filler;
+    const uint32_t col = (pos + 2 * p->Z) / p->Z;
+    int rows = static_cast<int>(col) - 22 + 1;
+    rows = rows < 4 ? 4 : (rows > AIRAN_BG1_ROWS ? AIRAN_BG1_ROWS : rows);
+    return static_cast<uint32_t>(rows);
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+"""ctypes access to the CPU oracle (oracle/lib/liboracle.so) and to the
+reference host sources built into oracle/_ref/libref.so.  Test-only."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE = ROOT / "oracle" / "lib" / "liboracle.so"
+REF = ROOT / "oracle" / "_ref" / "libref.so"
+
+_ORACLE = None
+
+
+class OrTbPlan(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("tbs", "C", "K", "Kprime", "Z", "filler", "G", "E_low", "E_high",
+                                          "n_low", "N", "Ncb", "k0", "bg", "crc_tb", "crc_cb")]
+
+
+class OrCeParams(C.Structure):
+    _fields_ = [("tau_s", C.c_double), ("fd_hz", C.c_double), ("scs_hz", C.c_double)]
+
+
+def load_oracle():
+    global _ORACLE
+    if _ORACLE is not None:
+        return _ORACLE
+    if not ORACLE.exists():
+        raise RuntimeError(f"{ORACLE} missing: run __graft_entry__.build()")
+    lib = C.CDLL(str(ORACLE))
+    vp, u16p, u32, u64 = C.c_void_p, C.POINTER(C.c_uint16), C.c_uint32, C.c_uint64
+    lib.or_f2h.restype = C.c_uint16
+    lib.or_f2h.argtypes = [C.c_float]
+    lib.or_h2f.restype = C.c_float
+    lib.or_h2f.argtypes = [C.c_uint16]
+    lib.or_gold.argtypes = [u32, u64, u64, vp]
+    lib.or_dmrs_cinit.restype = u32
+    lib.or_dmrs_cinit.argtypes = [vp, C.c_int]
+    lib.or_validate_runtime_config.argtypes = [vp, C.c_char_p, C.c_int]
+    lib.or_derive_config.argtypes = [vp, C.c_uint16, C.c_uint16, vp, C.c_char_p, C.c_int]
+    lib.or_re_demap_gather_index.argtypes = [vp, vp]
+    lib.or_re_demap.argtypes = [vp, vp, vp, vp, vp]
+    lib.or_dmrs_gen.argtypes = [vp, vp, vp]
+    lib.or_dmrs_ls.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.or_ce_lmmse.argtypes = [vp, vp, vp, vp, vp]
+    lib.or_detect_lmmse.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.or_qam_axis_llr.argtypes = [C.c_int, C.c_float, C.c_float, vp]
+    lib.or_qam_axis_llr_bruteforce.argtypes = [C.c_int, C.c_float, C.c_float, vp]
+    lib.or_qam_demod.argtypes = [vp, vp, vp, vp, vp]
+    lib.or_layer_demap.argtypes = [vp, vp, vp]
+    lib.or_descramble.argtypes = [vp, u32, u64, vp, vp]
+    lib.or_tbs.restype = u32
+    lib.or_tbs.argtypes = [u32, u32, u32, u32, C.c_double]
+    lib.or_tb_plan.argtypes = [u32, u32, u32, u32, u32, C.POINTER(OrTbPlan)]
+    lib.or_rate_dematch.argtypes = [C.POINTER(OrTbPlan), u32, u32, u32, u32, u32, vp, vp]
+    lib.or_ldpc_active_rows.argtypes = [u32, u32, u32, u32, u32]
+    lib.or_ldpc_decode.argtypes = [u32, u32, vp, C.c_int, C.c_int, C.c_int, u32, C.c_int, vp, vp]
+    lib.or_ldpc_encode.argtypes = [u32, vp, vp]
+    lib.or_ldpc_syndrome_ok.argtypes = [u32, vp]
+    lib.or_crc.restype = u32
+    lib.or_crc.argtypes = [C.c_int, vp, u64]
+    lib.or_ofdm_demod.argtypes = [vp, vp, vp]
+    _ORACLE = lib
+    return lib
+
+
+def load_ref():
+    if not REF.exists():
+        return None
+    lib = C.CDLL(str(REF))
+    vp = C.c_void_p
+    lib.ref_validate_runtime_config.argtypes = [vp, C.c_char_p, C.c_int]
+    lib.ref_derive_config.argtypes = [vp, C.c_uint, C.c_uint, vp]
+    lib.ref_re_demap_gather_index.argtypes = [vp, C.c_size_t]
+    lib.ref_re_demap_reference.argtypes = [vp, vp, C.c_uint, vp, vp, vp]
+    lib.ref_re_demap_profile.argtypes = [vp, vp]
+    lib.ref_layer_demap_reference.argtypes = [vp, vp, vp]
+    lib.ref_dmrs_gen_profile.argtypes = [vp, vp, vp]
+    lib.ref_dmrs_gold_basis.argtypes = [vp, vp]
+    lib.ref_dmrs_ls_profile.argtypes = [vp, vp, vp, vp]
+    lib.ref_pilot_sc_hash.restype = C.c_ulonglong
+    lib.ref_pilot_sc_hash.argtypes = [vp, C.c_uint]
+    return lib
+
+
+def ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def cref(s):
+    return C.cast(C.pointer(s), C.c_void_p)
+
+
+class OracleGeom:
+    """Python copy of OrGeom (oracle/oracle.h) for shapes in tests."""
+
+    def __init__(self, cfg):
+        a = lambda v, m: (v + m - 1) // m * m  # noqa: E731
+        self.nr, self.nl, self.qm = cfg.num_rx_antennas, cfg.num_layers, cfg.qm
+        self.fft, self.used, self.pad = cfg.fft_size, cfg.used_subcarriers, cfg.padded_subcarriers
+        self.dmrs_sym = [s for s in range(14) if (cfg.dmrs_symbol_mask >> s) & 1]
+        self.data_sym = [s for s in range(14) if not (cfg.dmrs_symbol_mask >> s) & 1]
+        self.ndmrs, self.ndata = len(self.dmrs_sym), len(self.data_sym)
+        self.n_dmrs_re = self.used // 2
+        self.ref_pad = a(self.n_dmrs_re, 128)
+        self.pilot_pad = a(self.n_dmrs_re, 64)
+        self.n_occ = self.used // 4
+        self.data_re = self.ndata * self.used
+        self.data_stride = a(self.data_re, 64)
+        self.qam_row = a(self.used, 64)
+        self.qam_plane = self.ndata * self.qam_row
+        self.cw_symbols = self.nl * self.data_re
+        self.cw_stride = self.nl * self.data_stride
+        self.cp_short = self.fft * 9 // 128
+        self.cp_long = self.cp_short + self.fft // 64
+        self.samples = 14 * self.fft + 13 * self.cp_short + self.cp_long
+
+
+def or_tb_plan(oracle, tbs, G, qm, layers, rv=0) -> OrTbPlan:
+    p = OrTbPlan()
+    st = oracle.or_tb_plan(tbs, G, qm, layers, rv, C.byref(p))
+    assert st == 0, st
+    return p
+
+
+def gold(oracle, cinit: int, start: int, count: int) -> np.ndarray:
+    c = np.zeros(count, np.uint8)
+    oracle.or_gold(cinit, start, count, ptr(c))
+    return c
