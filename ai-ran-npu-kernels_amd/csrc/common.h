@@ -19,6 +19,9 @@ constexpr uint32_t kSymbols = 14;
 
 constexpr uint32_t AlignUp(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
 
+// Per-device FFT twiddle table (op_ofdm.hip); idempotent.
+int InitOfdmTables();
+
 // Geometry derived from a PuschMimoConfig; one definition for every operator
 // so that SameLayout checks agree across the chain.
 struct Geom {

@@ -51,5 +51,6 @@ int InitGoldTables()
 
 extern "C" int32_t airan_InitDevice(void)
 {
-    return airan::InitGoldTables();
+    const int32_t st = airan::InitGoldTables();
+    return st != AIRAN_OK ? st : airan::InitOfdmTables();
 }

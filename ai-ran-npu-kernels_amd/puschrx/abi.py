@@ -29,8 +29,6 @@ LAUNCH_FAILED = -5
 IQ_INT16_Q8_8 = 0
 IQ_COMPLEX64 = 1
 DFT_FP32 = 0
-DFT_FP16 = 1
-DFT_BF16 = 2
 LDPC_MODE_REF = 0
 LDPC_MODE_FP16 = 1
 DESCR_RESTART_PER_SLOT = 1
@@ -339,6 +337,10 @@ PROTOTYPES = {
     "airan_ldpc_ActiveRows": (u32, [P(AiranTbPlanV1)]),
     "airan_crc_check_WorkspaceBytes": (C.c_size_t, [u32]),
     "airan_crc_check_Enqueue": (i32, [P(AiranTbPlanV1), u32, vp, vp, vp, vp, vp, C.c_size_t, vp]),
+    # synthetic transmitter (include/airan_pusch_tx.h)
+    "airan_pusch_tx_EncodeTbs": (i32, [P(PuschMimoConfig), P(AiranTbPlanV1), u32, u32, vp, vp, vp]),
+    "airan_ldpc_bg1_Encode": (i32, [u32, vp, vp]),
+    "airan_crc_Compute": (u32, [u32, vp, C.c_uint64]),
 }
 
 _LIB = None

@@ -270,3 +270,134 @@ def crc_check(plan: abi.AiranTbPlanV1, num_tbs: int, bits, want_bytes: bool = Fa
                                                       _dp(tb_bytes), _dp(ws), ws.numel(),
                                                       C.c_void_p(stream_ptr())))
     return cb_ok, tb_ok, tb_bytes
+
+
+# ---------------------------------------------------------------- OFDM demod
+def ofdm_demod(cfg, iq, batch: int, iq_format: int = abi.IQ_INT16_Q8_8, fused_redemap: bool = True):
+    """CP removal + unitary DFT of `batch` antenna streams (one slot each).
+    fused_redemap: return the used-subcarrier grid [batch,14,padded];
+    otherwise the stage-4 planes [batch,14,fft]."""
+    lib = load_library()
+    g = Geometry(cfg)
+    dev = iq.device
+    n = batch * 14 * (g.pad if fused_redemap else g.fft)
+    ore, oim = _dev(n, torch.float16, dev), _dev(n, torch.float16, dev)
+    a = _hdr(abi.OfdmDemodBatchOpArgsV1, iq=_dp(iq), iq_format=iq_format, batch=batch, precision=abi.DFT_FP32,
+             config=C.pointer(cfg), stream=C.c_void_p(stream_ptr()))
+    if fused_redemap:
+        _check("ofdm_demod_redemap Enqueue", lib.airan_ofdm_demod_redemap_Enqueue(C.byref(a), _dp(ore), _dp(oim)))
+    else:
+        a.out_re, a.out_im = _dp(ore), _dp(oim)
+        _check("ofdm_demod Enqueue", lib.airan_ofdm_demod_batch_Enqueue(C.byref(a)))
+    return ore, oim
+
+
+# ------------------------------------------------------------------ CE LMMSE
+class LmmseFilter:
+    """Owns a device filter built by airan_channel_est_lmmse_CreateFilter."""
+
+    def __init__(self, cfg, tau_s: float = 100e-9, fd_hz: float = 0.0, scs_hz: float = 30e3,
+                 max_rank: int = 512, eig_floor: float = 0.0):
+        self._lib = load_library()
+        self.cfg = cfg
+        p = _hdr(abi.LmmseFilterParamsV1, max_rank=max_rank, tau_s=tau_s, fd_hz=fd_hz, scs_hz=scs_hz,
+                 eig_floor=eig_floor)
+        self.params = p
+        self.handle = C.c_void_p()
+        _check("ce CreateFilter", self._lib.airan_channel_est_lmmse_CreateFilter(
+            C.byref(cfg), C.byref(p), C.c_void_p(stream_ptr()), C.byref(self.handle)))
+
+    def rank(self, layer: int) -> int:
+        return int(self._lib.airan_channel_est_lmmse_FilterRank(self.handle, layer))
+
+    def close(self):
+        if self.handle:
+            self._lib.airan_channel_est_lmmse_DestroyFilter(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ce_lmmse(cfg, filt: LmmseFilter, h_ls_re, h_ls_im, pilot_sc, pilot_count, noise_var_rx, out=None,
+             workspace=None):
+    """LMMSE channel estimate [NR,16,14,padded] fp16 planes (layers >= L untouched)."""
+    lib = load_library()
+    g = Geometry(cfg)
+    dev = h_ls_re.device
+    lay = layout(cfg)
+    cnt_h = (C.c_uint16 * 16)()
+    sc_h = (C.c_uint16 * (g.nl * g.ndmrs * g.pilot_pad))()
+    for l in range(g.nl):
+        comb = (cfg.dmrs_ports[l] - 1000) // 2
+        occ = sum((cfg.dmrs_ports[q] - 1000) // 2 == comb for q in range(g.nl)) == 2
+        n = g.n_occ if occ else g.n_dmrs_re
+        for d in range(g.ndmrs):
+            cnt_h[l * g.ndmrs + d] = n
+            for p in range(n):
+                sc_h[(l * g.ndmrs + d) * g.pilot_pad + p] = comb + (1 + 4 * p if occ else 2 * p)
+    model = abi.LmmseWeightModelV1()
+    _check("ce BuildWeightModel", lib.airan_channel_est_lmmse_BuildWeightModel(C.byref(cfg), C.byref(lay), cnt_h,
+                                                                               sc_h, C.byref(model)))
+    if out is None:
+        out = torch.zeros(g.nr * 16 * 14 * g.pad * 2, dtype=torch.float16, device=dev)
+    hr, hi = out[: g.nr * 16 * 14 * g.pad], out[g.nr * 16 * 14 * g.pad:]
+    a = _hdr(abi.ChannelEstLmmseOpArgsV1, h_ls_re=_dp(h_ls_re), h_ls_im=_dp(h_ls_im), pilot_count=_dp(pilot_count),
+             pilot_sc=_dp(pilot_sc), pilot_count_host=cnt_h, pilot_sc_host=sc_h, weight_model=C.pointer(model),
+             h_grid_re=_dp(hr), h_grid_im=_dp(hi), config=C.pointer(cfg), layout=C.pointer(lay),
+             stream=C.c_void_p(stream_ptr()))
+    nws = int(lib.airan_channel_est_lmmse_WorkspaceBytes(C.byref(cfg)))
+    if workspace is None:
+        workspace = _dev(nws, torch.uint8, dev)
+    _check("ce Enqueue", lib.airan_channel_est_lmmse_Enqueue(C.byref(a), filt.handle, _dp(noise_var_rx),
+                                                             _dp(workspace), workspace.numel()))
+    return hr, hi, workspace
+
+
+# ----------------------------------------------------------------- detection
+def layer_plan(cfg) -> abi.MimoDetectLayerPlan:
+    p = _hdr(abi.MimoDetectLayerPlan, num_rx_antennas=cfg.num_rx_antennas, layer_capacity=16,
+             total_layers=cfg.num_layers, num_allocations=1)
+    p.allocations[0].pusch_index = 0
+    p.allocations[0].layer_offset = 0
+    p.allocations[0].num_layers = cfg.num_layers
+    p.allocations[0].num_tx_ports = cfg.num_tx_ports
+    return p
+
+
+def detect_io_pack(cfg, rx_re, rx_im, h_re, h_im, noise_var_rx):
+    lib = load_library()
+    g = Geometry(cfg)
+    dev = rx_re.device
+    plan = layer_plan(cfg)
+    meta = abi.MimoDetectIoPackKernelMetadata()
+    _check("io_pack BuildCurrentProfile",
+           lib.airan_mimo_detect_io_pack_BuildCurrentProfile(C.byref(cfg), 1, C.byref(plan), C.byref(meta)))
+    n_re = g.n_re
+    hrm_r, hrm_i = _dev(n_re * g.nr * 16, torch.float16, dev), _dev(n_re * g.nr * 16, torch.float16, dev)
+    yv_r, yv_i = _dev(n_re // 8 * g.nr * 16, torch.float16, dev), _dev(n_re // 8 * g.nr * 16, torch.float16, dev)
+    no = _dev(n_re, torch.float16, dev)
+    a = _hdr(abi.MimoDetectIoPackOpArgsV1, rx_grid_re=_dp(rx_re), rx_grid_im=_dp(rx_im), h_grid_re=_dp(h_re),
+             h_grid_im=_dp(h_im), noise_var_rx=_dp(noise_var_rx), hrm_re=_dp(hrm_r), hrm_im=_dp(hrm_i),
+             yvpad_re=_dp(yv_r), yvpad_im=_dp(yv_i), no=_dp(no), configs=C.pointer(cfg), num_configs=1,
+             layer_plan=C.pointer(plan), stream=C.c_void_p(stream_ptr()))
+    meta_d = torch.frombuffer(bytearray(bytes(meta)), dtype=torch.uint8).to(dev)
+    _check("io_pack Enqueue", lib.airan_mimo_detect_io_pack_Enqueue(C.byref(a), _dp(meta_d), meta_d.numel()))
+    return hrm_r, hrm_i, yv_r, yv_i, no
+
+
+def detect_lmmse(cfg, hrm_r, hrm_i, yv_r, yv_i, no):
+    lib = load_library()
+    g = Geometry(cfg)
+    dev = hrm_r.device
+    plan = layer_plan(cfg)
+    n = 16 * g.n_re
+    xr, xi, ne = _dev(n, torch.float16, dev), _dev(n, torch.float16, dev), _dev(n, torch.float16, dev)
+    a = _hdr(abi.MimoDetectLmmseOpArgsV1, hrm_re=_dp(hrm_r), hrm_im=_dp(hrm_i), yvpad_re=_dp(yv_r),
+             yvpad_im=_dp(yv_i), no=_dp(no), xhat_re=_dp(xr), xhat_im=_dp(xi), no_eff=_dp(ne),
+             config=C.pointer(cfg), layer_plan=C.pointer(plan), stream=C.c_void_p(stream_ptr()))
+    _check("detect Enqueue", lib.airan_mimo_detect_lmmse_Enqueue(C.byref(a)))
+    return xr, xi, ne

@@ -154,9 +154,10 @@ REFERENCE_FD16X64 = Profile(name="fd16x64_full_band", rx=64, rank=4, num_rb=133,
 FULL_CARRIER_4RX = Profile(name="fd4x4_273prb", rx=4, rank=4, num_rb=273, fft=4096, qm=8,
                            padded_subcarriers=3328, code_rate=873 / 1024)
 
-# Mid-size plain MIMO profile: 4 Rx, rank 2, 106 PRB, FFT 2048, 64QAM, R ~ 0.6.
+# Mid-size plain MIMO profile: 4 Rx, rank 2 on CDM ports 0/1 (one comb,
+# FD-OCC2), 106 PRB, FFT 2048, 64QAM, R ~ 0.6.
 MID_4RX_RANK2 = Profile(name="fd4x2_106prb", rx=4, rank=2, num_rb=106, fft=2048, qm=6,
-                        padded_subcarriers=1280, code_rate=0.6)
+                        padded_subcarriers=1280, code_rate=0.6, ports=(1000, 1001))
 
 # Small SISO profile: 1 Rx, rank 1, 52 PRB, FFT 1024, 256QAM, one DMRS symbol.
 SISO_52PRB = Profile(name="fd1x1_52prb", rx=1, rank=1, num_rb=52, fft=1024, qm=8, dmrs_symbols=(2,),

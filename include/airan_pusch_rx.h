@@ -64,13 +64,12 @@ int32_t airan_DerivePuschMimoLayout(const PuschMimoConfig *config, PuschMimoLayo
  * (chain/pusch_mimo_rx_chain/rx_chain.cpp:471-475; kernel
  * ofdm/ofdm_demod_batch/ofdm_demod_kernel.cpp:889).  The DFT factors the
  * reference loads from files are generated inside the library.
+ * Input: `batch` antenna streams of one slot each (15 * fft samples).
  * Output: stage-4 layout fp16 planes [batch,14,P,64], bin k at
  * [(k % P) * 64 + k / P], unitary DFT, no fftshift. */
 #define AIRAN_IQ_INT16_Q8_8 0   /* int16 [batch, samples, 2], value / 256 */
 #define AIRAN_IQ_COMPLEX64 1    /* float [batch, samples, 2] */
-#define AIRAN_DFT_FP32 0        /* fp32 operands, fp32 accumulation */
-#define AIRAN_DFT_FP16 1        /* fp16 MFMA operands, fp32 accumulation */
-#define AIRAN_DFT_BF16 2        /* bf16 MFMA operands, fp32 accumulation */
+#define AIRAN_DFT_FP32 0        /* fp32 FFT (the only precision; the op is HBM bound) */
 typedef struct OfdmDemodBatchOpArgsV1 {
     uint16_t abi_version;
     uint16_t struct_size;
@@ -79,7 +78,7 @@ typedef struct OfdmDemodBatchOpArgsV1 {
     uint32_t batch;             /* number of antenna streams */
     void *out_re;
     void *out_im;
-    uint32_t precision;         /* AIRAN_DFT_* */
+    uint32_t precision;         /* AIRAN_DFT_FP32 */
     uint32_t reserved;
     const PuschMimoConfig *config;
     void *stream;
@@ -244,7 +243,7 @@ typedef struct ChannelEstLmmseOpArgsV1 {
     const uint16_t *pilot_count_host;
     const uint16_t *pilot_sc_host;
     const LmmseWeightModelV1 *weight_model;
-    void *h_grid_re;            /* fp16 [NR,16,14,padded], layers >= L zero */
+    void *h_grid_re;            /* fp16 [NR,16,14,padded]; only layers < L written */
     void *h_grid_im;
     const PuschMimoConfig *config;
     const PuschMimoLayout *layout;
