@@ -1,0 +1,470 @@
+// chain.hip -- the batched PUSCH receive chain (include/airan_pusch_chain.h).
+//
+// One pipeline of batched launches over `slots` slots (one TB each); the only
+// kernel specific to the chain is the fused detector/demapper, the rest are
+// the per-operator kernels launched over all slots or code blocks at once.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "airan_pusch_chain.h"
+#include "common.h"
+#include "detect.h"
+#include "gold.h"
+#include "launch.h"
+#include "qam.h"
+
+using airan::Cf;
+using airan::Geom;
+
+namespace {
+
+struct DemapArgs {
+    uint32_t nr, qm, used, pad, nsym, cw_symbols, cw_stride;
+    uint32_t data_sym[14];
+    uint16_t floor16;  // fp16 bits of the no_eff floor
+};
+
+// Packed Gold bits c(0 .. 32 words) of the data scrambling sequence: word w
+// holds c(32 w .. 32 w + 31), bit j = c(32 w + j).
+__global__ void __launch_bounds__(256) GoldBitsKernel(uint32_t *__restrict__ bits, uint32_t words, uint32_t cinit)
+{
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    uint32_t s1, s2;
+    airan::GoldStateAt(32ull * w + 1600ull, cinit, &s1, &s2);
+    bits[w] = airan::GoldNext32(&s1, &s2);
+}
+
+__device__ inline int16_t SatNeg16(int v) { return static_cast<int16_t>(v == -32768 ? 32767 : -v); }
+
+// Detection + QAM demap + layer demap + descramble for one data RE of one
+// slot (grid (used / 256, ndata, slots)).  The arithmetic is that of the
+// operators, step by step:
+//   no     = fp16(sum_rx fp32(noise_rx) * (1 / NR))     (detector I/O pack)
+//   x, ne  = Detect<L>(...) rounded to fp16              (detect_lmmse)
+//   ne     = max(ne, fp16(floor))                        (no_eff floor)
+//   llr    = QamSymbolLlrs(x, ne)                        (qam256_demod)
+//   cw[b][re L + l] = +-llr[stream(b)] by c((re L + l) Qm + b)
+//                                                        (layer_demap, descramble)
+template <int NL>
+__global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restrict__ rx_re,
+                                                         const __half *__restrict__ rx_im,
+                                                         const __half *__restrict__ h_re,
+                                                         const __half *__restrict__ h_im,
+                                                         const __half *__restrict__ noise,
+                                                         const uint32_t *__restrict__ gold,
+                                                         int16_t *__restrict__ cw, DemapArgs a)
+{
+    const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc >= a.used) return;
+    const uint32_t ds = blockIdx.y, slot = blockIdx.z;
+    const uint32_t sym = a.data_sym[ds];
+    const uint32_t nr = a.nr;
+    float s = 0.0f;
+    for (uint32_t r = 0; r < nr; ++r) s += __half2float(noise[static_cast<size_t>(slot) * nr + r]);
+    const float no = __half2float(__float2half_rn(s * (1.0f / static_cast<float>(nr))));
+    const size_t plane = static_cast<size_t>(a.nsym) * a.pad;
+    const size_t off = static_cast<size_t>(sym) * a.pad + sc;
+    const size_t rx_slot = static_cast<size_t>(slot) * nr * plane;
+    const size_t h_slot = rx_slot * NL;
+    auto loadh = [&](int r, int l) {
+        const size_t i = h_slot + (static_cast<size_t>(r) * NL + l) * plane + off;
+        return Cf{__half2float(h_re[i]), __half2float(h_im[i])};
+    };
+    auto loady = [&](int r) {
+        const size_t i = rx_slot + static_cast<size_t>(r) * plane + off;
+        return Cf{__half2float(rx_re[i]), __half2float(rx_im[i])};
+    };
+    Cf xh[4];
+    float ne[4];
+    airan::Detect<NL>(static_cast<int>(nr), no, loadh, loady, xh, ne);
+    const uint32_t qm = a.qm, m = qm / 2;
+    const uint32_t re = ds * a.used + sc;
+    const uint64_t bit0 = static_cast<uint64_t>(re) * NL * qm;
+    const uint32_t w0 = static_cast<uint32_t>(bit0 >> 5);
+    const uint64_t gw = (static_cast<uint64_t>(gold[w0]) | (static_cast<uint64_t>(gold[w0 + 1]) << 32)) >> (bit0 & 31u);
+    const __half floor16 = __ushort_as_half(a.floor16);
+    int16_t *out = cw + static_cast<size_t>(slot) * qm * a.cw_stride + static_cast<size_t>(re) * NL;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        const __half xr16 = __float2half_rn(xh[l].r), xi16 = __float2half_rn(xh[l].i);
+        __half ne16 = __float2half_rn(ne[l]);
+        if (__half2float(ne16) < __half2float(floor16)) ne16 = floor16;
+        int llr[8];
+        airan::QamSymbolLlrs(qm, xr16, xi16, ne16, llr);
+        for (uint32_t b = 0; b < qm; ++b) {
+            const uint32_t stream = (b & 1u) ? m + b / 2 : b / 2;
+            const int v = llr[stream];
+            const bool flip = (gw >> (static_cast<uint32_t>(l) * qm + b)) & 1u;
+            out[static_cast<size_t>(b) * a.cw_stride + l] = flip ? SatNeg16(v) : static_cast<int16_t>(v);
+        }
+    }
+}
+
+// Host fp32 -> fp16 bits, round to nearest even (normal range only).
+uint16_t HostF2H(float f)
+{
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const int32_t exp = static_cast<int32_t>((x >> 23) & 0xff) - 127 + 15;
+    uint32_t mant = x & 0x7fffffu;
+    if (exp <= 0 || exp >= 31) return static_cast<uint16_t>(sign | (exp >= 31 ? 0x7c00u : 0u));
+    uint32_t h = sign | (static_cast<uint32_t>(exp) << 10) | (mant >> 13);
+    const uint32_t rem = mant & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+    return static_cast<uint16_t>(h);
+}
+
+template <typename T>
+bool Alloc(T **p, size_t count, size_t *total)
+{
+    *p = nullptr;
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    if (hipMalloc(reinterpret_cast<void **>(p), bytes) != hipSuccess) return false;
+    *total += bytes;
+    return true;
+}
+
+void Free(void *p)
+{
+    if (p != nullptr) (void)hipFree(p);
+}
+
+}  // namespace
+
+struct AiranPuschChain {
+    int device = 0;
+    PuschMimoConfig cfg{};
+    AiranTbPlanV1 plan{};
+    AiranPuschChainOptionsV1 opts{};
+    Geom g{};
+    uint32_t active_rows = 0;
+    uint32_t last_slots = 0;
+    size_t bytes = 0;
+    LmmseFilter *filter = nullptr;
+    __half *dmrs_re = nullptr, *dmrs_im = nullptr;
+    __half *rx_re = nullptr, *rx_im = nullptr;
+    __half *hls_re = nullptr, *hls_im = nullptr;
+    __half *noise_ls = nullptr, *noise_ce = nullptr;
+    uint8_t *ce_ws = nullptr;
+    size_t ce_ws_bytes = 0;
+    __half *h_re = nullptr, *h_im = nullptr;
+    uint32_t *gold = nullptr;
+    int16_t *cw = nullptr;
+    RateMatchDescriptor *desc = nullptr;
+    int16_t *lam = nullptr;
+    uint8_t *ldpc_ws = nullptr;
+    size_t ldpc_ws_bytes = 0;
+    int8_t *bits = nullptr;
+    uint8_t *iters = nullptr, *cb_ok = nullptr, *tb_ok = nullptr;
+    uint8_t *crc_ws = nullptr;
+    size_t crc_ws_bytes = 0;
+    void *zero_iq = nullptr;
+    // Stage timing (events before the first stage and after each stage).
+    bool timing = false;
+    hipEvent_t ev[AIRAN_CHAIN_STAGES + 1] = {};
+    // Cached graph for the last (iq, slots, stream).
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *graph_iq = nullptr;
+    uint32_t graph_slots = 0;
+    void *graph_stream = nullptr;
+};
+
+namespace {
+
+void DropGraph(AiranPuschChain *c)
+{
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->exec = nullptr;
+    c->graph = nullptr;
+    c->graph_iq = nullptr;
+    c->graph_slots = 0;
+    c->graph_stream = nullptr;
+}
+
+int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStream_t s, bool timed)
+{
+    const Geom &g = c->g;
+    int stage = 0;
+    auto mark = [&]() {
+        if (timed) (void)hipEventRecord(c->ev[stage++], s);
+    };
+    mark();
+    OfdmDemodBatchOpArgsV1 oa{};
+    oa.abi_version = AIRAN_PUSCH_MIMO_ABI_VERSION;
+    oa.struct_size = sizeof(oa);
+    oa.iq = iq;
+    oa.iq_format = c->opts.iq_format;
+    oa.batch = slots * g.nr;
+    oa.precision = AIRAN_DFT_FP32;
+    oa.config = &c->cfg;
+    oa.stream = s;
+    int32_t st = airan_ofdm_demod_redemap_Enqueue(&oa, c->rx_re, c->rx_im);
+    if (st != AIRAN_OK) return st;
+    mark();
+    st = airan::LaunchDmrsLs(g, c->rx_re, c->rx_im, c->dmrs_re, c->dmrs_im, c->hls_re, c->hls_im, c->noise_ls,
+                             slots, s);
+    if (st != AIRAN_OK) return st;
+    mark();
+    st = airan::LaunchCeApply(c->filter, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce, c->ce_ws, c->ce_ws_bytes,
+                              c->h_re, c->h_im, g.nl, slots, s);
+    if (st != AIRAN_OK) return st;
+    mark();
+    DemapArgs da{};
+    da.nr = g.nr;
+    da.qm = g.qm;
+    da.used = g.used;
+    da.pad = g.pad;
+    da.nsym = g.nsym;
+    da.cw_symbols = g.cw_symbols;
+    da.cw_stride = g.cw_stride;
+    for (uint32_t i = 0; i < g.ndata; ++i) da.data_sym[i] = g.data_sym[i];
+    da.floor16 = HostF2H(airan_pusch_NoEffFloor(g.qm));
+    const dim3 grid(airan::CeilDiv(g.used, 256), g.ndata, slots);
+    switch (g.nl) {
+    case 1: hipLaunchKernelGGL(DetectDemapKernel<1>, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw, da); break;
+    case 2: hipLaunchKernelGGL(DetectDemapKernel<2>, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw, da); break;
+    case 3: hipLaunchKernelGGL(DetectDemapKernel<3>, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw, da); break;
+    default: hipLaunchKernelGGL(DetectDemapKernel<4>, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw, da); break;
+    }
+    if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
+    mark();
+    const uint32_t ncb = slots * c->plan.C;
+    st = airan::LaunchRateDematch(g, c->plan, c->desc, c->cw, c->lam, ncb, s);
+    if (st != AIRAN_OK) return st;
+    mark();
+    LdpcDecodeOpArgsV1 la{};
+    la.abi_version = AIRAN_PUSCH_MIMO_ABI_VERSION;
+    la.struct_size = sizeof(la);
+    la.lam_in = c->lam;
+    la.bits = c->bits;
+    la.iterations_out = c->iters;
+    la.num_cb = ncb;
+    la.z = c->plan.Z;
+    la.k = c->plan.K;
+    la.max_iterations = c->opts.ldpc_iterations;
+    la.active_rows = c->active_rows;
+    la.mode = c->opts.ldpc_mode;
+    la.early_stop = c->opts.early_stop ? 1u : 0u;
+    la.stream = s;
+    st = airan_ldpc_decode_Enqueue(&la, c->ldpc_ws, c->ldpc_ws_bytes);
+    if (st != AIRAN_OK) return st;
+    mark();
+    st = airan_crc_check_Enqueue(&c->plan, slots, c->bits, c->cb_ok, c->tb_ok, nullptr, c->crc_ws,
+                                 c->crc_ws_bytes, s);
+    if (st != AIRAN_OK) return st;
+    mark();
+    return AIRAN_OK;
+}
+
+bool OptionsOk(const AiranPuschChainOptionsV1 &o)
+{
+    return o.abi_version == AIRAN_PUSCH_MIMO_ABI_VERSION && o.struct_size >= sizeof(AiranPuschChainOptionsV1) &&
+           o.max_slots >= 1 && o.max_slots <= 4096 &&
+           (o.iq_format == AIRAN_IQ_INT16_Q8_8 || o.iq_format == AIRAN_IQ_COMPLEX64) && o.ldpc_iterations >= 1 &&
+           o.ldpc_iterations <= 64 && (o.ldpc_mode == AIRAN_LDPC_MODE_REF || o.ldpc_mode == AIRAN_LDPC_MODE_FP16) &&
+           o.ce_max_rank >= 1 && o.ce_max_rank <= 512 && o.tau_s >= 0.0 && o.fd_hz >= 0.0 && o.scs_hz > 0.0 &&
+           o.eig_floor >= 0.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPlanV1 *plan,
+                                 const AiranPuschChainOptionsV1 *options, void *stream, AiranPuschChain **chain)
+{
+    if (config == nullptr || plan == nullptr || options == nullptr || stream == nullptr || chain == nullptr ||
+        !OptionsOk(*options))
+        return AIRAN_INVALID_ARGUMENT;
+    Geom g;
+    int32_t st = airan::DeriveGeom(*config, &g);
+    if (st != AIRAN_OK) return st;
+    if (config->flags != 0 || g.ndmrs > 2 || (g.pad & 1u) != 0) return AIRAN_UNSUPPORTED_PROFILE;
+    const AiranTbPlanV1 &p = *plan;
+    if (p.C == 0 || p.Z < 24 || p.K != 22 * p.Z || p.G != g.cw_symbols * g.qm ||
+        static_cast<uint64_t>(p.n_low) * p.E_low + static_cast<uint64_t>(p.C - p.n_low) * p.E_high != p.G ||
+        p.E_low % g.qm != 0 || p.E_high % g.qm != 0 || p.Ncb <= p.filler ||
+        static_cast<uint64_t>(options->max_slots) * p.C > 65535)
+        return AIRAN_INVALID_ARGUMENT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto *c = new AiranPuschChain{};
+    c->cfg = *config;
+    c->plan = p;
+    c->opts = *options;
+    c->g = g;
+    c->active_rows = airan_ldpc_ActiveRows(plan);
+    auto fail = [&](int32_t code) {
+        airan_pusch_chain_Destroy(c);
+        return code;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess || airan_InitDevice() != AIRAN_OK) return fail(AIRAN_LAUNCH_FAILED);
+    LmmseFilterParamsV1 fp{};
+    fp.abi_version = AIRAN_PUSCH_MIMO_ABI_VERSION;
+    fp.struct_size = sizeof(fp);
+    fp.max_rank = options->ce_max_rank;
+    fp.tau_s = options->tau_s;
+    fp.fd_hz = options->fd_hz;
+    fp.scs_hz = options->scs_hz;
+    fp.eig_floor = options->eig_floor;
+    if ((st = airan_channel_est_lmmse_CreateFilter(config, &fp, stream, &c->filter)) != AIRAN_OK) return fail(st);
+    const size_t S = options->max_slots;
+    const size_t plane = static_cast<size_t>(g.nsym) * g.pad;
+    const size_t ncb = S * p.C;
+    const size_t gold_words = (static_cast<size_t>(g.cw_symbols) * g.qm + 31) / 32 + 2;
+    c->ce_ws_bytes = airan::CeWorkspaceBytes(g, static_cast<uint32_t>(S));
+    c->ldpc_ws_bytes = airan_ldpc_decode_WorkspaceBytes(static_cast<uint32_t>(ncb), p.Z, c->active_rows);
+    c->crc_ws_bytes = airan_crc_check_WorkspaceBytes(static_cast<uint32_t>(S));
+    const size_t iq_elem = options->iq_format == AIRAN_IQ_COMPLEX64 ? 8 : 4;
+    size_t &b = c->bytes;
+    if (!Alloc(&c->dmrs_re, static_cast<size_t>(g.nl) * g.ndmrs * g.ref_pad, &b) ||
+        !Alloc(&c->dmrs_im, static_cast<size_t>(g.nl) * g.ndmrs * g.ref_pad, &b) ||
+        !Alloc(&c->rx_re, S * g.nr * plane, &b) || !Alloc(&c->rx_im, S * g.nr * plane, &b) ||
+        !Alloc(&c->hls_re, S * g.nr * g.nl * g.ndmrs * g.pilot_pad, &b) ||
+        !Alloc(&c->hls_im, S * g.nr * g.nl * g.ndmrs * g.pilot_pad, &b) ||
+        !Alloc(&c->noise_ls, S * g.nr, &b) || !Alloc(&c->noise_ce, S * g.nr, &b) ||
+        !Alloc(&c->ce_ws, c->ce_ws_bytes, &b) || !Alloc(&c->h_re, S * g.nr * g.nl * plane, &b) ||
+        !Alloc(&c->h_im, S * g.nr * g.nl * plane, &b) || !Alloc(&c->gold, gold_words, &b) ||
+        !Alloc(&c->cw, S * g.qm * g.cw_stride, &b) || !Alloc(&c->desc, ncb, &b) ||
+        !Alloc(&c->lam, ncb * 68 * p.Z, &b) || !Alloc(&c->ldpc_ws, c->ldpc_ws_bytes, &b) ||
+        !Alloc(&c->bits, ncb * p.K, &b) || !Alloc(&c->iters, ncb, &b) || !Alloc(&c->cb_ok, ncb, &b) ||
+        !Alloc(&c->tb_ok, S, &b) || !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
+        !Alloc(reinterpret_cast<uint8_t **>(&c->zero_iq), g.nr * g.samples * iq_elem, &b))
+        return fail(AIRAN_RESOURCE_TOO_SMALL);
+    // DMRS reference of this slot configuration.
+    MimoDmrsGenRuntimeV1 *rt = nullptr;
+    if ((st = airan_mimo_dmrs_gen_CreateRuntime(&rt)) != AIRAN_OK) return fail(st);
+    PuschMimoLayout lay{};
+    airan_DerivePuschMimoLayout(config, &lay);
+    MimoDmrsGenOpArgsV1 ga{};
+    ga.abi_version = AIRAN_PUSCH_MIMO_ABI_VERSION;
+    ga.struct_size = sizeof(ga);
+    ga.dmrs_re = c->dmrs_re;
+    ga.dmrs_im = c->dmrs_im;
+    ga.config = config;
+    ga.layout = &lay;
+    ga.stream = stream;
+    st = airan_mimo_dmrs_gen_Enqueue(rt, &ga);
+    const bool sync_ok = hipStreamSynchronize(s) == hipSuccess;
+    airan_mimo_dmrs_gen_DestroyRuntime(rt);
+    if (st != AIRAN_OK || !sync_ok) return fail(st != AIRAN_OK ? st : AIRAN_LAUNCH_FAILED);
+    // Gold bits of one slot's codeword (scrambling restarts every slot).
+    const uint32_t cinit = (static_cast<uint32_t>(config->rnti) << 15) |
+                           ((static_cast<uint32_t>(config->codeword_index) & 1u) << 14) | config->data_scrambling_id;
+    hipLaunchKernelGGL(GoldBitsKernel, dim3(airan::CeilDiv(gold_words, 256)), dim3(256), 0, s, c->gold,
+                       static_cast<uint32_t>(gold_words), cinit);
+    // Rate-match descriptors of every code block of every slot.
+    std::vector<RateMatchDescriptor> desc(ncb);
+    for (size_t slot = 0; slot < S; ++slot) {
+        uint64_t off = slot * static_cast<uint64_t>(g.cw_symbols);
+        for (uint32_t r = 0; r < p.C; ++r) {
+            const uint32_t e = r < p.n_low ? p.E_low : p.E_high;
+            desc[slot * p.C + r] = {e, p.k0, p.Ncb, static_cast<uint32_t>(off)};
+            off += e / g.qm;
+        }
+    }
+    if (S * g.cw_symbols > 0xffffffffull) return fail(AIRAN_INVALID_ARGUMENT);
+    if (hipMemcpyAsync(c->desc, desc.data(), ncb * sizeof(RateMatchDescriptor), hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemsetAsync(c->cw, 0, S * g.qm * g.cw_stride * sizeof(int16_t), s) != hipSuccess ||
+        hipMemsetAsync(c->zero_iq, 0, g.nr * g.samples * iq_elem, s) != hipSuccess)
+        return fail(AIRAN_LAUNCH_FAILED);
+    // One eager slot loads every per-device table before any graph capture.
+    for (hipEvent_t &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
+    if ((st = EnqueueStages(c, c->zero_iq, 1, s, false)) != AIRAN_OK) return fail(st);
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
+    c->last_slots = 1;
+    *chain = c;
+    return AIRAN_OK;
+}
+
+void airan_pusch_chain_Destroy(AiranPuschChain *c)
+{
+    if (c == nullptr) return;
+    DropGraph(c);
+    for (hipEvent_t e : c->ev)
+        if (e != nullptr) (void)hipEventDestroy(e);
+    airan_channel_est_lmmse_DestroyFilter(c->filter);
+    void *bufs[] = {c->dmrs_re, c->dmrs_im, c->rx_re, c->rx_im, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce,
+                    c->ce_ws, c->h_re, c->h_im, c->gold, c->cw, c->desc, c->lam, c->ldpc_ws, c->bits,
+                    c->iters, c->cb_ok, c->tb_ok, c->crc_ws, c->zero_iq};
+    for (void *p : bufs) Free(p);
+    delete c;
+}
+
+int32_t airan_pusch_chain_Enqueue(AiranPuschChain *c, const void *iq, uint32_t num_slots, void *stream)
+{
+    if (c == nullptr || iq == nullptr || stream == nullptr || num_slots == 0 || num_slots > c->opts.max_slots)
+        return AIRAN_INVALID_ARGUMENT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    c->last_slots = num_slots;
+    if (c->timing || !c->opts.use_graph) return EnqueueStages(c, iq, num_slots, s, c->timing);
+    if (c->exec == nullptr || c->graph_iq != iq || c->graph_slots != num_slots || c->graph_stream != stream) {
+        DropGraph(c);
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return AIRAN_LAUNCH_FAILED;
+        const int32_t st = EnqueueStages(c, iq, num_slots, s, false);
+        hipGraph_t graph = nullptr;
+        const hipError_t end = hipStreamEndCapture(s, &graph);
+        if (st != AIRAN_OK || end != hipSuccess || graph == nullptr) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st != AIRAN_OK ? st : AIRAN_LAUNCH_FAILED;
+        }
+        c->graph = graph;
+        if (hipGraphInstantiate(&c->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            DropGraph(c);
+            return AIRAN_LAUNCH_FAILED;
+        }
+        c->graph_iq = iq;
+        c->graph_slots = num_slots;
+        c->graph_stream = stream;
+    }
+    return hipGraphLaunch(c->exec, s) == hipSuccess ? AIRAN_OK : AIRAN_LAUNCH_FAILED;
+}
+
+int32_t airan_pusch_chain_Outputs(const AiranPuschChain *c, AiranPuschChainOutputsV1 *o)
+{
+    if (c == nullptr || o == nullptr || o->abi_version != AIRAN_PUSCH_MIMO_ABI_VERSION ||
+        o->struct_size < sizeof(AiranPuschChainOutputsV1))
+        return AIRAN_INVALID_ARGUMENT;
+    o->num_slots = c->last_slots;
+    o->bits = c->bits;
+    o->tb_ok = c->tb_ok;
+    o->cb_ok = c->cb_ok;
+    o->iterations = c->iters;
+    o->rx_grid_re = c->rx_re;
+    o->rx_grid_im = c->rx_im;
+    o->h_ls_re = c->hls_re;
+    o->h_ls_im = c->hls_im;
+    o->noise_ls = c->noise_ls;
+    o->noise_ce = c->noise_ce;
+    o->h_re = c->h_re;
+    o->h_im = c->h_im;
+    o->cw_llr = c->cw;
+    o->ldpc_llr = c->lam;
+    return AIRAN_OK;
+}
+
+size_t airan_pusch_chain_DeviceBytes(const AiranPuschChain *c) { return c == nullptr ? 0 : c->bytes; }
+
+int32_t airan_pusch_chain_SetTiming(AiranPuschChain *c, uint32_t enabled)
+{
+    if (c == nullptr) return AIRAN_INVALID_ARGUMENT;
+    c->timing = enabled != 0;
+    return AIRAN_OK;
+}
+
+int32_t airan_pusch_chain_StageTimes(const AiranPuschChain *c, float *ms, uint32_t count)
+{
+    if (c == nullptr || ms == nullptr || count < AIRAN_CHAIN_STAGES || !c->timing) return AIRAN_INVALID_ARGUMENT;
+    if (hipEventSynchronize(c->ev[AIRAN_CHAIN_STAGES]) != hipSuccess) return AIRAN_LAUNCH_FAILED;
+    for (uint32_t i = 0; i < AIRAN_CHAIN_STAGES; ++i)
+        if (hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]) != hipSuccess) return AIRAN_LAUNCH_FAILED;
+    return AIRAN_OK;
+}
+
+}  // extern "C"

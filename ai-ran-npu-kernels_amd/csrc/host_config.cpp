@@ -315,4 +315,17 @@ int32_t airan_pusch_ReferenceTbPlan(const PuschMimoConfig *config, uint32_t num_
     return AIRAN_OK;
 }
 
+float airan_pusch_NoEffFloor(uint32_t qm)
+{
+    float norm;
+    switch (qm) {
+    case 2: norm = 2.0f; break;
+    case 4: norm = 10.0f; break;
+    case 6: norm = 42.0f; break;
+    case 8: norm = 170.0f; break;
+    default: return 0.0f;
+    }
+    return 0.04f * static_cast<float>((1u << (qm / 2)) - 1u) / norm;
+}
+
 }  // extern "C"

@@ -1,0 +1,33 @@
+// launch.h -- internal launchers shared by the per-operator entry points
+// (one slot) and the batched receive chain (many slots per launch).
+#pragma once
+
+#include "common.h"
+
+struct LmmseFilter;
+
+namespace airan {
+
+// DMRS least squares + per-Rx noise for `slots` stacked grids
+// (rx [slots,NR,14,pad] -> h_ls [slots,NR,L,D,pilot_pad], noise [slots,NR]).
+int32_t LaunchDmrsLs(const Geom &g, const __half *rx_re, const __half *rx_im, const __half *ref_re,
+                     const __half *ref_im, __half *h_re, __half *h_im, __half *noise, uint32_t slots,
+                     hipStream_t s);
+
+// LMMSE channel estimate for `slots` stacked observations; output
+// [slots,NR,out_layers,14,pad] (layers >= L untouched).
+size_t CeWorkspaceBytes(const Geom &g, uint32_t slots);
+// noise_in [slots,NR] is the fallback noise (DMRS-LS estimate); noise_out
+// [slots,NR] (may be null) receives the estimate the filter used.
+int32_t LaunchCeApply(const LmmseFilter *f, const __half *h_re, const __half *h_im, const __half *noise_in,
+                      __half *noise_out, void *ws, size_t ws_bytes, __half *out_re, __half *out_im,
+                      uint32_t out_layers, uint32_t slots, hipStream_t s);
+// The filter's geometry must match `g` (same grant shape).
+bool CeFilterMatches(const LmmseFilter *f, const Geom &g);
+
+// Rate dematch of num_cb code blocks described by device descriptors
+// (cw_symbol_offset counts from the start of the stacked codeword buffer).
+int32_t LaunchRateDematch(const Geom &g, const AiranTbPlanV1 &plan, const RateMatchDescriptor *desc,
+                          const int16_t *in, int16_t *out, uint32_t num_cb, hipStream_t s);
+
+}  // namespace airan

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "common.h"
+#include "launch.h"
 
 using airan::DevGeom;
 using airan::Geom;
@@ -34,7 +35,8 @@ constexpr int kMaxRank = 512;
 
 struct CeLayerFactors {
     uint32_t n;       // observations
-    uint32_t rank;    // kept eigenpairs (multiple of 16, <= n rounded up)
+    uint32_t rank;    // stored eigenpairs (multiple of 16; columns >= kept are zero)
+    uint32_t kept;    // eigenpairs actually kept
     float2 *bt;       // [n][rank]   U_r^H transposed: bt[p][k] = conj(U[p][k])
     float2 *at;       // [rank][used] (R_hp U_r) transposed, sc contiguous
     float *lam;       // [rank] eigenvalues (0 for padded ranks)
@@ -155,76 +157,121 @@ __global__ void __launch_bounds__(256) PackAtKernel(const double2 *__restrict__ 
 struct CeApply {
     uint32_t nr, nl, ndmrs, pilot_pad, used, pad, nsym;
     uint32_t dmrs_sym[4];
-    uint32_t n[4], rank[4];
+    uint32_t n[4], rank[4], kept[4];
     float w_sum_sq[4];
     float rt[14][4];
     float rtt[4][4];
 };
 
-// Noise statistics: sigma^2 = mean over Rx of the DMRS-LS estimate; per-layer
-// channel power P_l = mean |h_obs|^2 - sigma^2 * sum(w^2); the filter uses
-// n_obs,l = sigma^2 * sum(w^2) / P_l (the prior R_f is normalised to unit
-// power).  stats[l] = n_obs,l, stats[4 + l] = 1 / sqrt(P_l)... (unused).
-__global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ hr, const __half *__restrict__ hi,
-                                                     const __half *__restrict__ noise, float *__restrict__ stats,
-                                                     CeApply c)
+// Projection onto the kept eigenvectors, unscaled: u[k][col] = sum_p
+// conj(U[p][k]) h_obs[col][p], col = rx * D + d, plus the column energies
+// |h_obs|^2 and |u|^2 (double) for the noise estimate.  One workgroup per
+// (col, slot * L + l); h_obs staged in LDS, thread k owns eigenvectors k,
+// k + 256.  T of (slot, l) at (slot * L + l) * t_layer_stride.
+constexpr uint32_t kProjThreads = 256;
+constexpr uint32_t kMaxObs = 2048;
+constexpr double kMinNoiseDims = 32.0;
+
+__global__ void __launch_bounds__(kProjThreads) CeProjectKernel(
+    const __half *__restrict__ hr, const __half *__restrict__ hi, const float2 *__restrict__ bt0,
+    const float2 *__restrict__ bt1, const float2 *__restrict__ bt2, const float2 *__restrict__ bt3,
+    float2 *__restrict__ t_out, double *__restrict__ energy, uint32_t t_layer_stride, CeApply c)
 {
-    const uint32_t l = blockIdx.x;
-    float s2 = 0.0f;
-    for (uint32_t r = 0; r < c.nr; ++r) s2 += __half2float(noise[r]);
-    s2 /= static_cast<float>(c.nr);
-    float acc = 0.0f;
-    const uint32_t n = c.n[l];
-    const uint32_t total = c.nr * c.ndmrs * n;
-    for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
-        const uint32_t p = t % n, rd = t / n;
-        const uint32_t d = rd % c.ndmrs, rx = rd / c.ndmrs;
-        const size_t idx = ((static_cast<size_t>(rx) * c.nl + l) * c.ndmrs + d) * c.pilot_pad + p;
-        const float a = __half2float(hr[idx]), b = __half2float(hi[idx]);
-        acc += a * a + b * b;
+    __shared__ float2 hs[kMaxObs];
+    __shared__ double red[2][kProjThreads / 64];
+    const uint32_t col = blockIdx.x;
+    const uint32_t slot = blockIdx.y / c.nl, l = blockIdx.y - slot * c.nl;
+    const uint32_t rx = col / c.ndmrs, d = col - rx * c.ndmrs;
+    const uint32_t ncol = c.nr * c.ndmrs;
+    const size_t hb = (((static_cast<size_t>(slot) * c.nr + rx) * c.nl + l) * c.ndmrs + d) * c.pilot_pad;
+    const uint32_t n = c.n[l], rank = c.rank[l];
+    const float2 *bt = l == 0 ? bt0 : (l == 1 ? bt1 : (l == 2 ? bt2 : bt3));
+    double eh = 0.0;
+    for (uint32_t p = threadIdx.x; p < n; p += kProjThreads) {
+        const float2 v = make_float2(__half2float(hr[hb + p]), __half2float(hi[hb + p]));
+        hs[p] = v;
+        eh += static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    __shared__ float part[4];
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = acc;
+    __syncthreads();
+    double eu = 0.0;
+    float2 *tcol = t_out + (static_cast<size_t>(slot) * c.nl + l) * t_layer_stride + col;
+    for (uint32_t k = threadIdx.x; k < rank; k += kProjThreads) {
+        float ar = 0.0f, ai = 0.0f;
+        for (uint32_t p = 0; p < n; ++p) {
+            const float2 b = bt[static_cast<size_t>(p) * rank + k];
+            const float2 x = hs[p];
+            ar += b.x * x.x - b.y * x.y;
+            ai += b.x * x.y + b.y * x.x;
+        }
+        tcol[static_cast<size_t>(k) * ncol] = make_float2(ar, ai);
+        eu += static_cast<double>(ar) * ar + static_cast<double>(ai) * ai;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        eh += __shfl_xor(eh, off);
+        eu += __shfl_xor(eu, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x / 64] = eh;
+        red[1][threadIdx.x / 64] = eu;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const float mean = ((part[0] + part[1]) + (part[2] + part[3])) / static_cast<float>(total);
-        const float nobs_abs = s2 * c.w_sum_sq[l];
-        const float p = fmaxf(mean - nobs_abs, 1e-3f * mean + 1e-30f);
-        stats[l] = nobs_abs / p;
+        double a = 0.0, b = 0.0;
+        for (uint32_t w = 0; w < kProjThreads / 64; ++w) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        double *e = energy + 2 * ((static_cast<size_t>(slot) * c.nl + l) * ncol + col);
+        e[0] = a;
+        e[1] = b;
     }
 }
 
-// T[l][k][col] = (1 / (lam_k + n_obs)) sum_p bt[p][k] h_obs[col][p],
-// col = rx * D + d.  Grid (rank / 64, NR * D, L), 64 threads.
-__global__ void __launch_bounds__(64) CeProjectKernel(const __half *__restrict__ hr, const __half *__restrict__ hi,
-                                                      const float2 *__restrict__ bt0, const float2 *__restrict__ bt1,
-                                                      const float2 *__restrict__ bt2, const float2 *__restrict__ bt3,
-                                                      const float *__restrict__ lam0, const float *__restrict__ lam1,
-                                                      const float *__restrict__ lam2, const float *__restrict__ lam3,
-                                                      const float *__restrict__ stats, float2 *__restrict__ t_out,
-                                                      uint32_t t_layer_stride, CeApply c)
+// Noise and per-layer statistics of one slot (thread 0, double, fixed order).
+// The observation noise lies in the complement of the kept eigenspace: per
+// Rx, sigma^2 = sum_{l,d} (|h|^2 - |u|^2) / sum(w^2)_l over
+// sum_{l,d} (n_l - kept_l) dimensions (the DMRS-LS estimate, which the
+// channel's frequency selectivity inflates, is only the fallback when fewer
+// than kMinNoiseDims dimensions are left).  P_l = mean |h_obs|^2 - sigma^2 sum(w^2)_l and the
+// filter's relative noise n_obs,l = sigma^2 sum(w^2)_l / P_l ->
+// stats[4 slot + l].  noise_out[slot][rx] = fp16(sigma^2_rx) when non-null.
+__global__ void __launch_bounds__(64) CeStatsKernel(const double *__restrict__ energy,
+                                                    const __half *__restrict__ noise_in,
+                                                    __half *__restrict__ noise_out, float *__restrict__ stats,
+                                                    CeApply c)
 {
-    const uint32_t l = blockIdx.z;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t rank = c.rank[l];
-    if (k >= rank) return;
-    const uint32_t col = blockIdx.y;
-    const uint32_t rx = col / c.ndmrs, d = col - rx * c.ndmrs;
-    const float2 *bt = l == 0 ? bt0 : (l == 1 ? bt1 : (l == 2 ? bt2 : bt3));
-    const float *lam = l == 0 ? lam0 : (l == 1 ? lam1 : (l == 2 ? lam2 : lam3));
-    const size_t hb = ((static_cast<size_t>(rx) * c.nl + l) * c.ndmrs + d) * c.pilot_pad;
-    float ar = 0.0f, ai = 0.0f;
-    const uint32_t n = c.n[l];
-    for (uint32_t p = 0; p < n; ++p) {
-        const float2 b = bt[static_cast<size_t>(p) * rank + k];
-        const float xr = __half2float(hr[hb + p]), xi = __half2float(hi[hb + p]);
-        ar += b.x * xr - b.y * xi;
-        ai += b.x * xi + b.y * xr;
+    if (threadIdx.x != 0) return;
+    const uint32_t slot = blockIdx.x;
+    const uint32_t ncol = c.nr * c.ndmrs;
+    const double *e = energy + 2 * static_cast<size_t>(slot) * c.nl * ncol;
+    double dims = 0.0;
+    for (uint32_t l = 0; l < c.nl; ++l) dims += static_cast<double>(c.n[l] - c.kept[l]) * c.ndmrs;
+    double s2_sum = 0.0;
+    for (uint32_t rx = 0; rx < c.nr; ++rx) {
+        double s2;
+        if (dims >= kMinNoiseDims) {
+            double res = 0.0;
+            for (uint32_t l = 0; l < c.nl; ++l)
+                for (uint32_t d = 0; d < c.ndmrs; ++d) {
+                    const double *x = e + 2 * (static_cast<size_t>(l) * ncol + rx * c.ndmrs + d);
+                    res += fmax(x[0] - x[1], 0.0) / c.w_sum_sq[l];
+                }
+            s2 = res / dims;
+        } else {
+            s2 = __half2float(noise_in[static_cast<size_t>(slot) * c.nr + rx]);
+        }
+        if (noise_out) noise_out[static_cast<size_t>(slot) * c.nr + rx] = __float2half_rn(static_cast<float>(s2));
+        s2_sum += s2;
     }
-    const float inv = 1.0f / (lam[k] + stats[l]);
-    t_out[static_cast<size_t>(l) * t_layer_stride + static_cast<size_t>(k) * (c.nr * c.ndmrs) + col] =
-        make_float2(ar * inv, ai * inv);
+    const double s2 = s2_sum / c.nr;
+    for (uint32_t l = 0; l < c.nl; ++l) {
+        double eh = 0.0;
+        for (uint32_t col = 0; col < ncol; ++col) eh += e[2 * (static_cast<size_t>(l) * ncol + col)];
+        const double mean = eh / (static_cast<double>(ncol) * c.n[l]);
+        const double nobs_abs = s2 * c.w_sum_sq[l];
+        const double p = fmax(mean - nobs_abs, 1e-3 * mean + 1e-30);
+        stats[4 * slot + l] = static_cast<float>(nobs_abs / p);
+    }
 }
 
 // Solve the D x D symmetric system (R_tt + e I) w = r for one right-hand side.
@@ -252,8 +299,8 @@ __device__ inline void SolveTime(const CeApply &c, float e, const float *r, floa
 }
 
 // Frequency interpolation + rescale + time interpolation.  One thread per
-// subcarrier of one layer; loops over Rx in chunks.  Writes
-// H[rx][l][sym][sc] fp16 for sc < pad (zero beyond used).
+// subcarrier of one (slot, layer) = blockIdx.y; loops over Rx in chunks.
+// Writes H[slot][rx][l][sym][sc] fp16 for sc < pad (zero beyond used).
 template <int RXC>
 __global__ void __launch_bounds__(256) CeInterpKernel(const float2 *__restrict__ at0, const float2 *__restrict__ at1,
                                                       const float2 *__restrict__ at2, const float2 *__restrict__ at3,
@@ -264,9 +311,13 @@ __global__ void __launch_bounds__(256) CeInterpKernel(const float2 *__restrict__
                                                       __half *__restrict__ out_i, uint32_t out_layers, CeApply c)
 {
     const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t l = blockIdx.y;
+    const uint32_t slot = blockIdx.y / c.nl, l = blockIdx.y - slot * c.nl;
     if (sc >= c.pad) return;
     const size_t plane = static_cast<size_t>(c.nsym) * c.pad;
+    stats += 4 * slot;
+    t_in += static_cast<size_t>(slot) * c.nl * t_layer_stride;
+    out_r += static_cast<size_t>(slot) * c.nr * out_layers * plane;
+    out_i += static_cast<size_t>(slot) * c.nr * out_layers * plane;
     if (sc >= c.used) {
         for (uint32_t rx = 0; rx < c.nr; ++rx)
             for (uint32_t s = 0; s < c.nsym; ++s) {
@@ -280,16 +331,9 @@ __global__ void __launch_bounds__(256) CeInterpKernel(const float2 *__restrict__
     const float *lam = l == 0 ? lam0 : (l == 1 ? lam1 : (l == 2 ? lam2 : lam3));
     const uint32_t rank = c.rank[l];
     const float nobs = stats[l];
-    // hhat_var and the rescale / error of ref.py:109-116.
+    // hhat_var and the rescale / error of ref.py:109-116 (hv accumulated in
+    // the first Rx chunk's pass over the eigenvectors).
     float hv = 0.0f;
-    for (uint32_t k = 0; k < rank; ++k) {
-        const float2 a = at[static_cast<size_t>(k) * c.used + sc];
-        hv += (a.x * a.x + a.y * a.y) / (lam[k] + nobs);
-    }
-    const float err = fmaxf(1.0f - hv, 0.0f);
-    const float den = hv + 1.0f - err;
-    const float scale = den != 0.0f ? 2.0f / den : 0.0f;
-    const float err2 = fmaxf(scale * (scale - 1.0f) * hv + (1.0f - scale) + scale * err, 0.0f);
     const uint32_t ncol = c.nr * c.ndmrs;
     const float2 *tl = t_in + static_cast<size_t>(l) * t_layer_stride;
     for (uint32_t rx0 = 0; rx0 < c.nr; rx0 += RXC) {
@@ -299,7 +343,10 @@ __global__ void __launch_bounds__(256) CeInterpKernel(const float2 *__restrict__
 #pragma unroll
             for (int d = 0; d < 4; ++d) hf[i][d] = make_float2(0.0f, 0.0f);
         for (uint32_t k = 0; k < rank; ++k) {
-            const float2 a = at[static_cast<size_t>(k) * c.used + sc];
+            const float2 a0 = at[static_cast<size_t>(k) * c.used + sc];
+            const float inv = 1.0f / (lam[k] + nobs);
+            if (rx0 == 0) hv += (a0.x * a0.x + a0.y * a0.y) * inv;
+            const float2 a = make_float2(a0.x * inv, a0.y * inv);
             const float2 *trow = tl + static_cast<size_t>(k) * ncol;
 #pragma unroll
             for (int i = 0; i < RXC; ++i) {
@@ -313,6 +360,10 @@ __global__ void __launch_bounds__(256) CeInterpKernel(const float2 *__restrict__
                 }
             }
         }
+        const float err = fmaxf(1.0f - hv, 0.0f);
+        const float den = hv + 1.0f - err;
+        const float scale = den != 0.0f ? 2.0f / den : 0.0f;
+        const float err2 = fmaxf(scale * (scale - 1.0f) * hv + (1.0f - scale) + scale * err, 0.0f);
         // Time weights (the same for every Rx of this subcarrier).
         for (uint32_t s = 0; s < c.nsym; ++s) {
             float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -421,6 +472,7 @@ int32_t BuildLayer(const Geom &g, uint32_t l, const LmmseFilterParamsV1 &prm, hi
     }
     out->n = n;
     out->rank = rank;
+    out->kept = kept;
     out->w_sum_sq = g.occ[l] ? 0.5f : 1.0f;
     if (hipMalloc(&out->bt, sizeof(float2) * static_cast<size_t>(n) * rank) != hipSuccess ||
         hipMalloc(&out->at, sizeof(float2) * static_cast<size_t>(rank) * g.used) != hipSuccess ||
@@ -462,6 +514,7 @@ CeApply MakeApply(const LmmseFilter &f)
         const CeLayerFactors &lf = f.layer[f.shared_with[l]];
         c.n[l] = lf.n;
         c.rank[l] = lf.rank;
+        c.kept[l] = lf.kept;
         c.w_sum_sq[l] = lf.w_sum_sq;
     }
     std::memcpy(c.rt, f.rt, sizeof(c.rt));
@@ -482,6 +535,59 @@ uint64_t PilotHash(const uint16_t *sc, uint32_t count)
 }
 
 }  // namespace
+
+namespace airan {
+
+// Workspace: stats [slots][4] fp32, column energies [slots][L][NR*D][2]
+// fp64, then T[slots][L] of kMaxRank x NR x D complex fp32 (each part
+// 256-byte aligned).
+namespace {
+size_t StatsBytes(uint32_t slots) { return airan::AlignUp(16u * slots, 256u); }
+size_t EnergyBytes(const Geom &g, uint32_t slots)
+{
+    return (static_cast<size_t>(slots) * g.nl * g.nr * g.ndmrs * 16 + 255) / 256 * 256;
+}
+}  // namespace
+
+size_t CeWorkspaceBytes(const Geom &g, uint32_t slots)
+{
+    return StatsBytes(slots) + EnergyBytes(g, slots) +
+           static_cast<size_t>(slots) * g.nl * kMaxRank * g.nr * g.ndmrs * sizeof(float2);
+}
+
+bool CeFilterMatches(const LmmseFilter *f, const Geom &g)
+{
+    return f != nullptr && g.nl == f->g.nl && g.nr == f->g.nr && g.used == f->g.used && g.pad == f->g.pad &&
+           g.ndmrs == f->g.ndmrs && std::memcmp(g.comb, f->g.comb, sizeof(g.comb)) == 0 &&
+           std::memcmp(g.occ, f->g.occ, sizeof(g.occ)) == 0 &&
+           std::memcmp(g.dmrs_sym, f->g.dmrs_sym, sizeof(g.dmrs_sym)) == 0;
+}
+
+int32_t LaunchCeApply(const LmmseFilter *f, const __half *hr, const __half *hi, const __half *noise_in,
+                      __half *noise_out, void *ws, size_t ws_bytes, __half *out_re, __half *out_im,
+                      uint32_t out_layers, uint32_t slots, hipStream_t s)
+{
+    const Geom &g = f->g;
+    if (slots == 0 || slots > 65535 || out_layers < g.nl) return AIRAN_INVALID_ARGUMENT;
+    if (ws_bytes < CeWorkspaceBytes(g, slots)) return AIRAN_RESOURCE_TOO_SMALL;
+    const CeApply c = MakeApply(*f);
+    auto *base = static_cast<uint8_t *>(ws);
+    auto *stats = reinterpret_cast<float *>(base);
+    auto *energy = reinterpret_cast<double *>(base + StatsBytes(slots));
+    auto *tbuf = reinterpret_cast<float2 *>(base + StatsBytes(slots) + EnergyBytes(g, slots));
+    const uint32_t t_stride = kMaxRank * g.nr * g.ndmrs;
+    const CeLayerFactors *L[4];
+    for (int l = 0; l < 4; ++l) L[l] = &f->layer[l < static_cast<int>(g.nl) ? f->shared_with[l] : 0];
+    hipLaunchKernelGGL(CeProjectKernel, dim3(g.nr * g.ndmrs, slots * g.nl), dim3(kProjThreads), 0, s, hr, hi,
+                       L[0]->bt, L[1]->bt, L[2]->bt, L[3]->bt, tbuf, energy, t_stride, c);
+    hipLaunchKernelGGL(CeStatsKernel, dim3(slots), dim3(64), 0, s, energy, noise_in, noise_out, stats, c);
+    hipLaunchKernelGGL(CeInterpKernel<4>, dim3(CeilDiv(g.pad, 256), slots * g.nl), dim3(256), 0, s, L[0]->at,
+                       L[1]->at, L[2]->at, L[3]->at, L[0]->lam, L[1]->lam, L[2]->lam, L[3]->lam, stats, tbuf,
+                       t_stride, out_re, out_im, out_layers, c);
+    return LaunchStatus();
+}
+
+}  // namespace airan
 
 extern "C" {
 
@@ -632,7 +738,7 @@ void airan_channel_est_lmmse_DestroyFilter(LmmseFilter *f)
 uint32_t airan_channel_est_lmmse_FilterRank(const LmmseFilter *f, uint32_t layer)
 {
     if (f == nullptr || layer >= f->g.nl) return 0;
-    return f->layer[f->shared_with[layer]].rank;
+    return f->layer[f->shared_with[layer]].kept;
 }
 
 size_t airan_channel_est_lmmse_WorkspaceBytes(const PuschMimoConfig *config)
@@ -640,43 +746,26 @@ size_t airan_channel_est_lmmse_WorkspaceBytes(const PuschMimoConfig *config)
     if (config == nullptr) return 0;
     Geom g;
     if (airan::DeriveGeom(*config, &g) != AIRAN_OK) return 0;
-    // stats (64 B) + T[L][kMaxRank][NR*D] complex fp32
-    return 256 + static_cast<size_t>(g.nl) * kMaxRank * g.nr * g.ndmrs * sizeof(float2);
+    return airan::CeWorkspaceBytes(g, 1);
 }
 
 int32_t airan_channel_est_lmmse_Enqueue(const ChannelEstLmmseOpArgsV1 *a, const LmmseFilter *f,
-                                        const void *noise_var_rx, void *workspace, size_t workspace_bytes)
+                                        const void *noise_var_rx, void *noise_out, void *workspace,
+                                        size_t workspace_bytes)
 {
     int32_t st = airan_channel_est_lmmse_ValidateOpArgs(a);
     if (st != AIRAN_OK) return st;
     if (f == nullptr || noise_var_rx == nullptr || workspace == nullptr) return AIRAN_INVALID_ARGUMENT;
     Geom g;
     airan::DeriveGeom(*a->config, &g);
-    if (g.nl != f->g.nl || g.nr != f->g.nr || g.used != f->g.used || g.pad != f->g.pad || g.ndmrs != f->g.ndmrs ||
-        std::memcmp(g.comb, f->g.comb, sizeof(g.comb)) != 0 || std::memcmp(g.occ, f->g.occ, sizeof(g.occ)) != 0)
-        return AIRAN_LAYOUT_MISMATCH;
-    if (workspace_bytes < airan_channel_est_lmmse_WorkspaceBytes(a->config)) return AIRAN_RESOURCE_TOO_SMALL;
-    hipStream_t s = static_cast<hipStream_t>(a->stream);
-    const CeApply c = MakeApply(*f);
-    auto *stats = static_cast<float *>(workspace);
-    auto *tbuf = reinterpret_cast<float2 *>(static_cast<uint8_t *>(workspace) + 256);
-    const uint32_t t_stride = kMaxRank * g.nr * g.ndmrs;
-    const auto *hr = static_cast<const __half *>(a->h_ls_re);
-    const auto *hi = static_cast<const __half *>(a->h_ls_im);
-    const CeLayerFactors *L[4];
-    for (int l = 0; l < 4; ++l) L[l] = &f->layer[l < static_cast<int>(g.nl) ? f->shared_with[l] : 0];
-    hipLaunchKernelGGL(CeStatsKernel, dim3(g.nl), dim3(256), 0, s, hr, hi, static_cast<const __half *>(noise_var_rx),
-                       stats, c);
-    uint32_t max_rank = 0;
-    for (uint32_t l = 0; l < g.nl; ++l) max_rank = std::max(max_rank, c.rank[l]);
-    hipLaunchKernelGGL(CeProjectKernel, dim3(airan::CeilDiv(max_rank, 64), g.nr * g.ndmrs, g.nl), dim3(64), 0, s,
-                       hr, hi, L[0]->bt, L[1]->bt, L[2]->bt, L[3]->bt, L[0]->lam, L[1]->lam, L[2]->lam, L[3]->lam,
-                       stats, tbuf, t_stride, c);
-    hipLaunchKernelGGL(CeInterpKernel<4>, dim3(airan::CeilDiv(g.pad, 256), g.nl), dim3(256), 0, s, L[0]->at,
-                       L[1]->at, L[2]->at, L[3]->at, L[0]->lam, L[1]->lam, L[2]->lam, L[3]->lam, stats, tbuf,
-                       t_stride, static_cast<__half *>(a->h_grid_re), static_cast<__half *>(a->h_grid_im),
-                       airan::kDetectLayers, c);
-    return airan::LaunchStatus();
+    if (!airan::CeFilterMatches(f, g)) return AIRAN_LAYOUT_MISMATCH;
+    if (noise_out != nullptr && (noise_out == noise_var_rx || noise_out == a->h_grid_re || noise_out == a->h_grid_im))
+        return AIRAN_INVALID_ARGUMENT;
+    return airan::LaunchCeApply(f, static_cast<const __half *>(a->h_ls_re), static_cast<const __half *>(a->h_ls_im),
+                                static_cast<const __half *>(noise_var_rx), static_cast<__half *>(noise_out),
+                                workspace, workspace_bytes,
+                                static_cast<__half *>(a->h_grid_re), static_cast<__half *>(a->h_grid_im),
+                                airan::kDetectLayers, 1, static_cast<hipStream_t>(a->stream));
 }
 
 }  // extern "C"

@@ -2,6 +2,7 @@
 // least-squares channel observation with per-Rx noise estimation.
 #include "common.h"
 #include "gold.h"
+#include "launch.h"
 
 using airan::DevGeom;
 using airan::Geom;
@@ -86,10 +87,11 @@ __global__ void __launch_bounds__(256) DmrsLsKernel(const __half *__restrict__ y
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.pilot_pad) return;
     const uint32_t d = blockIdx.y;
-    const uint32_t rx = blockIdx.z / g.nl, l = blockIdx.z - rx * g.nl;
-    const size_t out = ((static_cast<size_t>(rx) * g.nl + l) * g.ndmrs + d) * g.pilot_pad + p;
-    const __half *y_r = yr + (static_cast<size_t>(rx) * g.nsym + g.dmrs_sym[d]) * g.pad;
-    const __half *y_i = yi + (static_cast<size_t>(rx) * g.nsym + g.dmrs_sym[d]) * g.pad;
+    // blockIdx.z = (slot * NR + rx) * L + l; slots are stacked grids.
+    const uint32_t srx = blockIdx.z / g.nl, l = blockIdx.z - srx * g.nl;
+    const size_t out = (static_cast<size_t>(blockIdx.z) * g.ndmrs + d) * g.pilot_pad + p;
+    const __half *y_r = yr + (static_cast<size_t>(srx) * g.nsym + g.dmrs_sym[d]) * g.pad;
+    const __half *y_i = yi + (static_cast<size_t>(srx) * g.nsym + g.dmrs_sym[d]) * g.pad;
     const __half *x_r = xr + (static_cast<size_t>(l) * g.ndmrs + d) * g.ref_pad;
     const __half *x_i = xi + (static_cast<size_t>(l) * g.ndmrs + d) * g.ref_pad;
     const uint32_t comb = g.comb[l];
@@ -118,7 +120,8 @@ __global__ void __launch_bounds__(256) DmrsLsKernel(const __half *__restrict__ y
 
 // Noise per Rx from layer 0 (mimo_dmrs_ls_kernel.cpp:179-196,334-337):
 // sum_d sum_p |h_{p+1} - h_p|^2 / (D (cnt - 1)), halved for the comb model.
-// One workgroup per Rx, fp32 accumulation, fixed reduction tree.
+// One workgroup per (slot, Rx) = blockIdx.x, fp32 accumulation, fixed
+// reduction tree.
 __global__ void __launch_bounds__(256) DmrsNoiseKernel(const __half *__restrict__ hr,
                                                        const __half *__restrict__ hi,
                                                        __half *__restrict__ noise, DevGeom g)
@@ -169,6 +172,21 @@ __global__ void __launch_bounds__(256) PilotDescKernel(uint16_t *__restrict__ sc
 struct MimoDmrsGenRuntimeV1 {
     int device;
 };
+
+namespace airan {
+
+int32_t LaunchDmrsLs(const Geom &g, const __half *rx_re, const __half *rx_im, const __half *ref_re,
+                     const __half *ref_im, __half *h_re, __half *h_im, __half *noise, uint32_t slots,
+                     hipStream_t s)
+{
+    const DevGeom dg = ToDev(g);
+    const dim3 grid(CeilDiv(g.pilot_pad, 256), g.ndmrs, slots * g.nr * g.nl);
+    hipLaunchKernelGGL(DmrsLsKernel, grid, dim3(256), 0, s, rx_re, rx_im, ref_re, ref_im, h_re, h_im, dg);
+    hipLaunchKernelGGL(DmrsNoiseKernel, dim3(slots * g.nr), dim3(256), 0, s, h_re, h_im, noise, dg);
+    return LaunchStatus();
+}
+
+}  // namespace airan
 
 extern "C" {
 
@@ -342,14 +360,11 @@ int32_t airan_mimo_dmrs_ls_Enqueue(const MimoDmrsLsOpArgsV1 *a)
     const DevGeom dg = airan::ToDev(g);
     hipLaunchKernelGGL(PilotDescKernel, dim3(airan::CeilDiv(g.pilot_pad, 256), g.ndmrs, g.nl), dim3(256), 0, s,
                        static_cast<uint16_t *>(a->pilot_sc), static_cast<uint16_t *>(a->pilot_count), dg);
-    dim3 grid(airan::CeilDiv(g.pilot_pad, 256), g.ndmrs, g.nr * g.nl);
-    hipLaunchKernelGGL(DmrsLsKernel, grid, dim3(256), 0, s, static_cast<const __half *>(a->rx_grid_re),
-                       static_cast<const __half *>(a->rx_grid_im), static_cast<const __half *>(a->dmrs_ref_re),
-                       static_cast<const __half *>(a->dmrs_ref_im), static_cast<__half *>(a->h_ls_re),
-                       static_cast<__half *>(a->h_ls_im), dg);
-    hipLaunchKernelGGL(DmrsNoiseKernel, dim3(g.nr), dim3(256), 0, s, static_cast<const __half *>(a->h_ls_re),
-                       static_cast<const __half *>(a->h_ls_im), static_cast<__half *>(a->noise_var_rx), dg);
-    return airan::LaunchStatus();
+    return airan::LaunchDmrsLs(g, static_cast<const __half *>(a->rx_grid_re),
+                               static_cast<const __half *>(a->rx_grid_im),
+                               static_cast<const __half *>(a->dmrs_ref_re),
+                               static_cast<const __half *>(a->dmrs_ref_im), static_cast<__half *>(a->h_ls_re),
+                               static_cast<__half *>(a->h_ls_im), static_cast<__half *>(a->noise_var_rx), 1, s);
 }
 
 }  // extern "C"

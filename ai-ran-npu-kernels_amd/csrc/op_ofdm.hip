@@ -154,9 +154,11 @@ struct OfdmArgs {
 
 // One Stockham pass of radix R over N points held in LDS (natural-order
 // in, natural-order out after the last pass).  Each thread of a symbol owns
-// 16 / R butterflies.
-template <int LOGN, int R>
-__device__ inline void LdsPass(float *lre, float *lim, uint32_t j, uint32_t ns, bool active)
+// 16 / R butterflies.  The last pass applies the output scale before its
+// LDS store, so the write-out is a pure fp32 -> fp16 conversion (no multiply
+// the compiler could fold into a single-rounding v_fma_mix).
+template <int LOGN, int R, bool LAST>
+__device__ inline void LdsPass(float *lre, float *lim, uint32_t j, uint32_t ns, bool active, float scale)
 {
     constexpr uint32_t N = 1u << LOGN;
     constexpr uint32_t TPS = N / 16;
@@ -187,8 +189,8 @@ __device__ inline void LdsPass(float *lre, float *lim, uint32_t j, uint32_t ns, 
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const uint32_t i = Pad(base + r * ns);
-                lre[i] = v[t][r].x;
-                lim[i] = v[t][r].y;
+                lre[i] = LAST ? v[t][r].x * scale : v[t][r].x;
+                lim[i] = LAST ? v[t][r].y * scale : v[t][r].y;
             }
         }
     }
@@ -236,18 +238,22 @@ __global__ void __launch_bounds__(kThreads) OfdmKernel(const void *__restrict__ 
     }
     __syncthreads();
     uint32_t ns = 16;
+    constexpr int kPasses16 = LOGN / 4;  // radix-16 passes, the first included
+    const float sc = a.out_scale;
 #pragma unroll
-    for (int p = 1; p < LOGN / 4; ++p) {
-        LdsPass<LOGN, 16>(lre, lim, j, ns, active);
+    for (int p = 1; p < kPasses16; ++p) {
+        if (LOGN % 4 == 0 && p == kPasses16 - 1)
+            LdsPass<LOGN, 16, true>(lre, lim, j, ns, active, sc);
+        else
+            LdsPass<LOGN, 16, false>(lre, lim, j, ns, active, sc);
         ns *= 16;
     }
-    if constexpr (LOGN % 4 == 1) LdsPass<LOGN, 2>(lre, lim, j, ns, active);
-    if constexpr (LOGN % 4 == 2) LdsPass<LOGN, 4>(lre, lim, j, ns, active);
-    if constexpr (LOGN % 4 == 3) LdsPass<LOGN, 8>(lre, lim, j, ns, active);
+    if constexpr (LOGN % 4 == 1) LdsPass<LOGN, 2, true>(lre, lim, j, ns, active, sc);
+    if constexpr (LOGN % 4 == 2) LdsPass<LOGN, 4, true>(lre, lim, j, ns, active, sc);
+    if constexpr (LOGN % 4 == 3) LdsPass<LOGN, 8, true>(lre, lim, j, ns, active, sc);
     if (!active) return;
 
-    // Write-out: two adjacent outputs per thread per step (half2 stores).
-    const float sc = a.out_scale;
+    // Write-out (already scaled): two adjacent outputs per thread per step.
     if constexpr (OUT_MODE == 0) {
         constexpr uint32_t P = N / 64;
         __half2 *ore = reinterpret_cast<__half2 *>(out_re + static_cast<size_t>(gsym) * N);
@@ -255,8 +261,8 @@ __global__ void __launch_bounds__(kThreads) OfdmKernel(const void *__restrict__ 
         for (uint32_t q = j; q < N / 2; q += TPS) {
             const uint32_t e = 2 * q;
             const uint32_t k0 = (e & 63) * P + (e >> 6), k1 = ((e + 1) & 63) * P + ((e + 1) >> 6);
-            ore[q] = __floats2half2_rn(lre[Pad(k0)] * sc, lre[Pad(k1)] * sc);
-            oim[q] = __floats2half2_rn(lim[Pad(k0)] * sc, lim[Pad(k1)] * sc);
+            ore[q] = __floats2half2_rn(lre[Pad(k0)], lre[Pad(k1)]);
+            oim[q] = __floats2half2_rn(lim[Pad(k0)], lim[Pad(k1)]);
         }
     } else {
         const uint32_t half_used = a.used / 2;
@@ -267,13 +273,13 @@ __global__ void __launch_bounds__(kThreads) OfdmKernel(const void *__restrict__ 
             float r0 = 0.0f, i0 = 0.0f, r1 = 0.0f, i1 = 0.0f;
             if (u < a.used) {
                 const uint32_t k = u < half_used ? N - half_used + u : u - half_used;
-                r0 = lre[Pad(k)] * sc;
-                i0 = lim[Pad(k)] * sc;
+                r0 = lre[Pad(k)];
+                i0 = lim[Pad(k)];
             }
             if (u + 1 < a.used) {
                 const uint32_t k = u + 1 < half_used ? N - half_used + u + 1 : u + 1 - half_used;
-                r1 = lre[Pad(k)] * sc;
-                i1 = lim[Pad(k)] * sc;
+                r1 = lre[Pad(k)];
+                i1 = lim[Pad(k)];
             }
             ore[q] = __floats2half2_rn(r0, r1);
             oim[q] = __floats2half2_rn(i0, i1);

@@ -254,6 +254,20 @@ class LdpcDecodeOpArgsV1(C.Structure):
                      ("stream", vp))
 
 
+class AiranPuschChainOptionsV1(C.Structure):
+    _fields_ = _args(("max_slots", u32), ("iq_format", u32), ("ldpc_iterations", u32), ("ldpc_mode", u32),
+                     ("early_stop", u32), ("ce_max_rank", u32), ("use_graph", u32), ("tau_s", C.c_double),
+                     ("fd_hz", C.c_double), ("scs_hz", C.c_double), ("eig_floor", C.c_double),
+                     ("reserved", u32 * 8))
+
+
+class AiranPuschChainOutputsV1(C.Structure):
+    _fields_ = _args(("num_slots", u32), ("bits", vp), ("tb_ok", vp), ("cb_ok", vp), ("iterations", vp),
+                     ("rx_grid_re", vp), ("rx_grid_im", vp), ("h_ls_re", vp), ("h_ls_im", vp),
+                     ("noise_ls", vp), ("noise_ce", vp), ("h_re", vp), ("h_im", vp), ("cw_llr", vp),
+                     ("ldpc_llr", vp))
+
+
 P = C.POINTER
 
 # name -> (restype, argtypes).  Every entry is an exported symbol of the
@@ -296,7 +310,7 @@ PROTOTYPES = {
     "airan_channel_est_lmmse_DestroyFilter": (None, [vp]),
     "airan_channel_est_lmmse_FilterRank": (u32, [vp, u32]),
     "airan_channel_est_lmmse_WorkspaceBytes": (C.c_size_t, [P(PuschMimoConfig)]),
-    "airan_channel_est_lmmse_Enqueue": (i32, [P(ChannelEstLmmseOpArgsV1), vp, vp, vp, C.c_size_t]),
+    "airan_channel_est_lmmse_Enqueue": (i32, [P(ChannelEstLmmseOpArgsV1), vp, vp, vp, vp, C.c_size_t]),
     "airan_mimo_detect_io_pack_BuildCurrentProfile": (i32, [P(PuschMimoConfig), C.c_size_t,
                                                             P(MimoDetectLayerPlan),
                                                             P(MimoDetectIoPackKernelMetadata)]),
@@ -337,11 +351,23 @@ PROTOTYPES = {
     "airan_ldpc_ActiveRows": (u32, [P(AiranTbPlanV1)]),
     "airan_crc_check_WorkspaceBytes": (C.c_size_t, [u32]),
     "airan_crc_check_Enqueue": (i32, [P(AiranTbPlanV1), u32, vp, vp, vp, vp, vp, C.c_size_t, vp]),
+    "airan_pusch_NoEffFloor": (C.c_float, [u32]),
     # synthetic transmitter (include/airan_pusch_tx.h)
     "airan_pusch_tx_EncodeTbs": (i32, [P(PuschMimoConfig), P(AiranTbPlanV1), u32, u32, vp, vp, vp]),
     "airan_ldpc_bg1_Encode": (i32, [u32, vp, vp]),
     "airan_crc_Compute": (u32, [u32, vp, C.c_uint64]),
+    # batched chain (include/airan_pusch_chain.h)
+    "airan_pusch_chain_Create": (i32, [P(PuschMimoConfig), P(AiranTbPlanV1), P(AiranPuschChainOptionsV1), vp,
+                                       P(vp)]),
+    "airan_pusch_chain_Destroy": (None, [vp]),
+    "airan_pusch_chain_Enqueue": (i32, [vp, vp, u32, vp]),
+    "airan_pusch_chain_Outputs": (i32, [vp, P(AiranPuschChainOutputsV1)]),
+    "airan_pusch_chain_DeviceBytes": (C.c_size_t, [vp]),
+    "airan_pusch_chain_SetTiming": (i32, [vp, u32]),
+    "airan_pusch_chain_StageTimes": (i32, [vp, P(C.c_float), u32]),
 }
+
+CHAIN_STAGES = ("ofdm_demod", "dmrs_ls", "ce_lmmse", "detect_demap", "rate_dematch", "ldpc_decode", "crc_check")
 
 _LIB = None
 

@@ -1,0 +1,159 @@
+"""Op-by-op PUSCH receive chain over the C-ABI operators, in the order of the
+reference's chain/pusch_mimo_rx_chain/rx_chain.cpp:440-640:
+
+  OFDM demod (+RE demap) -> DMRS gen -> DMRS LS -> LMMSE CE -> detector
+  I/O pack -> LMMSE detection -> QAM demap -> layer demap -> descramble ->
+  rate dematch -> LDPC decode -> CRC.
+
+One slot per transport block.  Every stage is a separate launch with its
+buffers allocated per call; the native batched chain (airan_pusch_chain_*)
+is the production path and must reproduce this composition bit for bit.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from . import abi, ops
+
+
+@dataclass
+class ChainOptions:
+    ldpc_iterations: int = 8
+    ldpc_mode: int = abi.LDPC_MODE_FP16
+    early_stop: bool = True
+    tau_s: float = 300e-9
+    fd_hz: float = 100.0
+    scs_hz: float = 30e3
+    ce_rank: int = 96
+    iq_format: int = abi.IQ_INT16_Q8_8
+
+
+class OpChain:
+    def __init__(self, cfg, plan, opts: ChainOptions | None = None):
+        self.cfg = cfg
+        self.plan = plan
+        self.opts = opts or ChainOptions()
+        self.g = ops.Geometry(cfg)
+        self.filter = ops.LmmseFilter(cfg, tau_s=self.opts.tau_s, fd_hz=self.opts.fd_hz, scs_hz=self.opts.scs_hz,
+                                      max_rank=self.opts.ce_rank)
+        self.dmrs_re, self.dmrs_im = ops.dmrs_gen(cfg)
+        self.active_rows = ops.ldpc_active_rows(plan)
+        self.no_eff_floor = float(abi.load_library().airan_pusch_NoEffFloor(cfg.qm))
+
+    def run_slot(self, iq: torch.Tensor, keep: bool = False):
+        """iq: one slot, [NR, samples, 2] on the device.  Returns a dict with
+        bits [C*K] int8, cb_ok, tb_ok and (keep=True) the intermediates."""
+        cfg, g, plan, o = self.cfg, self.g, self.plan, self.opts
+        out = {}
+        rx_re, rx_im = ops.ofdm_demod(cfg, iq, g.nr, o.iq_format, fused_redemap=True)
+        h_re, h_im, sc, cnt, noise = ops.dmrs_ls(cfg, rx_re, rx_im, self.dmrs_re, self.dmrs_im)
+        hg_re, hg_im, _, noise_ce = ops.ce_lmmse(cfg, self.filter, h_re, h_im, sc, cnt, noise)
+        hrm_r, hrm_i, yv_r, yv_i, no = ops.detect_io_pack(cfg, rx_re, rx_im, hg_re, hg_im, noise_ce)
+        xr, xi, ne = ops.detect_lmmse(cfg, hrm_r, hrm_i, yv_r, yv_i, no)
+        # Keep the demapper's integer arithmetic in range (airan_pusch_NoEffFloor).
+        ne_q = torch.clamp_min(ne, self.no_eff_floor)
+        llr = ops.qam_demod(cfg, xr, xi, ne_q)
+        cw = ops.layer_demap(cfg, llr)
+        cw_nr = ops.descramble(cfg, cw, 1, restart_per_slot=True)
+        lam = ops.rate_dematch(cfg, cw_nr, 1, plan)
+        bits, _, iters = ops.ldpc_decode(lam, plan.C, plan.Z, plan.K, o.ldpc_iterations, self.active_rows,
+                                         o.ldpc_mode, o.early_stop)
+        cb_ok, tb_ok, _ = ops.crc_check(plan, 1, bits)
+        out.update(bits=bits, cb_ok=cb_ok, tb_ok=tb_ok, iterations=iters)
+        if keep:
+            out.update(rx_re=rx_re, rx_im=rx_im, h_ls_re=h_re, h_ls_im=h_im, noise=noise, noise_ce=noise_ce,
+                       h_re=hg_re, h_im=hg_im,
+                       xhat_re=xr, xhat_im=xi, no_eff=ne, layer_llr=llr, cw_llr=cw, cw_nr=cw_nr, ldpc_llr=lam)
+        return out
+
+    def close(self):
+        self.filter.close()
+
+
+_HIP = None
+
+
+def _hip():
+    """libamdhip64 for device-to-device copies out of chain-owned buffers."""
+    global _HIP
+    if _HIP is None:
+        import ctypes as C
+        lib = C.CDLL("libamdhip64.so")
+        lib.hipMemcpyAsync.restype = C.c_int
+        lib.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        _HIP = lib
+    return _HIP
+
+
+class NativeChain:
+    """The batched chain of include/airan_pusch_chain.h (slots per launch,
+    HIP-graph replay)."""
+
+    def __init__(self, cfg, plan, max_slots: int, opts: ChainOptions | None = None, use_graph: bool = True):
+        import ctypes as C
+        self._C = C
+        self.lib = abi.load_library()
+        self.cfg, self.plan = cfg, plan
+        self.opts = opts or ChainOptions()
+        self.g = ops.Geometry(cfg)
+        o = abi.AiranPuschChainOptionsV1()
+        o.abi_version, o.struct_size = abi.ABI_VERSION, C.sizeof(o)
+        o.max_slots = max_slots
+        o.iq_format = self.opts.iq_format
+        o.ldpc_iterations = self.opts.ldpc_iterations
+        o.ldpc_mode = self.opts.ldpc_mode
+        o.early_stop = 1 if self.opts.early_stop else 0
+        o.ce_max_rank = self.opts.ce_rank
+        o.use_graph = 1 if use_graph else 0
+        o.tau_s, o.fd_hz, o.scs_hz, o.eig_floor = self.opts.tau_s, self.opts.fd_hz, self.opts.scs_hz, 1e-12
+        self.handle = C.c_void_p()
+        ops._check("chain Create", self.lib.airan_pusch_chain_Create(C.byref(cfg), C.byref(plan), C.byref(o),
+                                                                    C.c_void_p(ops.stream_ptr()),
+                                                                    C.byref(self.handle)))
+
+    def enqueue(self, iq: torch.Tensor, num_slots: int):
+        ops._check("chain Enqueue", self.lib.airan_pusch_chain_Enqueue(self.handle, self._C.c_void_p(iq.data_ptr()),
+                                                                      num_slots,
+                                                                      self._C.c_void_p(ops.stream_ptr())))
+
+    def outputs(self) -> abi.AiranPuschChainOutputsV1:
+        o = abi.AiranPuschChainOutputsV1()
+        o.abi_version, o.struct_size = abi.ABI_VERSION, self._C.sizeof(o)
+        ops._check("chain Outputs", self.lib.airan_pusch_chain_Outputs(self.handle, self._C.byref(o)))
+        return o
+
+    def fetch(self, name: str, dtype, count: int) -> torch.Tensor:
+        """Copy `count` elements of output buffer `name` into a new tensor."""
+        src = getattr(self.outputs(), name)
+        out = torch.empty(count, dtype=dtype, device="cuda")
+        nbytes = count * out.element_size()
+        rc = _hip().hipMemcpyAsync(out.data_ptr(), src, nbytes, 3, ops.stream_ptr())
+        if rc != 0:
+            raise ops.AiranError("hipMemcpyAsync", rc)
+        return out
+
+    def device_bytes(self) -> int:
+        return int(self.lib.airan_pusch_chain_DeviceBytes(self.handle))
+
+    def set_timing(self, enabled: bool):
+        ops._check("chain SetTiming", self.lib.airan_pusch_chain_SetTiming(self.handle, 1 if enabled else 0))
+
+    def stage_times(self) -> dict:
+        """Milliseconds per stage of the last Enqueue (timing on)."""
+        n = len(abi.CHAIN_STAGES)
+        ms = (self._C.c_float * n)()
+        ops._check("chain StageTimes", self.lib.airan_pusch_chain_StageTimes(self.handle, ms, n))
+        return dict(zip(abi.CHAIN_STAGES, (float(v) for v in ms)))
+
+    def close(self):
+        if self.handle:
+            self.lib.airan_pusch_chain_Destroy(self.handle)
+            self.handle = self._C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

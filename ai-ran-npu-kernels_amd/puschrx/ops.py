@@ -297,7 +297,7 @@ class LmmseFilter:
     """Owns a device filter built by airan_channel_est_lmmse_CreateFilter."""
 
     def __init__(self, cfg, tau_s: float = 100e-9, fd_hz: float = 0.0, scs_hz: float = 30e3,
-                 max_rank: int = 512, eig_floor: float = 0.0):
+                 max_rank: int = 512, eig_floor: float = 1e-12):
         self._lib = load_library()
         self.cfg = cfg
         p = _hdr(abi.LmmseFilterParamsV1, max_rank=max_rank, tau_s=tau_s, fd_hz=fd_hz, scs_hz=scs_hz,
@@ -324,7 +324,8 @@ class LmmseFilter:
 
 def ce_lmmse(cfg, filt: LmmseFilter, h_ls_re, h_ls_im, pilot_sc, pilot_count, noise_var_rx, out=None,
              workspace=None):
-    """LMMSE channel estimate [NR,16,14,padded] fp16 planes (layers >= L untouched)."""
+    """LMMSE channel estimate [NR,16,14,padded] fp16 planes (layers >= L
+    untouched).  Returns (h_re, h_im, workspace, noise_est [NR] fp16)."""
     lib = load_library()
     g = Geometry(cfg)
     dev = h_ls_re.device
@@ -352,9 +353,10 @@ def ce_lmmse(cfg, filt: LmmseFilter, h_ls_re, h_ls_im, pilot_sc, pilot_count, no
     nws = int(lib.airan_channel_est_lmmse_WorkspaceBytes(C.byref(cfg)))
     if workspace is None:
         workspace = _dev(nws, torch.uint8, dev)
+    noise_est = _dev(g.nr, torch.float16, dev)
     _check("ce Enqueue", lib.airan_channel_est_lmmse_Enqueue(C.byref(a), filt.handle, _dp(noise_var_rx),
-                                                             _dp(workspace), workspace.numel()))
-    return hr, hi, workspace
+                                                             _dp(noise_est), _dp(workspace), workspace.numel()))
+    return hr, hi, workspace, noise_est
 
 
 # ----------------------------------------------------------------- detection

@@ -1,0 +1,266 @@
+#!/usr/bin/env python3
+"""Decoded Gbit/s of the batched PUSCH receive chain on one MI355X.
+
+Workload (4 Rx, rank 4, 273 PRB): FFT 4096 at 122.88 Msps, 14 symbols, DMRS
+type 1 on symbols {2, 11} with ports 1000-1003, 256QAM at R = 873/1024 (TBS
+1,081,512 bits, C = 129 code blocks of Z = 384, 40 filler bits), one TB per
+slot.  Synthetic slots from the library's transmitter through a 16-tap TDL
+channel (300 ns RMS, exponential PDP, Jakes 100 Hz, Kronecker Rx
+correlation 0.5) with complex64 IQ; a step is one pass of the chain over a
+batch of slots (IQ already in HBM -> decoded bits and CRC outcomes in HBM),
+replayed from a HIP graph.
+
+The operating SNR is the lowest point of a 20..32 dB sweep (step 2) at which
+the chain's BLER is <= 10 %, measured first on a smaller batch.  Reported
+beside the throughput: BLER and goodput at that SNR, the per-stage times
+(HIP events on the chain's stream), the roofline of the dominant kernel and
+the CPU chain (the C oracle, OpenMP) on a bounded sample of the same slots.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "ai-ran-npu-kernels_amd"
+for p in (str(PKG), str(ROOT / "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "decoded Gbit/s, 4Rx rank-4 273-PRB PUSCH at BLER<=10%; learned vs plain LMMSE"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stage_bytes(g, plan, S: int) -> dict:
+    """Algorithmic HBM bytes per launch of each stage for S slots (compulsory
+    reads of the stage's inputs and writes of its outputs, in the layouts the
+    chain keeps)."""
+    nr, nl, qm = g.nr, g.nl, g.qm
+    data_re = g.ndata * g.used
+    grid16 = S * nr * 14 * g.pad * 4                   # fp16 re+im planes
+    iq = S * nr * g.samples * 8                        # complex64
+    h_ls = S * nr * nl * g.ndmrs * g.pilot_pad * 4
+    h_used = S * nr * nl * 14 * g.used * 4
+    cw = S * qm * g.cw_symbols * 2
+    lam = S * plan.C * 68 * plan.Z * 2
+    bits = S * plan.C * plan.K
+    return {
+        "ofdm_demod": iq + grid16,
+        "dmrs_ls": S * nr * g.ndmrs * g.used * 4 // 2 + h_ls,
+        "ce_lmmse": h_ls + h_used,
+        "detect_demap": S * nr * data_re * 4 + S * nr * nl * data_re * 4 + cw,
+        "rate_dematch": cw + lam,
+        "ldpc_decode": lam + bits,
+        "crc_check": bits,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--slots", type=int, default=64, help="slots per step")
+    ap.add_argument("--snr", type=float, default=None, help="skip the sweep and use this SNR (dB)")
+    ap.add_argument("--sweep-slots", type=int, default=16)
+    ap.add_argument("--cpu-slots", type=int, default=4, help="CPU baseline sample size (slots)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=20260)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the chain shards by slot with no exchange, and runs on more than one "
+              f"GPU are not part of this build (asked for --gpus {args.gpus}).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from puschrx import abi, ops, synth
+    from puschrx import profiles as P
+    from puschrx.chain import ChainOptions, NativeChain
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    torch.cuda.set_device(0)
+    prof = P.FULL_CARRIER_4RX
+    cfg = prof.config()
+    g = ops.Geometry(cfg)
+    tbs = ops.tbs(prof.num_rb, 12 * g.ndata, g.qm, g.nl, prof.code_rate)
+    plan = ops.tb_plan(tbs, g.cw_symbols * g.qm, g.qm, g.nl)
+    spec = synth.ChannelSpec(taps=16, delay_spread_s=300e-9, doppler_hz=100.0, rx_corr=0.5)
+    opts = ChainOptions(ldpc_iterations=8, early_stop=True, tau_s=spec.delay_spread_s, fd_hz=spec.doppler_hz,
+                        ce_rank=96, iq_format=abi.IQ_COMPLEX64)
+    rng = np.random.default_rng(args.seed)
+    S = args.slots
+
+    def make_slots(n, snr, payload=None, grid=None, H=None):
+        if payload is None:
+            payload = synth.random_payload(rng, n, plan.tbs)
+            grid = synth.tx_grid(cfg, plan, payload)
+            H = synth.channel(rng, cfg, spec, n)
+        iq = synth.receive(rng, cfg, grid, H, snr, iq_format=abi.IQ_COMPLEX64)
+        return payload, grid, H, iq
+
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        chain = NativeChain(cfg, plan, max_slots=max(S, args.sweep_slots), opts=opts, use_graph=True)
+        log(f"chain ready, {chain.device_bytes() / 2**20:.0f} MiB on the device")
+
+        def run(iq_np, n):
+            dev = torch.from_numpy(iq_np).cuda()
+            chain.enqueue(dev, n)
+            ok = chain.fetch("tb_ok", torch.uint8, n)
+            torch.cuda.synchronize()
+            return ok.cpu().numpy(), dev
+
+        # 1. Operating point: lowest SNR with BLER <= 10 %.
+        sweep = {}
+        if args.snr is None:
+            n = args.sweep_slots
+            payload, grid, H, _ = make_slots(n, 30.0)
+            snr_op = None
+            for snr in range(20, 34, 2):
+                _, _, _, iq = make_slots(n, float(snr), payload, grid, H)
+                ok, _ = run(iq, n)
+                bler = 1.0 - float(ok.mean())
+                sweep[snr] = bler
+                log(f"sweep snr {snr} dB: BLER {bler:.3f}")
+                if bler <= 0.10:
+                    snr_op = float(snr)
+                    break
+            if snr_op is None:
+                snr_op = 32.0
+        else:
+            snr_op = args.snr
+
+        # 2. Throughput at the operating point.
+        payload, grid, H, iq = make_slots(S, snr_op)
+        iq_dev = torch.from_numpy(iq).cuda()
+        for _ in range(args.warmup):
+            chain.enqueue(iq_dev, S)
+        torch.cuda.synchronize()
+        log(f"warmup done ({args.warmup} steps of {S} slots)")
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start.record(stream)
+        for _ in range(args.steps):
+            chain.enqueue(iq_dev, S)
+        end.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms_per_step = start.elapsed_time(end) / args.steps
+        tb_ok = chain.fetch("tb_ok", torch.uint8, S)
+        bits = chain.fetch("bits", torch.int8, S * plan.C * plan.K)
+        torch.cuda.synchronize()
+        tb_ok = tb_ok.cpu().numpy()
+        b = bits.cpu().numpy().reshape(S, plan.C, plan.K).astype(np.uint8)
+        per_cb = plan.Kprime - plan.crc_cb
+        bit_err_tb = 0
+        for s in range(S):
+            tb = b[s, :, :per_cb].reshape(-1)[: plan.tbs]
+            bit_err_tb += int(not np.array_equal(tb, payload[s]))
+        log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, wall {wall:.3f} s, "
+            f"tb_ok {int(tb_ok.sum())}/{S}, payload mismatches {bit_err_tb}")
+
+        # 3. Per-stage times (HIP events on the chain's stream, direct launches).
+        chain.set_timing(True)
+        acc = {k: 0.0 for k in abi.CHAIN_STAGES}
+        for _ in range(args.steps):
+            chain.enqueue(iq_dev, S)
+            for k, v in chain.stage_times().items():
+                acc[k] += v
+        chain.set_timing(False)
+        torch.cuda.synchronize()
+        stage_ms = {k: v / args.steps for k, v in acc.items()}
+        log("stage ms: " + ", ".join(f"{k} {v:.3f}" for k, v in stage_ms.items()))
+        chain.close()
+
+    tbs_bits = S * plan.tbs
+    value = tbs_bits / (ms_per_step * 1e-3) / 1e9
+    goodput = float(tb_ok.sum()) * plan.tbs / (ms_per_step * 1e-3) / 1e9
+    bytes_per = stage_bytes(g, plan, S)
+    dominant = max(stage_ms, key=stage_ms.get)
+    achieved = bytes_per[dominant] / (stage_ms[dominant] * 1e-3) / 1e9
+    chain_bytes = S * (g.nr * g.samples * 8 + plan.C * plan.K)
+    result = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gbit/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (FFT, CE, detection) / fp16 (LDPC messages) / int16 (LLRs)",
+        "data": "synthetic (library transmitter + TDL channel)",
+        "config": {
+            "workload": "4Rx rank-4 273-PRB PUSCH, FFT 4096, 256QAM R=873/1024, TBS 1081512, C=129, "
+                        "TDL-16 300 ns / 100 Hz / rx-corr 0.5, plain LMMSE",
+            "slots_per_step": S,
+            "snr_db": snr_op,
+            "bler": round(1.0 - float(tb_ok.mean()), 4),
+            "goodput_gbps": round(goodput, 4),
+            "slots_per_s": round(S / (ms_per_step * 1e-3), 1),
+            "ldpc_max_iterations": opts.ldpc_iterations,
+            "ce_rank": opts.ce_rank,
+            "sweep_bler": {str(k): round(v, 4) for k, v in sweep.items()},
+            "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+            "chain_hbm_gbps": round(chain_bytes / (ms_per_step * 1e-3) / 1e9, 2),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+        },
+        "cpu_baseline": None,
+    }
+
+    # 4. CPU chain on a bounded sample of the same slots.
+    if not args.no_cpu:
+        try:
+            from oracle_lib import OrCeParams, cref, load_oracle, or_tb_plan, ptr
+            oracle = load_oracle()
+            op = or_tb_plan(oracle, plan.tbs, plan.G, g.qm, g.nl)
+            prm = OrCeParams(spec.delay_spread_s, spec.doppler_hz, 30e3)
+            n_cpu = min(args.cpu_slots, S)
+            out_bits = np.zeros(op.C * op.K, np.int8)
+            ok = np.zeros(1, np.uint8)
+            t0 = time.perf_counter()
+            for s in range(n_cpu):
+                oracle.or_chain_slot(cref(cfg), C.byref(op), C.byref(prm), opts.ldpc_iterations,
+                                     ptr(np.ascontiguousarray(iq[s])), ptr(out_bits), ptr(ok))
+            dt = time.perf_counter() - t0
+            result["cpu_baseline"] = {
+                "value": round(n_cpu * plan.tbs / dt / 1e9, 6),
+                "unit": "Gbit/s",
+                "cores": int(oracle.or_num_threads()),
+                "kind": "port",
+                "sample": f"{n_cpu} slots of the same workload through the C chain (oracle/or_chain.c), "
+                          f"{dt:.1f} s",
+            }
+            log(f"cpu baseline: {n_cpu} slots in {dt:.2f} s")
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline skipped: {e}")
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,108 @@
+/*
+ * airan_pusch_chain.h -- the batched PUSCH receive chain on one GPU.
+ *
+ * Replaces the slot loop of chain/pusch_mimo_rx_chain/rx_chain.cpp:316-724
+ * (Run: per slot OFDM demod -> RE demap -> DMRS gen -> DMRS LS -> LMMSE CE ->
+ * detector I/O pack -> detection -> QAM demap -> layer demap, then per TB
+ * descramble -> rate dematch -> LDPC) for grants of one shape: `num_slots`
+ * slots, one transport block per slot, run as one pipeline of batched
+ * kernels that a HIP graph replays:
+ *
+ *   OFDM demod + RE demap     airan_ofdm_demod_redemap_Enqueue, all slots
+ *   DMRS LS + noise           the mimo_dmrs_ls kernels, all slots
+ *   LMMSE CE                  the channel_est_lmmse kernels, all slots
+ *   detect+demap+descramble   one fused kernel: exact per-RE LMMSE, the
+ *                             QAM demapper, layer demap and Gold descramble,
+ *                             writing the descrambled codeword LLRs
+ *   rate dematch              the rate_dematch_mimo kernel, all code blocks
+ *   LDPC                      airan_ldpc_decode_Enqueue, all code blocks
+ *   CRC                       airan_crc_check_Enqueue, all transport blocks
+ *
+ * Every stage computes exactly what the per-operator entry points compute
+ * (the fused stage is bit-identical to detect_io_pack + detect_lmmse +
+ * no_eff floor + qam256_demod + layer_demap + descramble), so a slot decoded
+ * here equals the same slot run operator by operator.  No counterpart to the
+ * fused kernel or the batching in the reference.
+ */
+#ifndef AIRAN_PUSCH_CHAIN_H
+#define AIRAN_PUSCH_CHAIN_H
+
+#include "airan_pusch_rx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct AiranPuschChainOptionsV1 {
+    uint16_t abi_version;
+    uint16_t struct_size;
+    uint32_t max_slots;         /* slots per Enqueue, >= 1 */
+    uint32_t iq_format;         /* AIRAN_IQ_INT16_Q8_8 or AIRAN_IQ_COMPLEX64 */
+    uint32_t ldpc_iterations;   /* maximum iterations, 1..64 */
+    uint32_t ldpc_mode;         /* AIRAN_LDPC_MODE_* */
+    uint32_t early_stop;        /* stop a code block once its syndrome is zero */
+    uint32_t ce_max_rank;       /* LMMSE filter eigenpairs, 1..512 */
+    uint32_t use_graph;         /* replay the pipeline from a HIP graph */
+    double tau_s;               /* CE frequency prior: RMS delay spread */
+    double fd_hz;               /* CE time prior: maximum Doppler */
+    double scs_hz;              /* subcarrier spacing */
+    double eig_floor;           /* CE: drop eigenvalues below eig_floor * max */
+    uint32_t reserved[8];
+} AiranPuschChainOptionsV1;
+
+typedef struct AiranPuschChain AiranPuschChain;
+
+/* Device buffers of the last Enqueue (valid until the next one), slot-major:
+ * bits int8 [slots][C][K], tb_ok / cb_ok / iterations uint8, and the
+ * intermediates in the operator layouts for inspection. */
+typedef struct AiranPuschChainOutputsV1 {
+    uint16_t abi_version;
+    uint16_t struct_size;
+    uint32_t num_slots;
+    const void *bits;           /* int8 [slots][C][K] */
+    const void *tb_ok;          /* uint8 [slots] */
+    const void *cb_ok;          /* uint8 [slots][C] */
+    const void *iterations;     /* uint8 [slots][C] */
+    const void *rx_grid_re;     /* fp16 [slots][NR][14][padded] */
+    const void *rx_grid_im;
+    const void *h_ls_re;        /* fp16 [slots][NR][L][D][pilot_pad] */
+    const void *h_ls_im;
+    const void *noise_ls;       /* fp16 [slots][NR] (DMRS LS) */
+    const void *noise_ce;       /* fp16 [slots][NR] (CE subspace estimate) */
+    const void *h_re;           /* fp16 [slots][NR][L][14][padded] */
+    const void *h_im;
+    const void *cw_llr;         /* int16 [slots][Qm][codeword_stride], descrambled */
+    const void *ldpc_llr;       /* int16 [slots][C][68 Z] */
+} AiranPuschChainOutputsV1;
+
+/* Builds the CE filter, the DMRS reference, the Gold bits and every buffer
+ * for max_slots slots, then runs the pipeline once on a zero slot (loading
+ * the per-device tables) and synchronises `stream`.  plan->G must be one
+ * slot's coded bits (codeword_symbols * Qm). */
+int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPlanV1 *plan,
+                                 const AiranPuschChainOptionsV1 *options, void *stream,
+                                 AiranPuschChain **chain);
+void airan_pusch_chain_Destroy(AiranPuschChain *chain);
+/* iq: device [num_slots][NR][samples][2] (int16 Q8.8 or float per iq_format).
+ * Asynchronous on `stream`; with use_graph, the first call for a given
+ * (iq, num_slots, stream) captures the graph. */
+int32_t airan_pusch_chain_Enqueue(AiranPuschChain *chain, const void *iq, uint32_t num_slots,
+                                  void *stream);
+int32_t airan_pusch_chain_Outputs(const AiranPuschChain *chain, AiranPuschChainOutputsV1 *out);
+/* Device bytes the chain holds for max_slots (0 on a bad argument). */
+size_t airan_pusch_chain_DeviceBytes(const AiranPuschChain *chain);
+
+/* Stage timing.  With timing on, Enqueue launches the stages directly (no
+ * graph) and records a HIP event on the stream before the first stage and
+ * after each one; StageTimes waits for the last event and returns the
+ * milliseconds of each stage of the last Enqueue, in pipeline order:
+ * 0 OFDM, 1 DMRS LS, 2 CE, 3 detect+demap, 4 rate dematch, 5 LDPC, 6 CRC. */
+#define AIRAN_CHAIN_STAGES 7
+int32_t airan_pusch_chain_SetTiming(AiranPuschChain *chain, uint32_t enabled);
+int32_t airan_pusch_chain_StageTimes(const AiranPuschChain *chain, float *ms, uint32_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -276,12 +276,19 @@ int32_t airan_channel_est_lmmse_CreateFilter(const PuschMimoConfig *config,
                                              const LmmseFilterParamsV1 *params, void *stream,
                                              LmmseFilter **filter);
 void airan_channel_est_lmmse_DestroyFilter(LmmseFilter *filter);
+/* Eigenpairs kept for `layer` (top of R_pp, at most params->max_rank). */
 uint32_t airan_channel_est_lmmse_FilterRank(const LmmseFilter *filter, uint32_t layer);
-/* noise_var_rx: fp16 [NR] from mimo_dmrs_ls.  workspace >= WorkspaceBytes. */
+/* The noise variance the filter uses is estimated from the energy of h_ls
+ * outside the kept eigenspace (n - kept dimensions per layer and DMRS
+ * symbol); noise_var_rx (fp16 [NR] from mimo_dmrs_ls, whose adjacent-pilot
+ * differences also pick up the channel's frequency selectivity) is the
+ * fallback when the filter keeps every eigenvector.  noise_out (fp16 [NR],
+ * may be NULL) receives the per-Rx estimate used, for the detector.
+ * workspace >= WorkspaceBytes. */
 size_t airan_channel_est_lmmse_WorkspaceBytes(const PuschMimoConfig *config);
 int32_t airan_channel_est_lmmse_Enqueue(const ChannelEstLmmseOpArgsV1 *args,
                                         const LmmseFilter *filter, const void *noise_var_rx,
-                                        void *workspace, size_t workspace_bytes);
+                                        void *noise_out, void *workspace, size_t workspace_bytes);
 
 /* ======================================================= detector IO pack */
 /* mimo/mimo_detect_io_pack/mimo_detect_io_pack.h:69-121 */
@@ -491,6 +498,15 @@ int32_t airan_pusch_BuildTbPlan(uint32_t tbs, uint32_t G, uint32_t qm, uint32_t 
  * no CRC, rv0 (rate_dematch_mimo.cpp:90-112). */
 int32_t airan_pusch_ReferenceTbPlan(const PuschMimoConfig *config, uint32_t num_slots,
                                     AiranTbPlanV1 *plan);
+/* Smallest post-detection noise variance the QAM demapper's integer
+ * arithmetic represents: below it u = rint(y fp16(32 D / n)) saturates at
+ * +-1000 for the outer levels and t = rint(fp16(32 D^2 / n)) overflows, so
+ * the magnitude-bit LLRs collapse (qam_demod_256_mimo_batch/scripts/
+ * gen_data.py:20-27 at n -> 0).  The chain clamps no_eff to this floor
+ * before demapping: 0.04 (2^(Qm/2) - 1) D^2, i.e. |y| up to 1.25x the
+ * outermost level stays below |u| = 1000.  No counterpart in the reference,
+ * which leaves high-SNR operating points to the saturated arithmetic. */
+float airan_pusch_NoEffFloor(uint32_t qm);
 
 /* ========================================================= rate dematch */
 /* fec/rate_dematch_mimo/rate_dematch_mimo.h:58-137, generalised with filler

@@ -154,6 +154,13 @@ uint32_t or_crc(int kind, const uint8_t *bits, uint64_t nbits);
  * [NR, 14, fft] (unitary DFT, no fftshift). */
 void or_ofdm_demod(const PuschMimoConfig *cfg, const float *iq, double *grid);
 
+/* ----------------------------------------------------------------- chain */
+/* One slot, one TB: iq complex64 [NR, samples] -> bits int8 [C, K] and the
+ * TB CRC outcome (or_chain.c).  0 on success. */
+int or_chain_slot(const PuschMimoConfig *cfg, const OrTbPlan *p, const OrCeParams *prm, int ldpc_iters,
+                  const float *iq, int8_t *bits, uint8_t *tb_ok);
+int or_num_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,8 @@ def load_oracle():
     lib.or_crc.restype = u32
     lib.or_crc.argtypes = [C.c_int, vp, u64]
     lib.or_ofdm_demod.argtypes = [vp, vp, vp]
+    lib.or_chain_slot.argtypes = [vp, C.POINTER(OrTbPlan), C.POINTER(OrCeParams), C.c_int, vp, vp, vp]
+    lib.or_num_threads.restype = C.c_int
     _ORACLE = lib
     return lib
 

@@ -6,7 +6,12 @@ Tolerances (stated per stage, all against the FP64 oracle):
   OFDM     each fp16 output within 1 fp16 ulp (+1e-5 abs) of the exact DFT,
            NMSE <= -60 dB; fused re-demap output bit-identical to
            re_demap(stage-4 output).
-  CE       NMSE <= -50 dB over the used grid; noise statistics to 1e-4 rel.
+  CE       NMSE <= -50 dB over the used grid against the full-rank FP64
+           estimator fed the same relative noise (-40 dB when the filter is
+           truncated, which the bound then also covers); the subspace-residual noise
+           estimate within 5 % of a numpy eigh restatement (and within 15 %
+           of the injected noise); with every eigenvector kept, the DMRS-LS
+           noise passes through bit-exact and the statistics match to 1e-4.
   io_pack  bit-exact (layout transform + fp32 mean of fp16 noise).
   detect   NMSE <= -50 dB for xhat and no_eff (FP32 LDL^H vs FP64 inverse).
 """
@@ -120,7 +125,8 @@ def _tdl_freq_response(rng, n_sc, taps, tau_rms, scs):
 
 
 @pytest.mark.parametrize("name", list(CE_PROFILES))
-def test_ce_lmmse_matches_fp64(oracle, gpu_stream, name):
+@pytest.mark.parametrize("max_rank", [512, 96])
+def test_ce_lmmse_matches_fp64(oracle, gpu_stream, name, max_rank):
     cfg = CE_PROFILES[name].config()
     g = OracleGeom(cfg)
     rng = np.random.default_rng(5)
@@ -146,26 +152,54 @@ def test_ce_lmmse_matches_fp64(oracle, gpu_stream, name):
     h16r = h_ls.real.astype(np.float16)
     h16i = h_ls.imag.astype(np.float16)
     noise16 = np.full(g.nr, sigma2, np.float16)
-    filt = ops.LmmseFilter(cfg, tau_s=tau, fd_hz=0.0, scs_hz=scs, max_rank=512)
+    # eig_floor keeps the numerically null eigenvectors out of the kept set,
+    # so the kept subspace (and the residual noise estimate) is well defined.
+    filt = ops.LmmseFilter(cfg, tau_s=tau, fd_hz=0.0, scs_hz=scs, max_rank=max_rank, eig_floor=1e-12)
+    kept = [filt.rank(l) for l in range(g.nl)]
+    nobs = [g.n_occ if occ[l] else g.n_dmrs_re for l in range(g.nl)]
     for l in range(g.nl):
-        n = g.n_occ if occ[l] else g.n_dmrs_re
-        r = filt.rank(l)
-        assert r % 16 == 0 and 16 <= r <= min(512, (n + 15) // 16 * 16)
+        assert 1 <= kept[l] <= min(max_rank, nobs[l])
     sc_d = torch.zeros(g.nl * g.ndmrs * g.pilot_pad, dtype=torch.int16, device="cuda")
     cnt_d = torch.zeros(16, dtype=torch.int16, device="cuda")
-    hr, hi, ws = ops.ce_lmmse(cfg, filt, dev(h16r.ravel()), dev(h16i.ravel()), sc_d, cnt_d, dev(noise16))
-    # Noise statistics: n_obs,l = s2 sum(w^2) / P_l.
-    stats = host(ws[:16].view(torch.float32))[:g.nl]
-    s2 = float(np.float32(noise16.astype(np.float32).sum()) / np.float32(g.nr))
+    hr, hi, ws, noise_est = ops.ce_lmmse(cfg, filt, dev(h16r.ravel()), dev(h16i.ravel()), sc_d, cnt_d,
+                                         dev(noise16))
     hq = h16r.astype(np.float64) + 1j * h16i.astype(np.float64)
+    wss = [0.5 if occ[l] else 1.0 for l in range(g.nl)]
+    # Noise: energy outside the top-`kept` eigenspace of R_pp (numpy eigh of
+    # the observation model restated here), else the DMRS-LS fallback.
+    dims = sum((nobs[l] - kept[l]) * g.ndmrs for l in range(g.nl))
+    if dims >= 32:
+        res = np.zeros(g.nr)
+        for l in range(g.nl):
+            n = nobs[l]
+            if occ[l]:
+                sup = np.stack([comb[l] + 4 * np.arange(n), comb[l] + 4 * np.arange(n) + 2], 1)
+            else:
+                sup = (comb[l] + 2 * np.arange(n))[:, None]
+            dk = sup[:, None, :, None] - sup[None, :, None, :]
+            rpp = (wss[l] ** 2 * (1.0 / (1.0 + 2j * np.pi * dk * scs * tau))).sum(axis=(2, 3))
+            w, v = np.linalg.eigh(rpp)
+            u = v[:, ::-1][:, : kept[l]]
+            for rx in range(g.nr):
+                for d in range(g.ndmrs):
+                    h = hq[rx, l, d, :n]
+                    res[rx] += (np.sum(np.abs(h) ** 2) - np.sum(np.abs(u.conj().T @ h) ** 2)) / wss[l]
+        s2_rx = res / dims
+        assert np.allclose(host(noise_est).astype(np.float64), s2_rx, rtol=0.05)
+        # The estimate recovers the injected noise (not the DMRS-LS input).
+        assert np.mean(s2_rx) == pytest.approx(sigma2, rel=0.15)
+    else:
+        s2_rx = noise16.astype(np.float64)
+        assert np.array_equal(host(noise_est).view(np.uint16), noise16.view(np.uint16))
+    s2 = float(np.mean(s2_rx))
+    # Filter statistics: n_obs,l = s2 sum(w^2) / P_l.
+    stats = host(ws[:16].view(torch.float32))[:g.nl]
     noise_rel = np.zeros(g.nl)
     for l in range(g.nl):
-        n = g.n_occ if occ[l] else g.n_dmrs_re
-        mean = np.mean(np.abs(hq[:, l, :, :n]) ** 2)
-        wss = 0.5 if occ[l] else 1.0
-        p_l = max(mean - s2 * wss, 1e-3 * mean)
-        noise_rel[l] = s2 / p_l
-        assert stats[l] == pytest.approx(s2 * wss / p_l, rel=1e-4)
+        mean = np.mean(np.abs(hq[:, l, :, : nobs[l]]) ** 2)
+        p_l = max(mean - s2 * wss[l], 1e-3 * mean)
+        assert stats[l] == pytest.approx(s2 * wss[l] / p_l, rel=0.05 if dims >= 32 else 1e-4)
+        noise_rel[l] = stats[l] / wss[l]
     prm = OrCeParams(tau, 0.0, scs)
     inter = np.zeros(g.nr * g.nl * g.ndmrs * g.pilot_pad * 2)
     inter[0::2] = hq.real.ravel()
@@ -175,7 +209,10 @@ def test_ce_lmmse_matches_fp64(oracle, gpu_stream, name):
     want = (want[0::2] + 1j * want[1::2]).reshape(g.nr, g.nl, 14, g.used)
     got = (host(hr).astype(np.float64) + 1j * host(hi).astype(np.float64)).reshape(g.nr, 16, 14, g.pad)
     assert np.all(got[:, : g.nl, :, g.used:] == 0)
-    assert nmse_db(got[:, : g.nl, :, : g.used], want) <= -50.0
+    # Full rank: the same estimator as the oracle.  Truncated: the oracle is
+    # still full rank, so the bound also covers the truncation error.
+    full = all(kept[l] == nobs[l] for l in range(g.nl))
+    assert nmse_db(got[:, : g.nl, :, : g.used], want) <= (-50.0 if full else -40.0)
     filt.close()
 
 

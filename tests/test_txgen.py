@@ -102,3 +102,26 @@ def test_tx_dmrs_matches_oracle_generator(lib, oracle):
             got = grid[l, s, comb::2][: g.n_dmrs_re]
             assert np.allclose(got, ref[l, d, : g.n_dmrs_re], atol=1e-3)
             assert np.all(grid[l, s, 1 - comb::2] == 0)
+
+
+@pytest.mark.parametrize("prof", [P.SISO_52PRB, P.MID_4RX_RANK2], ids=lambda p: p.name)
+def test_oracle_chain_decodes_a_fading_slot(lib, oracle, prof):
+    """The CPU chain (the CPU baseline) decodes a transmitted slot."""
+    from oracle_lib import OrCeParams
+    cfg, g, plan = _plan_for(prof)
+    rng = np.random.default_rng(3)
+    payload = synth.random_payload(rng, 1, plan.tbs)
+    grid = synth.tx_grid(cfg, plan, payload)
+    spec = synth.ChannelSpec(taps=4, delay_spread_s=50e-9, doppler_hz=5.0, rx_corr=0.3, snr_db=32.0)
+    H = synth.channel(rng, cfg, spec, 1)
+    iq = synth.receive(rng, cfg, grid, H, spec.snr_db, iq_format=abi.IQ_COMPLEX64)
+    op = or_tb_plan(oracle, plan.tbs, plan.G, g.qm, g.nl)
+    bits = np.zeros(op.C * op.K, np.int8)
+    ok = np.zeros(1, np.uint8)
+    prm = OrCeParams(spec.delay_spread_s, spec.doppler_hz, 30e3)
+    assert oracle.or_chain_slot(cref(cfg), C.byref(op), C.byref(prm), 8, ptr(np.ascontiguousarray(iq[0])),
+                                ptr(bits), ptr(ok)) == 0
+    assert ok[0] == 1
+    b = bits.reshape(op.C, op.K).astype(np.uint8)
+    tb = np.concatenate([b[r, : op.Kprime - op.crc_cb] for r in range(op.C)])[: plan.tbs]
+    assert np.array_equal(tb, payload[0])
