@@ -1,20 +1,28 @@
 // op_ce.hip -- LMMSE channel estimation from DMRS least-squares observations.
 //
 // Estimator (channel_est/channel_est_lmmse_mimo/scripts/channel_est_lmmse_ref.py:
-// 50-117 with the observation model of observation_model_ref.py:41-79):
-//   R_f(dk) = 1 / (1 + j 2 pi dk scs tau)           exponential-PDP prior
-//   R_pp, R_hp from the observation geometry (comb, or FD-OCC2 pair mean)
-//   R_pp = U diag(lam) U^H  (top `rank` eigenpairs kept)
-//   H_f  = scale * (R_hp U) diag(1/(lam + n_obs)) U^H h_obs
-//   scale = 2 / (hhat_var + 1 - err), err = max(1 - hhat_var, 0),
-//   hhat_var(sc) = sum_k |(R_hp U)_sc,k|^2 / (lam_k + n_obs)   (ref.py:109-116)
-//   time: per-subcarrier Wiener weights over the DMRS symbols with
-//   R_t = J0(2 pi f_d dt) plus the rescaled error (ref.py:75-88).
-// The reference builds these factors offline for one fixed SNR; here the
-// eigen-factors are built once per grant geometry in FP64 on the device
-// (prior covariances by a kernel, rocSOLVER zheevd, rocBLAS zgemm) and the
-// noise-dependent parts (1/(lam + n_obs), rescale, time weights) are applied
-// per slot from the DMRS-LS noise estimate, so no host round trip is needed.
+// 50-117): exponential-PDP frequency prior R_f(dk) = 1 / (1 + j 2 pi dk scs
+// tau), top-`rank` eigenpairs of the observation covariance, the
+// Sionna-style rescale of ref.py:109-116 and per-subcarrier time Wiener
+// weights with R_t = J0(2 pi f_d dt) (ref.py:75-88).
+//
+// Observation models.  A layer alone on its comb is observed at comb + 2p
+// (one "comb model").  For two layers sharing a comb with FD-OCC2 the
+// reference averages each pair (observation_model_ref.py:41-79), which
+// assumes the channel flat over the pair and leaks the partner layer's
+// channel into the estimate (about -25 dB at 300 ns delay spread).  Here the
+// pair is estimated jointly instead, without that assumption: with
+// z_m = y_m r_m* on comb + 2m, z_{2p} = h0 + h1 and z_{2p+1} = h0 - h1
+// (FD-OCC signs +,+ and +,-), recovered exactly from the two layers' h_ls
+// (z_{2p} = h_ls0[p] + h_ls1[p], z_{2p+1} = h_ls0[p] - h_ls1[p]).  For
+// independent layers of equal power g+ = h0 + h1 and g- = h0 - h1 are
+// uncorrelated, each observed on a comb of spacing 4 (offsets comb and
+// comb + 2); each gets its own comb-model filter and h0,1 = (g+ +- g-) / 2.
+//
+// The eigen-factors of every observation model are built once per grant
+// shape in FP64 on the device (covariances by a kernel, rocSOLVER zheevd,
+// rocBLAS zgemm); the noise-dependent parts are applied per slot, with the
+// noise estimated from the energy outside the kept eigenspace.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -23,24 +31,32 @@
 #include <vector>
 
 #include "common.h"
+#include "cgemm.h"
 #include "launch.h"
 
-using airan::DevGeom;
 using airan::Geom;
 
 namespace {
 
 constexpr uint32_t kWeightModelMagic = 0x43455731u;  // "CEW1"
 constexpr int kMaxRank = 512;
+constexpr uint32_t kMaxModels = 4;
+constexpr uint32_t kMaxObs = 2048;         // observations per model (used / 2 at 273 PRB: 1638)
+constexpr double kMinNoiseDims = 32.0;     // residual dimensions for the subspace noise estimate
 
-struct CeLayerFactors {
-    uint32_t n;       // observations
-    uint32_t rank;    // stored eigenpairs (multiple of 16; columns >= kept are zero)
-    uint32_t kept;    // eigenpairs actually kept
-    float2 *bt;       // [n][rank]   U_r^H transposed: bt[p][k] = conj(U[p][k])
-    float2 *at;       // [rank][used] (R_hp U_r) transposed, sc contiguous
-    float *lam;       // [rank] eigenvalues (0 for padded ranks)
-    float w_sum_sq;   // sum of squared observation weights (1 or 0.5)
+// One observation model: n observations at offset + step * p.
+struct CeModelFactors {
+    uint32_t n = 0;      // observations
+    uint32_t step = 0;   // subcarrier spacing of the observations (2 or 4)
+    uint32_t offset = 0;
+    uint32_t in_a = 0;   // z = h_ls[in_a] + sign_b * h_ls[in_b]
+    uint32_t in_b = 0;
+    float sign_b = 0.0f;
+    uint32_t rank = 0;   // stored eigenpairs (multiple of 16; columns >= kept are zero)
+    uint32_t kept = 0;   // eigenpairs actually kept
+    float2 *bt = nullptr;  // [n][rank]    bt[p][k] = conj(U[p][k])
+    float2 *at = nullptr;  // [rank][used] (R_hp U) transposed, subcarrier contiguous
+    float *lam = nullptr;  // [rank] eigenvalues (1 for padded columns)
 };
 
 }  // namespace
@@ -49,10 +65,12 @@ struct LmmseFilter {
     int device;
     Geom g;
     LmmseFilterParamsV1 params;
-    CeLayerFactors layer[4];
-    int shared_with[4];    // layer whose factors this layer reuses (same comb/model)
-    float rt[14][4];       // J0(2 pi f_d (s - d_j) T)  per symbol and DMRS symbol
-    float rtt[4][4];       // J0 between DMRS symbols
+    uint32_t nm;                       // observation models
+    CeModelFactors model[kMaxModels];
+    int32_t layer_model[4][2];         // models of layer l (second = -1 for a comb model)
+    float layer_sign[4];               // h_l = (g_a + sign g_b) / 2 for an FD-OCC pair
+    float rt[14][4];                   // J0(2 pi f_d (s - d_j) T) per symbol and DMRS symbol
+    float rtt[4][4];                   // J0 between DMRS symbols
 };
 
 namespace {
@@ -65,24 +83,10 @@ __device__ inline double2 RfDev(double dsc, double scs_tau)
     return make_double2(den, -a * den);
 }
 
-// Observation support of pilot p: comb model -> one subcarrier comb + 2p;
-// FD-OCC2 -> mean of comb + 4p and comb + 4p + 2 (observation_model_ref.py:41-58).
 struct ObsGeom {
-    uint32_t comb, occ, n, used;
+    uint32_t n, step, offset, used;
     double scs_tau;
 };
-
-__device__ inline void Support(const ObsGeom &o, uint32_t p, int *s, int *w_count)
-{
-    if (o.occ) {
-        s[0] = static_cast<int>(o.comb + 4 * p);
-        s[1] = s[0] + 2;
-        *w_count = 2;
-    } else {
-        s[0] = static_cast<int>(o.comb + 2 * p);
-        *w_count = 1;
-    }
-}
 
 // R_pp, column-major n x n (element (p, q) at p + q n).
 __global__ void __launch_bounds__(256) BuildRppKernel(double2 *__restrict__ rpp, ObsGeom o)
@@ -90,18 +94,8 @@ __global__ void __launch_bounds__(256) BuildRppKernel(double2 *__restrict__ rpp,
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t q = blockIdx.y;
     if (p >= o.n) return;
-    int sp[2], sq[2], wp, wq;
-    Support(o, p, sp, &wp);
-    Support(o, q, sq, &wq);
-    const double w = o.occ ? 0.5 : 1.0;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int a = 0; a < wp; ++a)
-        for (int b = 0; b < wq; ++b) {
-            const double2 v = RfDev(static_cast<double>(sp[a] - sq[b]), o.scs_tau);
-            acc.x += w * w * v.x;
-            acc.y += w * w * v.y;
-        }
-    rpp[p + static_cast<size_t>(q) * o.n] = acc;
+    const double dk = (static_cast<double>(p) - static_cast<double>(q)) * o.step;
+    rpp[p + static_cast<size_t>(q) * o.n] = RfDev(dk, o.scs_tau);
 }
 
 // R_hp, column-major used x n.
@@ -110,16 +104,8 @@ __global__ void __launch_bounds__(256) BuildRhpKernel(double2 *__restrict__ rhp,
     const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t p = blockIdx.y;
     if (sc >= o.used) return;
-    int sp[2], wp;
-    Support(o, p, sp, &wp);
-    const double w = o.occ ? 0.5 : 1.0;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int a = 0; a < wp; ++a) {
-        const double2 v = RfDev(static_cast<double>(static_cast<int>(sc) - sp[a]), o.scs_tau);
-        acc.x += w * v.x;
-        acc.y += w * v.y;
-    }
-    rhp[sc + static_cast<size_t>(p) * o.used] = acc;
+    const double dk = static_cast<double>(sc) - static_cast<double>(o.offset + o.step * p);
+    rhp[sc + static_cast<size_t>(p) * o.used] = RfDev(dk, o.scs_tau);
 }
 
 // Keep the top `rank` eigenpairs (rocSOLVER returns ascending order) in
@@ -154,124 +140,244 @@ __global__ void __launch_bounds__(256) PackAtKernel(const double2 *__restrict__ 
 }
 
 // ---------------------------------------------------------------- apply
+struct CeModelDev {
+    const float2 *bt;
+    const float2 *at;
+    const float *lam;
+    uint32_t n, rank, kept, in_a, in_b;
+    float sign_b;
+};
+
 struct CeApply {
-    uint32_t nr, nl, ndmrs, pilot_pad, used, pad, nsym;
-    uint32_t dmrs_sym[4];
-    uint32_t n[4], rank[4], kept[4];
-    float w_sum_sq[4];
+    uint32_t nr, nl, nm, ndmrs, pilot_pad, used, pad, nsym;
+    CeModelDev m[kMaxModels];
+    int32_t layer_model[4][2];
+    float layer_sign[4];
     float rt[14][4];
     float rtt[4][4];
 };
 
-// Projection onto the kept eigenvectors, unscaled: u[k][col] = sum_p
-// conj(U[p][k]) h_obs[col][p], col = rx * D + d, plus the column energies
-// |h_obs|^2 and |u|^2 (double) for the noise estimate.  One workgroup per
-// (col, slot * L + l); h_obs staged in LDS, thread k owns eigenvectors k,
-// k + 256.  T of (slot, l) at (slot * L + l) * t_layer_stride.
-constexpr uint32_t kProjThreads = 256;
-constexpr uint32_t kMaxObs = 2048;
-constexpr double kMinNoiseDims = 32.0;
+// Workspace layout for `slots` slots (each part 256-byte aligned):
+//   stats  [slots][4]                       fp32  relative noise n_q
+//   energy [slots][models][NR*D][2]         fp64  |z|^2, |u|^2 per column
+//   inv    [slots][models][kMaxRank]        fp32  1 / (lam_k + n_q)
+//   hv     [slots][models][used]            fp32  hhat variance
+//   t      [slots][models][kMaxRank][NR*D]  c64   U^H z
+//   hf     [slots][models][NR*D][used]      c64   frequency estimate (unscaled)
+struct CeWs {
+    float *stats;
+    double *energy;
+    float *inv;
+    float *hv;
+    float2 *t;
+    float2 *hf;
+};
 
-__global__ void __launch_bounds__(kProjThreads) CeProjectKernel(
-    const __half *__restrict__ hr, const __half *__restrict__ hi, const float2 *__restrict__ bt0,
-    const float2 *__restrict__ bt1, const float2 *__restrict__ bt2, const float2 *__restrict__ bt3,
-    float2 *__restrict__ t_out, double *__restrict__ energy, uint32_t t_layer_stride, CeApply c)
+size_t Align256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t CeWsLayout(const Geom &g, uint32_t slots, uint8_t *base, CeWs *w)
 {
-    __shared__ float2 hs[kMaxObs];
-    __shared__ double red[2][kProjThreads / 64];
-    const uint32_t col = blockIdx.x;
-    const uint32_t slot = blockIdx.y / c.nl, l = blockIdx.y - slot * c.nl;
-    const uint32_t rx = col / c.ndmrs, d = col - rx * c.ndmrs;
-    const uint32_t ncol = c.nr * c.ndmrs;
-    const size_t hb = (((static_cast<size_t>(slot) * c.nr + rx) * c.nl + l) * c.ndmrs + d) * c.pilot_pad;
-    const uint32_t n = c.n[l], rank = c.rank[l];
-    const float2 *bt = l == 0 ? bt0 : (l == 1 ? bt1 : (l == 2 ? bt2 : bt3));
-    double eh = 0.0;
-    for (uint32_t p = threadIdx.x; p < n; p += kProjThreads) {
-        const float2 v = make_float2(__half2float(hr[hb + p]), __half2float(hi[hb + p]));
-        hs[p] = v;
-        eh += static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
+    const size_t sm = static_cast<size_t>(slots) * g.nl;  // models <= layers
+    const size_t ncol = static_cast<size_t>(g.nr) * g.ndmrs;
+    const size_t sizes[6] = {Align256(16u * slots), Align256(sm * ncol * 16), Align256(sm * kMaxRank * 4),
+                             Align256(sm * g.used * 4), Align256(sm * kMaxRank * ncol * 8),
+                             Align256(sm * ncol * g.used * 8)};
+    size_t off = 0;
+    void *ptrs[6];
+    for (int i = 0; i < 6; ++i) {
+        ptrs[i] = base ? base + off : nullptr;
+        off += sizes[i];
     }
-    __syncthreads();
-    double eu = 0.0;
-    float2 *tcol = t_out + (static_cast<size_t>(slot) * c.nl + l) * t_layer_stride + col;
-    for (uint32_t k = threadIdx.x; k < rank; k += kProjThreads) {
-        float ar = 0.0f, ai = 0.0f;
-        for (uint32_t p = 0; p < n; ++p) {
-            const float2 b = bt[static_cast<size_t>(p) * rank + k];
-            const float2 x = hs[p];
-            ar += b.x * x.x - b.y * x.y;
-            ai += b.x * x.y + b.y * x.x;
-        }
-        tcol[static_cast<size_t>(k) * ncol] = make_float2(ar, ai);
-        eu += static_cast<double>(ar) * ar + static_cast<double>(ai) * ai;
+    if (w) {
+        w->stats = static_cast<float *>(ptrs[0]);
+        w->energy = static_cast<double *>(ptrs[1]);
+        w->inv = static_cast<float *>(ptrs[2]);
+        w->hv = static_cast<float *>(ptrs[3]);
+        w->t = static_cast<float2 *>(ptrs[4]);
+        w->hf = static_cast<float2 *>(ptrs[5]);
+    }
+    return off;
+}
+
+__device__ inline float2 LoadZ(const __half *hr, const __half *hi, const CeApply &c, const CeModelDev &md,
+                               uint32_t slot, uint32_t col, uint32_t p)
+{
+    if (p >= md.n) return make_float2(0.0f, 0.0f);
+    const uint32_t rx = col / c.ndmrs, d = col - rx * c.ndmrs;
+    const size_t ha = (((static_cast<size_t>(slot) * c.nr + rx) * c.nl + md.in_a) * c.ndmrs + d) * c.pilot_pad + p;
+    float2 v = make_float2(__half2float(hr[ha]), __half2float(hi[ha]));
+    if (md.sign_b != 0.0f) {
+        const size_t hb = (((static_cast<size_t>(slot) * c.nr + rx) * c.nl + md.in_b) * c.ndmrs + d) * c.pilot_pad + p;
+        v.x += md.sign_b * __half2float(hr[hb]);
+        v.y += md.sign_b * __half2float(hi[hb]);
+    }
+    return v;
+}
+
+// t[slot, q][k][col] = sum_p conj(U[p][k]) z[slot, col][p]: per model q =
+// blockIdx.z a GEMM with M = rank (k), N = slots * NR * D, K = n.
+__global__ void __launch_bounds__(airan::kGemmThreads) CeProjectKernel(const __half *__restrict__ hr,
+                                                                       const __half *__restrict__ hi, CeWs w,
+                                                                       uint32_t slots, CeApply c)
+{
+    const uint32_t q = blockIdx.z;
+    const CeModelDev &md = c.m[q];
+    const uint32_t ncol = c.nr * c.ndmrs;
+    const uint32_t m0 = blockIdx.x * airan::kGemmTile, n0 = blockIdx.y * airan::kGemmTile;
+    if (m0 >= md.rank) return;
+    const uint32_t N = slots * ncol;
+    auto la = [&](uint32_t k, uint32_t p) {
+        return (k < md.rank && p < md.n) ? md.bt[static_cast<size_t>(p) * md.rank + k] : make_float2(0.0f, 0.0f);
+    };
+    auto lb = [&](uint32_t p, uint32_t n) {
+        if (n >= N) return make_float2(0.0f, 0.0f);
+        const uint32_t slot = n / ncol, col = n - slot * ncol;
+        return LoadZ(hr, hi, c, md, slot, col, p);
+    };
+    auto st = [&](uint32_t k, uint32_t n, float2 v) {
+        const uint32_t slot = n / ncol, col = n - slot * ncol;
+        w.t[((static_cast<size_t>(slot) * c.nm + q) * kMaxRank + k) * ncol + col] = v;
+    };
+    airan::CGemmTile<true>(m0, n0, md.rank, N, md.n, la, lb, st);
+}
+
+// Column energies: |z|^2 over the observations and |u|^2 over the kept
+// eigenvectors, fp64 with a fixed reduction tree.  Grid slots * models * NR*D.
+__global__ void __launch_bounds__(256) CeEnergyKernel(const __half *__restrict__ hr, const __half *__restrict__ hi,
+                                                      CeWs w, CeApply c)
+{
+    __shared__ double red[2][4];
+    const uint32_t ncol = c.nr * c.ndmrs;
+    const uint32_t b = blockIdx.x;
+    const uint32_t col = b % ncol, sq = b / ncol;
+    const uint32_t q = sq % c.nm, slot = sq / c.nm;
+    const CeModelDev &md = c.m[q];
+    double ez = 0.0, eu = 0.0;
+    for (uint32_t p = threadIdx.x; p < md.n; p += blockDim.x) {
+        const float2 v = LoadZ(hr, hi, c, md, slot, col, p);
+        ez += static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
+    }
+    const float2 *tc = w.t + (static_cast<size_t>(sq) * kMaxRank) * ncol + col;
+    for (uint32_t k = threadIdx.x; k < md.rank; k += blockDim.x) {
+        const float2 u = tc[static_cast<size_t>(k) * ncol];
+        eu += static_cast<double>(u.x) * u.x + static_cast<double>(u.y) * u.y;
     }
     for (int off = 32; off > 0; off >>= 1) {
-        eh += __shfl_xor(eh, off);
+        ez += __shfl_xor(ez, off);
         eu += __shfl_xor(eu, off);
     }
     if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x / 64] = eh;
+        red[0][threadIdx.x / 64] = ez;
         red[1][threadIdx.x / 64] = eu;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (uint32_t w = 0; w < kProjThreads / 64; ++w) {
-            a += red[0][w];
-            b += red[1][w];
-        }
-        double *e = energy + 2 * ((static_cast<size_t>(slot) * c.nl + l) * ncol + col);
-        e[0] = a;
-        e[1] = b;
+        double *e = w.energy + 2 * (static_cast<size_t>(sq) * ncol + col);
+        e[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        e[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     }
 }
 
-// Noise and per-layer statistics of one slot (thread 0, double, fixed order).
-// The observation noise lies in the complement of the kept eigenspace: per
-// Rx, sigma^2 = sum_{l,d} (|h|^2 - |u|^2) / sum(w^2)_l over
-// sum_{l,d} (n_l - kept_l) dimensions (the DMRS-LS estimate, which the
-// channel's frequency selectivity inflates, is only the fallback when fewer
-// than kMinNoiseDims dimensions are left).  P_l = mean |h_obs|^2 - sigma^2 sum(w^2)_l and the
-// filter's relative noise n_obs,l = sigma^2 sum(w^2)_l / P_l ->
-// stats[4 slot + l].  noise_out[slot][rx] = fp16(sigma^2_rx) when non-null.
-__global__ void __launch_bounds__(64) CeStatsKernel(const double *__restrict__ energy,
-                                                    const __half *__restrict__ noise_in,
-                                                    __half *__restrict__ noise_out, float *__restrict__ stats,
-                                                    CeApply c)
+// Noise and per-model statistics of one slot (thread 0, fp64, fixed order),
+// then inv[slot, q][k] = 1 / (lam_k + n_q) by all threads.
+// Every observation z carries the per-RE noise sigma^2 (comb: y r*; FD-OCC
+// pair: h_ls0 +- h_ls1, each the mean of two).  The noise lies in the
+// complement of the kept eigenspace: per Rx, sigma^2 = sum_{q,d} (|z|^2 -
+// |u|^2) over sum_{q,d} (n_q - kept_q) dimensions; the DMRS-LS estimate,
+// which the channel's frequency selectivity inflates, is the fallback below
+// kMinNoiseDims dimensions.  P_q = mean |z|^2 - sigma^2 and the filter's
+// relative noise n_q = sigma^2 / P_q -> stats[4 slot + q];
+// noise_out[slot][rx] = fp16(sigma^2_rx) when non-null.
+__global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ noise_in,
+                                                     __half *__restrict__ noise_out, CeWs w, CeApply c)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ float nq[kMaxModels];
     const uint32_t slot = blockIdx.x;
     const uint32_t ncol = c.nr * c.ndmrs;
-    const double *e = energy + 2 * static_cast<size_t>(slot) * c.nl * ncol;
-    double dims = 0.0;
-    for (uint32_t l = 0; l < c.nl; ++l) dims += static_cast<double>(c.n[l] - c.kept[l]) * c.ndmrs;
-    double s2_sum = 0.0;
-    for (uint32_t rx = 0; rx < c.nr; ++rx) {
-        double s2;
-        if (dims >= kMinNoiseDims) {
-            double res = 0.0;
-            for (uint32_t l = 0; l < c.nl; ++l)
-                for (uint32_t d = 0; d < c.ndmrs; ++d) {
-                    const double *x = e + 2 * (static_cast<size_t>(l) * ncol + rx * c.ndmrs + d);
-                    res += fmax(x[0] - x[1], 0.0) / c.w_sum_sq[l];
-                }
-            s2 = res / dims;
-        } else {
-            s2 = __half2float(noise_in[static_cast<size_t>(slot) * c.nr + rx]);
+    if (threadIdx.x == 0) {
+        const double *e = w.energy + 2 * static_cast<size_t>(slot) * c.nm * ncol;
+        double dims = 0.0;
+        for (uint32_t q = 0; q < c.nm; ++q) dims += static_cast<double>(c.m[q].n - c.m[q].kept) * c.ndmrs;
+        double s2_sum = 0.0;
+        for (uint32_t rx = 0; rx < c.nr; ++rx) {
+            double s2;
+            if (dims >= kMinNoiseDims) {
+                double res = 0.0;
+                for (uint32_t q = 0; q < c.nm; ++q)
+                    for (uint32_t d = 0; d < c.ndmrs; ++d) {
+                        const double *x = e + 2 * (static_cast<size_t>(q) * ncol + rx * c.ndmrs + d);
+                        res += fmax(x[0] - x[1], 0.0);
+                    }
+                s2 = res / dims;
+            } else {
+                s2 = __half2float(noise_in[static_cast<size_t>(slot) * c.nr + rx]);
+            }
+            if (noise_out) noise_out[static_cast<size_t>(slot) * c.nr + rx] = __float2half_rn(static_cast<float>(s2));
+            s2_sum += s2;
         }
-        if (noise_out) noise_out[static_cast<size_t>(slot) * c.nr + rx] = __float2half_rn(static_cast<float>(s2));
-        s2_sum += s2;
+        const double s2 = s2_sum / c.nr;
+        for (uint32_t q = 0; q < c.nm; ++q) {
+            double eh = 0.0;
+            for (uint32_t col = 0; col < ncol; ++col) eh += e[2 * (static_cast<size_t>(q) * ncol + col)];
+            const double mean = eh / (static_cast<double>(ncol) * c.m[q].n);
+            const double p = fmax(mean - s2, 1e-3 * mean + 1e-30);
+            const float v = static_cast<float>(s2 / p);
+            w.stats[4 * slot + q] = v;
+            nq[q] = v;
+        }
     }
-    const double s2 = s2_sum / c.nr;
-    for (uint32_t l = 0; l < c.nl; ++l) {
-        double eh = 0.0;
-        for (uint32_t col = 0; col < ncol; ++col) eh += e[2 * (static_cast<size_t>(l) * ncol + col)];
-        const double mean = eh / (static_cast<double>(ncol) * c.n[l]);
-        const double nobs_abs = s2 * c.w_sum_sq[l];
-        const double p = fmax(mean - nobs_abs, 1e-3 * mean + 1e-30);
-        stats[4 * slot + l] = static_cast<float>(nobs_abs / p);
+    __syncthreads();
+    for (uint32_t q = 0; q < c.nm; ++q)
+        for (uint32_t k = threadIdx.x; k < c.m[q].rank; k += blockDim.x)
+            w.inv[(static_cast<size_t>(slot) * c.nm + q) * kMaxRank + k] = 1.0f / (c.m[q].lam[k] + nq[q]);
+}
+
+// hhat_var[slot, q][sc] = sum_k |A[k][sc]|^2 / (lam_k + n_q) (ref.py:109-111),
+// eight slots per thread.  Grid (used / 256, models, slots / 8).
+__global__ void __launch_bounds__(256) CeHvKernel(CeWs w, uint32_t slots, CeApply c)
+{
+    const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc >= c.used) return;
+    const uint32_t q = blockIdx.y, s0 = blockIdx.z * 8;
+    const CeModelDev &md = c.m[q];
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t k = 0; k < md.rank; ++k) {
+        const float2 a = md.at[static_cast<size_t>(k) * c.used + sc];
+        const float a2 = a.x * a.x + a.y * a.y;
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i)
+            if (s0 + i < slots) acc[i] += a2 * w.inv[(static_cast<size_t>(s0 + i) * c.nm + q) * kMaxRank + k];
     }
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i)
+        if (s0 + i < slots) w.hv[(static_cast<size_t>(s0 + i) * c.nm + q) * c.used + sc] = acc[i];
+}
+
+// hf[slot, q][col][sc] = sum_k A[k][sc] t[slot, q][k][col] / (lam_k + n_q):
+// per model q = blockIdx.z a GEMM with M = used, N = slots * NR * D, K = rank.
+__global__ void __launch_bounds__(airan::kGemmThreads) CeInterpGemmKernel(CeWs w, uint32_t slots, CeApply c)
+{
+    const uint32_t q = blockIdx.z;
+    const CeModelDev &md = c.m[q];
+    const uint32_t ncol = c.nr * c.ndmrs;
+    const uint32_t N = slots * ncol;
+    const uint32_t m0 = blockIdx.x * airan::kGemmTile, n0 = blockIdx.y * airan::kGemmTile;
+    auto la = [&](uint32_t sc, uint32_t k) {
+        return sc < c.used ? md.at[static_cast<size_t>(k) * c.used + sc] : make_float2(0.0f, 0.0f);
+    };
+    auto lb = [&](uint32_t k, uint32_t n) {
+        if (n >= N) return make_float2(0.0f, 0.0f);
+        const uint32_t slot = n / ncol, col = n - slot * ncol;
+        const size_t sq = static_cast<size_t>(slot) * c.nm + q;
+        const float2 v = w.t[(sq * kMaxRank + k) * ncol + col];
+        const float s = w.inv[sq * kMaxRank + k];
+        return make_float2(v.x * s, v.y * s);
+    };
+    auto st = [&](uint32_t sc, uint32_t n, float2 v) {
+        const uint32_t slot = n / ncol, col = n - slot * ncol;
+        w.hf[((static_cast<size_t>(slot) * c.nm + q) * ncol + col) * c.used + sc] = v;
+    };
+    airan::CGemmTile<false>(m0, n0, c.used, N, md.rank, la, lb, st);
 }
 
 // Solve the D x D symmetric system (R_tt + e I) w = r for one right-hand side.
@@ -298,24 +404,28 @@ __device__ inline void SolveTime(const CeApply &c, float e, const float *r, floa
     }
 }
 
-// Frequency interpolation + rescale + time interpolation.  One thread per
-// subcarrier of one (slot, layer) = blockIdx.y; loops over Rx in chunks.
-// Writes H[slot][rx][l][sym][sc] fp16 for sc < pad (zero beyond used).
-template <int RXC>
-__global__ void __launch_bounds__(256) CeInterpKernel(const float2 *__restrict__ at0, const float2 *__restrict__ at1,
-                                                      const float2 *__restrict__ at2, const float2 *__restrict__ at3,
-                                                      const float *__restrict__ lam0, const float *__restrict__ lam1,
-                                                      const float *__restrict__ lam2, const float *__restrict__ lam3,
-                                                      const float *__restrict__ stats, const float2 *__restrict__ t_in,
-                                                      uint32_t t_layer_stride, __half *__restrict__ out_r,
-                                                      __half *__restrict__ out_i, uint32_t out_layers, CeApply c)
+// Rescale of ref.py:109-116: scale = 2 / (hv + 1 - err), err = max(1 - hv, 0),
+// and the rescaled error variance used by the time interpolation.
+__device__ inline void Rescale(float hv, float *scale, float *err2)
+{
+    const float err = fmaxf(1.0f - hv, 0.0f);
+    const float den = hv + 1.0f - err;
+    const float s = den != 0.0f ? 2.0f / den : 0.0f;
+    *scale = s;
+    *err2 = fmaxf(s * (s - 1.0f) * hv + (1.0f - s) + s * err, 0.0f);
+}
+
+// Rescale (per model), pair combination h = (g_a + sign g_b) / 2 with
+// relative error (err_a + err_b) / 2, and time interpolation.  One thread
+// per subcarrier of one (slot, layer) = blockIdx.y.  Writes
+// H[slot][rx][l][sym][sc] fp16 for sc < pad (zero beyond used).
+__global__ void __launch_bounds__(256) CePostKernel(CeWs w, __half *__restrict__ out_r, __half *__restrict__ out_i,
+                                                    uint32_t out_layers, CeApply c)
 {
     const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t slot = blockIdx.y / c.nl, l = blockIdx.y - slot * c.nl;
     if (sc >= c.pad) return;
     const size_t plane = static_cast<size_t>(c.nsym) * c.pad;
-    stats += 4 * slot;
-    t_in += static_cast<size_t>(slot) * c.nl * t_layer_stride;
     out_r += static_cast<size_t>(slot) * c.nr * out_layers * plane;
     out_i += static_cast<size_t>(slot) * c.nr * out_layers * plane;
     if (sc >= c.used) {
@@ -327,61 +437,52 @@ __global__ void __launch_bounds__(256) CeInterpKernel(const float2 *__restrict__
             }
         return;
     }
-    const float2 *at = l == 0 ? at0 : (l == 1 ? at1 : (l == 2 ? at2 : at3));
-    const float *lam = l == 0 ? lam0 : (l == 1 ? lam1 : (l == 2 ? lam2 : lam3));
-    const uint32_t rank = c.rank[l];
-    const float nobs = stats[l];
-    // hhat_var and the rescale / error of ref.py:109-116 (hv accumulated in
-    // the first Rx chunk's pass over the eigenvectors).
-    float hv = 0.0f;
     const uint32_t ncol = c.nr * c.ndmrs;
-    const float2 *tl = t_in + static_cast<size_t>(l) * t_layer_stride;
-    for (uint32_t rx0 = 0; rx0 < c.nr; rx0 += RXC) {
-        float2 hf[RXC][4];
+    const int qa = c.layer_model[l][0], qb = c.layer_model[l][1];
+    const size_t sqa = static_cast<size_t>(slot) * c.nm + qa;
+    float sa, ea;
+    Rescale(w.hv[sqa * c.used + sc], &sa, &ea);
+    float wa = sa, wb = 0.0f, err2 = ea;
+    size_t sqb = sqa;
+    if (qb >= 0) {
+        sqb = static_cast<size_t>(slot) * c.nm + qb;
+        float sb, eb;
+        Rescale(w.hv[sqb * c.used + sc], &sb, &eb);
+        wa = 0.5f * sa;
+        wb = 0.5f * c.layer_sign[l] * sb;
+        err2 = 0.5f * (ea + eb);
+    }
+    float tw[14][4];
+    for (uint32_t s = 0; s < c.nsym; ++s) {
+        tw[s][0] = tw[s][1] = tw[s][2] = tw[s][3] = 0.0f;
+        SolveTime(c, err2, c.rt[s], tw[s]);
+    }
+    for (uint32_t rx = 0; rx < c.nr; ++rx) {
+        float2 hf[4];
 #pragma unroll
-        for (int i = 0; i < RXC; ++i)
-#pragma unroll
-            for (int d = 0; d < 4; ++d) hf[i][d] = make_float2(0.0f, 0.0f);
-        for (uint32_t k = 0; k < rank; ++k) {
-            const float2 a0 = at[static_cast<size_t>(k) * c.used + sc];
-            const float inv = 1.0f / (lam[k] + nobs);
-            if (rx0 == 0) hv += (a0.x * a0.x + a0.y * a0.y) * inv;
-            const float2 a = make_float2(a0.x * inv, a0.y * inv);
-            const float2 *trow = tl + static_cast<size_t>(k) * ncol;
-#pragma unroll
-            for (int i = 0; i < RXC; ++i) {
-                if (rx0 + i >= c.nr) break;
-#pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    if (d >= static_cast<int>(c.ndmrs)) break;
-                    const float2 t = trow[(rx0 + i) * c.ndmrs + d];
-                    hf[i][d].x += a.x * t.x - a.y * t.y;
-                    hf[i][d].y += a.x * t.y + a.y * t.x;
+        for (int d = 0; d < 4; ++d) {
+            hf[d] = make_float2(0.0f, 0.0f);
+            if (d < static_cast<int>(c.ndmrs)) {
+                const uint32_t col = rx * c.ndmrs + d;
+                const float2 ga = w.hf[(sqa * ncol + col) * c.used + sc];
+                hf[d] = make_float2(wa * ga.x, wa * ga.y);
+                if (qb >= 0) {
+                    const float2 gb = w.hf[(sqb * ncol + col) * c.used + sc];
+                    hf[d].x += wb * gb.x;
+                    hf[d].y += wb * gb.y;
                 }
             }
         }
-        const float err = fmaxf(1.0f - hv, 0.0f);
-        const float den = hv + 1.0f - err;
-        const float scale = den != 0.0f ? 2.0f / den : 0.0f;
-        const float err2 = fmaxf(scale * (scale - 1.0f) * hv + (1.0f - scale) + scale * err, 0.0f);
-        // Time weights (the same for every Rx of this subcarrier).
         for (uint32_t s = 0; s < c.nsym; ++s) {
-            float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-            SolveTime(c, err2, c.rt[s], w);
+            float vr = 0.0f, vi = 0.0f;
 #pragma unroll
-            for (int i = 0; i < RXC; ++i) {
-                const uint32_t rx = rx0 + i;
-                if (rx >= c.nr) break;
-                float vr = 0.0f, vi = 0.0f;
-#pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    vr += w[d] * hf[i][d].x;
-                    vi += w[d] * hf[i][d].y;
-                }
-                const size_t o = (static_cast<size_t>(rx) * out_layers + l) * plane + static_cast<size_t>(s) * c.pad + sc;
-                out_r[o] = __float2half_rn(vr * scale);
-                out_i[o] = __float2half_rn(vi * scale);
+            for (int d = 0; d < 4; ++d) {
+                vr += tw[s][d] * hf[d].x;
+                vi += tw[s][d] * hf[d].y;
             }
+            const size_t o = (static_cast<size_t>(rx) * out_layers + l) * plane + static_cast<size_t>(s) * c.pad + sc;
+            out_r[o] = __float2half_rn(vr);
+            out_i[o] = __float2half_rn(vi);
         }
     }
 }
@@ -393,23 +494,23 @@ void FreeDev(void *p)
     if (p != nullptr) (void)hipFree(p);
 }
 
-int32_t BuildLayer(const Geom &g, uint32_t l, const LmmseFilterParamsV1 &prm, hipStream_t stream,
-                   CeLayerFactors *out)
+// Eigen-factors of one observation model (FP64 on the device).
+int32_t BuildModel(const Geom &g, const LmmseFilterParamsV1 &prm, hipStream_t stream, CeModelFactors *out)
 {
     ObsGeom o{};
-    o.comb = g.comb[l];
-    o.occ = g.occ[l];
-    o.n = g.occ[l] ? g.n_occ : g.n_dmrs_re;
+    o.n = out->n;
+    o.step = out->step;
+    o.offset = out->offset;
     o.used = g.used;
     o.scs_tau = prm.scs_hz * prm.tau_s;
     const uint32_t n = o.n;
+    if (n == 0 || n > kMaxObs) return AIRAN_UNSUPPORTED_PROFILE;
     double2 *rpp = nullptr, *rhp = nullptr, *av = nullptr;
     double *eig = nullptr, *work = nullptr;
     rocblas_int *info = nullptr;
-    int32_t status = AIRAN_LAUNCH_FAILED;
     rocblas_handle handle = nullptr;
     std::vector<double> lam_host(n);
-    uint32_t kept = 0, rank = 0;
+    uint32_t kept = 0;
     auto cleanup = [&]() {
         if (handle) rocblas_destroy_handle(handle);
         FreeDev(rpp);
@@ -419,44 +520,37 @@ int32_t BuildLayer(const Geom &g, uint32_t l, const LmmseFilterParamsV1 &prm, hi
         FreeDev(work);
         FreeDev(info);
     };
+    auto fail = [&]() {
+        cleanup();
+        return AIRAN_LAUNCH_FAILED;
+    };
     if (hipMalloc(&rpp, sizeof(double2) * n * n) != hipSuccess ||
         hipMalloc(&rhp, sizeof(double2) * static_cast<size_t>(g.used) * n) != hipSuccess ||
         hipMalloc(&eig, sizeof(double) * n) != hipSuccess || hipMalloc(&work, sizeof(double) * n) != hipSuccess ||
-        hipMalloc(&info, sizeof(rocblas_int)) != hipSuccess) {
-        cleanup();
-        return AIRAN_LAUNCH_FAILED;
-    }
+        hipMalloc(&info, sizeof(rocblas_int)) != hipSuccess)
+        return fail();
     hipLaunchKernelGGL(BuildRppKernel, dim3(airan::CeilDiv(n, 256), n), dim3(256), 0, stream, rpp, o);
     hipLaunchKernelGGL(BuildRhpKernel, dim3(airan::CeilDiv(g.used, 256), n), dim3(256), 0, stream, rhp, o);
-    if (!CheckRocblas(rocblas_create_handle(&handle)) || !CheckRocblas(rocblas_set_stream(handle, stream))) {
-        cleanup();
-        return AIRAN_LAUNCH_FAILED;
-    }
+    if (!CheckRocblas(rocblas_create_handle(&handle)) || !CheckRocblas(rocblas_set_stream(handle, stream)))
+        return fail();
     if (!CheckRocblas(rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_lower,
                                        static_cast<rocblas_int>(n), reinterpret_cast<rocblas_double_complex *>(rpp),
-                                       static_cast<rocblas_int>(n), eig, work, info))) {
-        cleanup();
-        return AIRAN_LAUNCH_FAILED;
-    }
+                                       static_cast<rocblas_int>(n), eig, work, info)))
+        return fail();
     rocblas_int info_h = 0;
     if (hipMemcpyAsync(lam_host.data(), eig, sizeof(double) * n, hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipMemcpyAsync(&info_h, info, sizeof(info_h), hipMemcpyDeviceToHost, stream) != hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess || info_h != 0) {
-        cleanup();
-        return AIRAN_LAUNCH_FAILED;
-    }
+        hipStreamSynchronize(stream) != hipSuccess || info_h != 0)
+        return fail();
     const double lmax = std::max(lam_host[n - 1], 1e-300);
     for (uint32_t k = 0; k < n && kept < static_cast<uint32_t>(std::min<int>(kMaxRank, prm.max_rank)); ++k) {
         if (lam_host[n - 1 - k] < prm.eig_floor * lmax) break;
         ++kept;
     }
     kept = std::max<uint32_t>(kept, 1);
-    rank = airan::AlignUp(kept, 16);
+    const uint32_t rank = airan::AlignUp(kept, 16);
     // av = R_hp (used x n) * V[:, n-kept .. n-1] (n x kept), column-major.
-    if (hipMalloc(&av, sizeof(double2) * static_cast<size_t>(g.used) * kept) != hipSuccess) {
-        cleanup();
-        return AIRAN_LAUNCH_FAILED;
-    }
+    if (hipMalloc(&av, sizeof(double2) * static_cast<size_t>(g.used) * kept) != hipSuccess) return fail();
     const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
     if (!CheckRocblas(rocblas_zgemm(handle, rocblas_operation_none, rocblas_operation_none,
                                     static_cast<rocblas_int>(g.used), static_cast<rocblas_int>(kept),
@@ -466,36 +560,72 @@ int32_t BuildLayer(const Geom &g, uint32_t l, const LmmseFilterParamsV1 &prm, hi
                                     reinterpret_cast<const rocblas_double_complex *>(rpp) + static_cast<size_t>(n - kept) * n,
                                     static_cast<rocblas_int>(n), &zero,
                                     reinterpret_cast<rocblas_double_complex *>(av),
-                                    static_cast<rocblas_int>(g.used)))) {
-        cleanup();
-        return AIRAN_LAUNCH_FAILED;
-    }
-    out->n = n;
+                                    static_cast<rocblas_int>(g.used))))
+        return fail();
     out->rank = rank;
     out->kept = kept;
-    out->w_sum_sq = g.occ[l] ? 0.5f : 1.0f;
     if (hipMalloc(&out->bt, sizeof(float2) * static_cast<size_t>(n) * rank) != hipSuccess ||
         hipMalloc(&out->at, sizeof(float2) * static_cast<size_t>(rank) * g.used) != hipSuccess ||
-        hipMalloc(&out->lam, sizeof(float) * rank) != hipSuccess) {
-        cleanup();
-        return AIRAN_LAUNCH_FAILED;
-    }
+        hipMalloc(&out->lam, sizeof(float) * rank) != hipSuccess)
+        return fail();
     hipLaunchKernelGGL(PackBtKernel, dim3(airan::CeilDiv(rank, 256), n), dim3(256), 0, stream, rpp, out->bt, n,
                        rank, kept);
     hipLaunchKernelGGL(PackAtKernel, dim3(airan::CeilDiv(g.used, 256), rank), dim3(256), 0, stream, av, out->at,
                        g.used, rank, kept);
-    std::vector<float> lam_f(rank, 0.0f);
+    std::vector<float> lam_f(rank, 1.0f);  // padded columns: zero factors, unit eigenvalue
     for (uint32_t k = 0; k < kept; ++k) lam_f[k] = static_cast<float>(std::max(lam_host[n - 1 - k], 0.0));
-    // Padded ranks: zero factors with a unit eigenvalue contribute nothing.
-    for (uint32_t k = kept; k < rank; ++k) lam_f[k] = 1.0f;
     if (hipMemcpyAsync(out->lam, lam_f.data(), sizeof(float) * rank, hipMemcpyHostToDevice, stream) != hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess) {
-        cleanup();
-        return AIRAN_LAUNCH_FAILED;
-    }
-    status = AIRAN_OK;
+        hipStreamSynchronize(stream) != hipSuccess)
+        return fail();
     cleanup();
-    return status;
+    return AIRAN_OK;
+}
+
+// Observation models of a grant, in layer order: a layer alone on its comb
+// gets one comb model (step 2); the first layer of an FD-OCC pair creates the
+// pair's two models g+ (offset comb) and g- (offset comb + 2), step 4.  The
+// pair's layer with FD-OCC signs (+,+) (port 1000 or 1002) is h_ls "a".
+void PlanModels(const Geom &g, const PuschMimoConfig &cfg, LmmseFilter *f)
+{
+    f->nm = 0;
+    bool done[4] = {};
+    for (uint32_t l = 0; l < g.nl; ++l) {
+        if (done[l]) continue;
+        if (!g.occ[l]) {
+            CeModelFactors &m = f->model[f->nm];
+            m.n = g.n_dmrs_re;
+            m.step = 2;
+            m.offset = g.comb[l];
+            m.in_a = m.in_b = l;
+            m.sign_b = 0.0f;
+            f->layer_model[l][0] = static_cast<int32_t>(f->nm);
+            f->layer_model[l][1] = -1;
+            f->layer_sign[l] = 1.0f;
+            ++f->nm;
+            done[l] = true;
+            continue;
+        }
+        uint32_t partner = l;
+        for (uint32_t q = l + 1; q < g.nl; ++q)
+            if (g.comb[q] == g.comb[l]) partner = q;
+        const bool l_plus = cfg.dmrs_ports[l] == 1000 || cfg.dmrs_ports[l] == 1002;
+        const uint32_t lp = l_plus ? l : partner, lm = l_plus ? partner : l;
+        for (int sgn = 0; sgn < 2; ++sgn) {
+            CeModelFactors &m = f->model[f->nm + sgn];
+            m.n = g.n_occ;
+            m.step = 4;
+            m.offset = g.comb[l] + 2u * sgn;
+            m.in_a = lp;
+            m.in_b = lm;
+            m.sign_b = sgn == 0 ? 1.0f : -1.0f;
+        }
+        f->layer_model[lp][0] = f->layer_model[lm][0] = static_cast<int32_t>(f->nm);
+        f->layer_model[lp][1] = f->layer_model[lm][1] = static_cast<int32_t>(f->nm + 1);
+        f->layer_sign[lp] = 1.0f;
+        f->layer_sign[lm] = -1.0f;
+        f->nm += 2;
+        done[lp] = done[lm] = true;
+    }
 }
 
 CeApply MakeApply(const LmmseFilter &f)
@@ -504,19 +634,18 @@ CeApply MakeApply(const LmmseFilter &f)
     const Geom &g = f.g;
     c.nr = g.nr;
     c.nl = g.nl;
+    c.nm = f.nm;
     c.ndmrs = g.ndmrs;
     c.pilot_pad = g.pilot_pad;
     c.used = g.used;
     c.pad = g.pad;
     c.nsym = g.nsym;
-    for (uint32_t d = 0; d < g.ndmrs; ++d) c.dmrs_sym[d] = g.dmrs_sym[d];
-    for (uint32_t l = 0; l < g.nl; ++l) {
-        const CeLayerFactors &lf = f.layer[f.shared_with[l]];
-        c.n[l] = lf.n;
-        c.rank[l] = lf.rank;
-        c.kept[l] = lf.kept;
-        c.w_sum_sq[l] = lf.w_sum_sq;
+    for (uint32_t q = 0; q < f.nm; ++q) {
+        const CeModelFactors &m = f.model[q];
+        c.m[q] = CeModelDev{m.bt, m.at, m.lam, m.n, m.rank, m.kept, m.in_a, m.in_b, m.sign_b};
     }
+    std::memcpy(c.layer_model, f.layer_model, sizeof(c.layer_model));
+    std::memcpy(c.layer_sign, f.layer_sign, sizeof(c.layer_sign));
     std::memcpy(c.rt, f.rt, sizeof(c.rt));
     std::memcpy(c.rtt, f.rtt, sizeof(c.rtt));
     return c;
@@ -538,22 +667,7 @@ uint64_t PilotHash(const uint16_t *sc, uint32_t count)
 
 namespace airan {
 
-// Workspace: stats [slots][4] fp32, column energies [slots][L][NR*D][2]
-// fp64, then T[slots][L] of kMaxRank x NR x D complex fp32 (each part
-// 256-byte aligned).
-namespace {
-size_t StatsBytes(uint32_t slots) { return airan::AlignUp(16u * slots, 256u); }
-size_t EnergyBytes(const Geom &g, uint32_t slots)
-{
-    return (static_cast<size_t>(slots) * g.nl * g.nr * g.ndmrs * 16 + 255) / 256 * 256;
-}
-}  // namespace
-
-size_t CeWorkspaceBytes(const Geom &g, uint32_t slots)
-{
-    return StatsBytes(slots) + EnergyBytes(g, slots) +
-           static_cast<size_t>(slots) * g.nl * kMaxRank * g.nr * g.ndmrs * sizeof(float2);
-}
+size_t CeWorkspaceBytes(const Geom &g, uint32_t slots) { return CeWsLayout(g, slots, nullptr, nullptr); }
 
 bool CeFilterMatches(const LmmseFilter *f, const Geom &g)
 {
@@ -571,19 +685,22 @@ int32_t LaunchCeApply(const LmmseFilter *f, const __half *hr, const __half *hi, 
     if (slots == 0 || slots > 65535 || out_layers < g.nl) return AIRAN_INVALID_ARGUMENT;
     if (ws_bytes < CeWorkspaceBytes(g, slots)) return AIRAN_RESOURCE_TOO_SMALL;
     const CeApply c = MakeApply(*f);
-    auto *base = static_cast<uint8_t *>(ws);
-    auto *stats = reinterpret_cast<float *>(base);
-    auto *energy = reinterpret_cast<double *>(base + StatsBytes(slots));
-    auto *tbuf = reinterpret_cast<float2 *>(base + StatsBytes(slots) + EnergyBytes(g, slots));
-    const uint32_t t_stride = kMaxRank * g.nr * g.ndmrs;
-    const CeLayerFactors *L[4];
-    for (int l = 0; l < 4; ++l) L[l] = &f->layer[l < static_cast<int>(g.nl) ? f->shared_with[l] : 0];
-    hipLaunchKernelGGL(CeProjectKernel, dim3(g.nr * g.ndmrs, slots * g.nl), dim3(kProjThreads), 0, s, hr, hi,
-                       L[0]->bt, L[1]->bt, L[2]->bt, L[3]->bt, tbuf, energy, t_stride, c);
-    hipLaunchKernelGGL(CeStatsKernel, dim3(slots), dim3(64), 0, s, energy, noise_in, noise_out, stats, c);
-    hipLaunchKernelGGL(CeInterpKernel<4>, dim3(CeilDiv(g.pad, 256), slots * g.nl), dim3(256), 0, s, L[0]->at,
-                       L[1]->at, L[2]->at, L[3]->at, L[0]->lam, L[1]->lam, L[2]->lam, L[3]->lam, stats, tbuf,
-                       t_stride, out_re, out_im, out_layers, c);
+    CeWs w{};
+    CeWsLayout(g, slots, static_cast<uint8_t *>(ws), &w);
+    const uint32_t ncol = g.nr * g.ndmrs;
+    uint32_t max_rank = 0;
+    for (uint32_t q = 0; q < f->nm; ++q) max_rank = std::max(max_rank, f->model[q].rank);
+    const uint32_t n_tiles = CeilDiv(static_cast<uint64_t>(slots) * ncol, kGemmTile);
+    hipLaunchKernelGGL(CeProjectKernel, dim3(CeilDiv(max_rank, kGemmTile), n_tiles, f->nm), dim3(kGemmThreads), 0,
+                       s, hr, hi, w, slots, c);
+    hipLaunchKernelGGL(CeEnergyKernel, dim3(slots * f->nm * ncol), dim3(256), 0, s, hr, hi, w, c);
+    hipLaunchKernelGGL(CeStatsKernel, dim3(slots), dim3(256), 0, s, noise_in, noise_out, w, c);
+    hipLaunchKernelGGL(CeHvKernel, dim3(CeilDiv(g.used, 256), f->nm, CeilDiv(slots, 8)), dim3(256), 0, s, w, slots,
+                       c);
+    hipLaunchKernelGGL(CeInterpGemmKernel, dim3(CeilDiv(g.used, kGemmTile), n_tiles, f->nm), dim3(kGemmThreads), 0,
+                       s, w, slots, c);
+    hipLaunchKernelGGL(CePostKernel, dim3(CeilDiv(g.pad, 256), slots * g.nl), dim3(256), 0, s, w, out_re, out_im,
+                       out_layers, c);
     return LaunchStatus();
 }
 
@@ -687,6 +804,7 @@ int32_t airan_channel_est_lmmse_CreateFilter(const PuschMimoConfig *config, cons
     Geom g;
     int32_t st = airan::DeriveGeom(*config, &g);
     if (st != AIRAN_OK) return st;
+    if (g.ndmrs > 4) return AIRAN_UNSUPPORTED_PROFILE;
     auto *f = new LmmseFilter{};
     if (hipGetDevice(&f->device) != hipSuccess) {
         delete f;
@@ -694,16 +812,10 @@ int32_t airan_channel_est_lmmse_CreateFilter(const PuschMimoConfig *config, cons
     }
     f->g = g;
     f->params = *params;
+    PlanModels(g, *config, f);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    for (uint32_t l = 0; l < g.nl; ++l) {
-        f->shared_with[l] = static_cast<int>(l);
-        for (uint32_t q = 0; q < l; ++q)
-            if (g.comb[q] == g.comb[l] && g.occ[q] == g.occ[l]) {
-                f->shared_with[l] = static_cast<int>(q);
-                break;
-            }
-        if (f->shared_with[l] != static_cast<int>(l)) continue;
-        st = BuildLayer(g, l, *params, s, &f->layer[l]);
+    for (uint32_t q = 0; q < f->nm; ++q) {
+        st = BuildModel(g, *params, s, &f->model[q]);
         if (st != AIRAN_OK) {
             airan_channel_est_lmmse_DestroyFilter(f);
             return st;
@@ -727,10 +839,10 @@ int32_t airan_channel_est_lmmse_CreateFilter(const PuschMimoConfig *config, cons
 void airan_channel_est_lmmse_DestroyFilter(LmmseFilter *f)
 {
     if (f == nullptr) return;
-    for (int l = 0; l < 4; ++l) {
-        FreeDev(f->layer[l].bt);
-        FreeDev(f->layer[l].at);
-        FreeDev(f->layer[l].lam);
+    for (uint32_t q = 0; q < kMaxModels; ++q) {
+        FreeDev(f->model[q].bt);
+        FreeDev(f->model[q].at);
+        FreeDev(f->model[q].lam);
     }
     delete f;
 }
@@ -738,7 +850,7 @@ void airan_channel_est_lmmse_DestroyFilter(LmmseFilter *f)
 uint32_t airan_channel_est_lmmse_FilterRank(const LmmseFilter *f, uint32_t layer)
 {
     if (f == nullptr || layer >= f->g.nl) return 0;
-    return f->layer[f->shared_with[layer]].kept;
+    return f->model[f->layer_model[layer][0]].kept;
 }
 
 size_t airan_channel_est_lmmse_WorkspaceBytes(const PuschMimoConfig *config)
@@ -763,9 +875,9 @@ int32_t airan_channel_est_lmmse_Enqueue(const ChannelEstLmmseOpArgsV1 *a, const 
         return AIRAN_INVALID_ARGUMENT;
     return airan::LaunchCeApply(f, static_cast<const __half *>(a->h_ls_re), static_cast<const __half *>(a->h_ls_im),
                                 static_cast<const __half *>(noise_var_rx), static_cast<__half *>(noise_out),
-                                workspace, workspace_bytes,
-                                static_cast<__half *>(a->h_grid_re), static_cast<__half *>(a->h_grid_im),
-                                airan::kDetectLayers, 1, static_cast<hipStream_t>(a->stream));
+                                workspace, workspace_bytes, static_cast<__half *>(a->h_grid_re),
+                                static_cast<__half *>(a->h_grid_im), airan::kDetectLayers, 1,
+                                static_cast<hipStream_t>(a->stream));
 }
 
 }  // extern "C"

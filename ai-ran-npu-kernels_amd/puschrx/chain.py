@@ -26,7 +26,7 @@ class ChainOptions:
     tau_s: float = 300e-9
     fd_hz: float = 100.0
     scs_hz: float = 30e3
-    ce_rank: int = 96
+    ce_rank: int = 384
     iq_format: int = abi.IQ_INT16_Q8_8
 
 

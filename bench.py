@@ -10,8 +10,10 @@ correlation 0.5) with complex64 IQ; a step is one pass of the chain over a
 batch of slots (IQ already in HBM -> decoded bits and CRC outcomes in HBM),
 replayed from a HIP graph.
 
-The operating SNR is the lowest point of a 20..32 dB sweep (step 2) at which
-the chain's BLER is <= 10 %, measured first on a smaller batch.  Reported
+The operating SNR is the lowest point of a 20..46 dB sweep (step 2) at which
+the chain's BLER is <= 10 %, measured first on a smaller batch (4 Rx carrying
+4 layers leaves the LMMSE detector no receive diversity, so plain LMMSE needs
+an SNR above the 20..32 dB range quoted for the workload).  Reported
 beside the throughput: BLER and goodput at that SNR, the per-stage times
 (HIP events on the chain's stream), the roofline of the dominant kernel and
 the CPU chain (the C oracle, OpenMP) on a bounded sample of the same slots.
@@ -99,7 +101,7 @@ def main() -> int:
     plan = ops.tb_plan(tbs, g.cw_symbols * g.qm, g.qm, g.nl)
     spec = synth.ChannelSpec(taps=16, delay_spread_s=300e-9, doppler_hz=100.0, rx_corr=0.5)
     opts = ChainOptions(ldpc_iterations=8, early_stop=True, tau_s=spec.delay_spread_s, fd_hz=spec.doppler_hz,
-                        ce_rank=96, iq_format=abi.IQ_COMPLEX64)
+                        ce_rank=384, iq_format=abi.IQ_COMPLEX64)
     rng = np.random.default_rng(args.seed)
     S = args.slots
 
@@ -129,7 +131,7 @@ def main() -> int:
             n = args.sweep_slots
             payload, grid, H, _ = make_slots(n, 30.0)
             snr_op = None
-            for snr in range(20, 34, 2):
+            for snr in range(20, 48, 2):
                 _, _, _, iq = make_slots(n, float(snr), payload, grid, H)
                 ok, _ = run(iq, n)
                 bler = 1.0 - float(ok.mean())
@@ -139,7 +141,7 @@ def main() -> int:
                     snr_op = float(snr)
                     break
             if snr_op is None:
-                snr_op = 32.0
+                snr_op = 46.0
         else:
             snr_op = args.snr
 

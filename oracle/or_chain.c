@@ -87,24 +87,7 @@ int or_chain_slot(const PuschMimoConfig *cfg, const OrTbPlan *p, const OrCeParam
         hls[2 * i + 1] = or_h2f(h_im[i]);
     }
     double noise_rel[4];
-    for (int l = 0; l < L; ++l) {
-        const int comb = (cfg->dmrs_ports[l] - 1000) / 2;
-        int share = 0;
-        for (int q = 0; q < L; ++q) share += (cfg->dmrs_ports[q] - 1000) / 2 == comb;
-        const int n = share == 2 ? g.n_occ : g.n_dmrs_re;
-        const double wss = share == 2 ? 0.5 : 1.0;
-        double e = 0.0;
-        for (int r = 0; r < NR; ++r)
-            for (int d = 0; d < g.ndmrs; ++d)
-                for (int q = 0; q < n; ++q) {
-                    const double *v = hls + 2 * ((((size_t)r * L + l) * g.ndmrs + d) * g.pilot_pad + q);
-                    e += v[0] * v[0] + v[1] * v[1];
-                }
-        const double mean = e / ((double)NR * g.ndmrs * n);
-        double pw = mean - s2 * wss;
-        if (pw < 1e-3 * mean + 1e-30) pw = 1e-3 * mean + 1e-30;
-        noise_rel[l] = s2 / pw;
-    }
+    or_ce_noise_rel(cfg, hls, s2, noise_rel);
     if (or_ce_lmmse(cfg, prm, noise_rel, hls, H) != 0) goto done;
     /* Detection (FP64) and the fp16 demapper inputs. */
     for (int r = 0; r < NR; ++r)

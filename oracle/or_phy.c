@@ -135,126 +135,202 @@ static void solve_real(double *m, double *b, int n)
     }
 }
 
-/* Full-rank FP64 restatement of the reference LMMSE estimator
- * (channel_est/channel_est_lmmse_mimo/scripts/channel_est_lmmse_ref.py:
- * 50-117 with the observation model of observation_model_ref.py:41-79):
- *   frequency:  H_f = R_hp (R_pp + n_obs I)^-1 h_obs, n_obs = noise*sum(w^2)
- *   rescale:    scale = 2 raa / (hhat_var + raa - error)   (ref.py:109-116)
+/* Observation models of a grant, in layer order: a layer alone on its comb
+ * is observed at comb + 2p (step 2); the first layer of an FD-OCC pair
+ * creates the pair's models g+ = h0 + h1 at comb + 4p and g- = h0 - h1 at
+ * comb + 2 + 4p (step 4), observed exactly as h_ls0 +- h_ls1 (h_ls0 the
+ * layer with FD-OCC signs +,+: port 1000 or 1002).  Returns the count. */
+typedef struct {
+    int n, step, offset, a, b;
+    double sb;
+} CeModel;
+
+static int ce_plan(const PuschMimoConfig *cfg, const OrGeom *g, CeModel *m, int lm[4][2], double ls[4])
+{
+    int comb[4], occ[4], done[4] = {0, 0, 0, 0}, nm = 0;
+    for (int l = 0; l < g->nl; ++l) comb[l] = (cfg->dmrs_ports[l] - 1000) / 2;
+    for (int l = 0; l < g->nl; ++l) {
+        int share = 0;
+        for (int p = 0; p < g->nl; ++p) share += comb[p] == comb[l];
+        occ[l] = share == 2;
+    }
+    for (int l = 0; l < g->nl; ++l) {
+        if (done[l]) continue;
+        if (!occ[l]) {
+            m[nm] = (CeModel){g->n_dmrs_re, 2, comb[l], l, l, 0.0};
+            lm[l][0] = nm;
+            lm[l][1] = -1;
+            ls[l] = 1.0;
+            ++nm;
+            done[l] = 1;
+            continue;
+        }
+        int partner = l;
+        for (int q = l + 1; q < g->nl; ++q)
+            if (comb[q] == comb[l]) partner = q;
+        const int plus = cfg->dmrs_ports[l] == 1000 || cfg->dmrs_ports[l] == 1002;
+        const int lp = plus ? l : partner, lmn = plus ? partner : l;
+        m[nm] = (CeModel){g->n_occ, 4, comb[l], lp, lmn, 1.0};
+        m[nm + 1] = (CeModel){g->n_occ, 4, comb[l] + 2, lp, lmn, -1.0};
+        lm[lp][0] = lm[lmn][0] = nm;
+        lm[lp][1] = lm[lmn][1] = nm + 1;
+        ls[lp] = 1.0;
+        ls[lmn] = -1.0;
+        nm += 2;
+        done[lp] = done[lmn] = 1;
+    }
+    return nm;
+}
+
+/* Relative noise of each observation model: sigma2 / (mean |z_q|^2 - sigma2),
+ * the power floored at 1e-3 of the mean; returns the model count. */
+int or_ce_noise_rel(const PuschMimoConfig *cfg, const double *h_ls, double sigma2, double *noise_rel)
+{
+    OrGeom g;
+    or_geom(cfg, &g);
+    CeModel mod[4];
+    int lm[4][2];
+    double ls[4];
+    const int nm = ce_plan(cfg, &g, mod, lm, ls);
+    for (int q = 0; q < nm; ++q) {
+        double e = 0.0;
+        for (int rx = 0; rx < g.nr; ++rx)
+            for (int d = 0; d < g.ndmrs; ++d) {
+                const double *sa = h_ls + 2 * ((((size_t)rx * g.nl + mod[q].a) * g.ndmrs + d) * g.pilot_pad);
+                const double *sb = h_ls + 2 * ((((size_t)rx * g.nl + mod[q].b) * g.ndmrs + d) * g.pilot_pad);
+                for (int p = 0; p < mod[q].n; ++p) {
+                    const double re = sa[2 * p] + mod[q].sb * sb[2 * p], im = sa[2 * p + 1] + mod[q].sb * sb[2 * p + 1];
+                    e += re * re + im * im;
+                }
+            }
+        const double mean = e / ((double)g.nr * g.ndmrs * mod[q].n);
+        double pw = mean - sigma2;
+        if (pw < 1e-3 * mean + 1e-30) pw = 1e-3 * mean + 1e-30;
+        noise_rel[q] = sigma2 / pw;
+    }
+    return nm;
+}
+
+/* Full-rank FP64 LMMSE channel estimation (channel_est/channel_est_lmmse_mimo/
+ * scripts/channel_est_lmmse_ref.py:50-117) over the observation models above:
+ *   frequency:  G_q = R_hp (R_pp + n_q I)^-1 z_q, R_f(dk) = 1/(1 + j2pi dk scs tau)
+ *   rescale:    scale = 2 / (hhat_var + 1 - err), err = max(1 - hhat_var, 0)
+ *               (ref.py:109-116), per model
+ *   layer:      H_f = G_a (comb model) or (G_a +- G_b) / 2 (FD-OCC pair),
+ *               relative error (err_a + err_b) / 2
  *   time:       per-subcarrier Wiener weights over the DMRS symbols with
  *               R_t = J0(2 pi f_d dt) and the rescaled error (ref.py:75-88).
- * The reference truncates R_pp to its top RANK eigenpairs; with RANK equal to
- * the observation count the two are identical. */
+ * noise_rel[q] is model q's relative noise sigma^2 / P_q.  The reference
+ * truncates R_pp to its top RANK eigenpairs; with RANK equal to the
+ * observation count the two are identical. */
 int or_ce_lmmse(const PuschMimoConfig *cfg, const OrCeParams *prm, const double *noise_rel,
                 const double *h_ls, double *h_out)
 {
     OrGeom g;
     or_geom(cfg, &g);
-    const int U = g.used, D = g.ndmrs;
+    const int U = g.used, D = g.ndmrs, NR = g.nr;
     const double tsym = (1.0 + 9.0 / 128.0) / prm->scs_hz;
     double rt[14][14];
     for (int i = 0; i < g.nsym; ++i)
         for (int j = 0; j < g.nsym; ++j) rt[i][j] = j0(2.0 * M_PI * prm->fd_hz * (i - j) * tsym);
-    int comb[4], occ[4];
-    for (int l = 0; l < g.nl; ++l) comb[l] = (cfg->dmrs_ports[l] - 1000) / 2;
-    for (int l = 0; l < g.nl; ++l) {
-        int share = 0;
-        for (int p = 0; p < g.nl; ++p) share += comb[p] == comb[l];
-        occ[l] = share == 2;
-    }
-    for (int l = 0; l < g.nl; ++l) {
-        const int n = occ[l] ? g.n_occ : g.n_dmrs_re;
-        const int W = occ[l] ? 2 : 1;
-        int *sup = (int *)malloc(sizeof(int) * (size_t)n * W);
-        for (int p = 0; p < n; ++p) {
-            if (occ[l]) {
-                sup[2 * p] = comb[l] + 4 * p;
-                sup[2 * p + 1] = comb[l] + 4 * p + 2;
-            } else {
-                sup[p] = comb[l] + 2 * p;
-            }
-        }
-        const double w = occ[l] ? 0.5 : 1.0;
-        const double nobs = noise_rel[l] * (double)W * w * w;
+    CeModel mod[4];
+    int lm[4][2];
+    double lsgn[4];
+    const int nm = ce_plan(cfg, &g, mod, lm, lsgn);
+    cd *G[4] = {NULL, NULL, NULL, NULL};     /* [NR][D][U] scaled frequency estimates */
+    double *err[4] = {NULL, NULL, NULL, NULL};
+    int rc = 0;
+    for (int q = 0; q < nm && rc == 0; ++q) {
+        const CeModel *m = &mod[q];
+        const int n = m->n;
         cd *M = (cd *)malloc(sizeof(cd) * (size_t)n * n);
-        for (int p = 0; p < n; ++p)
-            for (int q = 0; q < n; ++q) {
-                cd s = 0;
-                for (int a = 0; a < W; ++a)
-                    for (int b = 0; b < W; ++b) s += w * w * rf(sup[p * W + a] - sup[q * W + b], prm);
-                M[(size_t)p * n + q] = s + (p == q ? nobs : 0.0);
-            }
         cd *Rhp = (cd *)malloc(sizeof(cd) * (size_t)U * n);
-        for (int sc = 0; sc < U; ++sc)
-            for (int p = 0; p < n; ++p) {
-                cd s = 0;
-                for (int a = 0; a < W; ++a) s += w * rf(sc - sup[p * W + a], prm);
-                Rhp[(size_t)sc * n + p] = s;
-            }
-        if (cholesky(M, n) != 0) return -1;
         double *scale = (double *)malloc(sizeof(double) * U);
-        double *err = (double *)malloc(sizeof(double) * U);
+        G[q] = (cd *)malloc(sizeof(cd) * (size_t)NR * D * U);
+        err[q] = (double *)malloc(sizeof(double) * U);
+        for (int p = 0; p < n; ++p)
+            for (int r = 0; r < n; ++r)
+                M[(size_t)p * n + r] = rf((double)(p - r) * m->step, prm) + (p == r ? noise_rel[q] : 0.0);
+        for (int sc = 0; sc < U; ++sc)
+            for (int p = 0; p < n; ++p) Rhp[(size_t)sc * n + p] = rf(sc - (m->offset + m->step * p), prm);
+        if (cholesky(M, n) != 0) {
+            rc = -1;
+        } else {
 #pragma omp parallel
-        {
-            cd *y = (cd *)malloc(sizeof(cd) * (size_t)n);
+            {
+                cd *y = (cd *)malloc(sizeof(cd) * (size_t)n);
 #pragma omp for schedule(static)
-            for (int sc = 0; sc < U; ++sc) {
-                for (int p = 0; p < n; ++p) y[p] = conj(Rhp[(size_t)sc * n + p]);
-                forward_sub(M, n, y);
-                double hv = 0.0;
-                for (int p = 0; p < n; ++p) hv += creal(y[p] * conj(y[p]));
-                const double raa = 1.0;
-                double e = raa - hv > 0.0 ? raa - hv : 0.0;
-                double den = hv + raa - e;
-                double s = den != 0.0 ? 2.0 * raa / den : 0.0;
-                double e2 = s * (s - 1.0) * hv + (1.0 - s) * raa + s * e;
-                scale[sc] = s;
-                err[sc] = e2 > 0.0 ? e2 : 0.0;
-            }
-            free(y);
-        }
-        cd *hf = (cd *)malloc(sizeof(cd) * (size_t)D * U);
-        cd *z = (cd *)malloc(sizeof(cd) * (size_t)n);
-        for (int rx = 0; rx < g.nr; ++rx) {
-            for (int d = 0; d < D; ++d) {
-                const double *src = h_ls + 2 * ((((size_t)rx * g.nl + l) * D + d) * g.pilot_pad);
-                for (int p = 0; p < n; ++p) z[p] = src[2 * p] + I * src[2 * p + 1];
-                forward_sub(M, n, z);
-                backward_sub_h(M, n, z);
-#pragma omp parallel for schedule(static)
                 for (int sc = 0; sc < U; ++sc) {
-                    cd s = 0;
-                    const cd *row = Rhp + (size_t)sc * n;
-                    for (int p = 0; p < n; ++p) s += row[p] * z[p];
-                    hf[(size_t)d * U + sc] = s * scale[sc];
+                    for (int p = 0; p < n; ++p) y[p] = conj(Rhp[(size_t)sc * n + p]);
+                    forward_sub(M, n, y);
+                    double hv = 0.0;
+                    for (int p = 0; p < n; ++p) hv += creal(y[p] * conj(y[p]));
+                    const double e = 1.0 - hv > 0.0 ? 1.0 - hv : 0.0;
+                    const double den = hv + 1.0 - e;
+                    const double s = den != 0.0 ? 2.0 / den : 0.0;
+                    const double e2 = s * (s - 1.0) * hv + (1.0 - s) + s * e;
+                    scale[sc] = s;
+                    err[q][sc] = e2 > 0.0 ? e2 : 0.0;
                 }
+                free(y);
             }
+            cd *z = (cd *)malloc(sizeof(cd) * (size_t)n);
+            for (int rx = 0; rx < NR; ++rx)
+                for (int d = 0; d < D; ++d) {
+                    const double *sa = h_ls + 2 * ((((size_t)rx * g.nl + m->a) * D + d) * g.pilot_pad);
+                    const double *sb = h_ls + 2 * ((((size_t)rx * g.nl + m->b) * D + d) * g.pilot_pad);
+                    for (int p = 0; p < n; ++p)
+                        z[p] = (sa[2 * p] + m->sb * sb[2 * p]) + I * (sa[2 * p + 1] + m->sb * sb[2 * p + 1]);
+                    forward_sub(M, n, z);
+                    backward_sub_h(M, n, z);
+#pragma omp parallel for schedule(static)
+                    for (int sc = 0; sc < U; ++sc) {
+                        cd s = 0;
+                        const cd *row = Rhp + (size_t)sc * n;
+                        for (int p = 0; p < n; ++p) s += row[p] * z[p];
+                        G[q][((size_t)rx * D + d) * U + sc] = s * scale[sc];
+                    }
+                }
+            free(z);
+        }
+        free(M);
+        free(Rhp);
+        free(scale);
+    }
+    for (int l = 0; l < g.nl && rc == 0; ++l) {
+        const int qa = lm[l][0], qb = lm[l][1];
+        for (int rx = 0; rx < NR; ++rx) {
 #pragma omp parallel for schedule(static)
             for (int sc = 0; sc < U; ++sc) {
+                cd hf[4];
+                double e;
+                for (int d = 0; d < D; ++d) {
+                    const size_t i = ((size_t)rx * D + d) * U + sc;
+                    hf[d] = qb < 0 ? G[qa][i] : 0.5 * (G[qa][i] + lsgn[l] * G[qb][i]);
+                }
+                e = qb < 0 ? err[qa][sc] : 0.5 * (err[qa][sc] + err[qb][sc]);
                 for (int s = 0; s < g.nsym; ++s) {
-                    double m[16], b[4];
+                    double mm[16], b[4];
                     for (int i = 0; i < D; ++i) {
                         for (int j = 0; j < D; ++j)
-                            m[i * D + j] = rt[g.dmrs_sym[i]][g.dmrs_sym[j]] + (i == j ? err[sc] : 0.0);
+                            mm[i * D + j] = rt[g.dmrs_sym[i]][g.dmrs_sym[j]] + (i == j ? e : 0.0);
                         b[i] = rt[g.dmrs_sym[i]][s];
                     }
-                    solve_real(m, b, D); /* symmetric: weights = M^-1 r */
+                    solve_real(mm, b, D); /* symmetric: weights = M^-1 r */
                     cd v = 0;
-                    for (int d = 0; d < D; ++d) v += b[d] * hf[(size_t)d * U + sc];
+                    for (int d = 0; d < D; ++d) v += b[d] * hf[d];
                     double *o = h_out + 2 * ((((size_t)rx * g.nl + l) * g.nsym + s) * U + sc);
                     o[0] = creal(v);
                     o[1] = cimag(v);
                 }
             }
         }
-        free(z);
-        free(hf);
-        free(scale);
-        free(err);
-        free(Rhp);
-        free(M);
-        free(sup);
     }
-    return 0;
+    for (int q = 0; q < 4; ++q) {
+        free(G[q]);
+        free(err[q]);
+    }
+    return rc;
 }
 
 /* ------------------------------------------------------------- detection */

@@ -94,10 +94,14 @@ typedef struct OrCeParams {
     double scs_hz;      /* subcarrier spacing */
 } OrCeParams;
 /* h_ls natural [NR,L,D,pilot_pad] (double complex, interleaved) -> H
- * [NR,L,14,used] complex double.  noise_rel[l] is the per-layer relative
- * noise sigma^2 / P_h used by the filter. */
+ * [NR,L,14,used] complex double.  Observation models in layer order: a
+ * layer alone on its comb is one model (step 2); an FD-OCC pair is two
+ * models g+ = h_ls0 + h_ls1 (offset comb) and g- = h_ls0 - h_ls1 (offset
+ * comb + 2), step 4.  noise_rel[q] is model q's relative noise
+ * sigma^2 / P_q (or_ce_noise_rel computes it from h_ls and sigma^2). */
 int or_ce_lmmse(const PuschMimoConfig *cfg, const OrCeParams *prm, const double *noise_rel,
                 const double *h_ls, double *h_out);
+int or_ce_noise_rel(const PuschMimoConfig *cfg, const double *h_ls, double sigma2, double *noise_rel);
 
 /* ------------------------------------------------------------- detection */
 /* H [NR,L,14,used], y [NR,14,used] -> xhat [L,14,used] complex, no_eff

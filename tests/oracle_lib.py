@@ -45,6 +45,7 @@ def load_oracle():
     lib.or_dmrs_gen.argtypes = [vp, vp, vp]
     lib.or_dmrs_ls.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.or_ce_lmmse.argtypes = [vp, vp, vp, vp, vp]
+    lib.or_ce_noise_rel.argtypes = [vp, vp, C.c_double, vp]
     lib.or_detect_lmmse.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.or_qam_axis_llr.argtypes = [C.c_int, C.c_float, C.c_float, vp]
     lib.or_qam_axis_llr_bruteforce.argtypes = [C.c_int, C.c_float, C.c_float, vp]

@@ -124,6 +124,25 @@ def _tdl_freq_response(rng, n_sc, taps, tau_rms, scs):
     return (a[None, :] * np.exp(-2j * np.pi * sc[:, None] * scs * delays[None, :])).sum(axis=1)
 
 
+def _models(cfg, g):
+    """Observation models in the library's order: (n, step, offset, a, b, sign_b)."""
+    comb = [(cfg.dmrs_ports[l] - 1000) // 2 for l in range(g.nl)]
+    out, done = [], set()
+    for l in range(g.nl):
+        if l in done:
+            continue
+        mates = [q for q in range(g.nl) if comb[q] == comb[l]]
+        if len(mates) == 1:
+            out.append((g.n_dmrs_re, 2, comb[l], l, l, 0.0))
+        else:
+            lp = [q for q in mates if cfg.dmrs_ports[q] in (1000, 1002)][0]
+            lm = [q for q in mates if q != lp][0]
+            out.append((g.n_occ, 4, comb[l], lp, lm, 1.0))
+            out.append((g.n_occ, 4, comb[l] + 2, lp, lm, -1.0))
+        done.update(mates)
+    return out
+
+
 @pytest.mark.parametrize("name", list(CE_PROFILES))
 @pytest.mark.parametrize("max_rank", [512, 96])
 def test_ce_lmmse_matches_fp64(oracle, gpu_stream, name, max_rank):
@@ -134,56 +153,62 @@ def test_ce_lmmse_matches_fp64(oracle, gpu_stream, name, max_rank):
     sigma2 = 0.01
     comb = [(cfg.dmrs_ports[l] - 1000) // 2 for l in range(g.nl)]
     occ = [sum(c == comb[l] for c in comb) == 2 for l in range(g.nl)]
+    # Physical DMRS observations: z_m = sum over the comb's layers of
+    # w_l(m) h_l(k_m) + noise on k_m = comb + 2m, then the DMRS-LS outputs
+    # (pair means for FD-OCC layers, z itself otherwise).
     h_ls = np.zeros((g.nr, g.nl, g.ndmrs, g.pilot_pad), np.complex128)
     for rx in range(g.nr):
-        for l in range(g.nl):
-            h = _tdl_freq_response(rng, g.used, 8, tau, scs)
-            for d in range(g.ndmrs):
-                if occ[l]:
-                    n = g.n_occ
-                    p = np.arange(n)
-                    obs = 0.5 * (h[comb[l] + 4 * p] + h[comb[l] + 4 * p + 2])
-                    noise = np.sqrt(sigma2 / 4) * (rng.normal(size=n) + 1j * rng.normal(size=n))
-                else:
-                    n = g.n_dmrs_re
-                    obs = h[comb[l] + 2 * np.arange(n)]
-                    noise = np.sqrt(sigma2 / 2) * (rng.normal(size=n) + 1j * rng.normal(size=n))
-                h_ls[rx, l, d, :n] = obs + noise
+        h = [_tdl_freq_response(rng, g.used, 8, tau, scs) for _ in range(g.nl)]
+        for d in range(g.ndmrs):
+            for c in sorted(set(comb)):
+                ls = [l for l in range(g.nl) if comb[l] == c]
+                k = c + 2 * np.arange(g.n_dmrs_re)
+                sgn = (-1.0) ** np.arange(g.n_dmrs_re)
+                z = np.sqrt(sigma2 / 2) * (rng.normal(size=k.size) + 1j * rng.normal(size=k.size))
+                for l in ls:
+                    odd = cfg.dmrs_ports[l] in (1001, 1003)
+                    z = z + (sgn if odd else 1.0) * h[l][k]
+                for l in ls:
+                    if occ[l]:
+                        s = -1.0 if cfg.dmrs_ports[l] in (1001, 1003) else 1.0
+                        h_ls[rx, l, d, : g.n_occ] = 0.5 * (z[0::2] + s * z[1::2])
+                    else:
+                        h_ls[rx, l, d, : g.n_dmrs_re] = z
     h16r = h_ls.real.astype(np.float16)
     h16i = h_ls.imag.astype(np.float16)
     noise16 = np.full(g.nr, sigma2, np.float16)
     # eig_floor keeps the numerically null eigenvectors out of the kept set,
     # so the kept subspace (and the residual noise estimate) is well defined.
     filt = ops.LmmseFilter(cfg, tau_s=tau, fd_hz=0.0, scs_hz=scs, max_rank=max_rank, eig_floor=1e-12)
-    kept = [filt.rank(l) for l in range(g.nl)]
-    nobs = [g.n_occ if occ[l] else g.n_dmrs_re for l in range(g.nl)]
+    models = _models(cfg, g)
+    kept_layer = [filt.rank(l) for l in range(g.nl)]
     for l in range(g.nl):
-        assert 1 <= kept[l] <= min(max_rank, nobs[l])
+        n = g.n_occ if occ[l] else g.n_dmrs_re
+        assert 1 <= kept_layer[l] <= min(max_rank, n)
     sc_d = torch.zeros(g.nl * g.ndmrs * g.pilot_pad, dtype=torch.int16, device="cuda")
     cnt_d = torch.zeros(16, dtype=torch.int16, device="cuda")
     hr, hi, ws, noise_est = ops.ce_lmmse(cfg, filt, dev(h16r.ravel()), dev(h16i.ravel()), sc_d, cnt_d,
                                          dev(noise16))
     hq = h16r.astype(np.float64) + 1j * h16i.astype(np.float64)
-    wss = [0.5 if occ[l] else 1.0 for l in range(g.nl)]
-    # Noise: energy outside the top-`kept` eigenspace of R_pp (numpy eigh of
-    # the observation model restated here), else the DMRS-LS fallback.
-    dims = sum((nobs[l] - kept[l]) * g.ndmrs for l in range(g.nl))
+    zq = []
+    kept = []
+    for (n, step, off, a, b, sb) in models:
+        zq.append(hq[:, a, :, :n] + sb * hq[:, b, :, :n])
+        kept.append(kept_layer[a])  # a model's pair partner has the same covariance
+    # Noise: energy outside the top-`kept` eigenspace of each model's R_pp
+    # (numpy eigh of the observation covariance restated here), else the
+    # DMRS-LS fallback.
+    dims = sum((m[0] - kq) * g.ndmrs for m, kq in zip(models, kept))
     if dims >= 32:
         res = np.zeros(g.nr)
-        for l in range(g.nl):
-            n = nobs[l]
-            if occ[l]:
-                sup = np.stack([comb[l] + 4 * np.arange(n), comb[l] + 4 * np.arange(n) + 2], 1)
-            else:
-                sup = (comb[l] + 2 * np.arange(n))[:, None]
-            dk = sup[:, None, :, None] - sup[None, :, None, :]
-            rpp = (wss[l] ** 2 * (1.0 / (1.0 + 2j * np.pi * dk * scs * tau))).sum(axis=(2, 3))
+        for (n, step, off, a, b, sb), kq, z in zip(models, kept, zq):
+            dk = step * (np.arange(n)[:, None] - np.arange(n)[None, :])
+            rpp = 1.0 / (1.0 + 2j * np.pi * dk * scs * tau)
             w, v = np.linalg.eigh(rpp)
-            u = v[:, ::-1][:, : kept[l]]
+            u = v[:, ::-1][:, :kq]
             for rx in range(g.nr):
                 for d in range(g.ndmrs):
-                    h = hq[rx, l, d, :n]
-                    res[rx] += (np.sum(np.abs(h) ** 2) - np.sum(np.abs(u.conj().T @ h) ** 2)) / wss[l]
+                    res[rx] += np.sum(np.abs(z[rx, d]) ** 2) - np.sum(np.abs(u.conj().T @ z[rx, d]) ** 2)
         s2_rx = res / dims
         assert np.allclose(host(noise_est).astype(np.float64), s2_rx, rtol=0.05)
         # The estimate recovers the injected noise (not the DMRS-LS input).
@@ -192,14 +217,14 @@ def test_ce_lmmse_matches_fp64(oracle, gpu_stream, name, max_rank):
         s2_rx = noise16.astype(np.float64)
         assert np.array_equal(host(noise_est).view(np.uint16), noise16.view(np.uint16))
     s2 = float(np.mean(s2_rx))
-    # Filter statistics: n_obs,l = s2 sum(w^2) / P_l.
-    stats = host(ws[:16].view(torch.float32))[:g.nl]
-    noise_rel = np.zeros(g.nl)
-    for l in range(g.nl):
-        mean = np.mean(np.abs(hq[:, l, :, : nobs[l]]) ** 2)
-        p_l = max(mean - s2 * wss[l], 1e-3 * mean)
-        assert stats[l] == pytest.approx(s2 * wss[l] / p_l, rel=0.05 if dims >= 32 else 1e-4)
-        noise_rel[l] = stats[l] / wss[l]
+    # Filter statistics: n_q = s2 / P_q, P_q = mean |z_q|^2 - s2.
+    stats = host(ws[:16].view(torch.float32))[: len(models)]
+    noise_rel = np.zeros(4)
+    for q, z in enumerate(zq):
+        mean = np.mean(np.abs(z) ** 2)
+        p_q = max(mean - s2, 1e-3 * mean)
+        assert stats[q] == pytest.approx(s2 / p_q, rel=0.05 if dims >= 32 else 1e-4)
+        noise_rel[q] = stats[q]
     prm = OrCeParams(tau, 0.0, scs)
     inter = np.zeros(g.nr * g.nl * g.ndmrs * g.pilot_pad * 2)
     inter[0::2] = hq.real.ravel()
@@ -211,7 +236,7 @@ def test_ce_lmmse_matches_fp64(oracle, gpu_stream, name, max_rank):
     assert np.all(got[:, : g.nl, :, g.used:] == 0)
     # Full rank: the same estimator as the oracle.  Truncated: the oracle is
     # still full rank, so the bound also covers the truncation error.
-    full = all(kept[l] == nobs[l] for l in range(g.nl))
+    full = all(kq == m[0] for m, kq in zip(models, kept))
     assert nmse_db(got[:, : g.nl, :, : g.used], want) <= (-50.0 if full else -40.0)
     filt.close()
 
