@@ -47,7 +47,9 @@ __device__ inline int16_t SatNeg16(int v) { return static_cast<int16_t>(v == -32
 //   llr    = QamSymbolLlrs(x, ne)                        (qam256_demod)
 //   cw[b][re L + l] = +-llr[stream(b)] by c((re L + l) Qm + b)
 //                                                        (layer_demap, descramble)
-template <int NL>
+// QM = 8 fixes the modulation at compile time (the bit -> stream mapping then
+// indexes registers); QM = 0 takes it from the arguments.
+template <int NL, int QM>
 __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restrict__ rx_re,
                                                          const __half *__restrict__ rx_im,
                                                          const __half *__restrict__ h_re,
@@ -79,13 +81,16 @@ __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restric
     Cf xh[4];
     float ne[4];
     airan::Detect<NL>(static_cast<int>(nr), no, loadh, loady, xh, ne);
-    const uint32_t qm = a.qm, m = qm / 2;
+    const uint32_t qm = QM > 0 ? static_cast<uint32_t>(QM) : a.qm, m = qm / 2;
     const uint32_t re = ds * a.used + sc;
     const uint64_t bit0 = static_cast<uint64_t>(re) * NL * qm;
     const uint32_t w0 = static_cast<uint32_t>(bit0 >> 5);
     const uint64_t gw = (static_cast<uint64_t>(gold[w0]) | (static_cast<uint64_t>(gold[w0 + 1]) << 32)) >> (bit0 & 31u);
     const __half floor16 = __ushort_as_half(a.floor16);
     int16_t *out = cw + static_cast<size_t>(slot) * qm * a.cw_stride + static_cast<size_t>(re) * NL;
+    // LLRs of all layers first, then one contiguous NL-element store per bit
+    // plane (the thread's REs are adjacent within a plane).
+    int16_t v[8][NL];
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
         const __half xr16 = __float2half_rn(xh[l].r), xi16 = __float2half_rn(xh[l].i);
@@ -93,11 +98,26 @@ __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restric
         if (__half2float(ne16) < __half2float(floor16)) ne16 = floor16;
         int llr[8];
         airan::QamSymbolLlrs(qm, xr16, xi16, ne16, llr);
-        for (uint32_t b = 0; b < qm; ++b) {
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
             const uint32_t stream = (b & 1u) ? m + b / 2 : b / 2;
-            const int v = llr[stream];
+            const int x = b < qm ? llr[stream] : 0;
             const bool flip = (gw >> (static_cast<uint32_t>(l) * qm + b)) & 1u;
-            out[static_cast<size_t>(b) * a.cw_stride + l] = flip ? SatNeg16(v) : static_cast<int16_t>(v);
+            v[b][l] = flip ? SatNeg16(x) : static_cast<int16_t>(x);
+        }
+    }
+    for (uint32_t b = 0; b < qm; ++b) {
+        int16_t *o = out + static_cast<size_t>(b) * a.cw_stride;
+        if constexpr (NL == 4) {
+            *reinterpret_cast<uint2 *>(o) =
+                make_uint2(static_cast<uint16_t>(v[b][0]) | static_cast<uint32_t>(static_cast<uint16_t>(v[b][1])) << 16,
+                           static_cast<uint16_t>(v[b][2]) | static_cast<uint32_t>(static_cast<uint16_t>(v[b][3])) << 16);
+        } else if constexpr (NL == 2) {
+            *reinterpret_cast<uint32_t *>(o) =
+                static_cast<uint16_t>(v[b][0]) | static_cast<uint32_t>(static_cast<uint16_t>(v[b][1])) << 16;
+        } else {
+#pragma unroll
+            for (int l = 0; l < NL; ++l) o[l] = v[b][l];
         }
     }
 }
@@ -141,6 +161,7 @@ struct AiranPuschChain {
     AiranPuschChainOptionsV1 opts{};
     Geom g{};
     uint32_t active_rows = 0;
+    bool fused_ldpc = false;  // decoder reads the codeword LLRs (no lam buffer)
     uint32_t last_slots = 0;
     size_t bytes = 0;
     LmmseFilter *filter = nullptr;
@@ -225,15 +246,34 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
     for (uint32_t i = 0; i < g.ndata; ++i) da.data_sym[i] = g.data_sym[i];
     da.floor16 = HostF2H(airan_pusch_NoEffFloor(g.qm));
     const dim3 grid(airan::CeilDiv(g.used, 256), g.ndata, slots);
+    using DdKernel = void (*)(const __half *, const __half *, const __half *, const __half *, const __half *,
+                              const uint32_t *, int16_t *, DemapArgs);
+    const bool q8 = g.qm == 8;
+    DdKernel dd;
     switch (g.nl) {
-    case 1: hipLaunchKernelGGL(DetectDemapKernel<1>, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw, da); break;
-    case 2: hipLaunchKernelGGL(DetectDemapKernel<2>, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw, da); break;
-    case 3: hipLaunchKernelGGL(DetectDemapKernel<3>, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw, da); break;
-    default: hipLaunchKernelGGL(DetectDemapKernel<4>, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw, da); break;
+    case 1: dd = q8 ? DetectDemapKernel<1, 8> : DetectDemapKernel<1, 0>; break;
+    case 2: dd = q8 ? DetectDemapKernel<2, 8> : DetectDemapKernel<2, 0>; break;
+    case 3: dd = q8 ? DetectDemapKernel<3, 8> : DetectDemapKernel<3, 0>; break;
+    default: dd = q8 ? DetectDemapKernel<4, 8> : DetectDemapKernel<4, 0>; break;
     }
+    hipLaunchKernelGGL(dd, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw,
+                       da);
     if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
     mark();
     const uint32_t ncb = slots * c->plan.C;
+    if (c->fused_ldpc) {
+        // Rate dematch runs inside the decoder's load (the stage is empty).
+        mark();
+        st = airan::LaunchLdpcFromCodeword(g, c->plan, c->desc, c->cw, ncb, c->active_rows, c->opts.ldpc_iterations,
+                                           c->opts.early_stop != 0, c->bits, c->iters, s);
+        if (st != AIRAN_OK) return st;
+        mark();
+        st = airan_crc_check_Enqueue(&c->plan, slots, c->bits, c->cb_ok, c->tb_ok, nullptr, c->crc_ws,
+                                     c->crc_ws_bytes, s);
+        if (st != AIRAN_OK) return st;
+        mark();
+        return AIRAN_OK;
+    }
     st = airan::LaunchRateDematch(g, c->plan, c->desc, c->cw, c->lam, ncb, s);
     if (st != AIRAN_OK) return st;
     mark();
@@ -298,6 +338,7 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     c->opts = *options;
     c->g = g;
     c->active_rows = airan_ldpc_ActiveRows(plan);
+    c->fused_ldpc = options->ldpc_mode == AIRAN_LDPC_MODE_FP16 && c->active_rows <= 8 && p.Z % 24 == 0;
     auto fail = [&](int32_t code) {
         airan_pusch_chain_Destroy(c);
         return code;
@@ -317,7 +358,8 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     const size_t ncb = S * p.C;
     const size_t gold_words = (static_cast<size_t>(g.cw_symbols) * g.qm + 31) / 32 + 2;
     c->ce_ws_bytes = airan::CeWorkspaceBytes(g, static_cast<uint32_t>(S));
-    c->ldpc_ws_bytes = airan_ldpc_decode_WorkspaceBytes(static_cast<uint32_t>(ncb), p.Z, c->active_rows);
+    c->ldpc_ws_bytes =
+        c->fused_ldpc ? 0 : airan_ldpc_decode_WorkspaceBytes(static_cast<uint32_t>(ncb), p.Z, c->active_rows);
     c->crc_ws_bytes = airan_crc_check_WorkspaceBytes(static_cast<uint32_t>(S));
     const size_t iq_elem = options->iq_format == AIRAN_IQ_COMPLEX64 ? 8 : 4;
     size_t &b = c->bytes;
@@ -330,7 +372,7 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         !Alloc(&c->ce_ws, c->ce_ws_bytes, &b) || !Alloc(&c->h_re, S * g.nr * g.nl * plane, &b) ||
         !Alloc(&c->h_im, S * g.nr * g.nl * plane, &b) || !Alloc(&c->gold, gold_words, &b) ||
         !Alloc(&c->cw, S * g.qm * g.cw_stride, &b) || !Alloc(&c->desc, ncb, &b) ||
-        !Alloc(&c->lam, ncb * 68 * p.Z, &b) || !Alloc(&c->ldpc_ws, c->ldpc_ws_bytes, &b) ||
+        !Alloc(&c->lam, c->fused_ldpc ? 0 : ncb * 68 * p.Z, &b) || !Alloc(&c->ldpc_ws, c->ldpc_ws_bytes, &b) ||
         !Alloc(&c->bits, ncb * p.K, &b) || !Alloc(&c->iters, ncb, &b) || !Alloc(&c->cb_ok, ncb, &b) ||
         !Alloc(&c->tb_ok, S, &b) || !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
         !Alloc(reinterpret_cast<uint8_t **>(&c->zero_iq), g.nr * g.samples * iq_elem, &b))
@@ -445,7 +487,7 @@ int32_t airan_pusch_chain_Outputs(const AiranPuschChain *c, AiranPuschChainOutpu
     o->h_re = c->h_re;
     o->h_im = c->h_im;
     o->cw_llr = c->cw;
-    o->ldpc_llr = c->lam;
+    o->ldpc_llr = c->fused_ldpc ? nullptr : c->lam;
     return AIRAN_OK;
 }
 

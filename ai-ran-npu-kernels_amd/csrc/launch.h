@@ -30,4 +30,12 @@ bool CeFilterMatches(const LmmseFilter *f, const Geom &g);
 int32_t LaunchRateDematch(const Geom &g, const AiranTbPlanV1 &plan, const RateMatchDescriptor *desc,
                           const int16_t *in, int16_t *out, uint32_t num_cb, hipStream_t s);
 
+// LDPC decode (FP16 messages) of num_cb code blocks read straight from the
+// bit-plane codeword LLRs through the rate-match descriptors: the rate
+// dematch and decoder stages in one launch.  AIRAN_UNSUPPORTED_PROFILE when
+// the active rows exceed the fused decoder's limit (8) or Z is not 24 * 2^j.
+int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &plan, const RateMatchDescriptor *desc,
+                               const int16_t *cw, uint32_t num_cb, uint32_t rows, uint32_t iters, bool early_stop,
+                               int8_t *bits, uint8_t *iters_out, hipStream_t s);
+
 }  // namespace airan

@@ -43,6 +43,10 @@ constexpr int kMaxRank = 512;
 constexpr uint32_t kMaxModels = 4;
 constexpr uint32_t kMaxObs = 2048;         // observations per model (used / 2 at 273 PRB: 1638)
 constexpr double kMinNoiseDims = 32.0;     // residual dimensions for the subspace noise estimate
+// The projection's observation axis is cut into this many fixed ranges
+// (partial products summed in a fixed order), independent of the batch, so
+// results do not depend on how many slots share a launch.
+constexpr uint32_t kProjSplits = 4;
 
 // One observation model: n observations at offset + step * p.
 struct CeModelFactors {
@@ -164,6 +168,8 @@ struct CeApply {
 //   hv     [slots][models][used]            fp32  hhat variance
 //   t      [slots][models][kMaxRank][NR*D]  c64   U^H z
 //   hf     [slots][models][NR*D][used]      c64   frequency estimate (unscaled)
+//   tp     [kProjSplits][slots][models][kMaxRank][NR*D]  c64  partial U^H z
+//          over one of kProjSplits fixed ranges of the observations
 struct CeWs {
     float *stats;
     double *energy;
@@ -171,6 +177,7 @@ struct CeWs {
     float *hv;
     float2 *t;
     float2 *hf;
+    float2 *tp;
 };
 
 size_t Align256(size_t b) { return (b + 255) / 256 * 256; }
@@ -179,12 +186,12 @@ size_t CeWsLayout(const Geom &g, uint32_t slots, uint8_t *base, CeWs *w)
 {
     const size_t sm = static_cast<size_t>(slots) * g.nl;  // models <= layers
     const size_t ncol = static_cast<size_t>(g.nr) * g.ndmrs;
-    const size_t sizes[6] = {Align256(16u * slots), Align256(sm * ncol * 16), Align256(sm * kMaxRank * 4),
+    const size_t sizes[7] = {Align256(16u * slots), Align256(sm * ncol * 16), Align256(sm * kMaxRank * 4),
                              Align256(sm * g.used * 4), Align256(sm * kMaxRank * ncol * 8),
-                             Align256(sm * ncol * g.used * 8)};
+                             Align256(sm * ncol * g.used * 8), Align256(kProjSplits * sm * kMaxRank * ncol * 8)};
     size_t off = 0;
-    void *ptrs[6];
-    for (int i = 0; i < 6; ++i) {
+    void *ptrs[7];
+    for (int i = 0; i < 7; ++i) {
         ptrs[i] = base ? base + off : nullptr;
         off += sizes[i];
     }
@@ -195,6 +202,7 @@ size_t CeWsLayout(const Geom &g, uint32_t slots, uint8_t *base, CeWs *w)
         w->hv = static_cast<float *>(ptrs[3]);
         w->t = static_cast<float2 *>(ptrs[4]);
         w->hf = static_cast<float2 *>(ptrs[5]);
+        w->tp = static_cast<float2 *>(ptrs[6]);
     }
     return off;
 }
@@ -214,31 +222,65 @@ __device__ inline float2 LoadZ(const __half *hr, const __half *hi, const CeApply
     return v;
 }
 
-// t[slot, q][k][col] = sum_p conj(U[p][k]) z[slot, col][p]: per model q =
-// blockIdx.z a GEMM with M = rank (k), N = slots * NR * D, K = n.
+// Observations of split `part` of a model: [p0, p0 + len), len a multiple of
+// the GEMM K-step except for the last split.
+__device__ inline void ProjRange(uint32_t n, uint32_t part, uint32_t *p0, uint32_t *len)
+{
+    const uint32_t chunk = ((n + kProjSplits - 1) / kProjSplits + airan::kGemmK - 1) / airan::kGemmK * airan::kGemmK;
+    const uint32_t a = min(n, part * chunk), b = min(n, a + chunk);
+    *p0 = a;
+    *len = b - a;
+}
+
+// tp[part][slot, q][k][col] = sum_{p in range(part)} conj(U[p][k]) z[slot, col][p]:
+// per (model q, split part) = blockIdx.z a GEMM with M = rank (k),
+// N = slots * NR * D, K = the split's observations.
 __global__ void __launch_bounds__(airan::kGemmThreads) CeProjectKernel(const __half *__restrict__ hr,
                                                                        const __half *__restrict__ hi, CeWs w,
                                                                        uint32_t slots, CeApply c)
 {
-    const uint32_t q = blockIdx.z;
+    const uint32_t q = blockIdx.z / kProjSplits, part = blockIdx.z - q * kProjSplits;
     const CeModelDev &md = c.m[q];
     const uint32_t ncol = c.nr * c.ndmrs;
     const uint32_t m0 = blockIdx.x * airan::kGemmTile, n0 = blockIdx.y * airan::kGemmTile;
     if (m0 >= md.rank) return;
     const uint32_t N = slots * ncol;
+    uint32_t p0, len;
+    ProjRange(md.n, part, &p0, &len);
     auto la = [&](uint32_t k, uint32_t p) {
-        return (k < md.rank && p < md.n) ? md.bt[static_cast<size_t>(p) * md.rank + k] : make_float2(0.0f, 0.0f);
+        return (k < md.rank && p < len) ? md.bt[static_cast<size_t>(p0 + p) * md.rank + k] : make_float2(0.0f, 0.0f);
     };
     auto lb = [&](uint32_t p, uint32_t n) {
-        if (n >= N) return make_float2(0.0f, 0.0f);
+        if (n >= N || p >= len) return make_float2(0.0f, 0.0f);
         const uint32_t slot = n / ncol, col = n - slot * ncol;
-        return LoadZ(hr, hi, c, md, slot, col, p);
+        return LoadZ(hr, hi, c, md, slot, col, p0 + p);
     };
+    float2 *tp = w.tp + static_cast<size_t>(part) * slots * c.nm * kMaxRank * ncol;
     auto st = [&](uint32_t k, uint32_t n, float2 v) {
         const uint32_t slot = n / ncol, col = n - slot * ncol;
-        w.t[((static_cast<size_t>(slot) * c.nm + q) * kMaxRank + k) * ncol + col] = v;
+        tp[((static_cast<size_t>(slot) * c.nm + q) * kMaxRank + k) * ncol + col] = v;
     };
-    airan::CGemmTile<true>(m0, n0, md.rank, N, md.n, la, lb, st);
+    airan::CGemmTile<true>(m0, n0, md.rank, N, len, la, lb, st);
+}
+
+// t = ((tp[0] + tp[1]) + tp[2]) + tp[3] ... in split order.  Grid over
+// slots * models * kMaxRank * NR * D elements (entries k >= rank skipped).
+__global__ void __launch_bounds__(256) CeProjReduceKernel(CeWs w, uint32_t slots, CeApply c)
+{
+    const uint32_t ncol = c.nr * c.ndmrs;
+    const size_t total = static_cast<size_t>(slots) * c.nm * kMaxRank * ncol;
+    const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const uint32_t k = static_cast<uint32_t>((e / ncol) % kMaxRank);
+    const uint32_t q = static_cast<uint32_t>((e / (static_cast<size_t>(ncol) * kMaxRank)) % c.nm);
+    if (k >= c.m[q].rank) return;
+    float2 acc = w.tp[e];
+    for (uint32_t part = 1; part < kProjSplits; ++part) {
+        const float2 v = w.tp[static_cast<size_t>(part) * total + e];
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    w.t[e] = acc;
 }
 
 // Column energies: |z|^2 over the observations and |u|^2 over the kept
@@ -333,20 +375,26 @@ __global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ 
 }
 
 // hhat_var[slot, q][sc] = sum_k |A[k][sc]|^2 / (lam_k + n_q) (ref.py:109-111),
-// eight slots per thread.  Grid (used / 256, models, slots / 8).
+// eight slots per thread, their 1 / (lam_k + n_q) staged in LDS.
+// Grid (used / 256, models, slots / 8).
 __global__ void __launch_bounds__(256) CeHvKernel(CeWs w, uint32_t slots, CeApply c)
 {
-    const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sc >= c.used) return;
+    __shared__ float sinv[8][kMaxRank];
     const uint32_t q = blockIdx.y, s0 = blockIdx.z * 8;
     const CeModelDev &md = c.m[q];
+    for (uint32_t e = threadIdx.x; e < 8u * md.rank; e += blockDim.x) {
+        const uint32_t i = e / md.rank, k = e - i * md.rank;
+        sinv[i][k] = s0 + i < slots ? w.inv[(static_cast<size_t>(s0 + i) * c.nm + q) * kMaxRank + k] : 0.0f;
+    }
+    __syncthreads();
+    const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc >= c.used) return;
     float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t k = 0; k < md.rank; ++k) {
         const float2 a = md.at[static_cast<size_t>(k) * c.used + sc];
         const float a2 = a.x * a.x + a.y * a.y;
 #pragma unroll
-        for (uint32_t i = 0; i < 8; ++i)
-            if (s0 + i < slots) acc[i] += a2 * w.inv[(static_cast<size_t>(s0 + i) * c.nm + q) * kMaxRank + k];
+        for (uint32_t i = 0; i < 8; ++i) acc[i] += a2 * sinv[i][k];
     }
 #pragma unroll
     for (uint32_t i = 0; i < 8; ++i)
@@ -380,29 +428,51 @@ __global__ void __launch_bounds__(airan::kGemmThreads) CeInterpGemmKernel(CeWs w
     airan::CGemmTile<false>(m0, n0, c.used, N, md.rank, la, lb, st);
 }
 
-// Solve the D x D symmetric system (R_tt + e I) w = r for one right-hand side.
-__device__ inline void SolveTime(const CeApply &c, float e, const float *r, float *w)
-{
-    float m[4][4], b[4];
-    const int D = static_cast<int>(c.ndmrs);
-    for (int i = 0; i < D; ++i) {
-        for (int j = 0; j < D; ++j) m[i][j] = c.rtt[i][j] + (i == j ? e : 0.0f);
-        b[i] = r[i];
-    }
-    for (int col = 0; col < D; ++col) {
-        const float inv = 1.0f / m[col][col];
-        for (int row = col + 1; row < D; ++row) {
-            const float f = m[row][col] * inv;
-            for (int k = col; k < D; ++k) m[row][k] -= f * m[col][k];
-            b[row] -= f * b[col];
+// (R_tt + e I) for D DMRS symbols, eliminated once (no pivoting; the matrix
+// is symmetric positive definite) and applied to each symbol's right-hand
+// side r = R_t[s][.]: the same operations, in the same order, as eliminating
+// the augmented system per symbol.
+template <int D>
+struct TimeSolver {
+    float m[D][D];
+    float f[D][D];  // f[row][col]: elimination multipliers
+
+    __device__ inline void Factor(const CeApply &c, float e)
+    {
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) m[i][j] = c.rtt[i][j] + (i == j ? e : 0.0f);
+#pragma unroll
+        for (int col = 0; col < D; ++col) {
+            const float inv = 1.0f / m[col][col];
+#pragma unroll
+            for (int row = col + 1; row < D; ++row) {
+                f[row][col] = m[row][col] * inv;
+#pragma unroll
+                for (int k = col; k < D; ++k) m[row][k] -= f[row][col] * m[col][k];
+            }
         }
     }
-    for (int row = D - 1; row >= 0; --row) {
-        float s = b[row];
-        for (int k = row + 1; k < D; ++k) s -= m[row][k] * w[k];
-        w[row] = s / m[row][row];
+
+    __device__ inline void Solve(const float *r, float *w) const
+    {
+        float b[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) b[i] = r[i];
+#pragma unroll
+        for (int col = 0; col < D; ++col)
+#pragma unroll
+            for (int row = col + 1; row < D; ++row) b[row] -= f[row][col] * b[col];
+#pragma unroll
+        for (int row = D - 1; row >= 0; --row) {
+            float s = b[row];
+#pragma unroll
+            for (int k = row + 1; k < D; ++k) s -= m[row][k] * w[k];
+            w[row] = s / m[row][row];
+        }
     }
-}
+};
 
 // Rescale of ref.py:109-116: scale = 2 / (hv + 1 - err), err = max(1 - hv, 0),
 // and the rescaled error variance used by the time interpolation.
@@ -416,9 +486,10 @@ __device__ inline void Rescale(float hv, float *scale, float *err2)
 }
 
 // Rescale (per model), pair combination h = (g_a + sign g_b) / 2 with
-// relative error (err_a + err_b) / 2, and time interpolation.  One thread
-// per subcarrier of one (slot, layer) = blockIdx.y.  Writes
-// H[slot][rx][l][sym][sc] fp16 for sc < pad (zero beyond used).
+// relative error (err_a + err_b) / 2, and time interpolation over the D DMRS
+// symbols.  One thread per subcarrier of one (slot, layer) = blockIdx.y.
+// Writes H[slot][rx][l][sym][sc] fp16 for sc < pad (zero beyond used).
+template <int D>
 __global__ void __launch_bounds__(256) CePostKernel(CeWs w, __half *__restrict__ out_r, __half *__restrict__ out_i,
                                                     uint32_t out_layers, CeApply c)
 {
@@ -437,7 +508,7 @@ __global__ void __launch_bounds__(256) CePostKernel(CeWs w, __half *__restrict__
             }
         return;
     }
-    const uint32_t ncol = c.nr * c.ndmrs;
+    const uint32_t ncol = c.nr * D;
     const int qa = c.layer_model[l][0], qb = c.layer_model[l][1];
     const size_t sqa = static_cast<size_t>(slot) * c.nm + qa;
     float sa, ea;
@@ -452,33 +523,29 @@ __global__ void __launch_bounds__(256) CePostKernel(CeWs w, __half *__restrict__
         wb = 0.5f * c.layer_sign[l] * sb;
         err2 = 0.5f * (ea + eb);
     }
-    float tw[14][4];
-    for (uint32_t s = 0; s < c.nsym; ++s) {
-        tw[s][0] = tw[s][1] = tw[s][2] = tw[s][3] = 0.0f;
-        SolveTime(c, err2, c.rt[s], tw[s]);
-    }
+    TimeSolver<D> ts;
+    ts.Factor(c, err2);
     for (uint32_t rx = 0; rx < c.nr; ++rx) {
-        float2 hf[4];
+        float2 hf[D];
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            hf[d] = make_float2(0.0f, 0.0f);
-            if (d < static_cast<int>(c.ndmrs)) {
-                const uint32_t col = rx * c.ndmrs + d;
-                const float2 ga = w.hf[(sqa * ncol + col) * c.used + sc];
-                hf[d] = make_float2(wa * ga.x, wa * ga.y);
-                if (qb >= 0) {
-                    const float2 gb = w.hf[(sqb * ncol + col) * c.used + sc];
-                    hf[d].x += wb * gb.x;
-                    hf[d].y += wb * gb.y;
-                }
+        for (int d = 0; d < D; ++d) {
+            const uint32_t col = rx * D + d;
+            const float2 ga = w.hf[(sqa * ncol + col) * c.used + sc];
+            hf[d] = make_float2(wa * ga.x, wa * ga.y);
+            if (qb >= 0) {
+                const float2 gb = w.hf[(sqb * ncol + col) * c.used + sc];
+                hf[d].x += wb * gb.x;
+                hf[d].y += wb * gb.y;
             }
         }
         for (uint32_t s = 0; s < c.nsym; ++s) {
+            float tw[D];
+            ts.Solve(c.rt[s], tw);
             float vr = 0.0f, vi = 0.0f;
 #pragma unroll
-            for (int d = 0; d < 4; ++d) {
-                vr += tw[s][d] * hf[d].x;
-                vi += tw[s][d] * hf[d].y;
+            for (int d = 0; d < D; ++d) {
+                vr += tw[d] * hf[d].x;
+                vi += tw[d] * hf[d].y;
             }
             const size_t o = (static_cast<size_t>(rx) * out_layers + l) * plane + static_cast<size_t>(s) * c.pad + sc;
             out_r[o] = __float2half_rn(vr);
@@ -691,16 +758,23 @@ int32_t LaunchCeApply(const LmmseFilter *f, const __half *hr, const __half *hi, 
     uint32_t max_rank = 0;
     for (uint32_t q = 0; q < f->nm; ++q) max_rank = std::max(max_rank, f->model[q].rank);
     const uint32_t n_tiles = CeilDiv(static_cast<uint64_t>(slots) * ncol, kGemmTile);
-    hipLaunchKernelGGL(CeProjectKernel, dim3(CeilDiv(max_rank, kGemmTile), n_tiles, f->nm), dim3(kGemmThreads), 0,
-                       s, hr, hi, w, slots, c);
+    hipLaunchKernelGGL(CeProjectKernel, dim3(CeilDiv(max_rank, kGemmTile), n_tiles, f->nm * kProjSplits),
+                       dim3(kGemmThreads), 0, s, hr, hi, w, slots, c);
+    const uint64_t t_elems = static_cast<uint64_t>(slots) * f->nm * kMaxRank * ncol;
+    hipLaunchKernelGGL(CeProjReduceKernel, dim3(CeilDiv(t_elems, 256)), dim3(256), 0, s, w, slots, c);
     hipLaunchKernelGGL(CeEnergyKernel, dim3(slots * f->nm * ncol), dim3(256), 0, s, hr, hi, w, c);
     hipLaunchKernelGGL(CeStatsKernel, dim3(slots), dim3(256), 0, s, noise_in, noise_out, w, c);
     hipLaunchKernelGGL(CeHvKernel, dim3(CeilDiv(g.used, 256), f->nm, CeilDiv(slots, 8)), dim3(256), 0, s, w, slots,
                        c);
     hipLaunchKernelGGL(CeInterpGemmKernel, dim3(CeilDiv(g.used, kGemmTile), n_tiles, f->nm), dim3(kGemmThreads), 0,
                        s, w, slots, c);
-    hipLaunchKernelGGL(CePostKernel, dim3(CeilDiv(g.pad, 256), slots * g.nl), dim3(256), 0, s, w, out_re, out_im,
-                       out_layers, c);
+    const dim3 pgrid(CeilDiv(g.pad, 256), slots * g.nl);
+    switch (g.ndmrs) {
+    case 1: hipLaunchKernelGGL(CePostKernel<1>, pgrid, dim3(256), 0, s, w, out_re, out_im, out_layers, c); break;
+    case 2: hipLaunchKernelGGL(CePostKernel<2>, pgrid, dim3(256), 0, s, w, out_re, out_im, out_layers, c); break;
+    case 3: hipLaunchKernelGGL(CePostKernel<3>, pgrid, dim3(256), 0, s, w, out_re, out_im, out_layers, c); break;
+    default: hipLaunchKernelGGL(CePostKernel<4>, pgrid, dim3(256), 0, s, w, out_re, out_im, out_layers, c); break;
+    }
     return LaunchStatus();
 }
 

@@ -8,6 +8,7 @@
 #include "gold.h"
 #include "launch.h"
 #include "qam.h"
+#include "ratematch.h"
 
 using airan::DevGeom;
 using airan::Geom;
@@ -506,49 +507,17 @@ namespace {
 
 constexpr uint32_t kRateMagic = 0x52444d31u;  // "RDM1"
 
-struct RateDev {
-    uint32_t qm, cw_symbols, cw_stride, z, kprime, filler, ncb, n_cb;
-};
-
-// rate_dematch_mimo_kernel.cpp:33-143 as a gather: thread per LDPC input
-// position p of CB r.  Circular position k = p - 2Z; the transmitted bits
-// landing on k are kk, kk + NF, kk + 2 NF, ... (< E) with NF = Ncb - filler.
+// One thread per decoder input position p of code block blockIdx.y.
 __global__ void __launch_bounds__(256) RateDematchKernel(const int16_t *__restrict__ in,
                                                          const RateMatchDescriptor *__restrict__ desc,
-                                                         int16_t *__restrict__ out, RateDev r)
+                                                         int16_t *__restrict__ out, airan::RateDev r)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = 68u * r.z;
     if (p >= n) return;
     const uint32_t cb = blockIdx.y;
-    int16_t *o = out + static_cast<size_t>(cb) * n;
-    const uint32_t two_z = 2u * r.z;
-    if (p < two_z || p >= two_z + r.ncb) {
-        o[p] = 0;
-        return;
-    }
-    const uint32_t k = p - two_z;
-    const uint32_t fstart = r.kprime - two_z;
-    if (k >= fstart && k < fstart + r.filler) {
-        o[p] = 5120;
-        return;
-    }
-    const RateMatchDescriptor d = desc[cb];
-    const uint32_t nf = r.ncb - r.filler;
-    const uint32_t q = k < fstart ? k : k - r.filler;
-    const uint32_t q0 = d.k0 <= fstart ? d.k0 : d.k0 - r.filler;
-    uint32_t kk = q >= q0 ? q - q0 : q + nf - q0;
-    const uint32_t eq = d.e / r.qm;
-    int32_t acc = 0;
-    for (; kk < d.e; kk += nf) {
-        const uint32_t b = kk / eq, j = kk - b * eq;
-        const uint64_t logical = static_cast<uint64_t>(d.cw_symbol_offset) + j;
-        const uint64_t slot = logical / r.cw_symbols, sym = logical - slot * r.cw_symbols;
-        int32_t v = in[(slot * r.qm + b) * r.cw_stride + sym] * 8;
-        v = v > 5120 ? 5120 : (v < -5120 ? -5120 : v);
-        acc += v;
-    }
-    o[p] = static_cast<int16_t>(acc > 5120 ? 5120 : (acc < -5120 ? -5120 : acc));
+    const airan::RateCb c = airan::RateCbSetup(desc[cb], r);
+    out[static_cast<size_t>(cb) * n + p] = airan::RateDematchValue(p, c, r, in);
 }
 
 int32_t BuildRateProfile(const PuschMimoConfig *config, uint32_t num_slots, const AiranTbPlanV1 &p,
@@ -668,7 +637,7 @@ int32_t airan_rate_dematch_mimo_Enqueue(const RateDematchMimoOpArgsV1 *a,
     airan::DeriveGeom(*a->config, &g);
     if (m->qm != g.qm || m->codeword_symbols != g.cw_symbols || m->codeword_stride != g.cw_stride)
         return AIRAN_LAYOUT_MISMATCH;
-    RateDev r{g.qm, g.cw_symbols, g.cw_stride, m->z, m->kprime, m->filler, m->ncb, m->code_blocks};
+    airan::RateDev r{g.qm, g.cw_symbols, g.cw_stride, m->z, m->kprime, m->filler, m->ncb, m->code_blocks};
     dim3 grid(airan::CeilDiv(68u * m->z, 256), m->code_blocks);
     hipLaunchKernelGGL(RateDematchKernel, grid, dim3(256), 0, static_cast<hipStream_t>(a->stream),
                        static_cast<const int16_t *>(a->cw_llr),

@@ -13,6 +13,8 @@
 // registers when the active rows fit REG_ROWS, otherwise in a global workspace.
 #include "common.h"
 #include "airan_ldpc_bg1.h"
+#include "launch.h"
+#include "ratematch.h"
 
 // Posterior of the code block owned by the workgroup ([68][Z] x 16 bit).
 extern __shared__ uint16_t airan_ldpc_lam[];
@@ -214,6 +216,250 @@ __global__ void __launch_bounds__(384) LdpcKernel(const int16_t *__restrict__ la
 
 constexpr int kRegRows = 8;
 
+// ------------------------------------------------------------------ fast path
+// FP16 messages and at most kFastRows active rows (every code rate above
+// about 0.6), the same arithmetic as LdpcKernel<MODE_FP16> with:
+//  * row degrees fixed at compile time and per-lifting-size edge words
+//    (col * Z | (shift mod Z) << 16) in constant memory, so an edge costs a
+//    scalar load and three VALU ops of addressing;
+//  * fp16 VALU arithmetic on the posterior: a single fp16 add/sub/mul of
+//    fp16 operands rounds exactly like the fp32 operation rounded to fp16
+//    (fp32 has 24 >= 2*11 + 2 significand bits, so rounding twice is
+//    innocuous), and the compressed state keeps the scaled magnitudes
+//    fp16(0.75 * min), which are the message magnitudes themselves;
+//  * LDS holding only the (22 + rows) columns the active rows touch, so
+//    several code blocks share a CU; columns beyond never change.
+// With FUSED the decoder input is de-rate-matched on the fly from the
+// bit-plane codeword LLRs (the standalone rate-dematch arithmetic).
+constexpr int kNumLift = 5;  // Z = 24 << zi
+constexpr int kFastRows = 8;
+constexpr int kFastDeg[kFastRows] = {19, 19, 19, 19, 3, 8, 9, 7};
+
+constexpr int FastRowStart(int r)
+{
+    int s = 0;
+    for (int i = 0; i < r; ++i) s += kFastDeg[i];
+    return s;
+}
+
+constexpr int kFastEdges = FastRowStart(kFastRows);
+
+struct Bg1LiftDev {
+    uint32_t w[kNumLift][kFastEdges];
+};
+
+__constant__ Bg1LiftDev kLift;
+
+bool g_lift_loaded[64] = {};
+
+int32_t LoadLiftTable()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return AIRAN_LAUNCH_FAILED;
+    if (g_lift_loaded[dev]) return AIRAN_OK;
+    for (int r = 0; r < kFastRows; ++r)
+        if (AIRAN_BG1_ROW_START[r] != FastRowStart(r) ||
+            AIRAN_BG1_ROW_START[r + 1] - AIRAN_BG1_ROW_START[r] != kFastDeg[r])
+            return AIRAN_LAUNCH_FAILED;
+    Bg1LiftDev t{};
+    for (int zi = 0; zi < kNumLift; ++zi) {
+        const uint32_t z = 24u << zi;
+        for (int e = 0; e < kFastEdges; ++e)
+            t.w[zi][e] = static_cast<uint32_t>(AIRAN_BG1_COL[e]) * z |
+                         (static_cast<uint32_t>(AIRAN_BG1_SHIFT[e]) % z) << 16;
+    }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(kLift), &t, sizeof(t)) != hipSuccess) return AIRAN_LAUNCH_FAILED;
+    g_lift_loaded[dev] = true;
+    return AIRAN_OK;
+}
+
+using h16 = _Float16;
+
+__device__ __forceinline__ h16 HBits(uint32_t b) { return __builtin_bit_cast(h16, static_cast<uint16_t>(b)); }
+__device__ __forceinline__ uint32_t BitsH(h16 h) { return __builtin_bit_cast(uint16_t, h); }
+
+// Negative and non-zero (fp16 bits), as `value < 0` on the decoded posterior.
+__device__ __forceinline__ uint32_t NegBit(uint32_t h) { return (h & 0xffffu) - 0x8001u <= 0x7bffu ? 1u : 0u; }
+
+__device__ __forceinline__ uint32_t LiftAddr(uint32_t w, uint32_t i, uint32_t Z)
+{
+    uint32_t p = i + (w >> 16);
+    p = min(p, p - Z);
+    return (w & 0xffffu) + p;
+}
+
+// Check i of row R: gather, extrinsic, min1/min2/signs, scatter.  State
+// sa = fp16(0.75 min1) | fp16(0.75 min2) << 16, sb = signs | index of min1 << 24.
+// Branch-free on fp16 bit patterns: magnitudes (|x| bits, +inf = 0x7c00,
+// NaN above) order like unsigned integers, so min1 and its first index are
+// the minimum of (|x| << 5 | k), and min2 = min(min2, max(min1, |x|)).  The
+// sign bit recorded is !(x >= 0), i.e. bits above 0x7c00 (the posterior is
+// never -0: it starts from integer LLRs and IEEE sums give -0 only from -0).
+template <int R>
+__device__ __forceinline__ void FastRow(h16 *lam, const uint32_t *lw, uint32_t i, uint32_t Z, uint32_t &sa,
+                                        uint32_t &sb)
+{
+    constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
+    const uint32_t o1 = sa & 0xffffu, o2 = sa >> 16;
+    const uint32_t osg = sb & 0x7ffffu, oidx = sb >> 24;
+    const uint32_t opar = __popc(osg) & 1u;
+    uint32_t w[DEG];
+#pragma unroll
+    for (int k = 0; k < DEG; ++k) w[k] = lw[E0 + k];
+    uint32_t addr[DEG];
+    h16 ext[DEG];
+    uint32_t m1key = 0x7c00u << 5, m2 = 0x7c00u, sg = 0;
+#pragma unroll
+    for (int k = 0; k < DEG; ++k) {
+        addr[k] = LiftAddr(w[k], i, Z);
+        const uint32_t omag = static_cast<uint32_t>(k) == oidx ? o2 : o1;
+        const uint32_t omsg = omag ^ ((((osg >> k) ^ opar) & 1u) << 15);
+        const h16 x = lam[addr[k]] - HBits(omsg);
+        ext[k] = x;
+        const uint32_t xb = BitsH(x);
+        const uint32_t key = ((xb & 0x7fffu) << 5) | static_cast<uint32_t>(k);
+        const uint32_t hi = max(m1key, key);
+        m1key = min(m1key, key);
+        m2 = min(m2, hi >> 5);
+        sg |= ((0x7c00u - xb) >> 31) << k;
+    }
+    const uint32_t idx = m1key & 31u;
+    const uint32_t s1 = BitsH(static_cast<h16>(0.75f) * HBits(m1key >> 5));
+    const uint32_t s2 = BitsH(static_cast<h16>(0.75f) * HBits(m2));
+    const uint32_t par = __popc(sg) & 1u;
+#pragma unroll
+    for (int k = 0; k < DEG; ++k) {
+        const uint32_t mag = static_cast<uint32_t>(k) == idx ? s2 : s1;
+        const uint32_t nm = mag ^ ((((sg >> k) ^ par) & 1u) << 15);
+        lam[addr[k]] = ext[k] + HBits(nm);
+    }
+    sa = s1 | (s2 << 16);
+    sb = sg | (idx << 24);
+}
+
+template <int R>
+__device__ __forceinline__ void FastIteration(h16 *lam, const uint32_t *lw, uint32_t i, uint32_t Z, int rows,
+                                              uint32_t *sa, uint32_t *sb)
+{
+    if constexpr (R < kFastRows) {
+        if (R < rows) {
+            if (i < Z) FastRow<R>(lam, lw, i, Z, sa[R], sb[R]);
+            __syncthreads();
+            FastIteration<R + 1>(lam, lw, i, Z, rows, sa, sb);
+        }
+    }
+}
+
+// OR over the active rows of check i's parity (hard decisions).
+template <int R>
+__device__ __forceinline__ uint32_t FastSyndrome(const uint16_t *lam, const uint32_t *lw, uint32_t i, uint32_t Z,
+                                                 int rows)
+{
+    if constexpr (R < kFastRows) {
+        if (R < rows) {
+            constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
+            uint32_t par = 0;
+#pragma unroll
+            for (int k = 0; k < DEG; ++k) par ^= NegBit(lam[LiftAddr(lw[E0 + k], i, Z)]);
+            // One row's loads at a time (keeps the register count at the
+            // occupancy target instead of every row's loads in flight).
+            asm volatile("" : "+v"(par) : : "memory");
+            return par | FastSyndrome<R + 1>(lam, lw, i, Z, rows);
+        }
+    }
+    return 0;
+}
+
+struct LdpcFastDev {
+    uint32_t z, zi, k, iters, rows, early_stop;
+    airan::RateDev rate;  // FUSED: codeword geometry
+};
+
+// One workgroup per code block.  src: int16 decoder LLRs [cb][68 Z]
+// (!FUSED) or bit-plane codeword LLRs with per-block descriptors (FUSED).
+// Six waves per SIMD (80 VGPRs): four Z = 384 blocks per CU.
+template <bool FUSED>
+__global__ void __launch_bounds__(384, 6) LdpcFastKernel(const int16_t *__restrict__ src,
+                                                      const RateMatchDescriptor *__restrict__ desc,
+                                                      int8_t *__restrict__ bits, int16_t *__restrict__ lam_out,
+                                                      uint8_t *__restrict__ iters_out, LdpcFastDev d)
+{
+    uint16_t *lam16 = airan_ldpc_lam;
+    h16 *lam = reinterpret_cast<h16 *>(airan_ldpc_lam);
+    const uint32_t Z = d.z, N = 68u * Z, i = threadIdx.x, cb = blockIdx.x;
+    const uint32_t nl = (22u + d.rows) * Z;  // posterior entries held in LDS
+    const int16_t *blk = FUSED ? src : src + static_cast<size_t>(cb) * N;
+    airan::RateCb rc{};
+    if (FUSED) rc = airan::RateCbSetup(desc[cb], d.rate);
+    auto input = [&](uint32_t t) -> h16 {
+        const int16_t v = FUSED ? airan::RateDematchValue(t, rc, d.rate, src) : blk[t];
+        return static_cast<h16>(static_cast<float>(v) * (1.0f / 256.0f));
+    };
+    if (FUSED) {
+        for (uint32_t t = i; t < nl; t += blockDim.x) lam[t] = input(t);
+    } else {
+        // nl is a multiple of 4 (Z is a multiple of 24) and blk 8-byte aligned.
+        const uint2 *s4 = reinterpret_cast<const uint2 *>(blk);
+        uint2 *l4 = reinterpret_cast<uint2 *>(lam16);
+        for (uint32_t t = i; t < nl / 4u; t += blockDim.x) {
+            const uint2 v = s4[t];
+            auto cv = [](uint32_t x) {
+                return BitsH(static_cast<h16>(static_cast<float>(static_cast<int16_t>(x)) * (1.0f / 256.0f)));
+            };
+            l4[t] = make_uint2(cv(v.x & 0xffffu) | cv(v.x >> 16) << 16, cv(v.y & 0xffffu) | cv(v.y >> 16) << 16);
+        }
+    }
+    uint32_t sa[kFastRows], sb[kFastRows];
+#pragma unroll
+    for (int r = 0; r < kFastRows; ++r) sa[r] = sb[r] = 0;
+    __syncthreads();
+    const uint32_t *lw = kLift.w[d.zi];
+    const int rows = static_cast<int>(d.rows);
+    uint32_t it = 0;
+    for (; it < d.iters; ++it) {
+        FastIteration<0>(lam, lw, i, Z, rows, sa, sb);
+        if (d.early_stop) {
+            const uint32_t bad = i < Z ? FastSyndrome<0>(lam16, lw, i, Z, rows) : 0u;
+            if (__syncthreads_and(bad == 0 ? 1 : 0)) {
+                ++it;
+                break;
+            }
+        }
+    }
+    if (iters_out != nullptr && i == 0) iters_out[cb] = static_cast<uint8_t>(it);
+    int8_t *bo = bits + static_cast<size_t>(cb) * d.k;
+    if ((d.k & 3u) == 0) {
+        const uint2 *l4 = reinterpret_cast<const uint2 *>(lam16);
+        for (uint32_t t = i; t < d.k / 4u; t += blockDim.x) {
+            const uint2 v = l4[t];
+            reinterpret_cast<uint32_t *>(bo)[t] = NegBit(v.x) | NegBit(v.x >> 16) << 8 | NegBit(v.y) << 16 |
+                                                  NegBit(v.y >> 16) << 24;
+        }
+    } else {
+        for (uint32_t t = i; t < d.k; t += blockDim.x) bo[t] = static_cast<int8_t>(NegBit(lam16[t]));
+    }
+    if (lam_out != nullptr) {
+        int16_t *lo = lam_out + static_cast<size_t>(cb) * N;
+        for (uint32_t t = i; t < N; t += blockDim.x) {
+            float v = static_cast<float>(t < nl ? lam[t] : input(t));
+            v = v > 20.0f ? 20.0f : (v < -20.0f ? -20.0f : v);
+            lo[t] = static_cast<int16_t>(rintf(v * 256.0f));
+        }
+    }
+}
+
+bool FastEligible(uint32_t mode, uint32_t z, uint32_t rows)
+{
+    return mode == AIRAN_LDPC_MODE_FP16 && rows <= static_cast<uint32_t>(kFastRows) && z >= 24 && z <= 384;
+}
+
+uint32_t LiftIndex(uint32_t z)
+{
+    uint32_t zi = 0;
+    while ((24u << zi) < z) ++zi;
+    return zi;
+}
+
 }  // namespace
 
 extern "C" {
@@ -257,6 +503,15 @@ int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, 
     auto *lo = static_cast<int16_t *>(a->lam_out);
     auto *io = static_cast<uint8_t *>(a->iterations_out);
     auto *ws = static_cast<uint2 *>(workspace);
+    const bool aligned = (reinterpret_cast<uintptr_t>(in) & 7u) == 0 && (reinterpret_cast<uintptr_t>(bits) & 3u) == 0;
+    if (aligned && FastEligible(a->mode, a->z, a->active_rows)) {
+        if ((st = LoadLiftTable()) != AIRAN_OK) return st;
+        LdpcFastDev fd{a->z, LiftIndex(a->z), a->k, a->max_iterations, a->active_rows, a->early_stop ? 1u : 0u, {}};
+        const size_t fast_lds = static_cast<size_t>(22 + a->active_rows) * a->z * sizeof(uint16_t);
+        hipLaunchKernelGGL(LdpcFastKernel<false>, dim3(a->num_cb), dim3(threads), fast_lds, s, in, nullptr, bits, lo,
+                           io, fd);
+        return airan::LaunchStatus();
+    }
     const bool reg = a->active_rows <= static_cast<uint32_t>(kRegRows);
     if (a->mode == AIRAN_LDPC_MODE_FP16) {
         if (reg)
@@ -271,6 +526,31 @@ int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, 
     }
     return airan::LaunchStatus();
 }
+
+}  // extern "C"
+
+namespace airan {
+
+int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const RateMatchDescriptor *desc,
+                               const int16_t *cw, uint32_t num_cb, uint32_t rows, uint32_t iters, bool early_stop,
+                               int8_t *bits, uint8_t *iters_out, hipStream_t s)
+{
+    if (num_cb == 0 || !FastEligible(AIRAN_LDPC_MODE_FP16, p.Z, rows) || rows < 4 || iters == 0 || iters > 100 ||
+        (p.Z % 24) != 0 || p.K == 0 || p.K > 22 * p.Z || p.Ncb <= p.filler || (reinterpret_cast<uintptr_t>(bits) & 3u))
+        return AIRAN_UNSUPPORTED_PROFILE;
+    int32_t st = LoadLiftTable();
+    if (st != AIRAN_OK) return st;
+    LdpcFastDev fd{p.Z, LiftIndex(p.Z), p.K, iters, rows, early_stop ? 1u : 0u,
+                   {g.qm, g.cw_symbols, g.cw_stride, p.Z, p.Kprime, p.filler, p.Ncb, num_cb}};
+    const size_t lds = static_cast<size_t>(22 + rows) * p.Z * sizeof(uint16_t);
+    hipLaunchKernelGGL(LdpcFastKernel<true>, dim3(num_cb), dim3(AlignUp(p.Z, 64)), lds, s, cw, desc, bits,
+                       static_cast<int16_t *>(nullptr), iters_out, fd);
+    return LaunchStatus();
+}
+
+}  // namespace airan
+
+extern "C" {
 
 // Rows 0..3 always; extension row r >= 4 only when its parity column 22 + r
 // holds a transmitted circular-buffer position.

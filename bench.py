@@ -42,10 +42,11 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def stage_bytes(g, plan, S: int) -> dict:
+def stage_bytes(g, plan, S: int, fused_ldpc: bool) -> dict:
     """Algorithmic HBM bytes per launch of each stage for S slots (compulsory
     reads of the stage's inputs and writes of its outputs, in the layouts the
-    chain keeps)."""
+    chain keeps).  With fused_ldpc the decoder reads the codeword LLRs itself
+    and the rate-dematch stage moves nothing."""
     nr, nl, qm = g.nr, g.nl, g.qm
     data_re = g.ndata * g.used
     grid16 = S * nr * 14 * g.pad * 4                   # fp16 re+im planes
@@ -60,8 +61,8 @@ def stage_bytes(g, plan, S: int) -> dict:
         "dmrs_ls": S * nr * g.ndmrs * g.used * 4 // 2 + h_ls,
         "ce_lmmse": h_ls + h_used,
         "detect_demap": S * nr * data_re * 4 + S * nr * nl * data_re * 4 + cw,
-        "rate_dematch": cw + lam,
-        "ldpc_decode": lam + bits,
+        "rate_dematch": 0 if fused_ldpc else cw + lam,
+        "ldpc_decode": (cw if fused_ldpc else lam) + bits,
         "crc_check": bits,
     }
 
@@ -74,8 +75,11 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=64, help="slots per step")
     ap.add_argument("--snr", type=float, default=None, help="skip the sweep and use this SNR (dB)")
     ap.add_argument("--sweep-slots", type=int, default=16)
-    ap.add_argument("--cpu-slots", type=int, default=4, help="CPU baseline sample size (slots)")
+    ap.add_argument("--cpu-slots", type=int, default=32, help="CPU baseline sample size (slots)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--seed", type=int, default=20260)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -94,6 +98,16 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        n = args.copy_probe << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 5 copies of {n} bytes")
+        return 0
     prof = P.FULL_CARRIER_4RX
     cfg = prof.config()
     g = ops.Geometry(cfg)
@@ -116,7 +130,9 @@ def main() -> int:
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         chain = NativeChain(cfg, plan, max_slots=max(S, args.sweep_slots), opts=opts, use_graph=True)
-        log(f"chain ready, {chain.device_bytes() / 2**20:.0f} MiB on the device")
+        fused_ldpc = not chain.outputs().ldpc_llr
+        log(f"chain ready, {chain.device_bytes() / 2**20:.0f} MiB on the device, "
+            f"{'fused rate dematch + LDPC' if fused_ldpc else 'separate rate dematch'}")
 
         def run(iq_np, n):
             dev = torch.from_numpy(iq_np).cuda()
@@ -164,7 +180,9 @@ def main() -> int:
         ms_per_step = start.elapsed_time(end) / args.steps
         tb_ok = chain.fetch("tb_ok", torch.uint8, S)
         bits = chain.fetch("bits", torch.int8, S * plan.C * plan.K)
+        iters = chain.fetch("iterations", torch.uint8, S * plan.C)
         torch.cuda.synchronize()
+        mean_iters = float(iters.float().mean().item())
         tb_ok = tb_ok.cpu().numpy()
         b = bits.cpu().numpy().reshape(S, plan.C, plan.K).astype(np.uint8)
         per_cb = plan.Kprime - plan.crc_cb
@@ -173,7 +191,7 @@ def main() -> int:
             tb = b[s, :, :per_cb].reshape(-1)[: plan.tbs]
             bit_err_tb += int(not np.array_equal(tb, payload[s]))
         log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, wall {wall:.3f} s, "
-            f"tb_ok {int(tb_ok.sum())}/{S}, payload mismatches {bit_err_tb}")
+            f"tb_ok {int(tb_ok.sum())}/{S}, payload mismatches {bit_err_tb}, mean LDPC iterations {mean_iters:.2f}")
 
         # 3. Per-stage times (HIP events on the chain's stream, direct launches).
         chain.set_timing(True)
@@ -191,7 +209,7 @@ def main() -> int:
     tbs_bits = S * plan.tbs
     value = tbs_bits / (ms_per_step * 1e-3) / 1e9
     goodput = float(tb_ok.sum()) * plan.tbs / (ms_per_step * 1e-3) / 1e9
-    bytes_per = stage_bytes(g, plan, S)
+    bytes_per = stage_bytes(g, plan, S, fused_ldpc)
     dominant = max(stage_ms, key=stage_ms.get)
     achieved = bytes_per[dominant] / (stage_ms[dominant] * 1e-3) / 1e9
     chain_bytes = S * (g.nr * g.samples * 8 + plan.C * plan.K)
@@ -217,6 +235,7 @@ def main() -> int:
             "goodput_gbps": round(goodput, 4),
             "slots_per_s": round(S / (ms_per_step * 1e-3), 1),
             "ldpc_max_iterations": opts.ldpc_iterations,
+            "ldpc_mean_iterations": round(mean_iters, 3),
             "ce_rank": opts.ce_rank,
             "sweep_bler": {str(k): round(v, 4) for k, v in sweep.items()},
             "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},

@@ -1,0 +1,98 @@
+/*
+ * airan_ce_refiner.h -- learned channel-estimation refiner (no counterpart
+ * in the reference): a convolutional network that maps the DMRS
+ * least-squares estimate, linearly interpolated onto the 14 x used grid, to
+ * the full-grid channel, with BF16 training and inference kernels.
+ *
+ * Network (4 Rx x 4 layers): input X [slots][14][used][32] bf16, channel
+ * c = 2 (rx * 4 + layer) + {0: Re, 1: Im}; six 3x3 convolutions (stride 1,
+ * zero padding) 32 -> 64 -> 64 -> 64 -> 64 -> 64 -> 32 with ReLU after the
+ * first five, and a residual: H = X + conv6(...).  Weights and activations
+ * bf16, accumulation fp32 (MFMA), master weights fp32 trained with Adam on
+ * the device.  Parameters, in order per layer: weights [9][cout][cin]
+ * (tap = 3 kh + kw over symbol offset kh - 1 and subcarrier offset kw - 1),
+ * then bias [cout]; AIRAN_REFINER_PARAMS floats in all.
+ *
+ * Training data is synthesised on the device: true channels from per-slot
+ * tapped-delay-line gains, the DMRS symbols of the received grid with
+ * AWGN, the library's DMRS least-squares kernels, then the interpolation
+ * that forms X; the loss is the mean squared error against the true channel.
+ */
+#ifndef AIRAN_CE_REFINER_H
+#define AIRAN_CE_REFINER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "airan_pusch_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AIRAN_REFINER_PARAMS 184672u
+#define AIRAN_REFINER_CHANNELS 32u
+
+typedef struct AiranCeRefiner AiranCeRefiner;
+
+/* Refiner for grants of `config`'s shape (4 Rx, 4 layers, DMRS type 1 on
+ * ports 1000-1003) and batches of up to max_slots slots.  params: host fp32
+ * [AIRAN_REFINER_PARAMS], or NULL for He-normal weights from `seed`.
+ * Synchronises `stream`. */
+int32_t airan_ce_refiner_Create(const PuschMimoConfig *config, uint32_t max_slots, const float *params,
+                                uint64_t seed, void *stream, AiranCeRefiner **refiner);
+void airan_ce_refiner_Destroy(AiranCeRefiner *refiner);
+/* Master weights to / from the host (synchronous). */
+int32_t airan_ce_refiner_GetParams(AiranCeRefiner *refiner, float *params);
+int32_t airan_ce_refiner_SetParams(AiranCeRefiner *refiner, const float *params);
+
+/* Inference on DMRS least-squares estimates in the chain layout
+ * (h_ls fp16 [slots][NR][L][D][pilot_pad], from mimo_dmrs_ls): writes the
+ * channel grid fp16 [slots][NR][L][14][padded] (zero beyond used) for every
+ * slot whose flag keep[slot] is non-zero (keep may be NULL: all slots). */
+int32_t airan_ce_refiner_Enqueue(AiranCeRefiner *refiner, const void *h_ls_re, const void *h_ls_im,
+                                 uint32_t slots, const uint8_t *keep, void *h_re, void *h_im, void *stream);
+
+/* Synthesises a training or evaluation batch on the device into the
+ * refiner's batch buffers: gains host complex64 [slots][NR][L][taps][14]
+ * (tapped-delay-line gains per OFDM symbol), tap_delay_s [taps],
+ * snr_db [slots] (AWGN per resource element relative to unit channel
+ * power).  Also leaves the DMRS least-squares estimate and its noise
+ * estimate (the LMMSE estimator's inputs) in the batch buffers. */
+int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *refiner, const float *gains, const float *tap_delay_s,
+                                    uint32_t taps, const float *snr_db, uint32_t slots, uint64_t seed,
+                                    void *stream);
+/* Device pointers of the batch buffers (valid until Destroy). */
+typedef struct AiranCeRefinerBatchV1 {
+    uint16_t abi_version;
+    uint16_t struct_size;
+    const void *x;              /* bf16 [slots][14][used][32] network input */
+    const void *target;         /* fp32 [slots][14][used][32] true channel */
+    const void *out;            /* fp32 [slots][14][used][32] last forward output */
+    const void *h_ls_re;        /* fp16 [slots][NR][L][D][pilot_pad] */
+    const void *h_ls_im;
+    const void *noise_ls;       /* fp16 [slots][NR] */
+} AiranCeRefinerBatchV1;
+int32_t airan_ce_refiner_Batch(const AiranCeRefiner *refiner, AiranCeRefinerBatchV1 *batch);
+
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the synthesised batch;
+ * loss_out (device fp32, may be NULL) receives the batch loss before the
+ * update. */
+int32_t airan_ce_refiner_TrainStep(AiranCeRefiner *refiner, uint32_t slots, float lr, float *loss_out,
+                                   void *stream);
+/* Forward pass of the current weights on the synthesised batch (result in
+ * the batch `out` buffer). */
+int32_t airan_ce_refiner_Forward(AiranCeRefiner *refiner, uint32_t slots, void *stream);
+
+/* Loss and gradients of every parameter for an arbitrary input:
+ * x bf16 [slots][14][width][32], target fp32 [slots][14][width][32]
+ * (width <= used), grad device fp32 [AIRAN_REFINER_PARAMS], loss device
+ * fp32 [1].  The loss is the mean over all slots * 14 * width * 32 values. */
+int32_t airan_ce_refiner_Gradients(AiranCeRefiner *refiner, const void *x, const void *target, uint32_t slots,
+                                   uint32_t width, float *grad, float *loss, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AIRAN_CE_REFINER_H */

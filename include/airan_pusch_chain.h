@@ -14,8 +14,12 @@
  *   detect+demap+descramble   one fused kernel: exact per-RE LMMSE, the
  *                             QAM demapper, layer demap and Gold descramble,
  *                             writing the descrambled codeword LLRs
- *   rate dematch              the rate_dematch_mimo kernel, all code blocks
- *   LDPC                      airan_ldpc_decode_Enqueue, all code blocks
+ *   rate dematch + LDPC       FP16 messages with at most 8 active rows: one
+ *                             decoder launch that de-rate-matches the
+ *                             codeword LLRs while loading each code block
+ *                             (the rate_dematch_mimo arithmetic); otherwise
+ *                             the rate_dematch_mimo kernel then
+ *                             airan_ldpc_decode_Enqueue, all code blocks
  *   CRC                       airan_crc_check_Enqueue, all transport blocks
  *
  * Every stage computes exactly what the per-operator entry points compute
@@ -72,7 +76,8 @@ typedef struct AiranPuschChainOutputsV1 {
     const void *h_re;           /* fp16 [slots][NR][L][14][padded] */
     const void *h_im;
     const void *cw_llr;         /* int16 [slots][Qm][codeword_stride], descrambled */
-    const void *ldpc_llr;       /* int16 [slots][C][68 Z] */
+    const void *ldpc_llr;       /* int16 [slots][C][68 Z]; NULL when the decoder reads cw_llr directly
+                                   (FP16 messages and at most 8 active rows) */
 } AiranPuschChainOutputsV1;
 
 /* Builds the CE filter, the DMRS reference, the Gold bits and every buffer

@@ -19,6 +19,8 @@
  *   or_ce_lmmse                pinned (noiseless/flat closed forms, full-rank
  *                              identity against a brute-force solve)
  *   or_detect_lmmse            pinned (closed form vs brute-force inverse)
+ *   or_refiner_grad            pinned (exact mode against central finite
+ *                              differences of its own loss)
  *   or_ldpc_decode             pinned at outcome level only (encode -> decode
  *                              round trip, syndrome); the reference's fp16/int8
  *                              message trajectory has no host oracle anywhere
@@ -164,6 +166,18 @@ void or_ofdm_demod(const PuschMimoConfig *cfg, const float *iq, double *grid);
 int or_chain_slot(const PuschMimoConfig *cfg, const OrTbPlan *p, const OrCeParams *prm, int ldpc_iters,
                   const float *iq, int8_t *bits, uint8_t *tb_ok);
 int or_num_threads(void);
+
+/* --------------------------------------------------------- CE refiner */
+/* The refiner network of include/airan_ce_refiner.h (or_refiner.c):
+ * params fp32 [or_refiner_param_offset(OR_RF_LAYERS)], x and target fp32
+ * [S][14][W][32] -> loss, gradients (double, same layout as params) and the
+ * output (fp32 [S][14][W][32], may be NULL).  device_rounding: round to bf16
+ * where the device stores bf16. */
+#define OR_RF_LAYERS 6
+size_t or_refiner_param_offset(int layer);
+int or_refiner_grad(const float *params, const float *x, const float *target, int S, int W, int device_rounding,
+                    double *grad, double *loss, float *out);
+double or_bf16_round(double v);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,13 @@ def load_oracle():
     lib.or_ofdm_demod.argtypes = [vp, vp, vp]
     lib.or_chain_slot.argtypes = [vp, C.POINTER(OrTbPlan), C.POINTER(OrCeParams), C.c_int, vp, vp, vp]
     lib.or_num_threads.restype = C.c_int
+    lib.or_refiner_param_offset.restype = C.c_size_t
+    lib.or_refiner_param_offset.argtypes = [C.c_int]
+    lib.or_refiner_grad.restype = C.c_int
+    lib.or_refiner_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]
+    lib.or_bf16_round.restype = C.c_double
+    lib.or_bf16_round.argtypes = [C.c_double]
     _ORACLE = lib
     return lib
 

@@ -111,10 +111,15 @@ def test_native_chain_matches_op_chain(gpu_stream, prof, snr):
             "noise_ce": nat.fetch("noise_ce", torch.float16, S * g.nr),
             "h_re": nat.fetch("h_re", torch.float16, S * g.nr * g.nl * plane),
             "cw_llr": nat.fetch("cw_llr", torch.int16, S * g.qm * g.cw_stride),
-            "ldpc_llr": nat.fetch("ldpc_llr", torch.int16, S * plan.C * 68 * plan.Z),
             "bits": nat.fetch("bits", torch.int8, S * plan.C * plan.K),
+            "iterations": nat.fetch("iterations", torch.uint8, S * plan.C),
             "tb_ok": nat.fetch("tb_ok", torch.uint8, S),
         }
+        # The decoder reads the codeword LLRs directly when it can (no
+        # rate-dematched buffer); otherwise that buffer is compared too.
+        fused = not nat.outputs().ldpc_llr
+        if not fused:
+            out["ldpc_llr"] = nat.fetch("ldpc_llr", torch.int16, S * plan.C * 68 * plan.Z)
         torch.cuda.synchronize()
         nat_np = {k: v.cpu().numpy() for k, v in out.items()}
         for s in range(S):
@@ -133,8 +138,10 @@ def test_native_chain_matches_op_chain(gpu_stream, prof, snr):
             h_nat = part("h_re", g.nr * g.nl * plane).reshape(g.nr, g.nl, plane)
             assert np.array_equal(h_nat.view(np.uint16), h_op.view(np.uint16))
             assert np.array_equal(part("cw_llr", g.qm * g.cw_stride), r["cw_nr"].cpu().numpy())
-            assert np.array_equal(part("ldpc_llr", plan.C * 68 * plan.Z), r["ldpc_llr"].cpu().numpy())
+            if not fused:
+                assert np.array_equal(part("ldpc_llr", plan.C * 68 * plan.Z), r["ldpc_llr"].cpu().numpy())
             assert np.array_equal(part("bits", plan.C * plan.K), r["bits"].cpu().numpy())
+            assert np.array_equal(part("iterations", plan.C), r["iterations"].cpu().numpy())
             assert int(nat_np["tb_ok"][s]) == int(r["tb_ok"].item()) == 1
             assert np.array_equal(tb_bits(plan, torch.from_numpy(part("bits", plan.C * plan.K))), payload[s])
     nat.close()

@@ -220,14 +220,39 @@ def test_ldpc_bit_exact(oracle, gpu_stream, mode, rows, iters, early):
         assert np.array_equal(want_bits.reshape(ncb, -1), infos.astype(np.int8))
 
 
-def test_ldpc_small_lifting_sizes(oracle, gpu_stream):
-    for Z in (24, 96, 192):
-        infos, lam = _encoded_llrs(oracle, Z, 3, 3.0, Z)
-        want = np.zeros(3 * 22 * Z, np.int8)
-        l2 = lam.copy()
-        oracle.or_ldpc_decode(Z, 3, ptr(l2), 1, 6, 46, 22 * Z, 0, ptr(want), None)
-        bits, _, _ = ops.ldpc_decode(dev(lam), 3, Z, 22 * Z, 6, 46, abi.LDPC_MODE_FP16)
-        assert np.array_equal(host(bits), want)
+@pytest.mark.parametrize("rows,early", [(46, False), (6, False), (8, True), (4, True)])
+def test_ldpc_small_lifting_sizes(oracle, gpu_stream, rows, early):
+    for Z in (24, 48, 96, 192):
+        ncb = 3
+        infos, lam = _encoded_llrs(oracle, Z, ncb, 3.0, Z + rows)
+        want = np.zeros(ncb * 22 * Z, np.int8)
+        want_lam = lam.copy()
+        want_it = np.zeros(ncb, np.uint8)
+        oracle.or_ldpc_decode(Z, ncb, ptr(want_lam), 1, 6, rows, 22 * Z, 1 if early else 0, ptr(want), ptr(want_it))
+        bits, lam_out, it = ops.ldpc_decode(dev(lam), ncb, Z, 22 * Z, 6, rows, abi.LDPC_MODE_FP16, early,
+                                            want_lam=True)
+        assert np.array_equal(host(bits), want), Z
+        assert np.array_equal(host(lam_out), want_lam), Z
+        assert np.array_equal(host(it), want_it), Z
+
+
+def test_ldpc_fp16_saturated_posteriors(oracle, gpu_stream):
+    """Clean high-SNR blocks decoded for many iterations without early stop:
+    the layered posteriors grow past the fp16 range (inf, then NaN extrinsics)
+    and the decoder must still follow the oracle's fp16 arithmetic."""
+    Z, ncb = 384, 4
+    _, lam = _encoded_llrs(oracle, Z, ncb, 12.0, 77)
+    for rows in (6, 46):
+        want_lam = lam.copy()
+        want_bits = np.zeros(ncb * 22 * Z, np.int8)
+        want_it = np.zeros(ncb, np.uint8)
+        oracle.or_ldpc_decode(Z, ncb, ptr(want_lam), abi.LDPC_MODE_FP16, 12, rows, 22 * Z, 0, ptr(want_bits),
+                              ptr(want_it))
+        bits, lam_out, it = ops.ldpc_decode(dev(lam), ncb, Z, 22 * Z, 12, rows, abi.LDPC_MODE_FP16, False,
+                                            want_lam=True)
+        assert np.array_equal(host(bits), want_bits), rows
+        assert np.array_equal(host(lam_out), want_lam), rows
+        assert np.array_equal(host(it), want_it), rows
 
 
 def test_crc_check_matches_oracle(oracle, gpu_stream):
