@@ -7,6 +7,7 @@
 #include <cmath>
 #include <vector>
 
+#include "airan_ce_refiner.h"
 #include "airan_pusch_chain.h"
 #include "common.h"
 #include "detect.h"
@@ -137,6 +138,20 @@ uint16_t HostF2H(float f)
     return static_cast<uint16_t>(h);
 }
 
+// keep[slot] = 1 when the slot's SNR estimate, 10 log10(1 / mean_rx noise),
+// lies in [snr_min_db, snr_max_db] (the refiner's training range).
+__global__ void __launch_bounds__(64) RefinerKeepKernel(const __half *__restrict__ noise, uint8_t *__restrict__ keep,
+                                                        uint32_t nr, uint32_t slots, float snr_min_db,
+                                                        float snr_max_db)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slots) return;
+    float acc = 0.0f;
+    for (uint32_t r = 0; r < nr; ++r) acc += __half2float(noise[static_cast<size_t>(s) * nr + r]);
+    const float snr = -10.0f * log10f(fmaxf(acc / static_cast<float>(nr), 1e-30f));
+    keep[s] = snr >= snr_min_db && snr <= snr_max_db ? 1 : 0;
+}
+
 template <typename T>
 bool Alloc(T **p, size_t count, size_t *total)
 {
@@ -183,6 +198,11 @@ struct AiranPuschChain {
     uint8_t *crc_ws = nullptr;
     size_t crc_ws_bytes = 0;
     void *zero_iq = nullptr;
+    // Learned refiner (optional): replaces the LMMSE estimate of the slots
+    // whose SNR estimate lies in [snr_min, snr_max].
+    AiranCeRefiner *refiner = nullptr;
+    float refiner_snr_min = 0.0f, refiner_snr_max = 0.0f;
+    uint8_t *refiner_keep = nullptr;
     // Stage timing (events before the first stage and after each stage).
     bool timing = false;
     hipEvent_t ev[AIRAN_CHAIN_STAGES + 1] = {};
@@ -234,6 +254,12 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
     st = airan::LaunchCeApply(c->filter, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce, c->ce_ws, c->ce_ws_bytes,
                               c->h_re, c->h_im, g.nl, slots, s);
     if (st != AIRAN_OK) return st;
+    if (c->refiner != nullptr) {
+        hipLaunchKernelGGL(RefinerKeepKernel, dim3(airan::CeilDiv(slots, 64)), dim3(64), 0, s, c->noise_ce,
+                           c->refiner_keep, g.nr, slots, c->refiner_snr_min, c->refiner_snr_max);
+        st = airan_ce_refiner_Enqueue(c->refiner, c->hls_re, c->hls_im, slots, c->refiner_keep, c->h_re, c->h_im, s);
+        if (st != AIRAN_OK) return st;
+    }
     mark();
     DemapArgs da{};
     da.nr = g.nr;
@@ -374,7 +400,7 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         !Alloc(&c->cw, S * g.qm * g.cw_stride, &b) || !Alloc(&c->desc, ncb, &b) ||
         !Alloc(&c->lam, c->fused_ldpc ? 0 : ncb * 68 * p.Z, &b) || !Alloc(&c->ldpc_ws, c->ldpc_ws_bytes, &b) ||
         !Alloc(&c->bits, ncb * p.K, &b) || !Alloc(&c->iters, ncb, &b) || !Alloc(&c->cb_ok, ncb, &b) ||
-        !Alloc(&c->tb_ok, S, &b) || !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
+        !Alloc(&c->tb_ok, S, &b) || !Alloc(&c->refiner_keep, S, &b) || !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
         !Alloc(reinterpret_cast<uint8_t **>(&c->zero_iq), g.nr * g.samples * iq_elem, &b))
         return fail(AIRAN_RESOURCE_TOO_SMALL);
     // DMRS reference of this slot configuration.
@@ -434,7 +460,7 @@ void airan_pusch_chain_Destroy(AiranPuschChain *c)
     airan_channel_est_lmmse_DestroyFilter(c->filter);
     void *bufs[] = {c->dmrs_re, c->dmrs_im, c->rx_re, c->rx_im, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce,
                     c->ce_ws, c->h_re, c->h_im, c->gold, c->cw, c->desc, c->lam, c->ldpc_ws, c->bits,
-                    c->iters, c->cb_ok, c->tb_ok, c->crc_ws, c->zero_iq};
+                    c->iters, c->cb_ok, c->tb_ok, c->crc_ws, c->zero_iq, c->refiner_keep};
     for (void *p : bufs) Free(p);
     delete c;
 }
@@ -492,6 +518,17 @@ int32_t airan_pusch_chain_Outputs(const AiranPuschChain *c, AiranPuschChainOutpu
 }
 
 size_t airan_pusch_chain_DeviceBytes(const AiranPuschChain *c) { return c == nullptr ? 0 : c->bytes; }
+
+int32_t airan_pusch_chain_SetRefiner(AiranPuschChain *c, AiranCeRefiner *refiner, float snr_min_db,
+                                     float snr_max_db)
+{
+    if (c == nullptr || (refiner != nullptr && !(snr_min_db <= snr_max_db))) return AIRAN_INVALID_ARGUMENT;
+    DropGraph(c);
+    c->refiner = refiner;
+    c->refiner_snr_min = snr_min_db;
+    c->refiner_snr_max = snr_max_db;
+    return AIRAN_OK;
+}
 
 int32_t airan_pusch_chain_SetTiming(AiranPuschChain *c, uint32_t enabled)
 {

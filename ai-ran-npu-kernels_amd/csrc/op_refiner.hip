@@ -1,0 +1,845 @@
+// op_refiner.hip -- learned channel-estimation refiner (include/airan_ce_refiner.h;
+// no counterpart in the reference).
+//
+// Tensors are NHWC bf16: [slots][14 symbols][width subcarriers][channels],
+// channels contiguous (32 or 64 = one 64/128-byte pixel vector).
+//
+// Convolution (forward, and the data gradient as a convolution with the
+// flipped, transposed weights): implicit GEMM on MFMA
+// v_mfma_f32_32x32x16_bf16, M = 32 pixels, N = 32 output channels, K = 16
+// input channels per instruction.  A workgroup (4 waves) owns a tile of one
+// symbol row: 64 pixels x 64 channels (or 128 x 32); each wave keeps the B
+// fragments of its 32 output channels for all 9 taps in registers for the
+// whole kernel (persistent grid), stages the input tile with its 1-pixel
+// halo (3 rows x (tile + 2) pixels) in LDS, and runs 9 * CIN / 16 MFMAs per
+// tile.  Weight gradients are fp32 VALU outer products over LDS tiles,
+// reduced across workgroups in a fixed order.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "airan_ce_refiner.h"
+#include "airan_pusch_rx.h"
+#include "common.h"
+#include "launch.h"
+
+using airan::Geom;
+
+namespace {
+
+using bf16 = __bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kLayers = 6;
+constexpr int kSym = 14;
+constexpr int kCh = 32;
+constexpr int kHid = 64;
+constexpr int kCin[kLayers] = {32, 64, 64, 64, 64, 64};
+constexpr int kCout[kLayers] = {64, 64, 64, 64, 64, 32};
+
+constexpr size_t ParamOffset(int layer)
+{
+    size_t off = 0;
+    for (int i = 0; i < layer; ++i) off += size_t(9) * kCout[i] * kCin[i] + kCout[i];
+    return off;
+}
+constexpr size_t kParams = ParamOffset(kLayers);
+static_assert(kParams == AIRAN_REFINER_PARAMS, "parameter count");
+constexpr size_t WeightCount(int layer) { return size_t(9) * kCout[layer] * kCin[layer]; }
+
+// ---------------------------------------------------------------- RNG
+__device__ inline uint64_t Mix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Two independent N(0, 1) values (Box-Muller) from a counter.
+__device__ inline float2 Gauss2(uint64_t seed, uint64_t ctr)
+{
+    const uint64_t r = Mix64(seed ^ Mix64(ctr));
+    const float u1 = (static_cast<float>(r >> 40) + 0.5f) * (1.0f / 16777216.0f);
+    const float u2 = static_cast<float>((r >> 16) & 0xffffffu) * (1.0f / 16777216.0f);
+    const float rad = sqrtf(-2.0f * logf(u1));
+    float sn, cs;
+    sincospif(2.0f * u2, &sn, &cs);
+    return make_float2(rad * cs, rad * sn);
+}
+
+// ----------------------------------------------------------- synthesis
+struct SynthArgs {
+    const float2 *gains;   // [slots][NR][L][taps][14]
+    const float *tau;      // [slots][taps]
+    const float *snr_db;   // [slots]
+    const __half *dmrs_re; // [L][D][ref_pad]
+    const __half *dmrs_im;
+    float *target;         // [slots][14][used][32]
+    __half *rx_re;         // [slots][NR][14][pad] (DMRS symbols written)
+    __half *rx_im;
+    uint32_t taps, used, pad, nr, nl, ndmrs, ref_pad;
+    uint32_t dmrs_sym[4], comb[4];
+    float scs;
+    uint64_t seed;
+};
+
+// True channel H[r][l][k][u] = sum_t g[r][l][t][k] exp(-j 2 pi (u - used/2) scs tau_t)
+// (the tapped-delay-line frequency response, subcarrier u at baseband
+// (u - used/2) scs) and the received DMRS symbols
+// Y[r][k][u] = sum_l H[r][l][k][u] x_l(u) + n.  Grid (used / 256, slots).
+__global__ void __launch_bounds__(256) SynthKernel(SynthArgs a)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = blockIdx.y;
+    if (u >= a.used) return;
+    float2 st[32];
+    for (uint32_t t = 0; t < a.taps; ++t) {
+        // Phase in cycles, reduced in double before the fp32 sincos.
+        const double cyc = (static_cast<double>(u) - 0.5 * a.used) * a.scs * a.tau[s * a.taps + t];
+        const float fr = static_cast<float>(cyc - floor(cyc));
+        float sn, cs;
+        sincospif(-2.0f * fr, &sn, &cs);
+        st[t] = make_float2(cs, sn);
+    }
+    const float sigma = sqrtf(0.5f * exp10f(-0.1f * a.snr_db[s]));
+    float *tg = a.target + (static_cast<size_t>(s) * kSym * a.used + u) * kCh;
+    for (uint32_t k = 0; k < kSym; ++k) {
+        int d = -1;
+        for (uint32_t j = 0; j < a.ndmrs; ++j)
+            if (a.dmrs_sym[j] == k) d = static_cast<int>(j);
+        float2 y[4] = {};
+        for (uint32_t r = 0; r < a.nr; ++r)
+            for (uint32_t l = 0; l < a.nl; ++l) {
+                const float2 *g = a.gains + ((((static_cast<size_t>(s) * a.nr + r) * a.nl + l) * a.taps) * kSym) + k;
+                float hr = 0.0f, hi = 0.0f;
+                for (uint32_t t = 0; t < a.taps; ++t) {
+                    const float2 gv = g[static_cast<size_t>(t) * kSym];
+                    hr += gv.x * st[t].x - gv.y * st[t].y;
+                    hi += gv.x * st[t].y + gv.y * st[t].x;
+                }
+                const uint32_t c = 2 * (r * a.nl + l);
+                tg[static_cast<size_t>(k) * a.used * kCh + c] = hr;
+                tg[static_cast<size_t>(k) * a.used * kCh + c + 1] = hi;
+                if (d >= 0 && u >= a.comb[l] && ((u - a.comb[l]) & 1u) == 0) {
+                    const size_t m = (static_cast<size_t>(l) * a.ndmrs + d) * a.ref_pad + (u - a.comb[l]) / 2;
+                    const float xr = __half2float(a.dmrs_re[m]), xi = __half2float(a.dmrs_im[m]);
+                    y[r].x += hr * xr - hi * xi;
+                    y[r].y += hr * xi + hi * xr;
+                }
+            }
+        if (d < 0) continue;
+        for (uint32_t r = 0; r < a.nr; ++r) {
+            const float2 n = Gauss2(a.seed, ((static_cast<uint64_t>(s) * 4 + r) * 4 + d) * 65536ull + u);
+            const size_t o = ((static_cast<size_t>(s) * a.nr + r) * kSym + k) * a.pad + u;
+            a.rx_re[o] = __float2half_rn(y[r].x + sigma * n.x);
+            a.rx_im[o] = __float2half_rn(y[r].y + sigma * n.y);
+        }
+    }
+}
+
+// ----------------------------------------------------------------- input
+struct InputArgs {
+    const __half *h_re;  // [slots][NR][L][D][pilot_pad]
+    const __half *h_im;
+    bf16 *x;             // [slots][14][used][32]
+    uint32_t used, nr, nl, ndmrs, pilot_pad;
+    uint32_t dmrs_sym[4], first[4], step[4], count[4];
+};
+
+// Network input: per link (rx, layer) and DMRS symbol, linear interpolation
+// of h_ls between the layer's pilot subcarriers (first + step p, constant
+// beyond the outermost pilots), then linear interpolation / extrapolation
+// in time through the DMRS symbols (constant for one DMRS symbol).
+// Grid (used / 256, slots).
+__global__ void __launch_bounds__(256) InputKernel(InputArgs a)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = blockIdx.y;
+    if (u >= a.used) return;
+    bf16 *xo = a.x + (static_cast<size_t>(s) * kSym * a.used + u) * kCh;
+    for (uint32_t r = 0; r < a.nr; ++r)
+        for (uint32_t l = 0; l < a.nl; ++l) {
+            float2 hd[4];
+            const float fp = (static_cast<float>(u) - static_cast<float>(a.first[l])) / static_cast<float>(a.step[l]);
+            const uint32_t cnt = a.count[l];
+            uint32_t p0;
+            float w;
+            if (fp <= 0.0f) {
+                p0 = 0;
+                w = 0.0f;
+            } else if (fp >= static_cast<float>(cnt - 1)) {
+                p0 = cnt - 2;
+                w = 1.0f;
+            } else {
+                p0 = static_cast<uint32_t>(fp);
+                w = fp - static_cast<float>(p0);
+            }
+            for (uint32_t d = 0; d < a.ndmrs; ++d) {
+                const size_t b = (((static_cast<size_t>(s) * a.nr + r) * a.nl + l) * a.ndmrs + d) * a.pilot_pad + p0;
+                const float r0 = __half2float(a.h_re[b]), r1 = __half2float(a.h_re[b + 1]);
+                const float i0 = __half2float(a.h_im[b]), i1 = __half2float(a.h_im[b + 1]);
+                hd[d] = make_float2(r0 + w * (r1 - r0), i0 + w * (i1 - i0));
+            }
+            const uint32_t c = 2 * (r * a.nl + l);
+            for (uint32_t k = 0; k < kSym; ++k) {
+                float2 v = hd[0];
+                if (a.ndmrs >= 2) {
+                    const float t0 = static_cast<float>(a.dmrs_sym[0]);
+                    const float t1 = static_cast<float>(a.dmrs_sym[a.ndmrs - 1]);
+                    const float wt = (static_cast<float>(k) - t0) / (t1 - t0);
+                    const float2 v1 = hd[a.ndmrs - 1];
+                    v = make_float2(hd[0].x + wt * (v1.x - hd[0].x), hd[0].y + wt * (v1.y - hd[0].y));
+                }
+                xo[static_cast<size_t>(k) * a.used * kCh + c] = static_cast<bf16>(v.x);
+                xo[static_cast<size_t>(k) * a.used * kCh + c + 1] = static_cast<bf16>(v.y);
+            }
+        }
+}
+
+// ----------------------------------------------------------------- conv
+enum { kEpiRelu = 0, kEpiResid = 1, kEpiMask = 2 };
+
+struct ConvArgs {
+    const bf16 *in;      // [S][14][W][CIN]
+    const bf16 *wt;      // [9][COUT][CIN]
+    const float *bias;   // [COUT] (kEpiRelu, kEpiResid)
+    const bf16 *aux;     // kEpiResid: X [S][14][W][COUT]; kEpiMask: activation [S][14][W][COUT]
+    bf16 *out;           // kEpiRelu, kEpiMask: [S][14][W][COUT]
+    float *out_f32;      // kEpiResid: [S][14][W][COUT]
+    uint32_t S, W;
+};
+
+// out[p][co] = epilogue(sum_{tap, ci} in[p + off(tap)][ci] wt[tap][co][ci]):
+//   kEpiRelu   bf16(max(acc + bias, 0))
+//   kEpiResid  acc + bias + aux          (fp32)
+//   kEpiMask   bf16(aux > 0 ? acc : 0)   (gradient through the ReLU)
+// MFMA 32x32x16 bf16 lane layout: A row (pixel) = lane % 32, its 8 K values
+// (channels) 8 (lane / 32) .. + 7; B column (output channel) = lane % 32,
+// same K; C column = lane % 32, row of accumulator i = 8 (i / 4) +
+// 4 (lane / 32) + i % 4.
+template <int CIN, int COUT, int EPI>
+__global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
+{
+    constexpr int CO_BLOCKS = COUT / 32;
+    constexpr int PX_WAVES = 4 / CO_BLOCKS;
+    constexpr int TILE = 32 * PX_WAVES;
+    constexpr int KS = CIN / 16;
+    constexpr int VPP = CIN / 8 + 1;  // bf16x8 vectors per LDS pixel (one of padding)
+    __shared__ bf16x8 tile[3][TILE + 2][VPP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wco = wave % CO_BLOCKS, wpx = wave / CO_BLOCKS;
+    const int co = wco * 32 + (lane & 31);
+    const int kh8 = lane >> 5;
+    bf16x8 wf[9][KS];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+            wf[tap][ks] = *reinterpret_cast<const bf16x8 *>(a.wt + (static_cast<size_t>(tap) * COUT + co) * CIN +
+                                                            ks * 16 + 8 * kh8);
+    const float bias = EPI == kEpiMask ? 0.0f : a.bias[co];
+    const uint32_t wtiles = (a.W + TILE - 1) / TILE;
+    const uint32_t ntiles = a.S * kSym * wtiles;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t wt_i = t % wtiles, sh = t / wtiles;
+        const uint32_t h = sh % kSym;
+        const size_t s = sh / kSym;
+        const int w0 = static_cast<int>(wt_i) * TILE;
+        constexpr int CH = CIN / 8;
+        for (int e = threadIdx.x; e < 3 * (TILE + 2) * CH; e += 256) {
+            const int c = e % CH, rp = e / CH, p = rp % (TILE + 2), r = rp / (TILE + 2);
+            const int hh = static_cast<int>(h) + r - 1, ww = w0 + p - 1;
+            bf16x8 v;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = static_cast<bf16>(0.0f);
+            if (hh >= 0 && hh < kSym && ww >= 0 && ww < static_cast<int>(a.W))
+                v = *reinterpret_cast<const bf16x8 *>(a.in + ((s * kSym + hh) * a.W + ww) * CIN + 8 * c);
+            tile[r][p][c] = v;
+        }
+        __syncthreads();
+        f32x16 acc;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+        const int pl = wpx * 32 + (lane & 31);
+#pragma unroll
+        for (int kh = 0; kh < 3; ++kh)
+#pragma unroll
+            for (int kw = 0; kw < 3; ++kw)
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(tile[kh][pl + kw][2 * ks + kh8],
+                                                                  wf[kh * 3 + kw][ks], acc, 0, 0, 0);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int m = 8 * (i >> 2) + 4 * kh8 + (i & 3);
+            const int w = w0 + wpx * 32 + m;
+            if (w >= static_cast<int>(a.W)) continue;
+            const size_t o = ((s * kSym + h) * a.W + w) * COUT + co;
+            const float v = acc[i];
+            if (EPI == kEpiRelu) {
+                a.out[o] = static_cast<bf16>(fmaxf(v + bias, 0.0f));
+            } else if (EPI == kEpiResid) {
+                a.out_f32[o] = v + bias + static_cast<float>(a.aux[o]);
+            } else {
+                a.out[o] = static_cast<float>(a.aux[o]) > 0.0f ? static_cast<bf16>(v) : static_cast<bf16>(0.0f);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------- weight gradient
+// Partial sums of one workgroup over its tiles (64 pixels of one symbol
+// row): part[tap][co][ci] = sum_p dz[p][co] a[p + off(tap)][ci] and
+// part[9 COUT CIN + co] = sum_p dz[p][co].  Thread: co = t % COUT and the
+// channel group g = t / COUT of CIN / (256 / COUT) input channels.
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) WgradKernel(const bf16 *__restrict__ act, const bf16 *__restrict__ dz,
+                                                   float *__restrict__ part, uint32_t S, uint32_t W)
+{
+    constexpr int TILE = 64;
+    constexpr int G = 256 / COUT;
+    constexpr int CG = CIN / G;
+    __shared__ float sa[3][TILE + 2][CIN];
+    __shared__ float sd[TILE][COUT];
+    const int t = threadIdx.x;
+    const int co = t % COUT, g = t / COUT;
+    float acc[9][CG];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+        for (int c = 0; c < CG; ++c) acc[k][c] = 0.0f;
+    float db = 0.0f;
+    const uint32_t wtiles = (W + TILE - 1) / TILE;
+    const uint32_t ntiles = S * kSym * wtiles;
+    for (uint32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const uint32_t wt_i = ti % wtiles, sh = ti / wtiles;
+        const uint32_t h = sh % kSym;
+        const size_t s = sh / kSym;
+        const int w0 = static_cast<int>(wt_i) * TILE;
+        for (int e = t; e < 3 * (TILE + 2) * CIN; e += 256) {
+            const int c = e % CIN, rp = e / CIN, p = rp % (TILE + 2), r = rp / (TILE + 2);
+            const int hh = static_cast<int>(h) + r - 1, ww = w0 + p - 1;
+            float v = 0.0f;
+            if (hh >= 0 && hh < kSym && ww >= 0 && ww < static_cast<int>(W))
+                v = static_cast<float>(act[((s * kSym + hh) * W + ww) * CIN + c]);
+            sa[r][p][c] = v;
+        }
+        for (int e = t; e < TILE * COUT; e += 256) {
+            const int c = e % COUT, p = e / COUT;
+            const int ww = w0 + p;
+            sd[p][c] = ww < static_cast<int>(W) ? static_cast<float>(dz[((s * kSym + h) * W + ww) * COUT + c]) : 0.0f;
+        }
+        __syncthreads();
+        for (int p = 0; p < TILE; ++p) {
+            const float d = sd[p][co];
+            db += d;
+#pragma unroll
+            for (int kh = 0; kh < 3; ++kh)
+#pragma unroll
+                for (int kw = 0; kw < 3; ++kw) {
+                    const float *ar = &sa[kh][p + kw][g * CG];
+#pragma unroll
+                    for (int c = 0; c < CG; ++c) acc[kh * 3 + kw][c] += d * ar[c];
+                }
+        }
+        __syncthreads();
+    }
+    float *o = part + static_cast<size_t>(blockIdx.x) * (9 * COUT * CIN + COUT);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+        for (int c = 0; c < CG; ++c) o[(static_cast<size_t>(k) * COUT + co) * CIN + g * CG + c] = acc[k][c];
+    if (g == 0) o[9 * COUT * CIN + co] = db;
+}
+
+// grad[i] = sum_b part[b][i] in block order.
+__global__ void __launch_bounds__(256) ReduceKernel(const float *__restrict__ part, uint32_t nb, uint32_t n,
+                                                    float *__restrict__ grad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (uint32_t b = 0; b < nb; ++b) s += part[static_cast<size_t>(b) * n + i];
+    grad[i] = s;
+}
+
+// ------------------------------------------------------------------ loss
+// dz = bf16(2 (out - target) / N); per-block sums of (out - target)^2.
+__global__ void __launch_bounds__(256) LossKernel(const float *__restrict__ out, const float *__restrict__ tgt,
+                                                  bf16 *__restrict__ dz, double *__restrict__ part, size_t n)
+{
+    __shared__ double red[4];
+    double acc = 0.0;
+    const float scale = static_cast<float>(2.0 / static_cast<double>(n));
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const float e = out[i] - tgt[i];
+        acc += static_cast<double>(e) * e;
+        dz[i] = static_cast<bf16>(e * scale);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(64) LossFinishKernel(const double *__restrict__ part, uint32_t nb, size_t n,
+                                                       float *__restrict__ loss)
+{
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (uint32_t b = 0; b < nb; ++b) s += part[b];
+    *loss = static_cast<float>(s / static_cast<double>(n));
+}
+
+// ------------------------------------------------------------- optimiser
+__global__ void __launch_bounds__(256) AdamKernel(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                                                  const float *__restrict__ g, uint32_t n, float lr, float bc1,
+                                                  float bc2)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + 1e-8f);
+}
+
+// bf16 copies of every layer's weights: forward [tap][co][ci] and the
+// data-gradient form [8 - tap][ci][co] (flipped taps, transposed channels).
+struct PackArgs {
+    uint32_t off[kLayers], cin[kLayers], cout[kLayers];
+};
+__global__ void __launch_bounds__(256) PackKernel(const float *__restrict__ p, bf16 *__restrict__ wf,
+                                                  bf16 *__restrict__ wb, PackArgs a)
+{
+    const uint32_t layer = blockIdx.y;
+    const uint32_t cin = a.cin[layer], cout = a.cout[layer];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 9 * cin * cout) return;
+    const uint32_t ci = i % cin, co = (i / cin) % cout, tap = i / (cin * cout);
+    const float v = p[a.off[layer] + i];
+    wf[a.off[layer] + i] = static_cast<bf16>(v);
+    wb[a.off[layer] + (static_cast<size_t>(8 - tap) * cin + ci) * cout + co] = static_cast<bf16>(v);
+}
+
+// ----------------------------------------------------------------- output
+// Network output [slots][14][used][32] -> channel grid fp16
+// [slots][NR][L][14][pad] (zero beyond used), slots with keep[s] != 0.
+__global__ void __launch_bounds__(256) OutToGridKernel(const float *__restrict__ out, const uint8_t *__restrict__ keep,
+                                                       __half *__restrict__ h_re, __half *__restrict__ h_im,
+                                                       uint32_t used, uint32_t pad, uint32_t nr, uint32_t nl)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = blockIdx.y, s = blockIdx.z;
+    if (u >= pad || (keep != nullptr && keep[s] == 0)) return;
+    const float *o = out + ((static_cast<size_t>(s) * kSym + k) * used + u) * kCh;
+    for (uint32_t r = 0; r < nr; ++r)
+        for (uint32_t l = 0; l < nl; ++l) {
+            const uint32_t c = 2 * (r * nl + l);
+            const size_t g = (((static_cast<size_t>(s) * nr + r) * nl + l) * kSym + k) * pad + u;
+            h_re[g] = __float2half_rn(u < used ? o[c] : 0.0f);
+            h_im[g] = __float2half_rn(u < used ? o[c + 1] : 0.0f);
+        }
+}
+
+void FreeDev(void *p)
+{
+    if (p != nullptr) (void)hipFree(p);
+}
+
+}  // namespace
+
+struct AiranCeRefiner {
+    int device = 0;
+    int cus = 256;
+    PuschMimoConfig cfg{};
+    Geom g{};
+    uint32_t max_slots = 0;
+    uint32_t step = 0;
+    float *params = nullptr, *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr;
+    bf16 *wf = nullptr, *wb = nullptr;
+    bf16 *x = nullptr;
+    float *target = nullptr, *out = nullptr;
+    bf16 *act[kLayers] = {};  // act[1..5] hidden activations (act[0] unused: the input is x)
+    bf16 *dz[2] = {};
+    float *part = nullptr;
+    size_t part_floats = 0;
+    double *loss_part = nullptr;
+    float *loss_dev = nullptr;
+    float2 *gains = nullptr;
+    float *tau = nullptr, *snr = nullptr;
+    uint32_t max_taps = 32;
+    __half *dmrs_re = nullptr, *dmrs_im = nullptr;
+    __half *rx_re = nullptr, *rx_im = nullptr;
+    __half *hls_re = nullptr, *hls_im = nullptr, *noise = nullptr;
+};
+
+namespace {
+
+constexpr uint32_t kLossBlocks = 256;
+
+template <int CIN, int COUT, int EPI>
+void LaunchConv(const AiranCeRefiner *r, const ConvArgs &a, hipStream_t s)
+{
+    constexpr int TILE = 32 * (4 / (COUT / 32));
+    const uint32_t ntiles = a.S * kSym * ((a.W + TILE - 1) / TILE);
+    const uint32_t grid = std::min<uint32_t>(ntiles, static_cast<uint32_t>(r->cus) * 2);
+    hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI>), dim3(grid), dim3(256), 0, s, a);
+}
+
+// Forward of the network on x [S][14][W][32] -> out [S][14][W][32] fp32,
+// keeping the hidden activations.
+void Forward(const AiranCeRefiner *r, const bf16 *x, uint32_t S, uint32_t W, hipStream_t s)
+{
+    ConvArgs a{};
+    a.S = S;
+    a.W = W;
+    const bf16 *in = x;
+    for (int i = 0; i < kLayers; ++i) {
+        a.in = in;
+        a.wt = r->wf + ParamOffset(i);
+        a.bias = r->params + ParamOffset(i) + WeightCount(i);
+        if (i < kLayers - 1) {
+            a.out = r->act[i + 1];
+            if (i == 0)
+                LaunchConv<32, 64, kEpiRelu>(r, a, s);
+            else
+                LaunchConv<64, 64, kEpiRelu>(r, a, s);
+            in = r->act[i + 1];
+        } else {
+            a.aux = x;
+            a.out_f32 = r->out;
+            LaunchConv<64, 32, kEpiResid>(r, a, s);
+        }
+    }
+}
+
+template <int CIN, int COUT>
+void LaunchWgrad(const AiranCeRefiner *r, const bf16 *act, const bf16 *dz, uint32_t S, uint32_t W, int layer,
+                 hipStream_t s)
+{
+    const uint32_t ntiles = S * kSym * ((W + 63) / 64);
+    const uint32_t nb = std::min<uint32_t>(ntiles, static_cast<uint32_t>(r->cus) * 2);
+    hipLaunchKernelGGL((WgradKernel<CIN, COUT>), dim3(nb), dim3(256), 0, s, act, dz, r->part, S, W);
+    const uint32_t n = static_cast<uint32_t>(WeightCount(layer) + kCout[layer]);
+    hipLaunchKernelGGL(ReduceKernel, dim3(airan::CeilDiv(n, 256)), dim3(256), 0, s, r->part, nb, n,
+                       r->grad + ParamOffset(layer));
+}
+
+// Loss and gradients of all parameters (after Forward on the same x).
+void Backward(const AiranCeRefiner *r, const bf16 *x, const float *target, uint32_t S, uint32_t W, float *loss,
+              hipStream_t s)
+{
+    const size_t n = static_cast<size_t>(S) * kSym * W * kCh;
+    hipLaunchKernelGGL(LossKernel, dim3(kLossBlocks), dim3(256), 0, s, r->out, target, r->dz[0], r->loss_part, n);
+    hipLaunchKernelGGL(LossFinishKernel, dim3(1), dim3(64), 0, s, r->loss_part, kLossBlocks, n,
+                       loss != nullptr ? loss : r->loss_dev);
+    int cur = 0;
+    for (int i = kLayers - 1; i >= 0; --i) {
+        const bf16 *a_in = i == 0 ? x : r->act[i];
+        const bf16 *dzi = r->dz[cur];
+        if (i == kLayers - 1)
+            LaunchWgrad<64, 32>(r, a_in, dzi, S, W, i, s);
+        else if (i == 0)
+            LaunchWgrad<32, 64>(r, a_in, dzi, S, W, i, s);
+        else
+            LaunchWgrad<64, 64>(r, a_in, dzi, S, W, i, s);
+        if (i == 0) break;
+        ConvArgs a{};
+        a.S = S;
+        a.W = W;
+        a.in = dzi;
+        a.wt = r->wb + ParamOffset(i);
+        a.aux = r->act[i];
+        a.out = r->dz[cur ^ 1];
+        if (i == kLayers - 1)
+            LaunchConv<32, 64, kEpiMask>(r, a, s);
+        else
+            LaunchConv<64, 64, kEpiMask>(r, a, s);
+        cur ^= 1;
+    }
+}
+
+void Pack(const AiranCeRefiner *r, hipStream_t s)
+{
+    PackArgs a{};
+    for (int i = 0; i < kLayers; ++i) {
+        a.off[i] = static_cast<uint32_t>(ParamOffset(i));
+        a.cin[i] = kCin[i];
+        a.cout[i] = kCout[i];
+    }
+    hipLaunchKernelGGL(PackKernel, dim3(airan::CeilDiv(9 * 64 * 64, 256), kLayers), dim3(256), 0, s, r->params, r->wf,
+                       r->wb, a);
+}
+
+bool ShapeOk(const Geom &g) { return g.nr == 4 && g.nl == 4 && g.ndmrs >= 1 && g.ndmrs <= 2; }
+
+InputArgs MakeInput(const AiranCeRefiner *r, const __half *hr, const __half *hi, bf16 *x)
+{
+    const Geom &g = r->g;
+    InputArgs a{};
+    a.h_re = hr;
+    a.h_im = hi;
+    a.x = x;
+    a.used = g.used;
+    a.nr = g.nr;
+    a.nl = g.nl;
+    a.ndmrs = g.ndmrs;
+    a.pilot_pad = g.pilot_pad;
+    for (uint32_t d = 0; d < g.ndmrs; ++d) a.dmrs_sym[d] = g.dmrs_sym[d];
+    for (uint32_t l = 0; l < g.nl; ++l) {
+        a.first[l] = g.comb[l] + (g.occ[l] ? 1u : 0u);
+        a.step[l] = g.occ[l] ? 4u : 2u;
+        a.count[l] = g.occ[l] ? g.n_occ : g.n_dmrs_re;
+    }
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t airan_ce_refiner_Create(const PuschMimoConfig *config, uint32_t max_slots, const float *params, uint64_t seed,
+                                void *stream, AiranCeRefiner **refiner)
+{
+    if (config == nullptr || refiner == nullptr || stream == nullptr || max_slots == 0 || max_slots > 1024)
+        return AIRAN_INVALID_ARGUMENT;
+    *refiner = nullptr;
+    Geom g;
+    int32_t st = airan::DeriveGeom(*config, &g);
+    if (st != AIRAN_OK) return st;
+    if (!ShapeOk(g)) return AIRAN_UNSUPPORTED_PROFILE;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto *r = new AiranCeRefiner{};
+    r->cfg = *config;
+    r->g = g;
+    r->max_slots = max_slots;
+    auto fail = [&](int32_t code) {
+        airan_ce_refiner_Destroy(r);
+        return code;
+    };
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&r->device) != hipSuccess || hipGetDeviceProperties(&prop, r->device) != hipSuccess)
+        return fail(AIRAN_LAUNCH_FAILED);
+    r->cus = prop.multiProcessorCount;
+    const size_t px = static_cast<size_t>(max_slots) * kSym * g.used;
+    const size_t nwg = static_cast<size_t>(r->cus) * 2;
+    r->part_floats = nwg * (9 * 64 * 64 + 64);
+    auto alloc = [](auto **p, size_t bytes) { return hipMalloc(reinterpret_cast<void **>(p), bytes) == hipSuccess; };
+    bool ok = alloc(&r->params, kParams * 4) && alloc(&r->adam_m, kParams * 4) && alloc(&r->adam_v, kParams * 4) &&
+              alloc(&r->grad, kParams * 4) && alloc(&r->wf, kParams * 2) && alloc(&r->wb, kParams * 2) &&
+              alloc(&r->x, px * kCh * 2) && alloc(&r->target, px * kCh * 4) && alloc(&r->out, px * kCh * 4) &&
+              alloc(&r->dz[0], px * kHid * 2) && alloc(&r->dz[1], px * kHid * 2) &&
+              alloc(&r->part, r->part_floats * 4) && alloc(&r->loss_part, kLossBlocks * 8) &&
+              alloc(&r->loss_dev, 4) &&
+              alloc(&r->gains, static_cast<size_t>(max_slots) * g.nr * g.nl * r->max_taps * kSym * 8) &&
+              alloc(&r->tau, static_cast<size_t>(max_slots) * r->max_taps * 4) && alloc(&r->snr, max_slots * 4) &&
+              alloc(&r->dmrs_re, static_cast<size_t>(g.nl) * g.ndmrs * g.ref_pad * 2) &&
+              alloc(&r->dmrs_im, static_cast<size_t>(g.nl) * g.ndmrs * g.ref_pad * 2) &&
+              alloc(&r->rx_re, static_cast<size_t>(max_slots) * g.nr * kSym * g.pad * 2) &&
+              alloc(&r->rx_im, static_cast<size_t>(max_slots) * g.nr * kSym * g.pad * 2) &&
+              alloc(&r->hls_re, static_cast<size_t>(max_slots) * g.nr * g.nl * g.ndmrs * g.pilot_pad * 2) &&
+              alloc(&r->hls_im, static_cast<size_t>(max_slots) * g.nr * g.nl * g.ndmrs * g.pilot_pad * 2) &&
+              alloc(&r->noise, static_cast<size_t>(max_slots) * g.nr * 2);
+    for (int i = 1; i < kLayers && ok; ++i) ok = alloc(&r->act[i], px * kHid * 2);
+    if (!ok) return fail(AIRAN_RESOURCE_TOO_SMALL);
+    // Initial weights: given, or He-normal (std sqrt(2 / (9 cin))) with zero
+    // biases from a host generator seeded by `seed`.
+    std::vector<float> p(kParams, 0.0f);
+    if (params != nullptr) {
+        std::copy(params, params + kParams, p.begin());
+    } else {
+        uint64_t z = seed;
+        auto next = [&z]() {
+            z += 0x9e3779b97f4a7c15ull;
+            uint64_t x = z;
+            x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+            x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+            return x ^ (x >> 31);
+        };
+        for (int i = 0; i < kLayers; ++i) {
+            const double sd = std::sqrt(2.0 / (9.0 * kCin[i]));
+            for (size_t k = 0; k < WeightCount(i); ++k) {
+                const double u1 = (static_cast<double>(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+                const double u2 = static_cast<double>(next() >> 11) * (1.0 / 9007199254740992.0);
+                p[ParamOffset(i) + k] =
+                    static_cast<float>(sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2));
+            }
+        }
+        // The last layer starts near zero so the untrained network is close
+        // to the identity (the LS interpolation).
+        for (size_t k = 0; k < WeightCount(kLayers - 1); ++k) p[ParamOffset(kLayers - 1) + k] *= 0.01f;
+    }
+    MimoDmrsGenRuntimeV1 *rt = nullptr;
+    if ((st = airan_mimo_dmrs_gen_CreateRuntime(&rt)) != AIRAN_OK) return fail(st);
+    PuschMimoLayout lay{};
+    airan_DerivePuschMimoLayout(config, &lay);
+    MimoDmrsGenOpArgsV1 ga{};
+    ga.abi_version = AIRAN_PUSCH_MIMO_ABI_VERSION;
+    ga.struct_size = sizeof(ga);
+    ga.dmrs_re = r->dmrs_re;
+    ga.dmrs_im = r->dmrs_im;
+    ga.config = config;
+    ga.layout = &lay;
+    ga.stream = stream;
+    st = airan_mimo_dmrs_gen_Enqueue(rt, &ga);
+    const bool copy_ok =
+        hipMemcpyAsync(r->params, p.data(), kParams * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+        hipMemsetAsync(r->adam_m, 0, kParams * 4, s) == hipSuccess &&
+        hipMemsetAsync(r->adam_v, 0, kParams * 4, s) == hipSuccess &&
+        hipMemsetAsync(r->rx_re, 0, static_cast<size_t>(max_slots) * g.nr * kSym * g.pad * 2, s) == hipSuccess &&
+        hipMemsetAsync(r->rx_im, 0, static_cast<size_t>(max_slots) * g.nr * kSym * g.pad * 2, s) == hipSuccess;
+    if (st == AIRAN_OK && copy_ok) Pack(r, s);
+    const bool sync_ok = hipStreamSynchronize(s) == hipSuccess;
+    airan_mimo_dmrs_gen_DestroyRuntime(rt);
+    if (st != AIRAN_OK) return fail(st);
+    if (!copy_ok || !sync_ok) return fail(AIRAN_LAUNCH_FAILED);
+    *refiner = r;
+    return AIRAN_OK;
+}
+
+void airan_ce_refiner_Destroy(AiranCeRefiner *r)
+{
+    if (r == nullptr) return;
+    void *bufs[] = {r->params, r->adam_m, r->adam_v, r->grad, r->wf, r->wb, r->x, r->target, r->out, r->dz[0],
+                    r->dz[1], r->part, r->loss_part, r->loss_dev, r->gains, r->tau, r->snr, r->dmrs_re, r->dmrs_im,
+                    r->rx_re, r->rx_im, r->hls_re, r->hls_im, r->noise};
+    for (void *b : bufs) FreeDev(b);
+    for (bf16 *a : r->act) FreeDev(a);
+    delete r;
+}
+
+int32_t airan_ce_refiner_GetParams(AiranCeRefiner *r, float *params)
+{
+    if (r == nullptr || params == nullptr) return AIRAN_INVALID_ARGUMENT;
+    return hipMemcpy(params, r->params, kParams * 4, hipMemcpyDeviceToHost) == hipSuccess ? AIRAN_OK
+                                                                                          : AIRAN_LAUNCH_FAILED;
+}
+
+int32_t airan_ce_refiner_SetParams(AiranCeRefiner *r, const float *params)
+{
+    if (r == nullptr || params == nullptr) return AIRAN_INVALID_ARGUMENT;
+    if (hipMemcpy(r->params, params, kParams * 4, hipMemcpyHostToDevice) != hipSuccess) return AIRAN_LAUNCH_FAILED;
+    Pack(r, nullptr);
+    return hipDeviceSynchronize() == hipSuccess ? AIRAN_OK : AIRAN_LAUNCH_FAILED;
+}
+
+int32_t airan_ce_refiner_Enqueue(AiranCeRefiner *r, const void *h_ls_re, const void *h_ls_im, uint32_t slots,
+                                 const uint8_t *keep, void *h_re, void *h_im, void *stream)
+{
+    if (r == nullptr || h_ls_re == nullptr || h_ls_im == nullptr || h_re == nullptr || h_im == nullptr ||
+        stream == nullptr || slots == 0 || slots > r->max_slots)
+        return AIRAN_INVALID_ARGUMENT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Geom &g = r->g;
+    const InputArgs ia = MakeInput(r, static_cast<const __half *>(h_ls_re), static_cast<const __half *>(h_ls_im), r->x);
+    hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
+    Forward(r, r->x, slots, g.used, s);
+    hipLaunchKernelGGL(OutToGridKernel, dim3(airan::CeilDiv(g.pad, 256), kSym, slots), dim3(256), 0, s, r->out, keep,
+                       static_cast<__half *>(h_re), static_cast<__half *>(h_im), g.used, g.pad, g.nr, g.nl);
+    return airan::LaunchStatus();
+}
+
+int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *r, const float *gains, const float *tap_delay_s, uint32_t taps,
+                                    const float *snr_db, uint32_t slots, float scs_hz, uint64_t seed, void *stream)
+{
+    if (r == nullptr || gains == nullptr || tap_delay_s == nullptr || snr_db == nullptr || stream == nullptr ||
+        slots == 0 || slots > r->max_slots || taps == 0 || taps > r->max_taps || !(scs_hz > 0.0f))
+        return AIRAN_INVALID_ARGUMENT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Geom &g = r->g;
+    const size_t ng = static_cast<size_t>(slots) * g.nr * g.nl * taps * kSym;
+    if (hipMemcpyAsync(r->gains, gains, ng * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(r->tau, tap_delay_s, static_cast<size_t>(slots) * taps * 4, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(r->snr, snr_db, slots * 4, hipMemcpyHostToDevice, s) != hipSuccess)
+        return AIRAN_LAUNCH_FAILED;
+    SynthArgs a{};
+    a.gains = r->gains;
+    a.tau = r->tau;
+    a.snr_db = r->snr;
+    a.dmrs_re = r->dmrs_re;
+    a.dmrs_im = r->dmrs_im;
+    a.target = r->target;
+    a.rx_re = r->rx_re;
+    a.rx_im = r->rx_im;
+    a.taps = taps;
+    a.used = g.used;
+    a.pad = g.pad;
+    a.nr = g.nr;
+    a.nl = g.nl;
+    a.ndmrs = g.ndmrs;
+    a.ref_pad = g.ref_pad;
+    for (uint32_t d = 0; d < g.ndmrs; ++d) a.dmrs_sym[d] = g.dmrs_sym[d];
+    for (uint32_t l = 0; l < g.nl; ++l) a.comb[l] = g.comb[l];
+    a.scs = scs_hz;
+    a.seed = seed;
+    hipLaunchKernelGGL(SynthKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, a);
+    int32_t st = airan::LaunchDmrsLs(g, r->rx_re, r->rx_im, r->dmrs_re, r->dmrs_im, r->hls_re, r->hls_im, r->noise,
+                                     slots, s);
+    if (st != AIRAN_OK) return st;
+    const InputArgs ia = MakeInput(r, r->hls_re, r->hls_im, r->x);
+    hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
+    return airan::LaunchStatus();
+}
+
+int32_t airan_ce_refiner_Batch(const AiranCeRefiner *r, AiranCeRefinerBatchV1 *b)
+{
+    if (r == nullptr || b == nullptr || b->abi_version != AIRAN_PUSCH_MIMO_ABI_VERSION ||
+        b->struct_size < sizeof(AiranCeRefinerBatchV1))
+        return AIRAN_INVALID_ARGUMENT;
+    b->x = r->x;
+    b->target = r->target;
+    b->out = r->out;
+    b->h_ls_re = r->hls_re;
+    b->h_ls_im = r->hls_im;
+    b->noise_ls = r->noise;
+    return AIRAN_OK;
+}
+
+int32_t airan_ce_refiner_TrainStep(AiranCeRefiner *r, uint32_t slots, float lr, float *loss_out, void *stream)
+{
+    if (r == nullptr || stream == nullptr || slots == 0 || slots > r->max_slots || !(lr > 0.0f))
+        return AIRAN_INVALID_ARGUMENT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t W = r->g.used;
+    Forward(r, r->x, slots, W, s);
+    Backward(r, r->x, r->target, slots, W, loss_out, s);
+    ++r->step;
+    const float bc1 = 1.0f - std::pow(0.9f, static_cast<float>(r->step));
+    const float bc2 = 1.0f - std::pow(0.999f, static_cast<float>(r->step));
+    hipLaunchKernelGGL(AdamKernel, dim3(airan::CeilDiv(kParams, 256)), dim3(256), 0, s, r->params, r->adam_m,
+                       r->adam_v, r->grad, static_cast<uint32_t>(kParams), lr, bc1, bc2);
+    Pack(r, s);
+    return airan::LaunchStatus();
+}
+
+int32_t airan_ce_refiner_Forward(AiranCeRefiner *r, uint32_t slots, void *stream)
+{
+    if (r == nullptr || stream == nullptr || slots == 0 || slots > r->max_slots) return AIRAN_INVALID_ARGUMENT;
+    Forward(r, r->x, slots, r->g.used, static_cast<hipStream_t>(stream));
+    return airan::LaunchStatus();
+}
+
+int32_t airan_ce_refiner_Gradients(AiranCeRefiner *r, const void *x, const void *target, uint32_t slots,
+                                   uint32_t width, float *grad, float *loss, void *stream)
+{
+    if (r == nullptr || x == nullptr || target == nullptr || grad == nullptr || loss == nullptr ||
+        stream == nullptr || slots == 0 || slots > r->max_slots || width == 0 || width > r->g.used)
+        return AIRAN_INVALID_ARGUMENT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bf16 *xb = static_cast<const bf16 *>(x);
+    Forward(r, xb, slots, width, s);
+    Backward(r, xb, static_cast<const float *>(target), slots, width, loss, s);
+    if (hipMemcpyAsync(grad, r->grad, kParams * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return AIRAN_LAUNCH_FAILED;
+    return airan::LaunchStatus();
+}
+
+}  // extern "C"

@@ -272,6 +272,12 @@ P = C.POINTER
 
 # name -> (restype, argtypes).  Every entry is an exported symbol of the
 # library; tests/test_abi.py checks the list against include/*.h.
+class AiranCeRefinerBatchV1(C.Structure):
+    _fields_ = [("abi_version", C.c_uint16), ("struct_size", C.c_uint16), ("x", C.c_void_p),
+                ("target", C.c_void_p), ("out", C.c_void_p), ("h_ls_re", C.c_void_p), ("h_ls_im", C.c_void_p),
+                ("noise_ls", C.c_void_p)]
+
+
 PROTOTYPES = {
     "airan_InitDevice": (i32, []),
     "airan_ValidateMimoRuntimeConfig": (i32, [P(PuschMimoRuntimeConfig), C.c_char_p, C.c_size_t]),
@@ -365,7 +371,21 @@ PROTOTYPES = {
     "airan_pusch_chain_DeviceBytes": (C.c_size_t, [vp]),
     "airan_pusch_chain_SetTiming": (i32, [vp, u32]),
     "airan_pusch_chain_StageTimes": (i32, [vp, P(C.c_float), u32]),
+    "airan_pusch_chain_SetRefiner": (i32, [vp, vp, C.c_float, C.c_float]),
+    # learned CE refiner (include/airan_ce_refiner.h)
+    "airan_ce_refiner_Create": (i32, [P(PuschMimoConfig), u32, vp, C.c_uint64, vp, P(vp)]),
+    "airan_ce_refiner_Destroy": (None, [vp]),
+    "airan_ce_refiner_GetParams": (i32, [vp, vp]),
+    "airan_ce_refiner_SetParams": (i32, [vp, vp]),
+    "airan_ce_refiner_Enqueue": (i32, [vp, vp, vp, u32, vp, vp, vp, vp]),
+    "airan_ce_refiner_SynthBatch": (i32, [vp, vp, vp, u32, vp, u32, C.c_float, C.c_uint64, vp]),
+    "airan_ce_refiner_Batch": (i32, [vp, P(AiranCeRefinerBatchV1)]),
+    "airan_ce_refiner_TrainStep": (i32, [vp, u32, C.c_float, vp, vp]),
+    "airan_ce_refiner_Forward": (i32, [vp, u32, vp]),
+    "airan_ce_refiner_Gradients": (i32, [vp, vp, vp, u32, u32, vp, vp, vp]),
 }
+
+REFINER_PARAMS = 184672
 
 CHAIN_STAGES = ("ofdm_demod", "dmrs_ls", "ce_lmmse", "detect_demap", "rate_dematch", "ldpc_decode", "crc_check")
 

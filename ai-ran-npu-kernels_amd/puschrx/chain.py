@@ -134,6 +134,13 @@ class NativeChain:
             raise ops.AiranError("hipMemcpyAsync", rc)
         return out
 
+    def set_refiner(self, refiner, snr_min_db: float = -100.0, snr_max_db: float = 100.0):
+        """Attach a puschrx.refiner.Refiner (None detaches): slots whose SNR
+        estimate lies in [snr_min_db, snr_max_db] use its channel estimate."""
+        h = refiner.handle if refiner is not None else self._C.c_void_p()
+        ops._check("chain SetRefiner", self.lib.airan_pusch_chain_SetRefiner(self.handle, h, snr_min_db, snr_max_db))
+        self._refiner = refiner
+
     def device_bytes(self) -> int:
         return int(self.lib.airan_pusch_chain_DeviceBytes(self.handle))
 

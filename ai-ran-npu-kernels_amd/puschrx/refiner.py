@@ -1,0 +1,225 @@
+"""Learned channel-estimation refiner (include/airan_ce_refiner.h): batches
+synthesised on the device, training with the library's BF16 kernels, and
+NMSE evaluation against the LMMSE estimator on the same slots.
+
+Training slots draw, per slot, an SNR, an RMS delay spread, a maximum
+Doppler and an Rx correlation from the TrainSpec ranges; the tapped-delay-
+line gains follow the same model as synth.channel (uniform delay grid over
+4x the delay spread, exponential power-delay profile, sum-of-sinusoids
+Jakes fading per OFDM symbol, exponential Kronecker Rx correlation).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import abi, ops
+from .chain import _hip
+
+WEIGHTS = Path(__file__).resolve().parents[1] / "weights" / "ce_refiner_4x4.f32"
+
+
+@dataclass
+class TrainSpec:
+    snr_db: tuple = (20.0, 46.0)
+    delay_spread_s: tuple = (30e-9, 500e-9)
+    doppler_hz: tuple = (0.0, 500.0)
+    rx_corr: tuple = (0.0, 0.7)
+    taps: int = 16
+    scs_hz: float = 30e3
+    sinusoids: int = 16
+
+
+def tdl_gains(rng: np.random.Generator, nr: int, nl: int, taps: int, delay_spread: np.ndarray,
+              doppler: np.ndarray, rx_corr: np.ndarray, scs_hz: float, sinusoids: int = 16):
+    """Per-slot gains complex64 [S][NR][L][taps][14] and tap delays float32 [S][taps]."""
+    S = delay_spread.size
+    delays = np.arange(taps)[None, :] * (4.0 * delay_spread[:, None] / taps)           # [S, T]
+    pdp = np.exp(-delays / delay_spread[:, None])
+    pdp /= pdp.sum(axis=1, keepdims=True)
+    tsym = (1.0 + 9.0 / 128.0) / scs_hz
+    t = np.arange(14) * tsym                                                          # [14]
+    M = sinusoids
+    alpha = rng.uniform(0, 2 * np.pi, (S, nr, nl, taps, M))
+    phi = rng.uniform(0, 2 * np.pi, (S, nr, nl, taps, M))
+    fd = doppler[:, None, None, None, None] * np.cos(alpha)
+    ph = 2 * np.pi * fd[..., None] * t + phi[..., None]                              # [S,r,l,T,M,14]
+    g = np.exp(1j * ph).sum(axis=4) / np.sqrt(M)                                      # [S,r,l,T,14]
+    g *= np.sqrt(pdp)[:, None, None, :, None]
+    out = np.empty_like(g)
+    for s in range(S):
+        idx = np.arange(nr)
+        R = rx_corr[s] ** np.abs(idx[:, None] - idx[None, :])
+        w, v = np.linalg.eigh(R)
+        rsq = (v * np.sqrt(np.maximum(w, 0.0))) @ v.T
+        out[s] = np.einsum("ij,jltk->iltk", rsq, g[s])
+    return out.astype(np.complex64), delays.astype(np.float32)
+
+
+def nmse_db(est: torch.Tensor, ref: torch.Tensor) -> float:
+    e = (est.double() - ref.double()).pow(2).sum()
+    return float(10.0 * torch.log10(e / ref.double().pow(2).sum()).item())
+
+
+class Refiner:
+    def __init__(self, cfg, max_slots: int, params: np.ndarray | None = None, seed: int = 1):
+        self.lib = abi.load_library()
+        self.cfg = cfg
+        self.g = ops.Geometry(cfg)
+        self.max_slots = max_slots
+        self.handle = C.c_void_p()
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        ops._check("refiner Create", self.lib.airan_ce_refiner_Create(
+            C.byref(cfg), max_slots, None if p is None else p.ctypes.data_as(C.c_void_p), seed,
+            C.c_void_p(ops.stream_ptr()), C.byref(self.handle)))
+        self._loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+
+    # ---------------------------------------------------------------- params
+    def params(self) -> np.ndarray:
+        p = np.zeros(abi.REFINER_PARAMS, np.float32)
+        ops._check("refiner GetParams", self.lib.airan_ce_refiner_GetParams(self.handle,
+                                                                          p.ctypes.data_as(C.c_void_p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == abi.REFINER_PARAMS
+        ops._check("refiner SetParams", self.lib.airan_ce_refiner_SetParams(self.handle,
+                                                                          p.ctypes.data_as(C.c_void_p)))
+
+    def save(self, path: Path = WEIGHTS) -> None:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        self.params().tofile(path)
+
+    @staticmethod
+    def load_params(path: Path = WEIGHTS) -> np.ndarray:
+        p = np.fromfile(path, np.float32)
+        if p.size != abi.REFINER_PARAMS:
+            raise ValueError(f"{path}: {p.size} values, expected {abi.REFINER_PARAMS}")
+        return p
+
+    # ----------------------------------------------------------------- batches
+    def synth(self, rng: np.random.Generator, slots: int, spec: TrainSpec, seed: int,
+              snr_db: np.ndarray | None = None, delay_spread: np.ndarray | None = None,
+              doppler: np.ndarray | None = None, rx_corr: np.ndarray | None = None) -> None:
+        u = lambda r: rng.uniform(r[0], r[1], slots)  # noqa: E731
+        snr = u(spec.snr_db) if snr_db is None else np.broadcast_to(snr_db, slots)
+        ds = u(spec.delay_spread_s) if delay_spread is None else np.broadcast_to(delay_spread, slots)
+        fd = u(spec.doppler_hz) if doppler is None else np.broadcast_to(doppler, slots)
+        rc = u(spec.rx_corr) if rx_corr is None else np.broadcast_to(rx_corr, slots)
+        gains, tau = tdl_gains(rng, self.g.nr, self.g.nl, spec.taps, np.asarray(ds, np.float64),
+                               np.asarray(fd, np.float64), np.asarray(rc, np.float64), spec.scs_hz, spec.sinusoids)
+        snr = np.ascontiguousarray(snr, np.float32)
+        self._keep = (gains, tau, snr)  # host buffers stay alive until the copies complete
+        ops._check("refiner SynthBatch", self.lib.airan_ce_refiner_SynthBatch(
+            self.handle, gains.ctypes.data_as(C.c_void_p), tau.ctypes.data_as(C.c_void_p), spec.taps,
+            snr.ctypes.data_as(C.c_void_p), slots, spec.scs_hz, seed, C.c_void_p(ops.stream_ptr())))
+        torch.cuda.current_stream().synchronize()
+
+    def batch(self) -> abi.AiranCeRefinerBatchV1:
+        b = abi.AiranCeRefinerBatchV1()
+        b.abi_version, b.struct_size = abi.ABI_VERSION, C.sizeof(b)
+        ops._check("refiner Batch", self.lib.airan_ce_refiner_Batch(self.handle, C.byref(b)))
+        return b
+
+    def fetch(self, name: str, dtype, count: int) -> torch.Tensor:
+        src = getattr(self.batch(), name)
+        out = torch.empty(count, dtype=dtype, device="cuda")
+        rc = _hip().hipMemcpyAsync(out.data_ptr(), src, count * out.element_size(), 3, ops.stream_ptr())
+        if rc != 0:
+            raise ops.AiranError("hipMemcpyAsync", rc)
+        return out
+
+    # ---------------------------------------------------------------- training
+    def train_step(self, slots: int, lr: float) -> torch.Tensor:
+        ops._check("refiner TrainStep", self.lib.airan_ce_refiner_TrainStep(
+            self.handle, slots, lr, C.c_void_p(self._loss.data_ptr()), C.c_void_p(ops.stream_ptr())))
+        return self._loss
+
+    def forward(self, slots: int) -> None:
+        ops._check("refiner Forward", self.lib.airan_ce_refiner_Forward(self.handle, slots,
+                                                                      C.c_void_p(ops.stream_ptr())))
+
+    def gradients(self, x: torch.Tensor, target: torch.Tensor, slots: int, width: int):
+        grad = torch.zeros(abi.REFINER_PARAMS, dtype=torch.float32, device="cuda")
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        ops._check("refiner Gradients", self.lib.airan_ce_refiner_Gradients(
+            self.handle, C.c_void_p(x.data_ptr()), C.c_void_p(target.data_ptr()), slots, width,
+            C.c_void_p(grad.data_ptr()), C.c_void_p(loss.data_ptr()), C.c_void_p(ops.stream_ptr())))
+        return grad, loss
+
+    def train(self, rng: np.random.Generator, steps: int, slots: int, spec: TrainSpec, lr: float = 1e-3,
+              log_every: int = 0, seed0: int = 1000, lr_final: float | None = None):
+        """Adam on freshly synthesised batches; returns the loss history."""
+        hist = []
+        for i in range(steps):
+            self.synth(rng, slots, spec, seed0 + i)
+            a = lr if lr_final is None else lr * (lr_final / lr) ** (i / max(steps - 1, 1))
+            loss = self.train_step(slots, a)
+            if log_every and (i % log_every == 0 or i == steps - 1):
+                hist.append((i, float(loss.item())))
+                print(f"[refiner] step {i} loss {hist[-1][1]:.3e}", flush=True)
+        return hist
+
+    # -------------------------------------------------------------- evaluation
+    def evaluate(self, slots: int, lmmse: ops.LmmseFilter | None = None) -> dict:
+        """NMSE (dB) against the true channel of the synthesised batch: the LS
+        interpolation (network input), the refiner, and (with a filter) LMMSE."""
+        n = slots * 14 * self.g.used * 32
+        self.forward(slots)
+        x = self.fetch("x", torch.bfloat16, n).float()
+        t = self.fetch("target", torch.float32, n)
+        o = self.fetch("out", torch.float32, n)
+        res = {"ls_interp": nmse_db(x, t), "refiner": nmse_db(o, t)}
+        if lmmse is not None:
+            res["lmmse"] = nmse_db(self.lmmse_grid(slots, lmmse), t)
+        return res
+
+    def lmmse_grid(self, slots: int, filt: ops.LmmseFilter) -> torch.Tensor:
+        """LMMSE estimate of the batch in the network layout [S][14][used][32]."""
+        g = self.g
+        per = g.nr * g.nl * g.ndmrs * g.pilot_pad
+        hr = self.fetch("h_ls_re", torch.float16, slots * per)
+        hi = self.fetch("h_ls_im", torch.float16, slots * per)
+        nz = self.fetch("noise_ls", torch.float16, slots * g.nr)
+        sc, cnt = pilot_tables(self.cfg)
+        out = torch.empty((slots, 14, g.used, 32), dtype=torch.float32, device="cuda")
+        ws = None
+        for s in range(slots):
+            h_re, h_im, ws, _ = ops.ce_lmmse(self.cfg, filt, hr[s * per:(s + 1) * per], hi[s * per:(s + 1) * per],
+                                             sc, cnt, nz[s * g.nr:(s + 1) * g.nr], workspace=ws)
+            H = torch.complex(h_re.float(), h_im.float()).reshape(g.nr, 16, 14, g.pad)[:, : g.nl, :, : g.used]
+            v = torch.stack([H.real, H.imag], -1)                       # [r, l, 14, used, 2]
+            out[s] = v.permute(2, 3, 0, 1, 4).reshape(14, g.used, 32)
+        return out.reshape(-1)
+
+    def close(self):
+        if self.handle:
+            self.lib.airan_ce_refiner_Destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pilot_tables(cfg):
+    """Device pilot subcarrier table and counts (the mimo_dmrs_ls profile)."""
+    g = ops.Geometry(cfg)
+    sc = np.zeros(g.nl * g.ndmrs * g.pilot_pad, np.uint16)
+    cnt = np.zeros(16, np.uint16)
+    for l in range(g.nl):
+        comb = (cfg.dmrs_ports[l] - 1000) // 2
+        occ = sum((cfg.dmrs_ports[q] - 1000) // 2 == comb for q in range(g.nl)) == 2
+        n = g.n_occ if occ else g.n_dmrs_re
+        for d in range(g.ndmrs):
+            cnt[l * g.ndmrs + d] = n
+            base = (l * g.ndmrs + d) * g.pilot_pad
+            sc[base: base + n] = comb + (1 + 4 * np.arange(n) if occ else 2 * np.arange(n))
+    return torch.from_numpy(sc.view(np.int16)).cuda(), torch.from_numpy(cnt.view(np.int16)).cuda()

@@ -81,6 +81,10 @@ def main() -> int:
                     help="only run 5 device-to-device copies of MIB MiB (the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--refiner", default=str(PKG / "weights" / "ce_refiner_4x4.f32"),
+                    help="learned CE refiner weights ('' to skip the learned variant)")
+    ap.add_argument("--refiner-snr-min", type=float, default=20.0)
+    ap.add_argument("--refiner-snr-max", type=float, default=46.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the chain shards by slot with no exchange, and runs on more than one "
@@ -141,59 +145,69 @@ def main() -> int:
             torch.cuda.synchronize()
             return ok.cpu().numpy(), dev
 
-        # 1. Operating point: lowest SNR with BLER <= 10 %.
-        sweep = {}
-        if args.snr is None:
-            n = args.sweep_slots
-            payload, grid, H, _ = make_slots(n, 30.0)
-            snr_op = None
+        sweep_rng = np.random.default_rng(args.seed + 1)
+        sweep_payload = synth.random_payload(sweep_rng, args.sweep_slots, plan.tbs)
+        sweep_grid = synth.tx_grid(cfg, plan, sweep_payload)
+        sweep_H = synth.channel(sweep_rng, cfg, spec, args.sweep_slots)
+
+        def operating_point(label):
+            """Lowest SNR of the sweep with BLER <= 10 % (same slots, same noise
+            seeds for every estimator)."""
+            sw = {}
+            noise_rng = np.random.default_rng(args.seed + 2)
             for snr in range(20, 48, 2):
-                _, _, _, iq = make_slots(n, float(snr), payload, grid, H)
-                ok, _ = run(iq, n)
-                bler = 1.0 - float(ok.mean())
-                sweep[snr] = bler
-                log(f"sweep snr {snr} dB: BLER {bler:.3f}")
-                if bler <= 0.10:
-                    snr_op = float(snr)
-                    break
+                iq = synth.receive(noise_rng, cfg, sweep_grid, sweep_H, float(snr), iq_format=abi.IQ_COMPLEX64)
+                ok, _ = run(iq, args.sweep_slots)
+                sw[snr] = 1.0 - float(ok.mean())
+                log(f"{label} sweep snr {snr} dB: BLER {sw[snr]:.3f}")
+                if sw[snr] <= 0.10:
+                    return float(snr), sw
+            return None, sw
+
+        def measure(snr_op, label):
+            """Throughput over `steps` graph replays of S slots at snr_op."""
+            payload, grid, H, iq = make_slots(S, snr_op)
+            iq_dev = torch.from_numpy(iq).cuda()
+            for _ in range(args.warmup):
+                chain.enqueue(iq_dev, S)
+            torch.cuda.synchronize()
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            start.record(stream)
+            for _ in range(args.steps):
+                chain.enqueue(iq_dev, S)
+            end.record(stream)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            ms = start.elapsed_time(end) / args.steps
+            tb_ok = chain.fetch("tb_ok", torch.uint8, S)
+            bits = chain.fetch("bits", torch.int8, S * plan.C * plan.K)
+            iters = chain.fetch("iterations", torch.uint8, S * plan.C)
+            torch.cuda.synchronize()
+            tb_ok = tb_ok.cpu().numpy()
+            b = bits.cpu().numpy().reshape(S, plan.C, plan.K).astype(np.uint8)
+            per_cb = plan.Kprime - plan.crc_cb
+            mism = sum(int(not np.array_equal(b[s, :, :per_cb].reshape(-1)[: plan.tbs], payload[s]))
+                       for s in range(S))
+            r = {"snr_db": snr_op, "ms_per_step": ms, "tb_ok": tb_ok, "mean_iters": float(iters.float().mean()),
+                 "wall": wall, "mismatches": mism, "iq_dev": iq_dev}
+            log(f"{label}: {args.steps} steps at {snr_op} dB: {ms:.3f} ms/step, wall {wall:.3f} s, "
+                f"tb_ok {int(tb_ok.sum())}/{S}, payload mismatches {mism}, mean LDPC iterations "
+                f"{r['mean_iters']:.2f}")
+            return r
+
+        # 1. Plain LMMSE chain: operating point, throughput, stage times.
+        if args.snr is None:
+            snr_op, sweep = operating_point("lmmse")
             if snr_op is None:
                 snr_op = 46.0
         else:
-            snr_op = args.snr
+            snr_op, sweep = args.snr, {}
+        plain = measure(snr_op, "lmmse")
+        ms_per_step, tb_ok, mean_iters = plain["ms_per_step"], plain["tb_ok"], plain["mean_iters"]
+        iq_dev = plain["iq_dev"]
 
-        # 2. Throughput at the operating point.
-        payload, grid, H, iq = make_slots(S, snr_op)
-        iq_dev = torch.from_numpy(iq).cuda()
-        for _ in range(args.warmup):
-            chain.enqueue(iq_dev, S)
-        torch.cuda.synchronize()
-        log(f"warmup done ({args.warmup} steps of {S} slots)")
-        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        start.record(stream)
-        for _ in range(args.steps):
-            chain.enqueue(iq_dev, S)
-        end.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        ms_per_step = start.elapsed_time(end) / args.steps
-        tb_ok = chain.fetch("tb_ok", torch.uint8, S)
-        bits = chain.fetch("bits", torch.int8, S * plan.C * plan.K)
-        iters = chain.fetch("iterations", torch.uint8, S * plan.C)
-        torch.cuda.synchronize()
-        mean_iters = float(iters.float().mean().item())
-        tb_ok = tb_ok.cpu().numpy()
-        b = bits.cpu().numpy().reshape(S, plan.C, plan.K).astype(np.uint8)
-        per_cb = plan.Kprime - plan.crc_cb
-        bit_err_tb = 0
-        for s in range(S):
-            tb = b[s, :, :per_cb].reshape(-1)[: plan.tbs]
-            bit_err_tb += int(not np.array_equal(tb, payload[s]))
-        log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, wall {wall:.3f} s, "
-            f"tb_ok {int(tb_ok.sum())}/{S}, payload mismatches {bit_err_tb}, mean LDPC iterations {mean_iters:.2f}")
-
-        # 3. Per-stage times (HIP events on the chain's stream, direct launches).
+        # Per-stage times (HIP events on the chain's stream, direct launches).
         chain.set_timing(True)
         acc = {k: 0.0 for k in abi.CHAIN_STAGES}
         for _ in range(args.steps):
@@ -204,6 +218,36 @@ def main() -> int:
         torch.cuda.synchronize()
         stage_ms = {k: v / args.steps for k, v in acc.items()}
         log("stage ms: " + ", ".join(f"{k} {v:.3f}" for k, v in stage_ms.items()))
+
+        # 2. The same chain with the learned estimator (LMMSE fallback outside
+        # its training SNR range).
+        learned = None
+        wpath = Path(args.refiner) if args.refiner else None
+        if wpath is not None and wpath.exists():
+            from puschrx.refiner import Refiner
+            ref = Refiner(cfg, max_slots=max(S, args.sweep_slots), params=Refiner.load_params(wpath))
+            chain.set_refiner(ref, args.refiner_snr_min, args.refiner_snr_max)
+            l_snr, l_sweep = (operating_point("learned") if args.snr is None else (args.snr, {}))
+            learned = {"weights": str(wpath.relative_to(ROOT)) if wpath.is_relative_to(ROOT) else str(wpath),
+                       "snr_range_db": [args.refiner_snr_min, args.refiner_snr_max],
+                       "sweep_bler": {str(k): round(v, 4) for k, v in l_sweep.items()}}
+            if l_snr is not None:
+                m = measure(l_snr, "learned")
+                chain.set_timing(True)
+                lacc = 0.0
+                for _ in range(args.steps):
+                    chain.enqueue(m["iq_dev"], S)
+                    lacc += chain.stage_times()["ce_lmmse"]
+                chain.set_timing(False)
+                learned.update({
+                    "snr_db": l_snr, "bler": round(1.0 - float(m["tb_ok"].mean()), 4),
+                    "value_gbps": round(S * plan.tbs / (m["ms_per_step"] * 1e-3) / 1e9, 4),
+                    "ms_per_step": round(m["ms_per_step"], 4),
+                    "ce_stage_ms": round(lacc / args.steps, 4)})
+            else:
+                learned.update({"snr_db": None, "value_gbps": 0.0})
+            chain.set_refiner(None)
+            ref.close()
         chain.close()
 
     tbs_bits = S * plan.tbs
@@ -238,6 +282,8 @@ def main() -> int:
             "ldpc_mean_iterations": round(mean_iters, 3),
             "ce_rank": opts.ce_rank,
             "sweep_bler": {str(k): round(v, 4) for k, v in sweep.items()},
+            "estimator": "lmmse",
+            "learned": learned,
             "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
             "chain_hbm_gbps": round(chain_bytes / (ms_per_step * 1e-3) / 1e9, 2),
         },

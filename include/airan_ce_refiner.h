@@ -55,13 +55,14 @@ int32_t airan_ce_refiner_Enqueue(AiranCeRefiner *refiner, const void *h_ls_re, c
 
 /* Synthesises a training or evaluation batch on the device into the
  * refiner's batch buffers: gains host complex64 [slots][NR][L][taps][14]
- * (tapped-delay-line gains per OFDM symbol), tap_delay_s [taps],
- * snr_db [slots] (AWGN per resource element relative to unit channel
- * power).  Also leaves the DMRS least-squares estimate and its noise
- * estimate (the LMMSE estimator's inputs) in the batch buffers. */
+ * (tapped-delay-line gains per OFDM symbol), tap_delay_s host [slots][taps]
+ * (at most 32 taps), snr_db host [slots] (AWGN per resource element
+ * relative to unit channel power), scs_hz the subcarrier spacing.  Also
+ * leaves the DMRS least-squares estimate and its noise estimate (the LMMSE
+ * estimator's inputs) in the batch buffers. */
 int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *refiner, const float *gains, const float *tap_delay_s,
-                                    uint32_t taps, const float *snr_db, uint32_t slots, uint64_t seed,
-                                    void *stream);
+                                    uint32_t taps, const float *snr_db, uint32_t slots, float scs_hz,
+                                    uint64_t seed, void *stream);
 /* Device pointers of the batch buffers (valid until Destroy). */
 typedef struct AiranCeRefinerBatchV1 {
     uint16_t abi_version;

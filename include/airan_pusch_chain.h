@@ -31,6 +31,7 @@
 #ifndef AIRAN_PUSCH_CHAIN_H
 #define AIRAN_PUSCH_CHAIN_H
 
+#include "airan_ce_refiner.h"
 #include "airan_pusch_rx.h"
 
 #ifdef __cplusplus
@@ -96,6 +97,13 @@ int32_t airan_pusch_chain_Enqueue(AiranPuschChain *chain, const void *iq, uint32
 int32_t airan_pusch_chain_Outputs(const AiranPuschChain *chain, AiranPuschChainOutputsV1 *out);
 /* Device bytes the chain holds for max_slots (0 on a bad argument). */
 size_t airan_pusch_chain_DeviceBytes(const AiranPuschChain *chain);
+/* Learned estimator (include/airan_ce_refiner.h; no reference counterpart):
+ * after the LMMSE CE, every slot whose SNR estimate 10 log10(1 / mean noise
+ * of the CE) lies in [snr_min_db, snr_max_db] gets the refiner's channel
+ * estimate instead; the others keep LMMSE.  The refiner must hold at least
+ * as many slots as each Enqueue; NULL detaches it.  Drops the cached graph. */
+int32_t airan_pusch_chain_SetRefiner(AiranPuschChain *chain, AiranCeRefiner *refiner, float snr_min_db,
+                                     float snr_max_db);
 
 /* Stage timing.  With timing on, Enqueue launches the stages directly (no
  * graph) and records a HIP event on the stream before the first stage and
