@@ -145,14 +145,35 @@ struct InputArgs {
     const __half *h_im;
     bf16 *x;             // [slots][14][used][32]
     uint32_t used, nr, nl, ndmrs, pilot_pad;
-    uint32_t dmrs_sym[4], first[4], step[4], count[4];
+    uint32_t dmrs_sym[4], comb[4], count[4];
+    int32_t partner[4];  // FD-OCC partner layer, -1 for a layer alone on its comb
+    float sign[4];       // +1: FD-OCC signs (+,+) (ports 1000 / 1002), -1: (+,-)
 };
 
-// Network input: per link (rx, layer) and DMRS symbol, linear interpolation
-// of h_ls between the layer's pilot subcarriers (first + step p, constant
-// beyond the outermost pilots), then linear interpolation / extrapolation
-// in time through the DMRS symbols (constant for one DMRS symbol).
-// Grid (used / 256, slots).
+// Linear interpolation at subcarrier u of samples v(p) at first + step p
+// (p < cnt), constant beyond the outermost samples.
+template <typename F>
+__device__ inline float2 Interp(F v, float first, float step, uint32_t cnt, uint32_t u)
+{
+    const float fp = (static_cast<float>(u) - first) / step;
+    if (fp <= 0.0f) return v(0);
+    if (fp >= static_cast<float>(cnt - 1)) return v(cnt - 1);
+    const uint32_t p0 = static_cast<uint32_t>(fp);
+    const float w = fp - static_cast<float>(p0);
+    const float2 a = v(p0), b = v(p0 + 1);
+    return make_float2(a.x + w * (b.x - a.x), a.y + w * (b.y - a.y));
+}
+
+// Network input: per link (rx, layer) and DMRS symbol, the least-squares
+// estimate linearly interpolated in frequency, then linearly interpolated /
+// extrapolated in time through the DMRS symbols (constant for one).  A layer
+// alone on its comb interpolates h_ls between its pilots (comb + 2p).  An
+// FD-OCC pair (layers a: signs +,+ and b: +,-) is first split into its exact
+// observations z(comb + 4p) = h_ls_a[p] + h_ls_b[p] = (h_a + h_b)(comb + 4p)
+// and z(comb + 4p + 2) = h_ls_a[p] - h_ls_b[p] = (h_a - h_b)(comb + 4p + 2);
+// g+ = h_a + h_b and g- = h_a - h_b are interpolated on their own combs and
+// h_a,b = (g+ +- g-) / 2, which avoids the pair mean's flat-channel
+// assumption.  Grid (used / 256, slots).
 __global__ void __launch_bounds__(256) InputKernel(InputArgs a)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -162,25 +183,36 @@ __global__ void __launch_bounds__(256) InputKernel(InputArgs a)
     for (uint32_t r = 0; r < a.nr; ++r)
         for (uint32_t l = 0; l < a.nl; ++l) {
             float2 hd[4];
-            const float fp = (static_cast<float>(u) - static_cast<float>(a.first[l])) / static_cast<float>(a.step[l]);
             const uint32_t cnt = a.count[l];
-            uint32_t p0;
-            float w;
-            if (fp <= 0.0f) {
-                p0 = 0;
-                w = 0.0f;
-            } else if (fp >= static_cast<float>(cnt - 1)) {
-                p0 = cnt - 2;
-                w = 1.0f;
-            } else {
-                p0 = static_cast<uint32_t>(fp);
-                w = fp - static_cast<float>(p0);
-            }
             for (uint32_t d = 0; d < a.ndmrs; ++d) {
-                const size_t b = (((static_cast<size_t>(s) * a.nr + r) * a.nl + l) * a.ndmrs + d) * a.pilot_pad + p0;
-                const float r0 = __half2float(a.h_re[b]), r1 = __half2float(a.h_re[b + 1]);
-                const float i0 = __half2float(a.h_im[b]), i1 = __half2float(a.h_im[b + 1]);
-                hd[d] = make_float2(r0 + w * (r1 - r0), i0 + w * (i1 - i0));
+                auto row = [&](uint32_t layer) {
+                    return (((static_cast<size_t>(s) * a.nr + r) * a.nl + layer) * a.ndmrs + d) * a.pilot_pad;
+                };
+                const size_t bl = row(l);
+                auto hl = [&](size_t base, uint32_t p) {
+                    return make_float2(__half2float(a.h_re[base + p]), __half2float(a.h_im[base + p]));
+                };
+                if (a.partner[l] < 0) {
+                    hd[d] = Interp([&](uint32_t p) { return hl(bl, p); }, static_cast<float>(a.comb[l]), 2.0f, cnt,
+                                   u);
+                } else {
+                    const size_t bq = row(static_cast<uint32_t>(a.partner[l]));
+                    // h_ls of the (+,+) layer and of the (+,-) layer.
+                    const size_t bp = a.sign[l] > 0.0f ? bl : bq, bm = a.sign[l] > 0.0f ? bq : bl;
+                    const float2 gp = Interp(
+                        [&](uint32_t p) {
+                            const float2 x = hl(bp, p), y = hl(bm, p);
+                            return make_float2(x.x + y.x, x.y + y.y);
+                        },
+                        static_cast<float>(a.comb[l]), 4.0f, cnt, u);
+                    const float2 gm = Interp(
+                        [&](uint32_t p) {
+                            const float2 x = hl(bp, p), y = hl(bm, p);
+                            return make_float2(x.x - y.x, x.y - y.y);
+                        },
+                        static_cast<float>(a.comb[l] + 2), 4.0f, cnt, u);
+                    hd[d] = make_float2(0.5f * (gp.x + a.sign[l] * gm.x), 0.5f * (gp.y + a.sign[l] * gm.y));
+                }
             }
             const uint32_t c = 2 * (r * a.nl + l);
             for (uint32_t k = 0; k < kSym; ++k) {
@@ -594,9 +626,16 @@ InputArgs MakeInput(const AiranCeRefiner *r, const __half *hr, const __half *hi,
     a.pilot_pad = g.pilot_pad;
     for (uint32_t d = 0; d < g.ndmrs; ++d) a.dmrs_sym[d] = g.dmrs_sym[d];
     for (uint32_t l = 0; l < g.nl; ++l) {
-        a.first[l] = g.comb[l] + (g.occ[l] ? 1u : 0u);
-        a.step[l] = g.occ[l] ? 4u : 2u;
+        a.comb[l] = g.comb[l];
         a.count[l] = g.occ[l] ? g.n_occ : g.n_dmrs_re;
+        a.partner[l] = -1;
+        a.sign[l] = 1.0f;
+        if (g.occ[l]) {
+            for (uint32_t q = 0; q < g.nl; ++q)
+                if (q != l && g.comb[q] == g.comb[l]) a.partner[l] = static_cast<int32_t>(q);
+            const uint16_t port = r->cfg.dmrs_ports[l];
+            a.sign[l] = port == 1000 || port == 1002 ? 1.0f : -1.0f;
+        }
     }
     return a;
 }

@@ -36,6 +36,11 @@ for p in (str(PKG), str(ROOT / "tests")):
 
 METRIC = "decoded Gbit/s, 4Rx rank-4 273-PRB PUSCH at BLER<=10%; learned vs plain LMMSE"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per 64-slot launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (scaled by a 256 MiB device copy: x2.0 / x1.0), from
+# profiles/chain_kernels_4rx_273prb_64slots.md; reported as roofline.traffic
+# when the dominant stage and batch match that profile.
+MEASURED_TRAFFIC_64 = {"ldpc_decode": (168.4 + 69.8) * 1e6, "ce_lmmse": None, "detect_demap": (205.0 + 161.0) * 1e6}
 
 
 def log(msg: str) -> None:
@@ -190,7 +195,7 @@ def main() -> int:
             mism = sum(int(not np.array_equal(b[s, :, :per_cb].reshape(-1)[: plan.tbs], payload[s]))
                        for s in range(S))
             r = {"snr_db": snr_op, "ms_per_step": ms, "tb_ok": tb_ok, "mean_iters": float(iters.float().mean()),
-                 "wall": wall, "mismatches": mism, "iq_dev": iq_dev}
+                 "wall": wall, "mismatches": mism, "iq_dev": iq_dev, "iq": iq}
             log(f"{label}: {args.steps} steps at {snr_op} dB: {ms:.3f} ms/step, wall {wall:.3f} s, "
                 f"tb_ok {int(tb_ok.sum())}/{S}, payload mismatches {mism}, mean LDPC iterations "
                 f"{r['mean_iters']:.2f}")
@@ -205,7 +210,7 @@ def main() -> int:
             snr_op, sweep = args.snr, {}
         plain = measure(snr_op, "lmmse")
         ms_per_step, tb_ok, mean_iters = plain["ms_per_step"], plain["tb_ok"], plain["mean_iters"]
-        iq_dev = plain["iq_dev"]
+        iq_dev, iq = plain["iq_dev"], plain["iq"]
 
         # Per-stage times (HIP events on the chain's stream, direct launches).
         chain.set_timing(True)
@@ -294,7 +299,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": (MEASURED_TRAFFIC_64.get(dominant) if S == 64 and fused_ldpc else None),
         },
         "cpu_baseline": None,
     }
