@@ -10,6 +10,7 @@
 #include "airan_ce_refiner.h"
 #include "airan_pusch_chain.h"
 #include "common.h"
+#include "crc.h"
 #include "detect.h"
 #include "gold.h"
 #include "launch.h"
@@ -288,15 +289,20 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
     mark();
     const uint32_t ncb = slots * c->plan.C;
     if (c->fused_ldpc) {
-        // Rate dematch runs inside the decoder's load (the stage is empty).
+        // Rate dematch runs inside the decoder's load (the stage is empty)
+        // and the CRC checks inside its epilogue (the CRC stage only turns
+        // the transport-block remainders into tb_ok).
         mark();
+        airan::CrcFuse crc{};
+        if ((st = airan::CrcFuseSetup(c->plan, c->cb_ok, reinterpret_cast<uint32_t *>(c->crc_ws), &crc)) != AIRAN_OK)
+            return st;
+        airan::LaunchCrcZero(reinterpret_cast<uint32_t *>(c->crc_ws), slots, s);
         st = airan::LaunchLdpcFromCodeword(g, c->plan, c->desc, c->cw, ncb, c->active_rows, c->opts.ldpc_iterations,
-                                           c->opts.early_stop != 0, c->bits, c->iters, s);
+                                           c->opts.early_stop != 0, c->bits, c->iters, &crc, s);
         if (st != AIRAN_OK) return st;
         mark();
-        st = airan_crc_check_Enqueue(&c->plan, slots, c->bits, c->cb_ok, c->tb_ok, nullptr, c->crc_ws,
-                                     c->crc_ws_bytes, s);
-        if (st != AIRAN_OK) return st;
+        airan::LaunchCrcFinish(reinterpret_cast<const uint32_t *>(c->crc_ws), c->tb_ok, slots, s);
+        if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
         mark();
         return AIRAN_OK;
     }

@@ -8,6 +8,8 @@ struct LmmseFilter;
 
 namespace airan {
 
+struct CrcFuse;
+
 // DMRS least squares + per-Rx noise for `slots` stacked grids
 // (rx [slots,NR,14,pad] -> h_ls [slots,NR,L,D,pilot_pad], noise [slots,NR]).
 int32_t LaunchDmrsLs(const Geom &g, const __half *rx_re, const __half *rx_im, const __half *ref_re,
@@ -34,8 +36,16 @@ int32_t LaunchRateDematch(const Geom &g, const AiranTbPlanV1 &plan, const RateMa
 // bit-plane codeword LLRs through the rate-match descriptors: the rate
 // dematch and decoder stages in one launch.  AIRAN_UNSUPPORTED_PROFILE when
 // the active rows exceed the fused decoder's limit (8) or Z is not 24 * 2^j.
+// With crc non-null the decoder also checks each code block's CRC24B
+// (cb_ok) and accumulates the transport blocks' CRC remainders (tb_acc,
+// zeroed by LaunchCrcZero before, turned into tb_ok by LaunchCrcFinish after).
 int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &plan, const RateMatchDescriptor *desc,
                                const int16_t *cw, uint32_t num_cb, uint32_t rows, uint32_t iters, bool early_stop,
-                               int8_t *bits, uint8_t *iters_out, hipStream_t s);
+                               int8_t *bits, uint8_t *iters_out, const CrcFuse *crc, hipStream_t s);
+
+// CRC check of decoded bits inside the decoder (crc.h).
+int32_t CrcFuseSetup(const AiranTbPlanV1 &plan, uint8_t *cb_ok, uint32_t *tb_acc, CrcFuse *out);
+void LaunchCrcZero(uint32_t *tb_acc, uint32_t num_tbs, hipStream_t s);
+void LaunchCrcFinish(const uint32_t *tb_acc, uint8_t *tb_ok, uint32_t num_tbs, hipStream_t s);
 
 }  // namespace airan

@@ -11,6 +11,8 @@
 #include <vector>
 
 #include "common.h"
+#include "crc.h"
+#include "launch.h"
 
 namespace {
 
@@ -75,20 +77,6 @@ int32_t EnsureTables(CrcTables **out)
     return AIRAN_OK;
 }
 
-// Carry-less product a*b mod g for len-bit residues.
-__device__ inline uint32_t MulMod(uint32_t a, uint32_t b, uint32_t poly, uint32_t len)
-{
-    const uint32_t mask = (1u << len) - 1u;
-    uint32_t r = 0;
-    for (int i = static_cast<int>(len) - 1; i >= 0; --i) {
-        const uint32_t top = (r >> (len - 1)) & 1u;
-        r = (r << 1) & mask;
-        if (top) r ^= poly;
-        if ((b >> i) & 1u) r ^= a;
-    }
-    return r;
-}
-
 struct CrcDev {
     uint32_t C, K, kprime, crc_cb, crc_tb, tbs;
     uint32_t poly_tb, len_tb;
@@ -100,23 +88,13 @@ __global__ void __launch_bounds__(256) ZeroKernel(uint32_t *p, uint32_t n)
     if (t < n) p[t] = 0;
 }
 
-__global__ void __launch_bounds__(256) CbCrcKernel(const int8_t *__restrict__ bits,
-                                                   const uint32_t *__restrict__ pow_b,
-                                                   const uint32_t *__restrict__ pow_tb,
-                                                   uint8_t *__restrict__ cb_ok,
-                                                   uint32_t *__restrict__ tb_acc, CrcDev d)
+__global__ void __launch_bounds__(256) CbCrcKernel(const int8_t *__restrict__ bits, airan::CrcFuse c, uint32_t K)
 {
     const uint32_t cbg = blockIdx.x;            // global CB index over all TBs
-    const uint32_t tb = cbg / d.C, r = cbg - tb * d.C;
-    const int8_t *b = bits + static_cast<size_t>(cbg) * d.K;
-    const uint32_t L = d.kprime - d.crc_cb;     // TB bits carried by this CB
+    const int8_t *b = bits + static_cast<size_t>(cbg) * K;
     uint32_t rb = 0, ra = 0;
-    for (uint32_t j = threadIdx.x; j < d.kprime; j += blockDim.x) {
-        if (b[j] & 1) {
-            if (d.crc_cb) rb ^= pow_b[d.kprime - 1 - j];
-            if (j < L) ra ^= pow_tb[L - 1 - j];
-        }
-    }
+    for (uint32_t j = threadIdx.x; j < c.kprime; j += blockDim.x)
+        if (b[j] & 1) airan::CrcAddBit(c, j, rb, ra);
     // Wavefront then block XOR reduction.
     for (int off = 32; off > 0; off >>= 1) {
         rb ^= __shfl_xor(rb, off);
@@ -135,18 +113,7 @@ __global__ void __launch_bounds__(256) CbCrcKernel(const int8_t *__restrict__ bi
             xb ^= sb[k];
             xa ^= sa[k];
         }
-        cb_ok[cbg] = d.crc_cb ? static_cast<uint8_t>(xb == 0) : static_cast<uint8_t>(1);
-        // Shift the local remainder past the TB bits that follow this CB:
-        // multiply by x^after mod g (square-and-multiply from x^1).
-        const uint32_t after = (d.C - 1 - r) * L;
-        uint32_t xpow = 1u, base = 2u, e = after;
-        while (e != 0) {
-            if (e & 1u) xpow = MulMod(xpow, base, d.poly_tb, d.len_tb);
-            base = MulMod(base, base, d.poly_tb, d.len_tb);
-            e >>= 1;
-        }
-        xa = MulMod(xa, xpow, d.poly_tb, d.len_tb);
-        atomicXor(&tb_acc[tb], xa);
+        airan::CrcFinishBlock(c, cbg, xb, xa);
     }
 }
 
@@ -181,6 +148,34 @@ __global__ void __launch_bounds__(256) PackKernel(const int8_t *__restrict__ bit
 
 }  // namespace
 
+namespace airan {
+
+int32_t CrcFuseSetup(const AiranTbPlanV1 &p, uint8_t *cb_ok, uint32_t *tb_acc, CrcFuse *out)
+{
+    if (p.C == 0 || p.Kprime > p.K || p.K > 8448 || p.Kprime < p.crc_cb || (p.crc_tb != 24 && p.crc_tb != 16) ||
+        (p.crc_cb != 0 && p.crc_cb != 24) ||
+        static_cast<uint64_t>(p.C) * (p.Kprime - p.crc_cb) != static_cast<uint64_t>(p.tbs) + p.crc_tb)
+        return AIRAN_INVALID_ARGUMENT;
+    CrcTables *t = nullptr;
+    const int32_t st = EnsureTables(&t);
+    if (st != AIRAN_OK) return st;
+    *out = CrcFuse{t->pow_b, p.crc_tb == 24 ? t->pow_a : t->pow_16, cb_ok, tb_acc, p.C, p.Kprime, p.crc_cb,
+                   p.Kprime - p.crc_cb, p.crc_tb == 24 ? kPolyA : kPoly16, p.crc_tb};
+    return AIRAN_OK;
+}
+
+void LaunchCrcZero(uint32_t *tb_acc, uint32_t num_tbs, hipStream_t s)
+{
+    hipLaunchKernelGGL(ZeroKernel, dim3(CeilDiv(num_tbs, 256)), dim3(256), 0, s, tb_acc, num_tbs);
+}
+
+void LaunchCrcFinish(const uint32_t *tb_acc, uint8_t *tb_ok, uint32_t num_tbs, hipStream_t s)
+{
+    hipLaunchKernelGGL(TbFinishKernel, dim3(CeilDiv(num_tbs, 256)), dim3(256), 0, s, tb_acc, tb_ok, num_tbs);
+}
+
+}  // namespace airan
+
 extern "C" size_t airan_crc_check_WorkspaceBytes(uint32_t num_tbs)
 {
     return static_cast<size_t>(num_tbs) * sizeof(uint32_t);
@@ -202,11 +197,12 @@ extern "C" int32_t airan_crc_check_Enqueue(const AiranTbPlanV1 *p, uint32_t num_
     if (st != AIRAN_OK) return st;
     CrcDev d{p->C, p->K, p->Kprime, p->crc_cb, p->crc_tb, p->tbs,
              p->crc_tb == 24 ? kPolyA : kPoly16, p->crc_tb};
-    const uint32_t *pow_tb = p->crc_tb == 24 ? t->pow_a : t->pow_16;
+    airan::CrcFuse c{t->pow_b, p->crc_tb == 24 ? t->pow_a : t->pow_16, static_cast<uint8_t *>(cb_ok),
+                     static_cast<uint32_t *>(workspace), p->C, p->Kprime, p->crc_cb, p->Kprime - p->crc_cb,
+                     d.poly_tb, d.len_tb};
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(ZeroKernel, dim3(airan::CeilDiv(num_tbs, 256)), dim3(256), 0, s, static_cast<uint32_t *>(workspace), num_tbs);
-    hipLaunchKernelGGL(CbCrcKernel, dim3(num_tbs * p->C), dim3(256), 0, s, static_cast<const int8_t *>(bits),
-                       t->pow_b, pow_tb, static_cast<uint8_t *>(cb_ok), static_cast<uint32_t *>(workspace), d);
+    hipLaunchKernelGGL(CbCrcKernel, dim3(num_tbs * p->C), dim3(256), 0, s, static_cast<const int8_t *>(bits), c, p->K);
     hipLaunchKernelGGL(TbFinishKernel, dim3(airan::CeilDiv(num_tbs, 256)), dim3(256), 0, s, static_cast<uint32_t *>(workspace),
                        static_cast<uint8_t *>(tb_ok), num_tbs);
     if (tb_bytes != nullptr) {

@@ -13,6 +13,7 @@
 // registers when the active rows fit REG_ROWS, otherwise in a global workspace.
 #include "common.h"
 #include "airan_ldpc_bg1.h"
+#include "crc.h"
 #include "launch.h"
 #include "ratematch.h"
 
@@ -289,52 +290,54 @@ __device__ __forceinline__ uint32_t LiftAddr(uint32_t w, uint32_t i, uint32_t Z)
 }
 
 // Check i of row R: gather, extrinsic, min1/min2/signs, scatter.  State
-// sa = fp16(0.75 min1) | fp16(0.75 min2) << 16, sb = signs | index of min1 << 24.
+// sa = fp16(0.75 min1) | fp16(0.75 min2) << 16, sb = message sign bits
+// (bit k: sign of the message on edge k) | index of min1 << 24.
 // Branch-free on fp16 bit patterns: magnitudes (|x| bits, +inf = 0x7c00,
 // NaN above) order like unsigned integers, so min1 and its first index are
-// the minimum of (|x| << 5 | k), and min2 = min(min2, max(min1, |x|)).  The
-// sign bit recorded is !(x >= 0), i.e. bits above 0x7c00 (the posterior is
-// never -0: it starts from integer LLRs and IEEE sums give -0 only from -0).
+// the minimum of the keys (|x| << 5 | k) and min2 is the median of
+// (min1 key, key, min2 key) accumulated over the edges.  The recorded sign
+// is !(x >= 0), i.e. bits above 0x7c00 (the posterior is never -0: it starts
+// from integer LLRs and IEEE sums give -0 only from -0).  The scatter takes
+// the new message's sign from the extrinsic's sign bit, which differs from
+// the recorded one only for a NaN extrinsic, whose posterior is NaN anyway.
 template <int R>
 __device__ __forceinline__ void FastRow(h16 *lam, const uint32_t *lw, uint32_t i, uint32_t Z, uint32_t &sa,
                                         uint32_t &sb)
 {
     constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
     const uint32_t o1 = sa & 0xffffu, o2 = sa >> 16;
-    const uint32_t osg = sb & 0x7ffffu, oidx = sb >> 24;
-    const uint32_t opar = __popc(osg) & 1u;
+    const uint32_t omsg = sb & 0x7ffffu, oidx = sb >> 24;
     uint32_t w[DEG];
 #pragma unroll
     for (int k = 0; k < DEG; ++k) w[k] = lw[E0 + k];
     uint32_t addr[DEG];
     h16 ext[DEG];
-    uint32_t m1key = 0x7c00u << 5, m2 = 0x7c00u, sg = 0;
+    uint32_t m1key = 0x7c00u << 5, m2key = 0x7c00u << 5, sg = 0;
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
         addr[k] = LiftAddr(w[k], i, Z);
-        const uint32_t omag = static_cast<uint32_t>(k) == oidx ? o2 : o1;
-        const uint32_t omsg = omag ^ ((((osg >> k) ^ opar) & 1u) << 15);
-        const h16 x = lam[addr[k]] - HBits(omsg);
+        const uint32_t mag = static_cast<uint32_t>(k) == oidx ? o2 : o1;
+        const uint32_t sbit = k <= 15 ? omsg << (15 - k) : omsg >> (k - 15);
+        const h16 x = lam[addr[k]] - HBits(mag | (sbit & 0x8000u));
         ext[k] = x;
         const uint32_t xb = BitsH(x);
         const uint32_t key = ((xb & 0x7fffu) << 5) | static_cast<uint32_t>(k);
-        const uint32_t hi = max(m1key, key);
+        m2key = min(m2key, max(m1key, key));
         m1key = min(m1key, key);
-        m2 = min(m2, hi >> 5);
         sg |= ((0x7c00u - xb) >> 31) << k;
     }
     const uint32_t idx = m1key & 31u;
     const uint32_t s1 = BitsH(static_cast<h16>(0.75f) * HBits(m1key >> 5));
-    const uint32_t s2 = BitsH(static_cast<h16>(0.75f) * HBits(m2));
+    const uint32_t s2 = BitsH(static_cast<h16>(0.75f) * HBits(m2key >> 5));
     const uint32_t par = __popc(sg) & 1u;
+    const uint32_t s1p = s1 | (par << 15), s2p = s2 | (par << 15);
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
-        const uint32_t mag = static_cast<uint32_t>(k) == idx ? s2 : s1;
-        const uint32_t nm = mag ^ ((((sg >> k) ^ par) & 1u) << 15);
-        lam[addr[k]] = ext[k] + HBits(nm);
+        const uint32_t msel = static_cast<uint32_t>(k) == idx ? s2p : s1p;
+        lam[addr[k]] = ext[k] + HBits(msel ^ (BitsH(ext[k]) & 0x8000u));
     }
     sa = s1 | (s2 << 16);
-    sb = sg | (idx << 24);
+    sb = (sg ^ (par ? 0x7ffffu : 0u)) | (idx << 24);
 }
 
 template <int R>
@@ -378,11 +381,13 @@ struct LdpcFastDev {
 // One workgroup per code block.  src: int16 decoder LLRs [cb][68 Z]
 // (!FUSED) or bit-plane codeword LLRs with per-block descriptors (FUSED).
 // Six waves per SIMD (80 VGPRs): four Z = 384 blocks per CU.
+// With crc.cb_ok set, the epilogue also checks the block's CRC (crc.h).
 template <bool FUSED>
 __global__ void __launch_bounds__(384, 6) LdpcFastKernel(const int16_t *__restrict__ src,
                                                       const RateMatchDescriptor *__restrict__ desc,
                                                       int8_t *__restrict__ bits, int16_t *__restrict__ lam_out,
-                                                      uint8_t *__restrict__ iters_out, LdpcFastDev d)
+                                                      uint8_t *__restrict__ iters_out, LdpcFastDev d,
+                                                      airan::CrcFuse crc)
 {
     uint16_t *lam16 = airan_ldpc_lam;
     h16 *lam = reinterpret_cast<h16 *>(airan_ldpc_lam);
@@ -428,15 +433,58 @@ __global__ void __launch_bounds__(384, 6) LdpcFastKernel(const int16_t *__restri
     }
     if (iters_out != nullptr && i == 0) iters_out[cb] = static_cast<uint8_t>(it);
     int8_t *bo = bits + static_cast<size_t>(cb) * d.k;
+    const bool check = crc.cb_ok != nullptr;
+    uint32_t rb = 0, ra = 0;
     if ((d.k & 3u) == 0) {
         const uint2 *l4 = reinterpret_cast<const uint2 *>(lam16);
+        // With kprime and L multiples of 4, the power-table entries of bits
+        // 4t .. 4t+3 are four consecutive words (in reverse order): one
+        // 16-byte load per table instead of a branch and a load per bit.
+        const bool vec = check && (crc.kprime & 3u) == 0 && (crc.L & 3u) == 0;
         for (uint32_t t = i; t < d.k / 4u; t += blockDim.x) {
             const uint2 v = l4[t];
-            reinterpret_cast<uint32_t *>(bo)[t] = NegBit(v.x) | NegBit(v.x >> 16) << 8 | NegBit(v.y) << 16 |
-                                                  NegBit(v.y >> 16) << 24;
+            const uint32_t b4 = NegBit(v.x) | NegBit(v.x >> 16) << 8 | NegBit(v.y) << 16 | NegBit(v.y >> 16) << 24;
+            reinterpret_cast<uint32_t *>(bo)[t] = b4;
+            if (vec) {
+                const uint32_t j = 4 * t;
+                auto sel = [b4](uint4 p) {
+                    return ((b4 & 1u) ? p.w : 0u) ^ ((b4 & 0x100u) ? p.z : 0u) ^ ((b4 & 0x10000u) ? p.y : 0u) ^
+                           ((b4 & 0x1000000u) ? p.x : 0u);
+                };
+                if (crc.crc_cb && j < crc.kprime)
+                    rb ^= sel(*reinterpret_cast<const uint4 *>(crc.pow_b + (crc.kprime - 4 - j)));
+                if (j < crc.L) ra ^= sel(*reinterpret_cast<const uint4 *>(crc.pow_tb + (crc.L - 4 - j)));
+            } else if (check) {
+                for (uint32_t q = 0; q < 4; ++q)
+                    if ((b4 >> (8 * q)) & 1u) airan::CrcAddBit(crc, 4 * t + q, rb, ra);
+            }
         }
     } else {
-        for (uint32_t t = i; t < d.k; t += blockDim.x) bo[t] = static_cast<int8_t>(NegBit(lam16[t]));
+        for (uint32_t t = i; t < d.k; t += blockDim.x) {
+            const uint32_t b = NegBit(lam16[t]);
+            bo[t] = static_cast<int8_t>(b);
+            if (check && b) airan::CrcAddBit(crc, t, rb, ra);
+        }
+    }
+    if (check) {
+        __shared__ uint32_t red[2][8];
+        for (int off = 32; off > 0; off >>= 1) {
+            rb ^= __shfl_xor(rb, off);
+            ra ^= __shfl_xor(ra, off);
+        }
+        if ((i & 63u) == 0) {
+            red[0][i >> 6] = rb;
+            red[1][i >> 6] = ra;
+        }
+        __syncthreads();
+        if (i == 0) {
+            uint32_t xb = 0, xa = 0;
+            for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
+                xb ^= red[0][w];
+                xa ^= red[1][w];
+            }
+            airan::CrcFinishBlock(crc, cb, xb, xa);
+        }
     }
     if (lam_out != nullptr) {
         int16_t *lo = lam_out + static_cast<size_t>(cb) * N;
@@ -509,7 +557,7 @@ int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, 
         LdpcFastDev fd{a->z, LiftIndex(a->z), a->k, a->max_iterations, a->active_rows, a->early_stop ? 1u : 0u, {}};
         const size_t fast_lds = static_cast<size_t>(22 + a->active_rows) * a->z * sizeof(uint16_t);
         hipLaunchKernelGGL(LdpcFastKernel<false>, dim3(a->num_cb), dim3(threads), fast_lds, s, in, nullptr, bits, lo,
-                           io, fd);
+                           io, fd, airan::CrcFuse{});
         return airan::LaunchStatus();
     }
     const bool reg = a->active_rows <= static_cast<uint32_t>(kRegRows);
@@ -533,7 +581,7 @@ namespace airan {
 
 int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const RateMatchDescriptor *desc,
                                const int16_t *cw, uint32_t num_cb, uint32_t rows, uint32_t iters, bool early_stop,
-                               int8_t *bits, uint8_t *iters_out, hipStream_t s)
+                               int8_t *bits, uint8_t *iters_out, const CrcFuse *crc, hipStream_t s)
 {
     if (num_cb == 0 || !FastEligible(AIRAN_LDPC_MODE_FP16, p.Z, rows) || rows < 4 || iters == 0 || iters > 100 ||
         (p.Z % 24) != 0 || p.K == 0 || p.K > 22 * p.Z || p.Ncb <= p.filler || (reinterpret_cast<uintptr_t>(bits) & 3u))
@@ -544,7 +592,7 @@ int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const Rate
                    {g.qm, g.cw_symbols, g.cw_stride, p.Z, p.Kprime, p.filler, p.Ncb, num_cb}};
     const size_t lds = static_cast<size_t>(22 + rows) * p.Z * sizeof(uint16_t);
     hipLaunchKernelGGL(LdpcFastKernel<true>, dim3(num_cb), dim3(AlignUp(p.Z, 64)), lds, s, cw, desc, bits,
-                       static_cast<int16_t *>(nullptr), iters_out, fd);
+                       static_cast<int16_t *>(nullptr), iters_out, fd, crc != nullptr ? *crc : CrcFuse{});
     return LaunchStatus();
 }
 

@@ -399,8 +399,11 @@ __global__ void __launch_bounds__(256) ReduceKernel(const float *__restrict__ pa
 }
 
 // ------------------------------------------------------------------ loss
-// dz = bf16(2 (out - target) / N); per-block sums of (out - target)^2.
+// dz = bf16(2 w_s (out - target) / N); per-block sums of w_s (out - target)^2,
+// w_s the weight of the element's slot (1 without weights; the training
+// weights have mean 1 over the batch).
 __global__ void __launch_bounds__(256) LossKernel(const float *__restrict__ out, const float *__restrict__ tgt,
+                                                  const float *__restrict__ slot_w, size_t per_slot,
                                                   bf16 *__restrict__ dz, double *__restrict__ part, size_t n)
 {
     __shared__ double red[4];
@@ -408,9 +411,10 @@ __global__ void __launch_bounds__(256) LossKernel(const float *__restrict__ out,
     const float scale = static_cast<float>(2.0 / static_cast<double>(n));
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const float w = slot_w != nullptr ? slot_w[i / per_slot] : 1.0f;
         const float e = out[i] - tgt[i];
-        acc += static_cast<double>(e) * e;
-        dz[i] = static_cast<bf16>(e * scale);
+        acc += static_cast<double>(w) * e * e;
+        dz[i] = static_cast<bf16>(w * e * scale);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -505,8 +509,9 @@ struct AiranCeRefiner {
     double *loss_part = nullptr;
     float *loss_dev = nullptr;
     float2 *gains = nullptr;
-    float *tau = nullptr, *snr = nullptr;
+    float *tau = nullptr, *snr = nullptr, *slot_w = nullptr;
     uint32_t max_taps = 32;
+    std::vector<float> host_w;  // staged slot weights of the last SynthBatch
     __half *dmrs_re = nullptr, *dmrs_im = nullptr;
     __half *rx_re = nullptr, *rx_im = nullptr;
     __half *hls_re = nullptr, *hls_im = nullptr, *noise = nullptr;
@@ -565,11 +570,12 @@ void LaunchWgrad(const AiranCeRefiner *r, const bf16 *act, const bf16 *dz, uint3
 }
 
 // Loss and gradients of all parameters (after Forward on the same x).
-void Backward(const AiranCeRefiner *r, const bf16 *x, const float *target, uint32_t S, uint32_t W, float *loss,
-              hipStream_t s)
+void Backward(const AiranCeRefiner *r, const bf16 *x, const float *target, const float *slot_w, uint32_t S,
+              uint32_t W, float *loss, hipStream_t s)
 {
     const size_t n = static_cast<size_t>(S) * kSym * W * kCh;
-    hipLaunchKernelGGL(LossKernel, dim3(kLossBlocks), dim3(256), 0, s, r->out, target, r->dz[0], r->loss_part, n);
+    hipLaunchKernelGGL(LossKernel, dim3(kLossBlocks), dim3(256), 0, s, r->out, target, slot_w,
+                       static_cast<size_t>(kSym) * W * kCh, r->dz[0], r->loss_part, n);
     hipLaunchKernelGGL(LossFinishKernel, dim3(1), dim3(64), 0, s, r->loss_part, kLossBlocks, n,
                        loss != nullptr ? loss : r->loss_dev);
     int cur = 0;
@@ -678,7 +684,7 @@ int32_t airan_ce_refiner_Create(const PuschMimoConfig *config, uint32_t max_slot
               alloc(&r->part, r->part_floats * 4) && alloc(&r->loss_part, kLossBlocks * 8) &&
               alloc(&r->loss_dev, 4) &&
               alloc(&r->gains, static_cast<size_t>(max_slots) * g.nr * g.nl * r->max_taps * kSym * 8) &&
-              alloc(&r->tau, static_cast<size_t>(max_slots) * r->max_taps * 4) && alloc(&r->snr, max_slots * 4) &&
+              alloc(&r->tau, static_cast<size_t>(max_slots) * r->max_taps * 4) && alloc(&r->snr, max_slots * 4) && alloc(&r->slot_w, max_slots * 4) &&
               alloc(&r->dmrs_re, static_cast<size_t>(g.nl) * g.ndmrs * g.ref_pad * 2) &&
               alloc(&r->dmrs_im, static_cast<size_t>(g.nl) * g.ndmrs * g.ref_pad * 2) &&
               alloc(&r->rx_re, static_cast<size_t>(max_slots) * g.nr * kSym * g.pad * 2) &&
@@ -747,7 +753,8 @@ void airan_ce_refiner_Destroy(AiranCeRefiner *r)
 {
     if (r == nullptr) return;
     void *bufs[] = {r->params, r->adam_m, r->adam_v, r->grad, r->wf, r->wb, r->x, r->target, r->out, r->dz[0],
-                    r->dz[1], r->part, r->loss_part, r->loss_dev, r->gains, r->tau, r->snr, r->dmrs_re, r->dmrs_im,
+                    r->dz[1], r->part, r->loss_part, r->loss_dev, r->gains, r->tau, r->snr, r->slot_w, r->dmrs_re,
+                    r->dmrs_im,
                     r->rx_re, r->rx_im, r->hls_re, r->hls_im, r->noise};
     for (void *b : bufs) FreeDev(b);
     for (bf16 *a : r->act) FreeDev(a);
@@ -794,7 +801,16 @@ int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *r, const float *gains, const
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Geom &g = r->g;
     const size_t ng = static_cast<size_t>(slots) * g.nr * g.nl * taps * kSym;
+    // Training weights: each slot's squared error relative to its noise
+    // level (weight proportional to the linear SNR, mean 1), so every SNR of
+    // the batch counts alike instead of the noisiest slots dominating.
+    double wsum = 0.0;
+    for (uint32_t i = 0; i < slots; ++i) wsum += std::pow(10.0, 0.1 * snr_db[i]);
+    r->host_w.resize(slots);
+    for (uint32_t i = 0; i < slots; ++i)
+        r->host_w[i] = static_cast<float>(std::pow(10.0, 0.1 * snr_db[i]) * slots / wsum);
     if (hipMemcpyAsync(r->gains, gains, ng * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(r->slot_w, r->host_w.data(), slots * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(r->tau, tap_delay_s, static_cast<size_t>(slots) * taps * 4, hipMemcpyHostToDevice, s) !=
             hipSuccess ||
         hipMemcpyAsync(r->snr, snr_db, slots * 4, hipMemcpyHostToDevice, s) != hipSuccess)
@@ -849,7 +865,7 @@ int32_t airan_ce_refiner_TrainStep(AiranCeRefiner *r, uint32_t slots, float lr, 
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t W = r->g.used;
     Forward(r, r->x, slots, W, s);
-    Backward(r, r->x, r->target, slots, W, loss_out, s);
+    Backward(r, r->x, r->target, r->slot_w, slots, W, loss_out, s);
     ++r->step;
     const float bc1 = 1.0f - std::pow(0.9f, static_cast<float>(r->step));
     const float bc2 = 1.0f - std::pow(0.999f, static_cast<float>(r->step));
@@ -875,7 +891,7 @@ int32_t airan_ce_refiner_Gradients(AiranCeRefiner *r, const void *x, const void 
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bf16 *xb = static_cast<const bf16 *>(x);
     Forward(r, xb, slots, width, s);
-    Backward(r, xb, static_cast<const float *>(target), slots, width, loss, s);
+    Backward(r, xb, static_cast<const float *>(target), nullptr, slots, width, loss, s);
     if (hipMemcpyAsync(grad, r->grad, kParams * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return AIRAN_LAUNCH_FAILED;
     return airan::LaunchStatus();

@@ -23,13 +23,16 @@ import torch
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--steps", type=int, default=12000)
     ap.add_argument("--slots", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=5e-4)
-    ap.add_argument("--lr-final", type=float, default=5e-5)
+    ap.add_argument("--lr", type=float, default=3e-4)
+    ap.add_argument("--lr-final", type=float, default=3e-5)
     ap.add_argument("--chunk", type=int, default=500, help="steps between held-out evaluations")
-    ap.add_argument("--snr-min", type=float, default=20.0)
+    ap.add_argument("--snr-min", type=float, default=30.0)
     ap.add_argument("--snr-max", type=float, default=46.0)
+    ap.add_argument("--delay-spread", type=float, nargs=2, default=(200e-9, 400e-9), metavar=("MIN", "MAX"))
+    ap.add_argument("--doppler", type=float, nargs=2, default=(50.0, 200.0), metavar=("MIN", "MAX"))
+    ap.add_argument("--rx-corr", type=float, nargs=2, default=(0.3, 0.7), metavar=("MIN", "MAX"))
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--init", type=Path, default=None, help="start from these weights")
     ap.add_argument("--out", type=Path, default=None)
@@ -43,7 +46,8 @@ def main(argv=None) -> int:
         print("train_refiner needs a GPU", file=sys.stderr)
         return 1
     cfg = P.FULL_CARRIER_4RX.config()
-    spec = TrainSpec(snr_db=(args.snr_min, args.snr_max))
+    spec = TrainSpec(snr_db=(args.snr_min, args.snr_max), delay_spread_s=tuple(args.delay_spread),
+                     doppler_hz=tuple(args.doppler), rx_corr=tuple(args.rx_corr))
     init = Refiner.load_params(args.init) if args.init else None
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):

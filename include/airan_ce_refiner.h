@@ -76,9 +76,10 @@ typedef struct AiranCeRefinerBatchV1 {
 } AiranCeRefinerBatchV1;
 int32_t airan_ce_refiner_Batch(const AiranCeRefiner *refiner, AiranCeRefinerBatchV1 *batch);
 
-/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the synthesised batch;
- * loss_out (device fp32, may be NULL) receives the batch loss before the
- * update. */
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the synthesised batch.
+ * The loss weights each slot's squared error by the slot's linear SNR
+ * (weights of mean 1 over the batch), so every SNR counts alike; loss_out
+ * (device fp32, may be NULL) receives it before the update. */
 int32_t airan_ce_refiner_TrainStep(AiranCeRefiner *refiner, uint32_t slots, float lr, float *loss_out,
                                    void *stream);
 /* Forward pass of the current weights on the synthesised batch (result in

@@ -113,6 +113,7 @@ def test_native_chain_matches_op_chain(gpu_stream, prof, snr):
             "cw_llr": nat.fetch("cw_llr", torch.int16, S * g.qm * g.cw_stride),
             "bits": nat.fetch("bits", torch.int8, S * plan.C * plan.K),
             "iterations": nat.fetch("iterations", torch.uint8, S * plan.C),
+            "cb_ok": nat.fetch("cb_ok", torch.uint8, S * plan.C),
             "tb_ok": nat.fetch("tb_ok", torch.uint8, S),
         }
         # The decoder reads the codeword LLRs directly when it can (no
@@ -142,6 +143,7 @@ def test_native_chain_matches_op_chain(gpu_stream, prof, snr):
                 assert np.array_equal(part("ldpc_llr", plan.C * 68 * plan.Z), r["ldpc_llr"].cpu().numpy())
             assert np.array_equal(part("bits", plan.C * plan.K), r["bits"].cpu().numpy())
             assert np.array_equal(part("iterations", plan.C), r["iterations"].cpu().numpy())
+            assert np.array_equal(part("cb_ok", plan.C), r["cb_ok"].cpu().numpy())
             assert int(nat_np["tb_ok"][s]) == int(r["tb_ok"].item()) == 1
             assert np.array_equal(tb_bits(plan, torch.from_numpy(part("bits", plan.C * plan.K))), payload[s])
     nat.close()
