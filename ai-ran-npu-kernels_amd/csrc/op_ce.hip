@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(airan::kGemmThreads) CeProjectKernel(const __h
         const uint32_t slot = n / ncol, col = n - slot * ncol;
         tp[((static_cast<size_t>(slot) * c.nm + q) * kMaxRank + k) * ncol + col] = v;
     };
-    airan::CGemmTile<true>(m0, n0, md.rank, N, len, la, lb, st);
+    airan::CGemmTileMfma<true>(m0, n0, md.rank, N, len, la, lb, st);
 }
 
 // t = ((tp[0] + tp[1]) + tp[2]) + tp[3] ... in split order.  Grid over
@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(airan::kGemmThreads) CeInterpGemmKernel(CeWs w
         const uint32_t slot = n / ncol, col = n - slot * ncol;
         w.hf[((static_cast<size_t>(slot) * c.nm + q) * ncol + col) * c.used + sc] = v;
     };
-    airan::CGemmTile<false>(m0, n0, c.used, N, md.rank, la, lb, st);
+    airan::CGemmTileMfma<false>(m0, n0, c.used, N, md.rank, la, lb, st);
 }
 
 // (R_tt + e I) for D DMRS symbols, eliminated once (no pivoting; the matrix

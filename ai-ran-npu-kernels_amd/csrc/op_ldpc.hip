@@ -400,7 +400,29 @@ __global__ void __launch_bounds__(384, 6) LdpcFastKernel(const int16_t *__restri
         const int16_t v = FUSED ? airan::RateDematchValue(t, rc, d.rate, src) : blk[t];
         return static_cast<h16>(static_cast<float>(v) * (1.0f / 256.0f));
     };
-    if (FUSED) {
+    if (FUSED && airan::RateSingle(rc, d.rate)) {
+        // One LLR (or a constant) per position: four positions per thread
+        // per pass with their loads in flight together.
+        const int16_t *base = src + static_cast<size_t>(rc.slot0) * d.rate.qm * d.rate.cw_stride + rc.rem0;
+        for (uint32_t t0 = i; t0 < nl; t0 += 4u * blockDim.x) {
+            int32_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t t = t0 + static_cast<uint32_t>(u) * blockDim.x;
+                const int32_t idx = t < nl ? airan::RateSingleIndex(t, rc, d.rate) : -1;
+                v[u] = idx == -2 ? 5120 : 0;
+                if (idx >= 0) {
+                    const int32_t x = base[idx] * 8;
+                    v[u] = x > 5120 ? 5120 : (x < -5120 ? -5120 : x);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t t = t0 + static_cast<uint32_t>(u) * blockDim.x;
+                if (t < nl) lam[t] = static_cast<h16>(static_cast<float>(v[u]) * (1.0f / 256.0f));
+            }
+        }
+    } else if (FUSED) {
         for (uint32_t t = i; t < nl; t += blockDim.x) lam[t] = input(t);
     } else {
         // nl is a multiple of 4 (Z is a multiple of 24) and blk 8-byte aligned.

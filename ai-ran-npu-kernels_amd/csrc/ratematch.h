@@ -50,6 +50,32 @@ __device__ inline uint32_t RateDivEq(uint32_t kk, const RateCb &c)
     return b;
 }
 
+// Every transmitted bit lands on its own circular-buffer position (no
+// repetition: E <= Ncb - filler) and the block's codeword symbols lie in one
+// slot: position p then takes at most one LLR (RateDematchSingle).
+__device__ inline bool RateSingle(const RateCb &c, const RateDev &r)
+{
+    return c.e <= r.ncb - r.filler && c.rem0 + c.eq <= r.cw_symbols;
+}
+
+// RateDematchValue under RateSingle: the source element index of position p
+// in the block's slot (planes [Qm][cw_stride] from the block's first
+// symbol), or -1 for a zero LLR, -2 for a filler bit (+5120).
+__device__ inline int32_t RateSingleIndex(uint32_t p, const RateCb &c, const RateDev &r)
+{
+    const uint32_t two_z = 2u * r.z;
+    if (p < two_z || p >= two_z + r.ncb) return -1;
+    const uint32_t k = p - two_z;
+    const uint32_t fstart = r.kprime - two_z;
+    if (k >= fstart && k < fstart + r.filler) return -2;
+    const uint32_t nf = r.ncb - r.filler;
+    const uint32_t q = k < fstart ? k : k - r.filler;
+    const uint32_t kk = q >= c.q0 ? q - c.q0 : q + nf - c.q0;
+    if (kk >= c.e) return -1;
+    const uint32_t b = RateDivEq(kk, c);
+    return static_cast<int32_t>(b * r.cw_stride + (kk - b * c.eq));
+}
+
 __device__ inline int16_t RateDematchValue(uint32_t p, const RateCb &c, const RateDev &r,
                                            const int16_t *__restrict__ in)
 {
