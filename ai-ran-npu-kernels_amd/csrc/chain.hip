@@ -204,6 +204,7 @@ struct AiranPuschChain {
     AiranCeRefiner *refiner = nullptr;
     float refiner_snr_min = 0.0f, refiner_snr_max = 0.0f;
     uint8_t *refiner_keep = nullptr;
+    uint32_t *crc_xpow = nullptr;  // [C] code-block shifts of the TB CRC remainder
     // Stage timing (events before the first stage and after each stage).
     bool timing = false;
     hipEvent_t ev[AIRAN_CHAIN_STAGES + 1] = {};
@@ -296,6 +297,7 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
         airan::CrcFuse crc{};
         if ((st = airan::CrcFuseSetup(c->plan, c->cb_ok, reinterpret_cast<uint32_t *>(c->crc_ws), &crc)) != AIRAN_OK)
             return st;
+        crc.xpow = c->crc_xpow;
         airan::LaunchCrcZero(reinterpret_cast<uint32_t *>(c->crc_ws), slots, s);
         st = airan::LaunchLdpcFromCodeword(g, c->plan, c->desc, c->cw, ncb, c->active_rows, c->opts.ldpc_iterations,
                                            c->opts.early_stop != 0, c->bits, c->iters, &crc, s);
@@ -406,9 +408,16 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         !Alloc(&c->cw, S * g.qm * g.cw_stride, &b) || !Alloc(&c->desc, ncb, &b) ||
         !Alloc(&c->lam, c->fused_ldpc ? 0 : ncb * 68 * p.Z, &b) || !Alloc(&c->ldpc_ws, c->ldpc_ws_bytes, &b) ||
         !Alloc(&c->bits, ncb * p.K, &b) || !Alloc(&c->iters, ncb, &b) || !Alloc(&c->cb_ok, ncb, &b) ||
-        !Alloc(&c->tb_ok, S, &b) || !Alloc(&c->refiner_keep, S, &b) || !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
+        !Alloc(&c->tb_ok, S, &b) || !Alloc(&c->refiner_keep, S, &b) || !Alloc(&c->crc_xpow, p.C, &b) || !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
         !Alloc(reinterpret_cast<uint8_t **>(&c->zero_iq), g.nr * g.samples * iq_elem, &b))
         return fail(AIRAN_RESOURCE_TOO_SMALL);
+    // Per-code-block shifts of the transport-block CRC remainder.
+    {
+        std::vector<uint32_t> xp(p.C);
+        for (uint32_t r = 0; r < p.C; ++r) xp[r] = airan::CrcXpowHost(p, r);
+        if (hipMemcpy(c->crc_xpow, xp.data(), p.C * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(AIRAN_LAUNCH_FAILED);
+    }
     // DMRS reference of this slot configuration.
     MimoDmrsGenRuntimeV1 *rt = nullptr;
     if ((st = airan_mimo_dmrs_gen_CreateRuntime(&rt)) != AIRAN_OK) return fail(st);
@@ -466,7 +475,7 @@ void airan_pusch_chain_Destroy(AiranPuschChain *c)
     airan_channel_est_lmmse_DestroyFilter(c->filter);
     void *bufs[] = {c->dmrs_re, c->dmrs_im, c->rx_re, c->rx_im, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce,
                     c->ce_ws, c->h_re, c->h_im, c->gold, c->cw, c->desc, c->lam, c->ldpc_ws, c->bits,
-                    c->iters, c->cb_ok, c->tb_ok, c->crc_ws, c->zero_iq, c->refiner_keep};
+                    c->iters, c->cb_ok, c->tb_ok, c->crc_ws, c->zero_iq, c->refiner_keep, c->crc_xpow};
     for (void *p : bufs) Free(p);
     delete c;
 }

@@ -18,6 +18,7 @@ struct CrcFuse {
     const uint32_t *pow_tb;  // x^(len + e) mod g_TB (CRC24A or CRC16)
     uint8_t *cb_ok;          // [blocks]; nullptr: no check
     uint32_t *tb_acc;        // [TBs] XOR accumulators, zero before the decoder
+    const uint32_t *xpow;    // [C] x^((C - 1 - r) L) mod g_TB per code block r, or nullptr
     uint32_t C, kprime, crc_cb, L, poly_tb, len_tb;
 };
 
@@ -49,12 +50,17 @@ __device__ inline void CrcFinishBlock(const CrcFuse &c, uint32_t cbg, uint32_t r
 {
     const uint32_t tb = cbg / c.C, r = cbg - tb * c.C;
     c.cb_ok[cbg] = c.crc_cb ? static_cast<uint8_t>(rb == 0) : static_cast<uint8_t>(1);
-    const uint32_t after = (c.C - 1 - r) * c.L;
-    uint32_t xpow = 1u, base = 2u, e = after;
-    while (e != 0) {
-        if (e & 1u) xpow = CrcMulMod(xpow, base, c.poly_tb, c.len_tb);
-        base = CrcMulMod(base, base, c.poly_tb, c.len_tb);
-        e >>= 1;
+    uint32_t xpow = 1u;
+    if (c.xpow != nullptr) {
+        xpow = c.xpow[r];
+    } else {
+        // Square-and-multiply from x^1 (serial: a table avoids it).
+        uint32_t base = 2u, e = (c.C - 1 - r) * c.L;
+        while (e != 0) {
+            if (e & 1u) xpow = CrcMulMod(xpow, base, c.poly_tb, c.len_tb);
+            base = CrcMulMod(base, base, c.poly_tb, c.len_tb);
+            e >>= 1;
+        }
     }
     atomicXor(&c.tb_acc[tb], CrcMulMod(ra, xpow, c.poly_tb, c.len_tb));
 }

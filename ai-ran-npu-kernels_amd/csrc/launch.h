@@ -45,6 +45,9 @@ int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &plan, const R
 
 // CRC check of decoded bits inside the decoder (crc.h).
 int32_t CrcFuseSetup(const AiranTbPlanV1 &plan, uint8_t *cb_ok, uint32_t *tb_acc, CrcFuse *out);
+// x^((C - 1 - r) L) mod g_TB: the shift of code block r's remainder into its
+// transport block's (a per-plan table for CrcFuse::xpow).
+uint32_t CrcXpowHost(const AiranTbPlanV1 &plan, uint32_t r);
 void LaunchCrcZero(uint32_t *tb_acc, uint32_t num_tbs, hipStream_t s);
 void LaunchCrcFinish(const uint32_t *tb_acc, uint8_t *tb_ok, uint32_t num_tbs, hipStream_t s);
 

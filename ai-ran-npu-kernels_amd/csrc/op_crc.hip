@@ -150,6 +150,31 @@ __global__ void __launch_bounds__(256) PackKernel(const int8_t *__restrict__ bit
 
 namespace airan {
 
+uint32_t CrcXpowHost(const AiranTbPlanV1 &p, uint32_t r)
+{
+    const uint32_t poly = p.crc_tb == 24 ? kPolyA : kPoly16, len = p.crc_tb;
+    const uint32_t mask = (1u << len) - 1u;
+    uint64_t e = static_cast<uint64_t>(p.C - 1 - r) * (p.Kprime - p.crc_cb);
+    // x^e mod g by square-and-multiply on the host.
+    auto mul = [&](uint32_t a, uint32_t b) {
+        uint32_t res = 0;
+        for (int i = static_cast<int>(len) - 1; i >= 0; --i) {
+            const uint32_t top = (res >> (len - 1)) & 1u;
+            res = (res << 1) & mask;
+            if (top) res ^= poly;
+            if ((b >> i) & 1u) res ^= a;
+        }
+        return res;
+    };
+    uint32_t x = 1u, base = 2u;
+    while (e != 0) {
+        if (e & 1u) x = mul(x, base);
+        base = mul(base, base);
+        e >>= 1;
+    }
+    return x;
+}
+
 int32_t CrcFuseSetup(const AiranTbPlanV1 &p, uint8_t *cb_ok, uint32_t *tb_acc, CrcFuse *out)
 {
     if (p.C == 0 || p.Kprime > p.K || p.K > 8448 || p.Kprime < p.crc_cb || (p.crc_tb != 24 && p.crc_tb != 16) ||
@@ -159,7 +184,7 @@ int32_t CrcFuseSetup(const AiranTbPlanV1 &p, uint8_t *cb_ok, uint32_t *tb_acc, C
     CrcTables *t = nullptr;
     const int32_t st = EnsureTables(&t);
     if (st != AIRAN_OK) return st;
-    *out = CrcFuse{t->pow_b, p.crc_tb == 24 ? t->pow_a : t->pow_16, cb_ok, tb_acc, p.C, p.Kprime, p.crc_cb,
+    *out = CrcFuse{t->pow_b, p.crc_tb == 24 ? t->pow_a : t->pow_16, cb_ok, tb_acc, nullptr, p.C, p.Kprime, p.crc_cb,
                    p.Kprime - p.crc_cb, p.crc_tb == 24 ? kPolyA : kPoly16, p.crc_tb};
     return AIRAN_OK;
 }
@@ -198,8 +223,8 @@ extern "C" int32_t airan_crc_check_Enqueue(const AiranTbPlanV1 *p, uint32_t num_
     CrcDev d{p->C, p->K, p->Kprime, p->crc_cb, p->crc_tb, p->tbs,
              p->crc_tb == 24 ? kPolyA : kPoly16, p->crc_tb};
     airan::CrcFuse c{t->pow_b, p->crc_tb == 24 ? t->pow_a : t->pow_16, static_cast<uint8_t *>(cb_ok),
-                     static_cast<uint32_t *>(workspace), p->C, p->Kprime, p->crc_cb, p->Kprime - p->crc_cb,
-                     d.poly_tb, d.len_tb};
+                     static_cast<uint32_t *>(workspace), nullptr, p->C, p->Kprime, p->crc_cb,
+                     p->Kprime - p->crc_cb, d.poly_tb, d.len_tb};
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(ZeroKernel, dim3(airan::CeilDiv(num_tbs, 256)), dim3(256), 0, s, static_cast<uint32_t *>(workspace), num_tbs);
     hipLaunchKernelGGL(CbCrcKernel, dim3(num_tbs * p->C), dim3(256), 0, s, static_cast<const int8_t *>(bits), c, p->K);

@@ -401,26 +401,25 @@ __global__ void __launch_bounds__(384, 6) LdpcFastKernel(const int16_t *__restri
         return static_cast<h16>(static_cast<float>(v) * (1.0f / 256.0f));
     };
     if (FUSED && airan::RateSingle(rc, d.rate)) {
-        // One LLR (or a constant) per position: four positions per thread
-        // per pass with their loads in flight together.
+        // One LLR (or a constant) per position; a thread holds at most
+        // kMaxLoad positions (22 + rows <= 30 columns of Z, blockDim >= Z or
+        // 64 for Z < 64), so every load of the block is issued before the
+        // first is waited on.
+        constexpr int kMaxLoad = 30;
         const int16_t *base = src + static_cast<size_t>(rc.slot0) * d.rate.qm * d.rate.cw_stride + rc.rem0;
-        for (uint32_t t0 = i; t0 < nl; t0 += 4u * blockDim.x) {
-            int32_t v[4];
+        int32_t v[kMaxLoad];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t t = t0 + static_cast<uint32_t>(u) * blockDim.x;
-                const int32_t idx = t < nl ? airan::RateSingleIndex(t, rc, d.rate) : -1;
-                v[u] = idx == -2 ? 5120 : 0;
-                if (idx >= 0) {
-                    const int32_t x = base[idx] * 8;
-                    v[u] = x > 5120 ? 5120 : (x < -5120 ? -5120 : x);
-                }
-            }
+        for (int u = 0; u < kMaxLoad; ++u) {
+            const uint32_t t = i + static_cast<uint32_t>(u) * blockDim.x;
+            const int32_t idx = t < nl ? airan::RateSingleIndex(t, rc, d.rate) : -1;
+            v[u] = idx == -2 ? 5120 : 0;
+            if (idx >= 0) v[u] = base[idx] * 8;
+        }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t t = t0 + static_cast<uint32_t>(u) * blockDim.x;
-                if (t < nl) lam[t] = static_cast<h16>(static_cast<float>(v[u]) * (1.0f / 256.0f));
-            }
+        for (int u = 0; u < kMaxLoad; ++u) {
+            const uint32_t t = i + static_cast<uint32_t>(u) * blockDim.x;
+            const int32_t x = v[u] > 5120 ? 5120 : (v[u] < -5120 ? -5120 : v[u]);
+            if (t < nl) lam[t] = static_cast<h16>(static_cast<float>(x) * (1.0f / 256.0f));
         }
     } else if (FUSED) {
         for (uint32_t t = i; t < nl; t += blockDim.x) lam[t] = input(t);
@@ -606,7 +605,8 @@ int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const Rate
                                int8_t *bits, uint8_t *iters_out, const CrcFuse *crc, hipStream_t s)
 {
     if (num_cb == 0 || !FastEligible(AIRAN_LDPC_MODE_FP16, p.Z, rows) || rows < 4 || iters == 0 || iters > 100 ||
-        (p.Z % 24) != 0 || p.K == 0 || p.K > 22 * p.Z || p.Ncb <= p.filler || (reinterpret_cast<uintptr_t>(bits) & 3u))
+        (p.Z % 24) != 0 || p.K == 0 || p.K > 22 * p.Z || p.Ncb <= p.filler || (reinterpret_cast<uintptr_t>(bits) & 3u) ||
+        p.E_low >= (1u << 22) || p.E_high >= (1u << 22))
         return AIRAN_UNSUPPORTED_PROFILE;
     int32_t st = LoadLiftTable();
     if (st != AIRAN_OK) return st;

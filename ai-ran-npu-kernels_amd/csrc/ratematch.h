@@ -72,7 +72,11 @@ __device__ inline int32_t RateSingleIndex(uint32_t p, const RateCb &c, const Rat
     const uint32_t q = k < fstart ? k : k - r.filler;
     const uint32_t kk = q >= c.q0 ? q - c.q0 : q + nf - c.q0;
     if (kk >= c.e) return -1;
-    const uint32_t b = RateDivEq(kk, c);
+    // kk < E < 2^22 (checked by the launcher): the fp32 quotient is within
+    // one of kk / eq, so one correction each way.
+    uint32_t b = static_cast<uint32_t>(static_cast<float>(kk) * c.rcp_eq);
+    b = b * c.eq > kk ? b - 1u : b;
+    b = (b + 1u) * c.eq <= kk ? b + 1u : b;
     return static_cast<int32_t>(b * r.cw_stride + (kk - b * c.eq));
 }
 
