@@ -1,7 +1,9 @@
-// cgemm.h -- complex fp32 GEMM tiles for the channel estimator's batched
-// projections: VALU FMA (CGemmTile) and fp32 MFMA (CGemmTileMfma), both with
-// LDS-staged operands (fp32 throughout: the filter's dynamic range is too
-// wide for bf16/fp16 operands).
+// cgemm.h -- complex GEMM building blocks for the channel estimator's batched
+// projections: VALU FMA (CGemmTile) and fp32 MFMA (CGemmTileMfma) tiles with
+// LDS-staged fp32 operands, and split-precision bf16 MFMA blocks (below) for
+// operands pre-packed in fragment order.  Plain bf16/fp16 operands would not
+// do: the filter's dynamic range and the subspace noise estimate need more
+// than 11 significand bits.
 //
 // One workgroup of 256 threads computes a 64 x 64 tile of C = A B with
 // K-steps of 16: A (64 x 16) and B (16 x 64) complex tiles are staged in LDS
@@ -146,6 +148,109 @@ __device__ inline void CGemmTileMfma(uint32_t m0, uint32_t n0, uint32_t M, uint3
                 const uint32_t m = m0 + wm + 16 * i + 4 * kq + q, n = n0 + wn + 16 * j + l16;
                 if (m < M && n < N) st(m, n, make_float2(acc_r[i][j][q], acc_i[i][j][q]));
             }
+}
+
+// ---------------------------------------------------------------------------
+// Split-precision complex products on bf16 MFMA (v_mfma_f32_32x32x16_bf16).
+// An fp32 value x is carried as x_hi = bf16(x) and x_lo = bf16(x - x_hi)
+// (16 significand bits, |x - x_hi - x_lo| <= 2^-17 |x|) and a product as
+// x_hi y_hi + x_hi y_lo + x_lo y_hi, accumulated in fp32: relative error
+// about 2^-16 per product (the dropped x_lo y_lo is below 2^-16 |x y|), the
+// fp32 exponent range throughout.
+//
+// Operands live in "fragment planes": a complex matrix X with rows r and K
+// columns is stored as bf16
+//   [r / 32][k / 16][plane][(k % 16) / 8][r % 32][k % 8]
+// with planes re_hi, re_lo, im_hi, im_lo.  The eight K values that lane l
+// of a 32x32x16 MFMA takes for row l % 32 (k = 8 (l / 32) + j, for both the
+// A and the B operand) are then one 16-byte load, and a wave's load of one
+// plane of a 32 x 16 block is 1 KB contiguous.  Accumulator element i of lane
+// l holds C row 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t kFragPlane = 32 * 16;       // bf16 per plane of a 32 x 16 block
+constexpr uint32_t kFragBlock = 4 * kFragPlane;
+
+__device__ inline size_t FragIndex(uint32_t r, uint32_t k, uint32_t plane, uint32_t kblocks)
+{
+    return (((static_cast<size_t>(r >> 5) * kblocks + (k >> 4)) * 4 + plane) * 2 + ((k >> 3) & 1u)) * 256 +
+           (r & 31u) * 8 + (k & 7u);
+}
+
+// Round-to-nearest-even bf16 of a finite fp32 value (bits).
+__device__ inline uint32_t Bf16Bits(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+// hi | lo << 16 of x.
+__device__ inline uint32_t Bf16Split(float x)
+{
+    const uint32_t hi = Bf16Bits(x);
+    return hi | Bf16Bits(x - __uint_as_float(hi << 16)) << 16;
+}
+
+// Splits v[0..7] (row r, K values k0 .. k0 + 7, k0 a multiple of 8) into the
+// four planes of `base` (fragment order, `kblocks` = K / 16).
+__device__ inline void StoreFrag8(uint16_t *__restrict__ base, uint32_t r, uint32_t k0, uint32_t kblocks,
+                                  const float2 *v)
+{
+    uint32_t w[4][4] = {};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t sr = Bf16Split(v[j].x), si = Bf16Split(v[j].y), sh = (j & 1) * 16;
+        w[0][j >> 1] |= (sr & 0xffffu) << sh;
+        w[1][j >> 1] |= (sr >> 16) << sh;
+        w[2][j >> 1] |= (si & 0xffffu) << sh;
+        w[3][j >> 1] |= (si >> 16) << sh;
+    }
+#pragma unroll
+    for (uint32_t p = 0; p < 4; ++p)
+        *reinterpret_cast<uint4 *>(base + FragIndex(r, k0, p, kblocks)) = make_uint4(w[p][0], w[p][1], w[p][2], w[p][3]);
+}
+
+// One complex 32 x 16 operand block, 8 K values per plane for this lane.
+struct CFrag {
+    uint4 rh, rl, ih, il;
+};
+
+__device__ inline CFrag LoadFrag(const uint16_t *__restrict__ block, uint32_t lane)
+{
+    const uint4 *p = reinterpret_cast<const uint4 *>(block) + lane;
+    return CFrag{p[0], p[kFragPlane / 8], p[2 * kFragPlane / 8], p[3 * kFragPlane / 8]};
+}
+
+__device__ inline uint4 NegBf16x8(uint4 v)
+{
+    return make_uint4(v.x ^ 0x80008000u, v.y ^ 0x80008000u, v.z ^ 0x80008000u, v.w ^ 0x80008000u);
+}
+
+__device__ inline f32x16_t MfmaBf16(uint4 a, uint4 b, f32x16_t c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
+}
+
+// C += A B for one 32 x 32 complex block over 16 K values (12 MFMAs):
+// re += ar br - ai bi, im += ar bi + ai br, each product split as above;
+// nai_h / nai_l are -ai's planes.  The order of the terms is fixed.
+__device__ inline void CMfmaSplit(f32x16_t &cr, f32x16_t &ci, const CFrag &a, uint4 nai_h, uint4 nai_l,
+                                  const CFrag &b)
+{
+    cr = MfmaBf16(a.rh, b.rh, cr);
+    cr = MfmaBf16(a.rh, b.rl, cr);
+    cr = MfmaBf16(a.rl, b.rh, cr);
+    cr = MfmaBf16(nai_h, b.ih, cr);
+    cr = MfmaBf16(nai_h, b.il, cr);
+    cr = MfmaBf16(nai_l, b.ih, cr);
+    ci = MfmaBf16(a.rh, b.ih, ci);
+    ci = MfmaBf16(a.rh, b.il, ci);
+    ci = MfmaBf16(a.rl, b.ih, ci);
+    ci = MfmaBf16(a.ih, b.rh, ci);
+    ci = MfmaBf16(a.ih, b.rl, ci);
+    ci = MfmaBf16(a.il, b.rh, ci);
 }
 
 }  // namespace airan

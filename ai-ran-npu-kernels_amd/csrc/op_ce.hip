@@ -35,6 +35,7 @@
 #include "launch.h"
 
 using airan::Geom;
+using airan::f32x16_t;
 
 namespace {
 
@@ -47,6 +48,9 @@ constexpr double kMinNoiseDims = 32.0;     // residual dimensions for the subspa
 // (partial products summed in a fixed order), independent of the batch, so
 // results do not depend on how many slots share a launch.
 constexpr uint32_t kProjSplits = 4;
+// Interpolation GEMM: a workgroup computes kInterpTile columns (slot,
+// receive antenna, DMRS symbol) x kInterpTile subcarriers.
+constexpr uint32_t kInterpTile = 128;
 
 // One observation model: n observations at offset + step * p.
 struct CeModelFactors {
@@ -60,6 +64,7 @@ struct CeModelFactors {
     uint32_t kept = 0;   // eigenpairs actually kept
     float2 *bt = nullptr;  // [n][rank]    bt[p][k] = conj(U[p][k])
     float2 *at = nullptr;  // [rank][used] (R_hp U) transposed, subcarrier contiguous
+    uint16_t *a16 = nullptr;  // R_hp U in bf16 fragment planes: rows sc < AlignUp(used, kInterpTile), K = rank
     float *lam = nullptr;  // [rank] eigenvalues (1 for padded columns)
 };
 
@@ -143,10 +148,26 @@ __global__ void __launch_bounds__(256) PackAtKernel(const double2 *__restrict__ 
     at[static_cast<size_t>(k) * used + sc] = o;
 }
 
+// a16 = at transposed into bf16 fragment planes (rows sc < sc_pad, zero
+// beyond used; K = rank), one thread per subcarrier and 8 eigenvectors.
+__global__ void __launch_bounds__(256) PackA16Kernel(const float2 *__restrict__ at, uint16_t *__restrict__ a16,
+                                                     uint32_t used, uint32_t sc_pad, uint32_t rank)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= sc_pad * (rank / 8u)) return;
+    const uint32_t sc = e % sc_pad, k0 = (e / sc_pad) * 8u;
+    float2 v[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+        v[j] = sc < used ? at[static_cast<size_t>(k0 + j) * used + sc] : make_float2(0.0f, 0.0f);
+    airan::StoreFrag8(a16, sc, k0, rank / 16u, v);
+}
+
 // ---------------------------------------------------------------- apply
 struct CeModelDev {
     const float2 *bt;
     const float2 *at;
+    const uint16_t *a16;
     const float *lam;
     uint32_t n, rank, kept, in_a, in_b;
     float sign_b;
@@ -170,6 +191,9 @@ struct CeApply {
 //   hf     [slots][models][NR*D][used]      c64   frequency estimate (unscaled)
 //   tp     [kProjSplits][slots][models][kMaxRank][NR*D]  c64  partial U^H z
 //          over one of kProjSplits fixed ranges of the observations
+//   tb     [models][TbModelElems]            bf16  t / (lam_k + n_q) in fragment
+//          planes: rows n = slot * NR*D + col < AlignUp(slots * NR*D,
+//          kInterpTile) (zero beyond), K = the model's rank
 struct CeWs {
     float *stats;
     double *energy;
@@ -178,20 +202,28 @@ struct CeWs {
     float2 *t;
     float2 *hf;
     float2 *tp;
+    uint16_t *tb;
+    uint32_t npad;  // rows of tb per model
 };
 
 size_t Align256(size_t b) { return (b + 255) / 256 * 256; }
+
+uint32_t TbRows(const Geom &g, uint32_t slots) { return airan::AlignUp(slots * g.nr * g.ndmrs, kInterpTile); }
+
+__host__ __device__ inline size_t TbModelElems(uint32_t npad) { return static_cast<size_t>(npad) * kMaxRank * 4; }
 
 size_t CeWsLayout(const Geom &g, uint32_t slots, uint8_t *base, CeWs *w)
 {
     const size_t sm = static_cast<size_t>(slots) * g.nl;  // models <= layers
     const size_t ncol = static_cast<size_t>(g.nr) * g.ndmrs;
-    const size_t sizes[7] = {Align256(16u * slots), Align256(sm * ncol * 16), Align256(sm * kMaxRank * 4),
+    const uint32_t npad = TbRows(g, slots);
+    const size_t sizes[8] = {Align256(16u * slots), Align256(sm * ncol * 16), Align256(sm * kMaxRank * 4),
                              Align256(sm * g.used * 4), Align256(sm * kMaxRank * ncol * 8),
-                             Align256(sm * ncol * g.used * 8), Align256(kProjSplits * sm * kMaxRank * ncol * 8)};
+                             Align256(sm * ncol * g.used * 8), Align256(kProjSplits * sm * kMaxRank * ncol * 8),
+                             Align256(g.nl * TbModelElems(npad) * 2)};
     size_t off = 0;
-    void *ptrs[7];
-    for (int i = 0; i < 7; ++i) {
+    void *ptrs[8];
+    for (int i = 0; i < 8; ++i) {
         ptrs[i] = base ? base + off : nullptr;
         off += sizes[i];
     }
@@ -203,6 +235,8 @@ size_t CeWsLayout(const Geom &g, uint32_t slots, uint8_t *base, CeWs *w)
         w->t = static_cast<float2 *>(ptrs[4]);
         w->hf = static_cast<float2 *>(ptrs[5]);
         w->tp = static_cast<float2 *>(ptrs[6]);
+        w->tb = static_cast<uint16_t *>(ptrs[7]);
+        w->npad = npad;
     }
     return off;
 }
@@ -401,31 +435,95 @@ __global__ void __launch_bounds__(256) CeHvKernel(CeWs w, uint32_t slots, CeAppl
         if (s0 + i < slots) w.hv[(static_cast<size_t>(s0 + i) * c.nm + q) * c.used + sc] = acc[i];
 }
 
-// hf[slot, q][col][sc] = sum_k A[k][sc] t[slot, q][k][col] / (lam_k + n_q):
-// per model q = blockIdx.z a GEMM with M = used, N = slots * NR * D, K = rank.
-__global__ void __launch_bounds__(airan::kGemmThreads) CeInterpGemmKernel(CeWs w, uint32_t slots, CeApply c)
+// tb[q] = t[slot, q][k][col] / (lam_k + n_q) in bf16 fragment planes, row
+// n = slot * NR*D + col (zero for n >= slots * NR*D), one thread per row and
+// 8 eigenvectors.  Grid (npad * kMaxRank / 8 / 256, models).
+__global__ void __launch_bounds__(256) CeScaleSplitKernel(CeWs w, uint32_t slots, CeApply c)
 {
-    const uint32_t q = blockIdx.z;
+    const uint32_t q = blockIdx.y;
     const CeModelDev &md = c.m[q];
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = e % w.npad, k0 = (e / w.npad) * 8u;
+    if (k0 >= md.rank) return;
     const uint32_t ncol = c.nr * c.ndmrs;
-    const uint32_t N = slots * ncol;
-    const uint32_t m0 = blockIdx.x * airan::kGemmTile, n0 = blockIdx.y * airan::kGemmTile;
-    auto la = [&](uint32_t sc, uint32_t k) {
-        return sc < c.used ? md.at[static_cast<size_t>(k) * c.used + sc] : make_float2(0.0f, 0.0f);
-    };
-    auto lb = [&](uint32_t k, uint32_t n) {
-        if (n >= N) return make_float2(0.0f, 0.0f);
+    float2 v[8];
+    if (n < slots * ncol) {
         const uint32_t slot = n / ncol, col = n - slot * ncol;
         const size_t sq = static_cast<size_t>(slot) * c.nm + q;
-        const float2 v = w.t[(sq * kMaxRank + k) * ncol + col];
-        const float s = w.inv[sq * kMaxRank + k];
-        return make_float2(v.x * s, v.y * s);
-    };
-    auto st = [&](uint32_t sc, uint32_t n, float2 v) {
-        const uint32_t slot = n / ncol, col = n - slot * ncol;
-        w.hf[((static_cast<size_t>(slot) * c.nm + q) * ncol + col) * c.used + sc] = v;
-    };
-    airan::CGemmTileMfma<false>(m0, n0, c.used, N, md.rank, la, lb, st);
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            const float2 x = w.t[(sq * kMaxRank + k0 + j) * ncol + col];
+            const float s = w.inv[sq * kMaxRank + k0 + j];
+            v[j] = make_float2(x.x * s, x.y * s);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) v[j] = make_float2(0.0f, 0.0f);
+    }
+    airan::StoreFrag8(w.tb + q * TbModelElems(w.npad), n, k0, md.rank / 16u, v);
+}
+
+// hf[slot, q][col][sc] = sum_k (t[slot, q][k][col] / (lam_k + n_q)) A[k][sc]
+// on split bf16 MFMA: C = TB A^T with rows n (slot, col) and columns sc.
+// Workgroup: a kInterpTile x kInterpTile tile of one model, four waves of
+// 64 x 64 (2 x 2 blocks of 32 x 32), operands loaded straight from the
+// fragment planes.  Workgroups are handed out so that each XCD works through
+// a contiguous range of tiles with the row tiles of one subcarrier tile
+// adjacent: the subcarrier-side operand is read once per XCD, the small
+// row-side operand stays in every L2.  Linear grid of 8 * per_xcd.
+__global__ void __launch_bounds__(256, 2) CeInterpMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t sc_tiles,
+                                                          uint32_t per_xcd)
+{
+    const uint32_t xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
+    const uint32_t n_tiles = w.npad / kInterpTile;
+    const uint32_t tile = xcd * per_xcd + j;
+    if (tile >= c.nm * sc_tiles * n_tiles) return;
+    const uint32_t nt = tile % n_tiles, rest = tile / n_tiles;
+    const uint32_t st = rest % sc_tiles, q = rest / sc_tiles;
+    const CeModelDev &md = c.m[q];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t nb0 = (nt * kInterpTile) / 32u + (wave & 1u) * 2u;
+    const uint32_t sb0 = (st * kInterpTile) / 32u + (wave >> 1) * 2u;
+    const uint32_t kb_count = md.rank / 16u;
+    const uint16_t *ta = w.tb + q * TbModelElems(w.npad);
+    f32x16_t acc_r[2][2], acc_i[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc_r[a][b][i] = acc_i[a][b][i] = 0.0f;
+    for (uint32_t kb = 0; kb < kb_count; ++kb) {
+        airan::CFrag fa[2], fb[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+            fa[a] = airan::LoadFrag(ta + (static_cast<size_t>(nb0 + a) * kb_count + kb) * airan::kFragBlock, lane);
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            fb[b] = airan::LoadFrag(md.a16 + (static_cast<size_t>(sb0 + b) * kb_count + kb) * airan::kFragBlock, lane);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const uint4 nh = airan::NegBf16x8(fa[a].ih), nl = airan::NegBf16x8(fa[a].il);
+#pragma unroll
+            for (int b = 0; b < 2; ++b) airan::CMfmaSplit(acc_r[a][b], acc_i[a][b], fa[a], nh, nl, fb[b]);
+        }
+    }
+    const uint32_t ncol = c.nr * c.ndmrs, N = slots * ncol;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const uint32_t sc = (sb0 + b) * 32u + (lane & 31u);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t n = (nb0 + a) * 32u + 8u * (i / 4) + 4u * (lane >> 5) + (i & 3);
+                if (n < N && sc < c.used) {
+                    const uint32_t slot = n / ncol, col = n - slot * ncol;
+                    w.hf[((static_cast<size_t>(slot) * c.nm + q) * ncol + col) * c.used + sc] =
+                        make_float2(acc_r[a][b][i], acc_i[a][b][i]);
+                }
+            }
+        }
 }
 
 // (R_tt + e I) for D DMRS symbols, eliminated once (no pivoting; the matrix
@@ -639,6 +737,10 @@ int32_t BuildModel(const Geom &g, const LmmseFilterParamsV1 &prm, hipStream_t st
                        rank, kept);
     hipLaunchKernelGGL(PackAtKernel, dim3(airan::CeilDiv(g.used, 256), rank), dim3(256), 0, stream, av, out->at,
                        g.used, rank, kept);
+    const uint32_t sc_pad = airan::AlignUp(g.used, kInterpTile);
+    if (hipMalloc(&out->a16, sizeof(uint16_t) * 4 * static_cast<size_t>(sc_pad) * rank) != hipSuccess) return fail();
+    hipLaunchKernelGGL(PackA16Kernel, dim3(airan::CeilDiv(sc_pad * (rank / 8), 256)), dim3(256), 0, stream, out->at,
+                       out->a16, g.used, sc_pad, rank);
     std::vector<float> lam_f(rank, 1.0f);  // padded columns: zero factors, unit eigenvalue
     for (uint32_t k = 0; k < kept; ++k) lam_f[k] = static_cast<float>(std::max(lam_host[n - 1 - k], 0.0));
     if (hipMemcpyAsync(out->lam, lam_f.data(), sizeof(float) * rank, hipMemcpyHostToDevice, stream) != hipSuccess ||
@@ -709,7 +811,7 @@ CeApply MakeApply(const LmmseFilter &f)
     c.nsym = g.nsym;
     for (uint32_t q = 0; q < f.nm; ++q) {
         const CeModelFactors &m = f.model[q];
-        c.m[q] = CeModelDev{m.bt, m.at, m.lam, m.n, m.rank, m.kept, m.in_a, m.in_b, m.sign_b};
+        c.m[q] = CeModelDev{m.bt, m.at, m.a16, m.lam, m.n, m.rank, m.kept, m.in_a, m.in_b, m.sign_b};
     }
     std::memcpy(c.layer_model, f.layer_model, sizeof(c.layer_model));
     std::memcpy(c.layer_sign, f.layer_sign, sizeof(c.layer_sign));
@@ -766,8 +868,11 @@ int32_t LaunchCeApply(const LmmseFilter *f, const __half *hr, const __half *hi, 
     hipLaunchKernelGGL(CeStatsKernel, dim3(slots), dim3(256), 0, s, noise_in, noise_out, w, c);
     hipLaunchKernelGGL(CeHvKernel, dim3(CeilDiv(g.used, 256), f->nm, CeilDiv(slots, 8)), dim3(256), 0, s, w, slots,
                        c);
-    hipLaunchKernelGGL(CeInterpGemmKernel, dim3(CeilDiv(g.used, kGemmTile), n_tiles, f->nm), dim3(kGemmThreads), 0,
-                       s, w, slots, c);
+    hipLaunchKernelGGL(CeScaleSplitKernel, dim3(CeilDiv(static_cast<uint64_t>(w.npad) * kMaxRank / 8, 256), f->nm),
+                       dim3(256), 0, s, w, slots, c);
+    const uint32_t sc_tiles = AlignUp(g.used, kInterpTile) / kInterpTile;
+    const uint32_t per_xcd = CeilDiv(f->nm * sc_tiles * (w.npad / kInterpTile), 8);
+    hipLaunchKernelGGL(CeInterpMfmaKernel, dim3(8 * per_xcd), dim3(256), 0, s, w, slots, c, sc_tiles, per_xcd);
     const dim3 pgrid(CeilDiv(g.pad, 256), slots * g.nl);
     switch (g.ndmrs) {
     case 1: hipLaunchKernelGGL(CePostKernel<1>, pgrid, dim3(256), 0, s, w, out_re, out_im, out_layers, c); break;
@@ -916,6 +1021,7 @@ void airan_channel_est_lmmse_DestroyFilter(LmmseFilter *f)
     for (uint32_t q = 0; q < kMaxModels; ++q) {
         FreeDev(f->model[q].bt);
         FreeDev(f->model[q].at);
+        FreeDev(f->model[q].a16);
         FreeDev(f->model[q].lam);
     }
     delete f;
