@@ -44,10 +44,14 @@ constexpr int kMaxRank = 512;
 constexpr uint32_t kMaxModels = 4;
 constexpr uint32_t kMaxObs = 2048;         // observations per model (used / 2 at 273 PRB: 1638)
 constexpr double kMinNoiseDims = 32.0;     // residual dimensions for the subspace noise estimate
-// The projection's observation axis is cut into this many fixed ranges
-// (partial products summed in a fixed order), independent of the batch, so
-// results do not depend on how many slots share a launch.
+// The projection's observation axis is cut into this many fixed ranges of
+// whole 16-observation blocks (partial products summed in a fixed order),
+// independent of the batch, so results do not depend on how many slots share
+// a launch.
 constexpr uint32_t kProjSplits = 4;
+// Projection GEMM: a workgroup computes 64 columns (slot, receive antenna,
+// DMRS symbol) x kProjKTile eigenvectors of one model and split.
+constexpr uint32_t kProjKTile = 128;
 // Interpolation GEMM: a workgroup computes kInterpTile columns (slot,
 // receive antenna, DMRS symbol) x kInterpTile subcarriers.
 constexpr uint32_t kInterpTile = 128;
@@ -63,8 +67,10 @@ struct CeModelFactors {
     uint32_t rank = 0;   // stored eigenpairs (multiple of 16; columns >= kept are zero)
     uint32_t kept = 0;   // eigenpairs actually kept
     float2 *bt = nullptr;  // [n][rank]    bt[p][k] = conj(U[p][k])
-    float2 *at = nullptr;  // [rank][used] (R_hp U) transposed, subcarrier contiguous
-    uint16_t *a16 = nullptr;  // R_hp U in bf16 fragment planes: rows sc < AlignUp(used, kInterpTile), K = rank
+    float *a2 = nullptr;   // [rank][used] |(R_hp U)[sc][k]|^2, subcarrier contiguous
+    uint16_t *a16 = nullptr;  // R_hp U, 2-part fragment planes: rows sc < AlignUp(used, kInterpTile), K = rank
+    uint16_t *w16 = nullptr;  // conj(U) transposed, 3-part fragment planes: rows k < AlignUp(rank, kProjKTile),
+                              // K = AlignUp(n, 16) observations (zero beyond n)
     float *lam = nullptr;  // [rank] eigenvalues (1 for padded columns)
 };
 
@@ -149,25 +155,47 @@ __global__ void __launch_bounds__(256) PackAtKernel(const double2 *__restrict__ 
 }
 
 // a16 = at transposed into bf16 fragment planes (rows sc < sc_pad, zero
-// beyond used; K = rank), one thread per subcarrier and 8 eigenvectors.
+// beyond used; K = rank) and a2 = |at|^2, one thread per subcarrier and 8
+// eigenvectors.
 __global__ void __launch_bounds__(256) PackA16Kernel(const float2 *__restrict__ at, uint16_t *__restrict__ a16,
-                                                     uint32_t used, uint32_t sc_pad, uint32_t rank)
+                                                     float *__restrict__ a2, uint32_t used, uint32_t sc_pad,
+                                                     uint32_t rank)
 {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= sc_pad * (rank / 8u)) return;
     const uint32_t sc = e % sc_pad, k0 = (e / sc_pad) * 8u;
     float2 v[8];
 #pragma unroll
-    for (uint32_t j = 0; j < 8; ++j)
+    for (uint32_t j = 0; j < 8; ++j) {
         v[j] = sc < used ? at[static_cast<size_t>(k0 + j) * used + sc] : make_float2(0.0f, 0.0f);
-    airan::StoreFrag8(a16, sc, k0, rank / 16u, v);
+        if (sc < used) a2[static_cast<size_t>(k0 + j) * used + sc] = v[j].x * v[j].x + v[j].y * v[j].y;
+    }
+    airan::StoreFrag8<2>(a16, sc, k0, rank / 16u, v);
+}
+
+// w16 = bt (= conj(U), [n][rank]) transposed into 3-part fragment planes:
+// rows k < k_pad (zero beyond rank), K = AlignUp(n, 16) (zero beyond n); one
+// thread per eigenvector and 8 observations.
+__global__ void __launch_bounds__(256) PackW16Kernel(const float2 *__restrict__ bt, uint16_t *__restrict__ w16,
+                                                     uint32_t n, uint32_t rank, uint32_t k_pad)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t pk = airan::AlignUp(n, 16u);
+    if (e >= k_pad * (pk / 8u)) return;
+    const uint32_t k = e % k_pad, p0 = (e / k_pad) * 8u;
+    float2 v[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+        v[j] = k < rank && p0 + j < n ? bt[static_cast<size_t>(p0 + j) * rank + k] : make_float2(0.0f, 0.0f);
+    airan::StoreFrag8<3>(w16, k, p0, pk / 16u, v);
 }
 
 // ---------------------------------------------------------------- apply
 struct CeModelDev {
     const float2 *bt;
-    const float2 *at;
+    const float *a2;
     const uint16_t *a16;
+    const uint16_t *w16;
     const float *lam;
     uint32_t n, rank, kept, in_a, in_b;
     float sign_b;
@@ -187,10 +215,13 @@ struct CeApply {
 //   energy [slots][models][NR*D][2]         fp64  |z|^2, |u|^2 per column
 //   inv    [slots][models][kMaxRank]        fp32  1 / (lam_k + n_q)
 //   hv     [slots][models][used]            fp32  hhat variance
-//   t      [slots][models][kMaxRank][NR*D]  c64   U^H z
+//   t      [slots][models][NR*D][kMaxRank]  c64   U^H z
 //   hf     [slots][models][NR*D][used]      c64   frequency estimate (unscaled)
-//   tp     [kProjSplits][slots][models][kMaxRank][NR*D]  c64  partial U^H z
+//   tp     [kProjSplits][slots][models][NR*D][kMaxRank]  c64  partial U^H z
 //          over one of kProjSplits fixed ranges of the observations
+//   zb     [models][zb_stride]               bf16  z in 3-part fragment planes:
+//          rows n = slot * NR*D + col < npad (zero beyond slots * NR*D),
+//          K = AlignUp(n_q, 16) observations (zero beyond n_q)
 //   tb     [models][TbModelElems]            bf16  t / (lam_k + n_q) in fragment
 //          planes: rows n = slot * NR*D + col < AlignUp(slots * NR*D,
 //          kInterpTile) (zero beyond), K = the model's rank
@@ -203,7 +234,9 @@ struct CeWs {
     float2 *hf;
     float2 *tp;
     uint16_t *tb;
-    uint32_t npad;  // rows of tb per model
+    uint16_t *zb;
+    size_t zb_stride;  // bf16 per model in zb
+    uint32_t npad;     // rows of tb and zb per model
 };
 
 size_t Align256(size_t b) { return (b + 255) / 256 * 256; }
@@ -212,18 +245,24 @@ uint32_t TbRows(const Geom &g, uint32_t slots) { return airan::AlignUp(slots * g
 
 __host__ __device__ inline size_t TbModelElems(uint32_t npad) { return static_cast<size_t>(npad) * kMaxRank * 4; }
 
+// Observations of the largest model: a comb model observes every DMRS RE.
+size_t ZbModelElems(const Geom &g, uint32_t npad)
+{
+    return static_cast<size_t>(npad) * airan::AlignUp(g.n_dmrs_re, 16) * 6;
+}
+
 size_t CeWsLayout(const Geom &g, uint32_t slots, uint8_t *base, CeWs *w)
 {
     const size_t sm = static_cast<size_t>(slots) * g.nl;  // models <= layers
     const size_t ncol = static_cast<size_t>(g.nr) * g.ndmrs;
     const uint32_t npad = TbRows(g, slots);
-    const size_t sizes[8] = {Align256(16u * slots), Align256(sm * ncol * 16), Align256(sm * kMaxRank * 4),
+    const size_t sizes[9] = {Align256(16u * slots), Align256(sm * ncol * 16), Align256(sm * kMaxRank * 4),
                              Align256(sm * g.used * 4), Align256(sm * kMaxRank * ncol * 8),
                              Align256(sm * ncol * g.used * 8), Align256(kProjSplits * sm * kMaxRank * ncol * 8),
-                             Align256(g.nl * TbModelElems(npad) * 2)};
+                             Align256(g.nl * TbModelElems(npad) * 2), Align256(g.nl * ZbModelElems(g, npad) * 2)};
     size_t off = 0;
-    void *ptrs[8];
-    for (int i = 0; i < 8; ++i) {
+    void *ptrs[9];
+    for (int i = 0; i < 9; ++i) {
         ptrs[i] = base ? base + off : nullptr;
         off += sizes[i];
     }
@@ -236,6 +275,8 @@ size_t CeWsLayout(const Geom &g, uint32_t slots, uint8_t *base, CeWs *w)
         w->hf = static_cast<float2 *>(ptrs[5]);
         w->tp = static_cast<float2 *>(ptrs[6]);
         w->tb = static_cast<uint16_t *>(ptrs[7]);
+        w->zb = static_cast<uint16_t *>(ptrs[8]);
+        w->zb_stride = ZbModelElems(g, npad);
         w->npad = npad;
     }
     return off;
@@ -256,56 +297,139 @@ __device__ inline float2 LoadZ(const __half *hr, const __half *hi, const CeApply
     return v;
 }
 
-// Observations of split `part` of a model: [p0, p0 + len), len a multiple of
-// the GEMM K-step except for the last split.
+// Observations of split `part` of a model: [p0, p0 + len), p0 and len
+// multiples of 16 except for the last split's len.
 __device__ inline void ProjRange(uint32_t n, uint32_t part, uint32_t *p0, uint32_t *len)
 {
-    const uint32_t chunk = ((n + kProjSplits - 1) / kProjSplits + airan::kGemmK - 1) / airan::kGemmK * airan::kGemmK;
+    const uint32_t chunk = airan::AlignUp((n + kProjSplits - 1) / kProjSplits, 16u);
     const uint32_t a = min(n, part * chunk), b = min(n, a + chunk);
     *p0 = a;
     *len = b - a;
 }
 
-// tp[part][slot, q][k][col] = sum_{p in range(part)} conj(U[p][k]) z[slot, col][p]:
-// per (model q, split part) = blockIdx.z a GEMM with M = rank (k),
-// N = slots * NR * D, K = the split's observations.
-__global__ void __launch_bounds__(airan::kGemmThreads) CeProjectKernel(const __half *__restrict__ hr,
-                                                                       const __half *__restrict__ hi, CeWs w,
-                                                                       uint32_t slots, CeApply c)
+// zb[q] = z[slot, col][p] in 3-part fragment planes (rows n = slot * NR*D +
+// col, zero for n >= slots * NR*D and p >= n_q), one thread per row and 8
+// observations.  Grid (npad * AlignUp(max n, 16) / 8 / 256, models).
+__global__ void __launch_bounds__(256) CeZSplitKernel(const __half *__restrict__ hr, const __half *__restrict__ hi,
+                                                      CeWs w, uint32_t slots, CeApply c)
 {
-    const uint32_t q = blockIdx.z / kProjSplits, part = blockIdx.z - q * kProjSplits;
+    const uint32_t q = blockIdx.y;
     const CeModelDev &md = c.m[q];
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = e % w.npad, p0 = (e / w.npad) * 8u;
+    const uint32_t pk = airan::AlignUp(md.n, 16u);
+    if (p0 >= pk) return;
     const uint32_t ncol = c.nr * c.ndmrs;
-    const uint32_t m0 = blockIdx.x * airan::kGemmTile, n0 = blockIdx.y * airan::kGemmTile;
-    if (m0 >= md.rank) return;
-    const uint32_t N = slots * ncol;
+    float2 v[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) v[j] = make_float2(0.0f, 0.0f);
+    if (n < slots * ncol) {
+        // LoadZ of observations p0 .. p0 + 7 with 16-byte loads (p0 and
+        // pilot_pad are multiples of 8 and p0 + 8 <= pk <= pilot_pad).
+        const uint32_t slot = n / ncol, col = n - slot * ncol;
+        const uint32_t rx = col / c.ndmrs, d = col - rx * c.ndmrs;
+        auto row = [&](uint32_t layer) {
+            return (((static_cast<size_t>(slot) * c.nr + rx) * c.nl + layer) * c.ndmrs + d) * c.pilot_pad + p0;
+        };
+        auto h8 = [](const __half *base, size_t off, float *out) {
+            const uint4 u = *reinterpret_cast<const uint4 *>(base + off);
+            const uint32_t w4[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                out[j] = __half2float(__ushort_as_half(static_cast<unsigned short>(w4[j >> 1] >> ((j & 1) * 16))));
+        };
+        float ar[8], ai[8];
+        h8(hr, row(md.in_a), ar);
+        h8(hi, row(md.in_a), ai);
+        if (md.sign_b != 0.0f) {
+            float br[8], bi[8];
+            h8(hr, row(md.in_b), br);
+            h8(hi, row(md.in_b), bi);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                ar[j] += md.sign_b * br[j];
+                ai[j] += md.sign_b * bi[j];
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if (p0 + j < md.n) v[j] = make_float2(ar[j], ai[j]);
+    }
+    airan::StoreFrag8<3>(w.zb + q * w.zb_stride, n, p0, pk / 16u, v);
+}
+
+// tp[part][slot, q][col][k] = sum_{p in range(part)} z[slot, col][p] conj(U[p][k])
+// on split bf16 MFMA with 3 parts (fp32-accurate products, which the
+// subspace noise estimate needs): C = ZB W^T with rows n (slot, col) and
+// columns k.  Workgroup: 64 rows x kProjKTile eigenvectors of one (model,
+// split), four waves of 32 x 64 (1 x 2 blocks of 32 x 32).  Linear grid of
+// 8 * per_xcd workgroups; each XCD takes a contiguous range of tiles ordered
+// (model, split) major, so the operands of one (model, split) -- about 2 MB
+// at 273 PRB and 64 slots -- are read through one L2.
+__global__ void __launch_bounds__(256, 2) CeProjectMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t k_tiles,
+                                                              uint32_t per_xcd)
+{
+    const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    const uint32_t per_split = (w.npad / 64u) * k_tiles;
+    if (tile >= c.nm * kProjSplits * per_split) return;
+    const uint32_t qs = tile / per_split, rest = tile - qs * per_split;
+    const uint32_t q = qs / kProjSplits, part = qs - q * kProjSplits;
+    const CeModelDev &md = c.m[q];
+    const uint32_t kt = rest % k_tiles, nt = rest / k_tiles;
+    if (kt * kProjKTile >= md.rank) return;
     uint32_t p0, len;
     ProjRange(md.n, part, &p0, &len);
-    auto la = [&](uint32_t k, uint32_t p) {
-        return (k < md.rank && p < len) ? md.bt[static_cast<size_t>(p0 + p) * md.rank + k] : make_float2(0.0f, 0.0f);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t nb = nt * 2u + (wave & 1u);
+    const uint32_t kb0 = kt * (kProjKTile / 32u) + (wave >> 1) * 2u;
+    const uint32_t pblocks = airan::AlignUp(md.n, 16u) / 16u;
+    const uint16_t *za = w.zb + q * w.zb_stride;
+    f32x16_t acc_r[2], acc_i[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc_r[b][i] = acc_i[b][i] = 0.0f;
+    const uint32_t pb_begin = p0 / 16u, pb_end = (p0 + len + 15u) / 16u;
+    auto load = [&](uint32_t pb, airan::CFrag<3> &fa, airan::CFrag<3> *fb) {
+        fa = airan::LoadFrag<3>(za + (static_cast<size_t>(nb) * pblocks + pb) * airan::FragBlock<3>(), lane);
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            fb[b] = airan::LoadFrag<3>(md.w16 + (static_cast<size_t>(kb0 + b) * pblocks + pb) * airan::FragBlock<3>(),
+                                       lane);
     };
-    auto lb = [&](uint32_t p, uint32_t n) {
-        if (n >= N || p >= len) return make_float2(0.0f, 0.0f);
-        const uint32_t slot = n / ncol, col = n - slot * ncol;
-        return LoadZ(hr, hi, c, md, slot, col, p0 + p);
-    };
-    float2 *tp = w.tp + static_cast<size_t>(part) * slots * c.nm * kMaxRank * ncol;
-    auto st = [&](uint32_t k, uint32_t n, float2 v) {
-        const uint32_t slot = n / ncol, col = n - slot * ncol;
-        tp[((static_cast<size_t>(slot) * c.nm + q) * kMaxRank + k) * ncol + col] = v;
-    };
-    airan::CGemmTileMfma<true>(m0, n0, md.rank, N, len, la, lb, st);
+    for (uint32_t pb = pb_begin; pb < pb_end; ++pb) {
+        airan::CFrag<3> fa, fb[2];
+        load(pb, fa, fb);
+        const uint4 nai[3] = {airan::NegBf16x8(fa.p[3]), airan::NegBf16x8(fa.p[4]), airan::NegBf16x8(fa.p[5])};
+#pragma unroll
+        for (int b = 0; b < 2; ++b) airan::CMfmaSplit<3>(acc_r[b], acc_i[b], fa, nai, fb[b]);
+    }
+    const uint32_t ncol = c.nr * c.ndmrs, N = slots * ncol;
+    float2 *tp = w.tp + static_cast<size_t>(part) * slots * c.nm * ncol * kMaxRank;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const uint32_t k = (kb0 + b) * 32u + (lane & 31u);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t n = nb * 32u + 8u * (i / 4) + 4u * (lane >> 5) + (i & 3);
+            if (n < N && k < md.rank) {
+                const uint32_t slot = n / ncol, col = n - slot * ncol;
+                tp[((static_cast<size_t>(slot) * c.nm + q) * ncol + col) * kMaxRank + k] =
+                    make_float2(acc_r[b][i], acc_i[b][i]);
+            }
+        }
+    }
 }
 
 // t = ((tp[0] + tp[1]) + tp[2]) + tp[3] ... in split order.  Grid over
-// slots * models * kMaxRank * NR * D elements (entries k >= rank skipped).
+// slots * models * NR * D * kMaxRank elements (entries k >= rank skipped).
 __global__ void __launch_bounds__(256) CeProjReduceKernel(CeWs w, uint32_t slots, CeApply c)
 {
     const uint32_t ncol = c.nr * c.ndmrs;
     const size_t total = static_cast<size_t>(slots) * c.nm * kMaxRank * ncol;
     const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    const uint32_t k = static_cast<uint32_t>((e / ncol) % kMaxRank);
+    const uint32_t k = static_cast<uint32_t>(e % kMaxRank);
     const uint32_t q = static_cast<uint32_t>((e / (static_cast<size_t>(ncol) * kMaxRank)) % c.nm);
     if (k >= c.m[q].rank) return;
     float2 acc = w.tp[e];
@@ -333,9 +457,9 @@ __global__ void __launch_bounds__(256) CeEnergyKernel(const __half *__restrict__
         const float2 v = LoadZ(hr, hi, c, md, slot, col, p);
         ez += static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
     }
-    const float2 *tc = w.t + (static_cast<size_t>(sq) * kMaxRank) * ncol + col;
+    const float2 *tc = w.t + (static_cast<size_t>(sq) * ncol + col) * kMaxRank;
     for (uint32_t k = threadIdx.x; k < md.rank; k += blockDim.x) {
-        const float2 u = tc[static_cast<size_t>(k) * ncol];
+        const float2 u = tc[k];
         eu += static_cast<double>(u.x) * u.x + static_cast<double>(u.y) * u.y;
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -409,33 +533,53 @@ __global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ 
 }
 
 // hhat_var[slot, q][sc] = sum_k |A[k][sc]|^2 / (lam_k + n_q) (ref.py:109-111),
-// eight slots per thread, their 1 / (lam_k + n_q) staged in LDS.
-// Grid (used / 256, models, slots / 8).
-__global__ void __launch_bounds__(256) CeHvKernel(CeWs w, uint32_t slots, CeApply c)
+// kHvSlots slots per thread, their 1 / (lam_k + n_q) staged in LDS.  Linear
+// grid of 8 * per_xcd workgroups over (model, subcarrier chunk of 256, slot
+// group) tiles, slot groups fastest and each XCD taking a contiguous range:
+// the slot groups of one chunk read its |A|^2 through one L2.
+constexpr uint32_t kHvSlots = 16;
+
+__global__ void __launch_bounds__(256) CeHvKernel(CeWs w, uint32_t slots, CeApply c, uint32_t per_xcd)
 {
-    __shared__ float sinv[8][kMaxRank];
-    const uint32_t q = blockIdx.y, s0 = blockIdx.z * 8;
+    __shared__ float4 sinv[kMaxRank][kHvSlots / 4];
+    const uint32_t groups = (slots + kHvSlots - 1) / kHvSlots, chunks = (c.used + 255u) / 256u;
+    const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    if (tile >= groups * chunks * c.nm) return;
+    const uint32_t g = tile % groups, rest = tile / groups;
+    const uint32_t chunk = rest % chunks, q = rest / chunks;
+    const uint32_t s0 = g * kHvSlots;
     const CeModelDev &md = c.m[q];
-    for (uint32_t e = threadIdx.x; e < 8u * md.rank; e += blockDim.x) {
+    for (uint32_t e = threadIdx.x; e < kHvSlots * md.rank; e += blockDim.x) {
         const uint32_t i = e / md.rank, k = e - i * md.rank;
-        sinv[i][k] = s0 + i < slots ? w.inv[(static_cast<size_t>(s0 + i) * c.nm + q) * kMaxRank + k] : 0.0f;
+        reinterpret_cast<float *>(sinv[k])[i] =
+            s0 + i < slots ? w.inv[(static_cast<size_t>(s0 + i) * c.nm + q) * kMaxRank + k] : 0.0f;
     }
     __syncthreads();
-    const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t sc = chunk * 256u + threadIdx.x;
     if (sc >= c.used) return;
-    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t k = 0; k < md.rank; ++k) {
-        const float2 a = md.at[static_cast<size_t>(k) * c.used + sc];
-        const float a2 = a.x * a.x + a.y * a.y;
+    float acc[kHvSlots] = {};
+    // rank is a multiple of 16: eight independent loads per step.
+    for (uint32_t k0 = 0; k0 < md.rank; k0 += 8) {
+        float a2[8];
 #pragma unroll
-        for (uint32_t i = 0; i < 8; ++i) acc[i] += a2 * sinv[i][k];
+        for (uint32_t j = 0; j < 8; ++j) a2[j] = md.a2[static_cast<size_t>(k0 + j) * c.used + sc];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+#pragma unroll
+            for (uint32_t i = 0; i < kHvSlots / 4; ++i) {
+                const float4 v = sinv[k0 + j][i];
+                acc[4 * i] += a2[j] * v.x;
+                acc[4 * i + 1] += a2[j] * v.y;
+                acc[4 * i + 2] += a2[j] * v.z;
+                acc[4 * i + 3] += a2[j] * v.w;
+            }
     }
 #pragma unroll
-    for (uint32_t i = 0; i < 8; ++i)
+    for (uint32_t i = 0; i < kHvSlots; ++i)
         if (s0 + i < slots) w.hv[(static_cast<size_t>(s0 + i) * c.nm + q) * c.used + sc] = acc[i];
 }
 
-// tb[q] = t[slot, q][k][col] / (lam_k + n_q) in bf16 fragment planes, row
+// tb[q] = t[slot, q][col][k] / (lam_k + n_q) in 2-part fragment planes, row
 // n = slot * NR*D + col (zero for n >= slots * NR*D), one thread per row and
 // 8 eigenvectors.  Grid (npad * kMaxRank / 8 / 256, models).
 __global__ void __launch_bounds__(256) CeScaleSplitKernel(CeWs w, uint32_t slots, CeApply c)
@@ -452,7 +596,7 @@ __global__ void __launch_bounds__(256) CeScaleSplitKernel(CeWs w, uint32_t slots
         const size_t sq = static_cast<size_t>(slot) * c.nm + q;
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j) {
-            const float2 x = w.t[(sq * kMaxRank + k0 + j) * ncol + col];
+            const float2 x = w.t[(sq * ncol + col) * kMaxRank + k0 + j];
             const float s = w.inv[sq * kMaxRank + k0 + j];
             v[j] = make_float2(x.x * s, x.y * s);
         }
@@ -460,11 +604,12 @@ __global__ void __launch_bounds__(256) CeScaleSplitKernel(CeWs w, uint32_t slots
 #pragma unroll
         for (uint32_t j = 0; j < 8; ++j) v[j] = make_float2(0.0f, 0.0f);
     }
-    airan::StoreFrag8(w.tb + q * TbModelElems(w.npad), n, k0, md.rank / 16u, v);
+    airan::StoreFrag8<2>(w.tb + q * TbModelElems(w.npad), n, k0, md.rank / 16u, v);
 }
 
-// hf[slot, q][col][sc] = sum_k (t[slot, q][k][col] / (lam_k + n_q)) A[k][sc]
-// on split bf16 MFMA: C = TB A^T with rows n (slot, col) and columns sc.
+// hf[slot, q][col][sc] = sum_k (t[slot, q][col][k] / (lam_k + n_q)) A[k][sc]
+// on split bf16 MFMA with 2 parts: C = TB A^T with rows n (slot, col) and
+// columns sc.
 // Workgroup: a kInterpTile x kInterpTile tile of one model, four waves of
 // 64 x 64 (2 x 2 blocks of 32 x 32), operands loaded straight from the
 // fragment planes.  Workgroups are handed out so that each XCD works through
@@ -493,19 +638,24 @@ __global__ void __launch_bounds__(256, 2) CeInterpMfmaKernel(CeWs w, uint32_t sl
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int i = 0; i < 16; ++i) acc_r[a][b][i] = acc_i[a][b][i] = 0.0f;
-    for (uint32_t kb = 0; kb < kb_count; ++kb) {
-        airan::CFrag fa[2], fb[2];
+    auto load = [&](uint32_t kb, airan::CFrag<2> *fa, airan::CFrag<2> *fb) {
 #pragma unroll
         for (int a = 0; a < 2; ++a)
-            fa[a] = airan::LoadFrag(ta + (static_cast<size_t>(nb0 + a) * kb_count + kb) * airan::kFragBlock, lane);
+            fa[a] = airan::LoadFrag<2>(ta + (static_cast<size_t>(nb0 + a) * kb_count + kb) * airan::FragBlock<2>(),
+                                       lane);
 #pragma unroll
         for (int b = 0; b < 2; ++b)
-            fb[b] = airan::LoadFrag(md.a16 + (static_cast<size_t>(sb0 + b) * kb_count + kb) * airan::kFragBlock, lane);
+            fb[b] = airan::LoadFrag<2>(md.a16 + (static_cast<size_t>(sb0 + b) * kb_count + kb) * airan::FragBlock<2>(),
+                                       lane);
+    };
+    for (uint32_t kb = 0; kb < kb_count; ++kb) {
+        airan::CFrag<2> fa[2], fb[2];
+        load(kb, fa, fb);
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            const uint4 nh = airan::NegBf16x8(fa[a].ih), nl = airan::NegBf16x8(fa[a].il);
+            const uint4 nai[2] = {airan::NegBf16x8(fa[a].p[2]), airan::NegBf16x8(fa[a].p[3])};
 #pragma unroll
-            for (int b = 0; b < 2; ++b) airan::CMfmaSplit(acc_r[a][b], acc_i[a][b], fa[a], nh, nl, fb[b]);
+            for (int b = 0; b < 2; ++b) airan::CMfmaSplit<2>(acc_r[a][b], acc_i[a][b], fa[a], nai, fb[b]);
         }
     }
     const uint32_t ncol = c.nr * c.ndmrs, N = slots * ncol;
@@ -623,6 +773,10 @@ __global__ void __launch_bounds__(256) CePostKernel(CeWs w, __half *__restrict__
     }
     TimeSolver<D> ts;
     ts.Factor(c, err2);
+    float tw[14][D];  // time weights of every symbol (nsym <= 14)
+#pragma unroll
+    for (uint32_t s = 0; s < 14; ++s)
+        if (s < c.nsym) ts.Solve(c.rt[s], tw[s]);
     for (uint32_t rx = 0; rx < c.nr; ++rx) {
         float2 hf[D];
 #pragma unroll
@@ -636,14 +790,14 @@ __global__ void __launch_bounds__(256) CePostKernel(CeWs w, __half *__restrict__
                 hf[d].y += wb * gb.y;
             }
         }
-        for (uint32_t s = 0; s < c.nsym; ++s) {
-            float tw[D];
-            ts.Solve(c.rt[s], tw);
+#pragma unroll
+        for (uint32_t s = 0; s < 14; ++s) {
+            if (s >= c.nsym) break;
             float vr = 0.0f, vi = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                vr += tw[d] * hf[d].x;
-                vi += tw[d] * hf[d].y;
+                vr += tw[s][d] * hf[d].x;
+                vi += tw[s][d] * hf[d].y;
             }
             const size_t o = (static_cast<size_t>(rx) * out_layers + l) * plane + static_cast<size_t>(s) * c.pad + sc;
             out_r[o] = __float2half_rn(vr);
@@ -671,6 +825,7 @@ int32_t BuildModel(const Geom &g, const LmmseFilterParamsV1 &prm, hipStream_t st
     const uint32_t n = o.n;
     if (n == 0 || n > kMaxObs) return AIRAN_UNSUPPORTED_PROFILE;
     double2 *rpp = nullptr, *rhp = nullptr, *av = nullptr;
+    float2 *at = nullptr;  // (R_hp U) transposed, [rank][used], packed into a16 / a2
     double *eig = nullptr, *work = nullptr;
     rocblas_int *info = nullptr;
     rocblas_handle handle = nullptr;
@@ -681,6 +836,7 @@ int32_t BuildModel(const Geom &g, const LmmseFilterParamsV1 &prm, hipStream_t st
         FreeDev(rpp);
         FreeDev(rhp);
         FreeDev(av);
+        FreeDev(at);
         FreeDev(eig);
         FreeDev(work);
         FreeDev(info);
@@ -730,17 +886,22 @@ int32_t BuildModel(const Geom &g, const LmmseFilterParamsV1 &prm, hipStream_t st
     out->rank = rank;
     out->kept = kept;
     if (hipMalloc(&out->bt, sizeof(float2) * static_cast<size_t>(n) * rank) != hipSuccess ||
-        hipMalloc(&out->at, sizeof(float2) * static_cast<size_t>(rank) * g.used) != hipSuccess ||
+        hipMalloc(&at, sizeof(float2) * static_cast<size_t>(rank) * g.used) != hipSuccess ||
+        hipMalloc(&out->a2, sizeof(float) * static_cast<size_t>(rank) * g.used) != hipSuccess ||
         hipMalloc(&out->lam, sizeof(float) * rank) != hipSuccess)
         return fail();
     hipLaunchKernelGGL(PackBtKernel, dim3(airan::CeilDiv(rank, 256), n), dim3(256), 0, stream, rpp, out->bt, n,
                        rank, kept);
-    hipLaunchKernelGGL(PackAtKernel, dim3(airan::CeilDiv(g.used, 256), rank), dim3(256), 0, stream, av, out->at,
+    hipLaunchKernelGGL(PackAtKernel, dim3(airan::CeilDiv(g.used, 256), rank), dim3(256), 0, stream, av, at,
                        g.used, rank, kept);
     const uint32_t sc_pad = airan::AlignUp(g.used, kInterpTile);
     if (hipMalloc(&out->a16, sizeof(uint16_t) * 4 * static_cast<size_t>(sc_pad) * rank) != hipSuccess) return fail();
-    hipLaunchKernelGGL(PackA16Kernel, dim3(airan::CeilDiv(sc_pad * (rank / 8), 256)), dim3(256), 0, stream, out->at,
-                       out->a16, g.used, sc_pad, rank);
+    hipLaunchKernelGGL(PackA16Kernel, dim3(airan::CeilDiv(sc_pad * (rank / 8), 256)), dim3(256), 0, stream, at,
+                       out->a16, out->a2, g.used, sc_pad, rank);
+    const uint32_t k_pad = airan::AlignUp(rank, kProjKTile), pk = airan::AlignUp(n, 16u);
+    if (hipMalloc(&out->w16, sizeof(uint16_t) * 6 * static_cast<size_t>(k_pad) * pk) != hipSuccess) return fail();
+    hipLaunchKernelGGL(PackW16Kernel, dim3(airan::CeilDiv(k_pad * (pk / 8), 256)), dim3(256), 0, stream, out->bt,
+                       out->w16, n, rank, k_pad);
     std::vector<float> lam_f(rank, 1.0f);  // padded columns: zero factors, unit eigenvalue
     for (uint32_t k = 0; k < kept; ++k) lam_f[k] = static_cast<float>(std::max(lam_host[n - 1 - k], 0.0));
     if (hipMemcpyAsync(out->lam, lam_f.data(), sizeof(float) * rank, hipMemcpyHostToDevice, stream) != hipSuccess ||
@@ -811,7 +972,7 @@ CeApply MakeApply(const LmmseFilter &f)
     c.nsym = g.nsym;
     for (uint32_t q = 0; q < f.nm; ++q) {
         const CeModelFactors &m = f.model[q];
-        c.m[q] = CeModelDev{m.bt, m.at, m.a16, m.lam, m.n, m.rank, m.kept, m.in_a, m.in_b, m.sign_b};
+        c.m[q] = CeModelDev{m.bt, m.a2, m.a16, m.w16, m.lam, m.n, m.rank, m.kept, m.in_a, m.in_b, m.sign_b};
     }
     std::memcpy(c.layer_model, f.layer_model, sizeof(c.layer_model));
     std::memcpy(c.layer_sign, f.layer_sign, sizeof(c.layer_sign));
@@ -859,15 +1020,22 @@ int32_t LaunchCeApply(const LmmseFilter *f, const __half *hr, const __half *hi, 
     const uint32_t ncol = g.nr * g.ndmrs;
     uint32_t max_rank = 0;
     for (uint32_t q = 0; q < f->nm; ++q) max_rank = std::max(max_rank, f->model[q].rank);
-    const uint32_t n_tiles = CeilDiv(static_cast<uint64_t>(slots) * ncol, kGemmTile);
-    hipLaunchKernelGGL(CeProjectKernel, dim3(CeilDiv(max_rank, kGemmTile), n_tiles, f->nm * kProjSplits),
-                       dim3(kGemmThreads), 0, s, hr, hi, w, slots, c);
+    uint32_t max_n = 0;
+    for (uint32_t q = 0; q < f->nm; ++q) max_n = std::max(max_n, f->model[q].n);
+    hipLaunchKernelGGL(CeZSplitKernel, dim3(CeilDiv(static_cast<uint64_t>(w.npad) * AlignUp(max_n, 16u) / 8, 256),
+                                            f->nm),
+                       dim3(256), 0, s, hr, hi, w, slots, c);
+    const uint32_t k_tiles = AlignUp(max_rank, kProjKTile) / kProjKTile;
+    const uint32_t proj_per_xcd = CeilDiv(f->nm * kProjSplits * (w.npad / 64) * k_tiles, 8);
+    hipLaunchKernelGGL(CeProjectMfmaKernel, dim3(8 * proj_per_xcd), dim3(256), 0, s, w, slots, c, k_tiles,
+                       proj_per_xcd);
     const uint64_t t_elems = static_cast<uint64_t>(slots) * f->nm * kMaxRank * ncol;
     hipLaunchKernelGGL(CeProjReduceKernel, dim3(CeilDiv(t_elems, 256)), dim3(256), 0, s, w, slots, c);
     hipLaunchKernelGGL(CeEnergyKernel, dim3(slots * f->nm * ncol), dim3(256), 0, s, hr, hi, w, c);
     hipLaunchKernelGGL(CeStatsKernel, dim3(slots), dim3(256), 0, s, noise_in, noise_out, w, c);
-    hipLaunchKernelGGL(CeHvKernel, dim3(CeilDiv(g.used, 256), f->nm, CeilDiv(slots, 8)), dim3(256), 0, s, w, slots,
-                       c);
+    const uint32_t hv_tiles = CeilDiv(slots, kHvSlots) * CeilDiv(g.used, 256) * f->nm;
+    hipLaunchKernelGGL(CeHvKernel, dim3(8 * CeilDiv(hv_tiles, 8)), dim3(256), 0, s, w, slots, c,
+                       CeilDiv(hv_tiles, 8));
     hipLaunchKernelGGL(CeScaleSplitKernel, dim3(CeilDiv(static_cast<uint64_t>(w.npad) * kMaxRank / 8, 256), f->nm),
                        dim3(256), 0, s, w, slots, c);
     const uint32_t sc_tiles = AlignUp(g.used, kInterpTile) / kInterpTile;
@@ -1020,8 +1188,9 @@ void airan_channel_est_lmmse_DestroyFilter(LmmseFilter *f)
     if (f == nullptr) return;
     for (uint32_t q = 0; q < kMaxModels; ++q) {
         FreeDev(f->model[q].bt);
-        FreeDev(f->model[q].at);
+        FreeDev(f->model[q].a2);
         FreeDev(f->model[q].a16);
+        FreeDev(f->model[q].w16);
         FreeDev(f->model[q].lam);
     }
     delete f;

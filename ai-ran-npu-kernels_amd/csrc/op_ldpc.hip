@@ -11,6 +11,8 @@
 // barrier separates rows.  A check's messages are stored compressed
 // (min1, min2, index of min1, sign bits): 8 bytes per (row, check) in
 // registers when the active rows fit REG_ROWS, otherwise in a global workspace.
+#include <algorithm>
+
 #include "common.h"
 #include "airan_ldpc_bg1.h"
 #include "crc.h"
@@ -289,200 +291,303 @@ __device__ __forceinline__ uint32_t LiftAddr(uint32_t w, uint32_t i, uint32_t Z)
     return (w & 0xffffu) + p;
 }
 
-// Check i of row R: gather, extrinsic, min1/min2/signs, scatter.  State
-// sa = fp16(0.75 min1) | fp16(0.75 min2) << 16, sb = message sign bits
-// (bit k: sign of the message on edge k) | index of min1 << 24.
-// Branch-free on fp16 bit patterns: magnitudes (|x| bits, +inf = 0x7c00,
-// NaN above) order like unsigned integers, so min1 and its first index are
-// the minimum of the keys (|x| << 5 | k) and min2 is the median of
-// (min1 key, key, min2 key) accumulated over the edges.  The recorded sign
-// is !(x >= 0), i.e. bits above 0x7c00 (the posterior is never -0: it starts
-// from integer LLRs and IEEE sums give -0 only from -0).  The scatter takes
-// the new message's sign from the extrinsic's sign bit, which differs from
-// the recorded one only for a NaN extrinsic, whose posterior is NaN anyway.
+// Two code blocks per workgroup: block "A" in the low and block "B" in the
+// high fp16 half of every 32-bit posterior word, so one LDS access, one set
+// of edge addresses and one packed fp16 add (v_pk_add_f16) serve both.
+// Every message of the thread's check lives in a register (one packed pair
+// per edge), so the extrinsic is lam - msg with no reconstruction of the old
+// message from a compressed state.  Check i of row R, per half:
+//   x_k = lam_k - msg_k; min1/min2 of the magnitudes |x_k| (fp16
+//   magnitudes order like unsigned integers), sign bits !(x_k >= 0) (bits
+//   above 0x7c00: negative, or a NaN), parity p;
+//   msg_k = (|x_k| == |min1| ? 0.75 min2 : 0.75 min1) with sign
+//   p ^ sign(x_k), and lam_k = x_k + msg_k.
+// This is the compressed-state decoder's arithmetic exactly: an edge whose
+// magnitude equals min1 without being min1's index implies min2 == min1, so
+// selecting by magnitude instead of index changes no value; and a message's
+// sign differs from the recorded sign bit only for a NaN extrinsic, whose
+// posterior is NaN and stays NaN (every later extrinsic on it is NaN), so the
+// stored message sign is never observed.
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ h2_t H2(uint32_t x) { return __builtin_bit_cast(h2_t, x); }
+__device__ __forceinline__ uint32_t WH(h2_t x) { return __builtin_bit_cast(uint32_t, x); }
+
+// Packed unsigned 16-bit operations on both halves (VOP3P).
+__device__ __forceinline__ uint32_t PkMinU16(uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ uint32_t PkMaxU16(uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_pk_max_u16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// max(a - b, 0) per half.
+__device__ __forceinline__ uint32_t PkSubSatU16(uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_pk_sub_u16 %0, %1, %2 clamp" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// (a - b) mod 2^16 per half.
+__device__ __forceinline__ uint32_t PkSubU16(uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_pk_sub_u16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// (a * b + c) mod 2^16 per half.
+__device__ __forceinline__ uint32_t PkMadU16(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_pk_mad_u16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// Per half: 1 if the fp16 bits exceed 0x7c00 (negative, or a NaN), else 0.
+__device__ __forceinline__ uint32_t AboveInf2(uint32_t x) { return PkMinU16(PkSubSatU16(x, 0x7c007c00u), 0x00010001u); }
+
+// Per half: NegBit (negative, non-zero, not NaN) as 0 / 1: x - 0x8001 < 0x7c00.
+__device__ __forceinline__ uint32_t NegBit2(uint32_t x)
+{
+    return PkMinU16(PkSubSatU16(0x7c007c00u, PkSubU16(x, 0x80018001u)), 0x00010001u);
+}
+
+// Check i of row R for both blocks.  min1 / min2 are tracked on the fp16
+// magnitudes (with duplicates: two edges at the minimum give min2 == min1),
+// starting from two virtual edges of magnitude 0x7c00, which is what the
+// keys (|x| << 5 | k) from (0x7c00 << 5) give for the magnitudes.
 template <int R>
-__device__ __forceinline__ void FastRow(h16 *lam, const uint32_t *lw, uint32_t i, uint32_t Z, uint32_t &sa,
-                                        uint32_t &sb)
+__device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i, uint32_t Z, uint32_t *msg)
 {
     constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
-    const uint32_t o1 = sa & 0xffffu, o2 = sa >> 16;
-    const uint32_t omsg = sb & 0x7ffffu, oidx = sb >> 24;
-    uint32_t w[DEG];
-#pragma unroll
-    for (int k = 0; k < DEG; ++k) w[k] = lw[E0 + k];
-    uint32_t addr[DEG];
-    h16 ext[DEG];
-    uint32_t m1key = 0x7c00u << 5, m2key = 0x7c00u << 5, sg = 0;
+    const uint32_t *lw = kLift.w[zi];
+    uint32_t addr[DEG], ext[DEG];
+    uint32_t m1 = 0x7c007c00u, m2 = 0x7c007c00u;
+    uint32_t sgl = 0, sgh = 0;  // sign bits of edges 0..15 and 16.. (A low, B high half)
+    // Every gather of the row is issued before the first is consumed.
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
-        addr[k] = LiftAddr(w[k], i, Z);
-        const uint32_t mag = static_cast<uint32_t>(k) == oidx ? o2 : o1;
-        const uint32_t sbit = k <= 15 ? omsg << (15 - k) : omsg >> (k - 15);
-        const h16 x = lam[addr[k]] - HBits(mag | (sbit & 0x8000u));
+        addr[k] = LiftAddr(lw[E0 + k], i, Z);
+        ext[k] = lam2[addr[k]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < DEG; ++k) {
+        const uint32_t x = WH(H2(ext[k]) - H2(msg[E0 + k]));
         ext[k] = x;
-        const uint32_t xb = BitsH(x);
-        const uint32_t key = ((xb & 0x7fffu) << 5) | static_cast<uint32_t>(k);
-        m2key = min(m2key, max(m1key, key));
-        m1key = min(m1key, key);
-        sg |= ((0x7c00u - xb) >> 31) << k;
+        const uint32_t mg = x & 0x7fff7fffu;
+        m2 = PkMinU16(m2, PkMaxU16(m1, mg));
+        m1 = PkMinU16(m1, mg);
+        if (k < 16)
+            sgl |= AboveInf2(x) << k;
+        else
+            sgh |= AboveInf2(x) << (k - 16);
     }
-    const uint32_t idx = m1key & 31u;
-    const uint32_t s1 = BitsH(static_cast<h16>(0.75f) * HBits(m1key >> 5));
-    const uint32_t s2 = BitsH(static_cast<h16>(0.75f) * HBits(m2key >> 5));
-    const uint32_t par = __popc(sg) & 1u;
-    const uint32_t s1p = s1 | (par << 15), s2p = s2 | (par << 15);
+    const h2_t q = {static_cast<_Float16>(0.75f), static_cast<_Float16>(0.75f)};
+    const uint32_t pa = (__popc(sgl & 0xffffu) + __popc(sgh & 0xffffu)) & 1u;
+    const uint32_t pb = (__popc(sgl >> 16) + __popc(sgh >> 16)) & 1u;
+    const uint32_t par = pa << 15 | pb << 31;
+    const uint32_t s1p = WH(q * H2(m1)) | par, s2p = WH(q * H2(m2)) | par;
+    const uint32_t d12 = PkSubU16(s2p, s1p);
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
-        const uint32_t msel = static_cast<uint32_t>(k) == idx ? s2p : s1p;
-        lam[addr[k]] = ext[k] + HBits(msel ^ (BitsH(ext[k]) & 0x8000u));
-    }
-    sa = s1 | (s2 << 16);
-    sb = (sg ^ (par ? 0x7ffffu : 0u)) | (idx << 24);
-}
-
-template <int R>
-__device__ __forceinline__ void FastIteration(h16 *lam, const uint32_t *lw, uint32_t i, uint32_t Z, int rows,
-                                              uint32_t *sa, uint32_t *sb)
-{
-    if constexpr (R < kFastRows) {
-        if (R < rows) {
-            if (i < Z) FastRow<R>(lam, lw, i, Z, sa[R], sb[R]);
-            __syncthreads();
-            FastIteration<R + 1>(lam, lw, i, Z, rows, sa, sb);
-        }
+        // 1 per half where |x_k| == min1, then s1p + that * (s2p - s1p).
+        const uint32_t e = PkSubSatU16(0x00010001u, (ext[k] & 0x7fff7fffu) ^ m1);
+        const uint32_t m = PkMadU16(e, d12, s1p) ^ (ext[k] & 0x80008000u);
+        msg[E0 + k] = m;
+        lam2[addr[k]] = WH(H2(ext[k]) + H2(m));
     }
 }
 
-// OR over the active rows of check i's parity (hard decisions).
-template <int R>
-__device__ __forceinline__ uint32_t FastSyndrome(const uint16_t *lam, const uint32_t *lw, uint32_t i, uint32_t Z,
-                                                 int rows)
+template <int R, int ROWS>
+__device__ __forceinline__ void PairIteration(uint32_t *lam2, uint32_t zi, uint32_t i, uint32_t Z, uint32_t *msg)
 {
-    if constexpr (R < kFastRows) {
-        if (R < rows) {
-            constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
-            uint32_t par = 0;
+    if constexpr (R < ROWS) {
+        if (i < Z) PairRow<R>(lam2, zi, i, Z, msg);
+        __syncthreads();
+        PairIteration<R + 1, ROWS>(lam2, zi, i, Z, msg);
+    }
+}
+
+// Bit 0: A's check i fails a row's parity; bit 16: B's.
+template <int R, int ROWS>
+__device__ __forceinline__ uint32_t PairSyndrome(const uint32_t *lam2, const uint32_t *lw, uint32_t i, uint32_t Z)
+{
+    if constexpr (R < ROWS) {
+        constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
+        uint32_t v[DEG];
 #pragma unroll
-            for (int k = 0; k < DEG; ++k) par ^= NegBit(lam[LiftAddr(lw[E0 + k], i, Z)]);
-            // One row's loads at a time (keeps the register count at the
-            // occupancy target instead of every row's loads in flight).
-            asm volatile("" : "+v"(par) : : "memory");
-            return par | FastSyndrome<R + 1>(lam, lw, i, Z, rows);
-        }
+        for (int k = 0; k < DEG; ++k) v[k] = lam2[LiftAddr(lw[E0 + k], i, Z)];
+        __builtin_amdgcn_sched_barrier(0);
+        uint32_t par = 0;
+#pragma unroll
+        for (int k = 0; k < DEG; ++k) par ^= NegBit2(v[k]);
+        // One row's loads at a time (keeps the register count down).
+        asm volatile("" : "+v"(par) : : "memory");
+        return par | PairSyndrome<R + 1, ROWS>(lam2, lw, i, Z);
     }
     return 0;
 }
 
 struct LdpcFastDev {
-    uint32_t z, zi, k, iters, rows, early_stop;
+    uint32_t z, zi, k, iters, rows, early_stop, num_cb;
     airan::RateDev rate;  // FUSED: codeword geometry
 };
 
-// One workgroup per code block.  src: int16 decoder LLRs [cb][68 Z]
-// (!FUSED) or bit-plane codeword LLRs with per-block descriptors (FUSED).
-// Six waves per SIMD (80 VGPRs): four Z = 384 blocks per CU.
-// With crc.cb_ok set, the epilogue also checks the block's CRC (crc.h).
-template <bool FUSED>
-__global__ void __launch_bounds__(384, 6) LdpcFastKernel(const int16_t *__restrict__ src,
-                                                      const RateMatchDescriptor *__restrict__ desc,
-                                                      int8_t *__restrict__ bits, int16_t *__restrict__ lam_out,
-                                                      uint8_t *__restrict__ iters_out, LdpcFastDev d,
-                                                      airan::CrcFuse crc)
+// Decoder input LLR of position t of a block, as the fp16 posterior start
+// value (LLR / 256).
+__device__ inline uint32_t InputH(int32_t v)
 {
-    uint16_t *lam16 = airan_ldpc_lam;
-    h16 *lam = reinterpret_cast<h16 *>(airan_ldpc_lam);
-    const uint32_t Z = d.z, N = 68u * Z, i = threadIdx.x, cb = blockIdx.x;
-    const uint32_t nl = (22u + d.rows) * Z;  // posterior entries held in LDS
-    const int16_t *blk = FUSED ? src : src + static_cast<size_t>(cb) * N;
-    airan::RateCb rc{};
-    if (FUSED) rc = airan::RateCbSetup(desc[cb], d.rate);
-    auto input = [&](uint32_t t) -> h16 {
-        const int16_t v = FUSED ? airan::RateDematchValue(t, rc, d.rate, src) : blk[t];
-        return static_cast<h16>(static_cast<float>(v) * (1.0f / 256.0f));
-    };
-    if (FUSED && airan::RateSingle(rc, d.rate)) {
-        // One LLR (or a constant) per position; a thread holds at most
-        // kMaxLoad positions (22 + rows <= 30 columns of Z, blockDim >= Z or
-        // 64 for Z < 64), so every load of the block is issued before the
-        // first is waited on.
-        constexpr int kMaxLoad = 30;
-        const int16_t *base = src + static_cast<size_t>(rc.slot0) * d.rate.qm * d.rate.cw_stride + rc.rem0;
-        int32_t v[kMaxLoad];
-#pragma unroll
-        for (int u = 0; u < kMaxLoad; ++u) {
-            const uint32_t t = i + static_cast<uint32_t>(u) * blockDim.x;
-            const int32_t idx = t < nl ? airan::RateSingleIndex(t, rc, d.rate) : -1;
-            v[u] = idx == -2 ? 5120 : 0;
-            if (idx >= 0) v[u] = base[idx] * 8;
-        }
-#pragma unroll
-        for (int u = 0; u < kMaxLoad; ++u) {
-            const uint32_t t = i + static_cast<uint32_t>(u) * blockDim.x;
-            const int32_t x = v[u] > 5120 ? 5120 : (v[u] < -5120 ? -5120 : v[u]);
-            if (t < nl) lam[t] = static_cast<h16>(static_cast<float>(x) * (1.0f / 256.0f));
-        }
-    } else if (FUSED) {
-        for (uint32_t t = i; t < nl; t += blockDim.x) lam[t] = input(t);
-    } else {
-        // nl is a multiple of 4 (Z is a multiple of 24) and blk 8-byte aligned.
-        const uint2 *s4 = reinterpret_cast<const uint2 *>(blk);
-        uint2 *l4 = reinterpret_cast<uint2 *>(lam16);
-        for (uint32_t t = i; t < nl / 4u; t += blockDim.x) {
-            const uint2 v = s4[t];
-            auto cv = [](uint32_t x) {
-                return BitsH(static_cast<h16>(static_cast<float>(static_cast<int16_t>(x)) * (1.0f / 256.0f)));
-            };
-            l4[t] = make_uint2(cv(v.x & 0xffffu) | cv(v.x >> 16) << 16, cv(v.y & 0xffffu) | cv(v.y >> 16) << 16);
-        }
+    return BitsH(static_cast<h16>(static_cast<float>(v) * (1.0f / 256.0f)));
+}
+
+// Loads one block's posterior start values into half h of lam2 (nl
+// positions).  FUSED: de-rate-matched from the bit-plane codeword LLRs.
+template <bool FUSED>
+__device__ inline void PairLoad(uint16_t *lam16, uint32_t h, uint32_t nl, const int16_t *__restrict__ src,
+                                const airan::RateCb &rc, const LdpcFastDev &d, uint32_t cb)
+{
+    const uint32_t i = threadIdx.x;
+    if (cb >= d.num_cb) {
+        for (uint32_t t = i; t < nl; t += blockDim.x) lam16[2 * t + h] = 0;
+        return;
     }
-    uint32_t sa[kFastRows], sb[kFastRows];
+    if (FUSED && airan::RateSingle(rc, d.rate)) {
+        // Every transmitted bit kk < E lands on its own position (the
+        // inverse of RateSingleIndex): q = (q0 + kk) mod (Ncb - filler),
+        // position 2Z + (q < fstart ? q : q + filler).  So the block's
+        // codeword LLRs -- Qm planes of E/Qm consecutive symbols -- are read
+        // in aligned 16-byte chunks and scattered into LDS, after the other
+        // positions are set (zero, or +5120 for filler bits).
+        const uint32_t two_z = 2u * d.z, fstart = d.rate.kprime - two_z;
+        const uint32_t nf = d.rate.ncb - d.rate.filler;
+        const uint32_t fill = InputH(5120);
+        for (uint32_t t = i; t < nl; t += blockDim.x)
+            lam16[2 * t + h] = static_cast<uint16_t>(
+                t >= two_z + fstart && t < two_z + fstart + d.rate.filler ? fill : 0u);
+        __syncthreads();
+        // Symbols rem0 .. rem0 + eq - 1 of each plane of the slot, in chunks
+        // of 8 aligned to the plane start (cw_stride is a multiple of 8).
+        const int16_t *slot = src + static_cast<size_t>(rc.slot0) * d.rate.qm * d.rate.cw_stride;
+        const uint32_t c0 = rc.rem0 / 8u, chunks = (rc.rem0 + rc.eq + 7u) / 8u - c0;
+        const uint32_t total = chunks * d.rate.qm;
+        constexpr uint32_t kBatch = 4;  // chunk loads of a thread in flight together
+        for (uint32_t e0 = i; e0 < total; e0 += kBatch * blockDim.x) {
+            uint4 raw[kBatch];
 #pragma unroll
-    for (int r = 0; r < kFastRows; ++r) sa[r] = sb[r] = 0;
-    __syncthreads();
-    const uint32_t *lw = kLift.w[d.zi];
-    const int rows = static_cast<int>(d.rows);
-    uint32_t it = 0;
-    for (; it < d.iters; ++it) {
-        FastIteration<0>(lam, lw, i, Z, rows, sa, sb);
-        if (d.early_stop) {
-            const uint32_t bad = i < Z ? FastSyndrome<0>(lam16, lw, i, Z, rows) : 0u;
-            if (__syncthreads_and(bad == 0 ? 1 : 0)) {
-                ++it;
-                break;
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                const uint32_t e = e0 + u * blockDim.x;
+                if (e < total) {
+                    const uint32_t b = e / chunks, c = c0 + (e - b * chunks);
+                    raw[u] = *reinterpret_cast<const uint4 *>(slot + static_cast<size_t>(b) * d.rate.cw_stride + 8u * c);
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                const uint32_t e = e0 + u * blockDim.x;
+                if (e >= total) continue;
+                const uint32_t b = e / chunks, c = c0 + (e - b * chunks);
+                const uint32_t w[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
+#pragma unroll
+                for (uint32_t x = 0; x < 8; ++x) {
+                    const int32_t j = static_cast<int32_t>(8u * c + x) - static_cast<int32_t>(rc.rem0);
+                    if (j < 0 || j >= static_cast<int32_t>(rc.eq)) continue;
+                    const uint32_t kk = b * rc.eq + static_cast<uint32_t>(j);
+                    uint32_t q = rc.q0 + kk;
+                    q = q >= nf ? q - nf : q;
+                    const uint32_t t = two_z + (q < fstart ? q : q + d.rate.filler);
+                    if (t >= nl) continue;
+                    int32_t v = static_cast<int16_t>(w[x >> 1] >> (16u * (x & 1u))) * 8;
+                    v = v > 5120 ? 5120 : (v < -5120 ? -5120 : v);
+                    lam16[2 * t + h] = static_cast<uint16_t>(InputH(v));
+                }
             }
         }
+    } else if (FUSED) {
+        for (uint32_t t = i; t < nl; t += blockDim.x)
+            lam16[2 * t + h] = static_cast<uint16_t>(InputH(airan::RateDematchValue(t, rc, d.rate, src)));
+    } else {
+        // nl is a multiple of 4 (Z is a multiple of 24) and the block 8-byte aligned.
+        const uint2 *s4 = reinterpret_cast<const uint2 *>(src + static_cast<size_t>(cb) * 68u * d.z);
+        for (uint32_t t = i; t < nl / 4u; t += blockDim.x) {
+            const uint2 v = s4[t];
+            lam16[8 * t + h] = static_cast<uint16_t>(InputH(static_cast<int16_t>(v.x & 0xffffu)));
+            lam16[8 * t + 2 + h] = static_cast<uint16_t>(InputH(static_cast<int16_t>(v.x >> 16)));
+            lam16[8 * t + 4 + h] = static_cast<uint16_t>(InputH(static_cast<int16_t>(v.y & 0xffffu)));
+            lam16[8 * t + 6 + h] = static_cast<uint16_t>(InputH(static_cast<int16_t>(v.y >> 16)));
+        }
     }
+}
+
+// Writes block cb (half h of lam2): iteration count, hard bits (with the CRC
+// epilogue when crc.cb_ok is set) and, when lam_out is set, the int16
+// posterior (positions beyond the LDS part keep their input).  Called by the
+// whole workgroup; ends with a barrier.
+template <bool FUSED>
+__device__ inline void PairEmit(const uint32_t *lam2, uint32_t h, uint32_t cb, uint32_t it, uint32_t nl,
+                                const int16_t *__restrict__ src, const airan::RateCb &rc, int8_t *__restrict__ bits,
+                                int16_t *__restrict__ lam_out, uint8_t *__restrict__ iters_out, const LdpcFastDev &d,
+                                const airan::CrcFuse &crc)
+{
+    const uint32_t i = threadIdx.x;
+    const uint16_t *lam16 = reinterpret_cast<const uint16_t *>(lam2);
     if (iters_out != nullptr && i == 0) iters_out[cb] = static_cast<uint8_t>(it);
     int8_t *bo = bits + static_cast<size_t>(cb) * d.k;
     const bool check = crc.cb_ok != nullptr;
     uint32_t rb = 0, ra = 0;
     if ((d.k & 3u) == 0) {
-        const uint2 *l4 = reinterpret_cast<const uint2 *>(lam16);
+        const uint4 *l4 = reinterpret_cast<const uint4 *>(lam2);
         // With kprime and L multiples of 4, the power-table entries of bits
         // 4t .. 4t+3 are four consecutive words (in reverse order): one
         // 16-byte load per table instead of a branch and a load per bit.
         const bool vec = check && (crc.kprime & 3u) == 0 && (crc.L & 3u) == 0;
-        for (uint32_t t = i; t < d.k / 4u; t += blockDim.x) {
-            const uint2 v = l4[t];
-            const uint32_t b4 = NegBit(v.x) | NegBit(v.x >> 16) << 8 | NegBit(v.y) << 16 | NegBit(v.y >> 16) << 24;
-            reinterpret_cast<uint32_t *>(bo)[t] = b4;
-            if (vec) {
-                const uint32_t j = 4 * t;
-                auto sel = [b4](uint4 p) {
-                    return ((b4 & 1u) ? p.w : 0u) ^ ((b4 & 0x100u) ? p.z : 0u) ^ ((b4 & 0x10000u) ? p.y : 0u) ^
-                           ((b4 & 0x1000000u) ? p.x : 0u);
-                };
-                if (crc.crc_cb && j < crc.kprime)
-                    rb ^= sel(*reinterpret_cast<const uint4 *>(crc.pow_b + (crc.kprime - 4 - j)));
-                if (j < crc.L) ra ^= sel(*reinterpret_cast<const uint4 *>(crc.pow_tb + (crc.L - 4 - j)));
-            } else if (check) {
-                for (uint32_t q = 0; q < 4; ++q)
-                    if ((b4 >> (8 * q)) & 1u) airan::CrcAddBit(crc, 4 * t + q, rb, ra);
+        const uint32_t sh = 16u * h;
+        auto sel = [](uint32_t b4, uint4 p) {
+            return ((b4 & 1u) ? p.w : 0u) ^ ((b4 & 0x100u) ? p.z : 0u) ^ ((b4 & 0x10000u) ? p.y : 0u) ^
+                   ((b4 & 0x1000000u) ? p.x : 0u);
+        };
+        // kBatch groups of 4 bits per thread at a time, their power-table
+        // loads in flight together.
+        constexpr uint32_t kBatch = 3;
+        for (uint32_t t0 = i; t0 < d.k / 4u; t0 += kBatch * blockDim.x) {
+            uint32_t b4[kBatch];
+            uint4 pb[kBatch], pt[kBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                const uint32_t t = t0 + u * blockDim.x;
+                b4[u] = 0;
+                pb[u] = pt[u] = make_uint4(0, 0, 0, 0);
+                if (t < d.k / 4u) {
+                    const uint4 v = l4[t];
+                    b4[u] = NegBit(v.x >> sh) | NegBit(v.y >> sh) << 8 | NegBit(v.z >> sh) << 16 |
+                            NegBit(v.w >> sh) << 24;
+                    reinterpret_cast<uint32_t *>(bo)[t] = b4[u];
+                    const uint32_t j = 4 * t;
+                    if (vec && crc.crc_cb && j < crc.kprime)
+                        pb[u] = *reinterpret_cast<const uint4 *>(crc.pow_b + (crc.kprime - 4 - j));
+                    if (vec && j < crc.L) pt[u] = *reinterpret_cast<const uint4 *>(crc.pow_tb + (crc.L - 4 - j));
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                const uint32_t t = t0 + u * blockDim.x;
+                if (vec) {
+                    rb ^= sel(b4[u], pb[u]);
+                    ra ^= sel(b4[u], pt[u]);
+                } else if (check && t < d.k / 4u) {
+                    for (uint32_t q = 0; q < 4; ++q)
+                        if ((b4[u] >> (8 * q)) & 1u) airan::CrcAddBit(crc, 4 * t + q, rb, ra);
+                }
             }
         }
     } else {
         for (uint32_t t = i; t < d.k; t += blockDim.x) {
-            const uint32_t b = NegBit(lam16[t]);
+            const uint32_t b = NegBit(lam16[2 * t + h]);
             bo[t] = static_cast<int8_t>(b);
             if (check && b) airan::CrcAddBit(crc, t, rb, ra);
         }
@@ -508,13 +613,120 @@ __global__ void __launch_bounds__(384, 6) LdpcFastKernel(const int16_t *__restri
         }
     }
     if (lam_out != nullptr) {
+        const uint32_t N = 68u * d.z;
         int16_t *lo = lam_out + static_cast<size_t>(cb) * N;
         for (uint32_t t = i; t < N; t += blockDim.x) {
-            float v = static_cast<float>(t < nl ? lam[t] : input(t));
+            float v;
+            if (t < nl) {
+                v = static_cast<float>(HBits(lam16[2 * t + h]));
+            } else {
+                const int16_t x = FUSED ? airan::RateDematchValue(t, rc, d.rate, src)
+                                        : src[static_cast<size_t>(cb) * N + t];
+                v = static_cast<float>(HBits(InputH(x)));
+            }
             v = v > 20.0f ? 20.0f : (v < -20.0f ? -20.0f : v);
             lo[t] = static_cast<int16_t>(rintf(v * 256.0f));
         }
     }
+    __syncthreads();
+}
+
+// Persistent decoder: workgroup b of G = gridDim.x decodes blocks b, b + G,
+// b + 2 G, ... two at a time, one per half of the posterior words.  When a
+// half's block is done (syndrome clear under early stop, or the iteration
+// limit) its outputs are written and the next block of the list is loaded
+// into that half (its messages reset) while the other half runs on, so
+// neither half idles while the list lasts; a half past the list holds a
+// zero block and is never written.  Blocks are independent of the pairing
+// (the halves never interact), so every block's outputs equal a decode of
+// that block alone.  src: int16 decoder LLRs [cb][68 Z] (!FUSED) or
+// bit-plane codeword LLRs with per-block descriptors (FUSED).  LDS holds the
+// (22 + rows) Z posterior pairs the active rows touch.  With crc.cb_ok set,
+// the epilogue also checks each block's CRC (crc.h).
+template <bool FUSED, int ROWS>
+__global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
+    LdpcPairKernel(const int16_t *__restrict__ src, const RateMatchDescriptor *__restrict__ desc,
+                   int8_t *__restrict__ bits, int16_t *__restrict__ lam_out, uint8_t *__restrict__ iters_out,
+                   LdpcFastDev d, airan::CrcFuse crc)
+{
+    uint32_t *lam2 = reinterpret_cast<uint32_t *>(airan_ldpc_lam);
+    const uint32_t Z = d.z, i = threadIdx.x;
+    const uint32_t nl = (22u + ROWS) * Z;
+    constexpr int kEdges = FastRowStart(ROWS);
+    uint32_t msg[kEdges];
+    uint32_t next = blockIdx.x;
+    uint32_t cur[2], it[2];
+    airan::RateCb rc[2];
+    // Loads the next block of the list into half h (a zero block past it).
+    auto start = [&](uint32_t h) {
+        cur[h] = next;
+        next += gridDim.x;
+        rc[h] = airan::RateCb{};
+        if (FUSED && cur[h] < d.num_cb) rc[h] = airan::RateCbSetup(desc[cur[h]], d.rate);
+        PairLoad<FUSED>(airan_ldpc_lam, h, nl, src, rc[h], d, cur[h]);
+        it[h] = 0;
+    };
+    start(0);
+    start(1);
+#pragma unroll
+    for (int e = 0; e < kEdges; ++e) msg[e] = 0;
+    __syncthreads();
+    const uint32_t *lw = kLift.w[d.zi];
+    while (cur[0] < d.num_cb || cur[1] < d.num_cb) {
+        PairIteration<0, ROWS>(lam2, d.zi, i, Z, msg);
+        ++it[0];
+        ++it[1];
+        uint32_t ok = 0;  // bit h: half h's syndrome is clear
+        if (d.early_stop) {
+            const uint32_t bad = i < Z ? PairSyndrome<0, ROWS>(lam2, lw, i, Z) : 0u;
+            ok = (__syncthreads_or((bad & 1u) != 0) ? 0u : 1u) | (__syncthreads_or((bad >> 16) != 0) ? 0u : 2u);
+        }
+        bool refilled = false;
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            if (cur[h] < d.num_cb && (((ok >> h) & 1u) || it[h] >= d.iters)) {
+                PairEmit<FUSED>(lam2, h, cur[h], it[h], nl, src, rc[h], bits, lam_out, iters_out, d, crc);
+                start(h);
+                const uint32_t keep = h == 0 ? 0xffff0000u : 0x0000ffffu;
+#pragma unroll
+                for (int e = 0; e < kEdges; ++e) msg[e] &= keep;
+                refilled = true;
+            }
+        }
+        if (refilled) __syncthreads();
+    }
+}
+
+// Workgroups resident on the device at once for a pair-decoder launch shape
+// (0 when the runtime cannot tell).
+uint32_t PairCapacity(const void *kernel, uint32_t threads, size_t lds)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, static_cast<int>(threads), lds) != hipSuccess)
+        return 0;
+    return static_cast<uint32_t>(std::max(cus, 0)) * static_cast<uint32_t>(std::max(per_cu, 0));
+}
+
+// Launches LdpcPairKernel<FUSED, rows> (rows 4 .. kFastRows).
+template <bool FUSED>
+void LaunchPair(uint32_t rows, uint32_t threads, size_t lds, hipStream_t s, const int16_t *src,
+                const RateMatchDescriptor *desc, int8_t *bits, int16_t *lam_out, uint8_t *iters_out,
+                const LdpcFastDev &d, const airan::CrcFuse &crc)
+{
+    using Kern = void (*)(const int16_t *, const RateMatchDescriptor *, int8_t *, int16_t *, uint8_t *, LdpcFastDev,
+                          airan::CrcFuse);
+    Kern k = rows <= 4 ? LdpcPairKernel<FUSED, 4>
+           : rows == 5 ? LdpcPairKernel<FUSED, 5>
+           : rows == 6 ? LdpcPairKernel<FUSED, 6>
+           : rows == 7 ? LdpcPairKernel<FUSED, 7>
+                       : LdpcPairKernel<FUSED, 8>;
+    // One resident wave of workgroups, each walking its list of blocks.
+    uint32_t grid = (d.num_cb + 1u) / 2u;
+    const uint32_t cap = PairCapacity(reinterpret_cast<const void *>(k), threads, lds);
+    if (cap > 0) grid = std::min(grid, cap);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, s, src, desc, bits, lam_out, iters_out, d, crc);
 }
 
 bool FastEligible(uint32_t mode, uint32_t z, uint32_t rows)
@@ -575,10 +787,10 @@ int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, 
     const bool aligned = (reinterpret_cast<uintptr_t>(in) & 7u) == 0 && (reinterpret_cast<uintptr_t>(bits) & 3u) == 0;
     if (aligned && FastEligible(a->mode, a->z, a->active_rows)) {
         if ((st = LoadLiftTable()) != AIRAN_OK) return st;
-        LdpcFastDev fd{a->z, LiftIndex(a->z), a->k, a->max_iterations, a->active_rows, a->early_stop ? 1u : 0u, {}};
-        const size_t fast_lds = static_cast<size_t>(22 + a->active_rows) * a->z * sizeof(uint16_t);
-        hipLaunchKernelGGL(LdpcFastKernel<false>, dim3(a->num_cb), dim3(threads), fast_lds, s, in, nullptr, bits, lo,
-                           io, fd, airan::CrcFuse{});
+        const uint32_t rows = a->active_rows;  // validated >= 4
+        LdpcFastDev fd{a->z, LiftIndex(a->z), a->k, a->max_iterations, rows, a->early_stop ? 1u : 0u, a->num_cb, {}};
+        const size_t fast_lds = static_cast<size_t>(22 + rows) * a->z * sizeof(uint32_t);
+        LaunchPair<false>(rows, threads, fast_lds, s, in, nullptr, bits, lo, io, fd, airan::CrcFuse{});
         return airan::LaunchStatus();
     }
     const bool reg = a->active_rows <= static_cast<uint32_t>(kRegRows);
@@ -610,11 +822,11 @@ int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const Rate
         return AIRAN_UNSUPPORTED_PROFILE;
     int32_t st = LoadLiftTable();
     if (st != AIRAN_OK) return st;
-    LdpcFastDev fd{p.Z, LiftIndex(p.Z), p.K, iters, rows, early_stop ? 1u : 0u,
+    LdpcFastDev fd{p.Z, LiftIndex(p.Z), p.K, iters, rows, early_stop ? 1u : 0u, num_cb,
                    {g.qm, g.cw_symbols, g.cw_stride, p.Z, p.Kprime, p.filler, p.Ncb, num_cb}};
-    const size_t lds = static_cast<size_t>(22 + rows) * p.Z * sizeof(uint16_t);
-    hipLaunchKernelGGL(LdpcFastKernel<true>, dim3(num_cb), dim3(AlignUp(p.Z, 64)), lds, s, cw, desc, bits,
-                       static_cast<int16_t *>(nullptr), iters_out, fd, crc != nullptr ? *crc : CrcFuse{});
+    const size_t lds = static_cast<size_t>(22 + rows) * p.Z * sizeof(uint32_t);
+    LaunchPair<true>(rows, AlignUp(p.Z, 64), lds, s, cw, desc, bits, nullptr, iters_out, fd,
+                     crc != nullptr ? *crc : CrcFuse{});
     return LaunchStatus();
 }
 
