@@ -205,6 +205,7 @@ struct AiranPuschChain {
     float refiner_snr_min = 0.0f, refiner_snr_max = 0.0f;
     uint8_t *refiner_keep = nullptr;
     uint32_t *crc_xpow = nullptr;  // [C] code-block shifts of the TB CRC remainder
+    uint32_t *ldpc_queue = nullptr;  // decoder work counter, zeroed before each decode
     // Stage timing (events before the first stage and after each stage).
     bool timing = false;
     hipEvent_t ev[AIRAN_CHAIN_STAGES + 1] = {};
@@ -299,8 +300,9 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
             return st;
         crc.xpow = c->crc_xpow;
         airan::LaunchCrcZero(reinterpret_cast<uint32_t *>(c->crc_ws), slots, s);
+        airan::LaunchCrcZero(c->ldpc_queue, 1, s);
         st = airan::LaunchLdpcFromCodeword(g, c->plan, c->desc, c->cw, ncb, c->active_rows, c->opts.ldpc_iterations,
-                                           c->opts.early_stop != 0, c->bits, c->iters, &crc, s);
+                                           c->opts.early_stop != 0, c->bits, c->iters, &crc, c->ldpc_queue, s);
         if (st != AIRAN_OK) return st;
         mark();
         airan::LaunchCrcFinish(reinterpret_cast<const uint32_t *>(c->crc_ws), c->tb_ok, slots, s);
@@ -409,6 +411,7 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         !Alloc(&c->lam, c->fused_ldpc ? 0 : ncb * 68 * p.Z, &b) || !Alloc(&c->ldpc_ws, c->ldpc_ws_bytes, &b) ||
         !Alloc(&c->bits, ncb * p.K, &b) || !Alloc(&c->iters, ncb, &b) || !Alloc(&c->cb_ok, ncb, &b) ||
         !Alloc(&c->tb_ok, S, &b) || !Alloc(&c->refiner_keep, S, &b) || !Alloc(&c->crc_xpow, p.C, &b) || !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
+        !Alloc(&c->ldpc_queue, 1, &b) ||
         !Alloc(reinterpret_cast<uint8_t **>(&c->zero_iq), g.nr * g.samples * iq_elem, &b))
         return fail(AIRAN_RESOURCE_TOO_SMALL);
     // Per-code-block shifts of the transport-block CRC remainder.
@@ -475,7 +478,8 @@ void airan_pusch_chain_Destroy(AiranPuschChain *c)
     airan_channel_est_lmmse_DestroyFilter(c->filter);
     void *bufs[] = {c->dmrs_re, c->dmrs_im, c->rx_re, c->rx_im, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce,
                     c->ce_ws, c->h_re, c->h_im, c->gold, c->cw, c->desc, c->lam, c->ldpc_ws, c->bits,
-                    c->iters, c->cb_ok, c->tb_ok, c->crc_ws, c->zero_iq, c->refiner_keep, c->crc_xpow};
+                    c->iters, c->cb_ok, c->tb_ok, c->crc_ws, c->zero_iq, c->refiner_keep, c->crc_xpow,
+                    c->ldpc_queue};
     for (void *p : bufs) Free(p);
     delete c;
 }

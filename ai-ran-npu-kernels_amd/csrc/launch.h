@@ -39,9 +39,13 @@ int32_t LaunchRateDematch(const Geom &g, const AiranTbPlanV1 &plan, const RateMa
 // With crc non-null the decoder also checks each code block's CRC24B
 // (cb_ok) and accumulates the transport blocks' CRC remainders (tb_acc,
 // zeroed by LaunchCrcZero before, turned into tb_ok by LaunchCrcFinish after).
+// queue: null, or one device word zeroed before the launch (LaunchCrcZero
+// with n = 1): the decoder's workgroups then take code blocks from it as
+// they free up instead of walking fixed lists (same outputs, better balance).
 int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &plan, const RateMatchDescriptor *desc,
                                const int16_t *cw, uint32_t num_cb, uint32_t rows, uint32_t iters, bool early_stop,
-                               int8_t *bits, uint8_t *iters_out, const CrcFuse *crc, hipStream_t s);
+                               int8_t *bits, uint8_t *iters_out, const CrcFuse *crc, uint32_t *queue,
+                               hipStream_t s);
 
 // CRC check of decoded bits inside the decoder (crc.h).
 int32_t CrcFuseSetup(const AiranTbPlanV1 &plan, uint8_t *cb_ok, uint32_t *tb_acc, CrcFuse *out);

@@ -532,51 +532,56 @@ __global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ 
             w.inv[(static_cast<size_t>(slot) * c.nm + q) * kMaxRank + k] = 1.0f / (c.m[q].lam[k] + nq[q]);
 }
 
-// hhat_var[slot, q][sc] = sum_k |A[k][sc]|^2 / (lam_k + n_q) (ref.py:109-111),
-// kHvSlots slots per thread, their 1 / (lam_k + n_q) staged in LDS.  Linear
-// grid of 8 * per_xcd workgroups over (model, subcarrier chunk of 256, slot
-// group) tiles, slot groups fastest and each XCD taking a contiguous range:
-// the slot groups of one chunk read its |A|^2 through one L2.
-constexpr uint32_t kHvSlots = 16;
-
-__global__ void __launch_bounds__(256) CeHvKernel(CeWs w, uint32_t slots, CeApply c, uint32_t per_xcd)
+// hhat_var[slot, q][sc] = sum_k |A[k][sc]|^2 / (lam_k + n_q) (ref.py:109-111)
+// on fp32 MFMA (v_mfma_f32_32x32x2f32, exact fp32 products): a workgroup per
+// 32 slots x 32 subcarriers of one model, its four waves taking fixed
+// quarters of the rank (in steps of 16) and their partial sums added in
+// wave order.  Lane l takes A[slot l % 32][k + l / 32] = 1 / (lam + n) and
+// B[k + l / 32][sc l % 32] = |A|^2; accumulator i holds slot
+// 8 (i / 4) + 4 (l / 32) + i % 4.  Rows past `slots` and subcarriers past
+// `used` read clamped (valid) addresses and are not stored.  Grid: slot
+// tiles fastest (they share the |A|^2 tile), then subcarrier tiles, then
+// models.
+__global__ void __launch_bounds__(256) CeHvMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t slot_tiles,
+                                                      uint32_t sc_tiles)
 {
-    __shared__ float4 sinv[kMaxRank][kHvSlots / 4];
-    const uint32_t groups = (slots + kHvSlots - 1) / kHvSlots, chunks = (c.used + 255u) / 256u;
-    const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
-    if (tile >= groups * chunks * c.nm) return;
-    const uint32_t g = tile % groups, rest = tile / groups;
-    const uint32_t chunk = rest % chunks, q = rest / chunks;
-    const uint32_t s0 = g * kHvSlots;
+    __shared__ float part[3][16][64];
+    const uint32_t st = blockIdx.x % slot_tiles, rest = blockIdx.x / slot_tiles;
+    const uint32_t ct = rest % sc_tiles, q = rest / sc_tiles;
     const CeModelDev &md = c.m[q];
-    for (uint32_t e = threadIdx.x; e < kHvSlots * md.rank; e += blockDim.x) {
-        const uint32_t i = e / md.rank, k = e - i * md.rank;
-        reinterpret_cast<float *>(sinv[k])[i] =
-            s0 + i < slots ? w.inv[(static_cast<size_t>(s0 + i) * c.nm + q) * kMaxRank + k] : 0.0f;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, r = lane & 31u, kh = lane >> 5;
+    const uint32_t slot = min(st * 32u + r, slots - 1u), sc = min(ct * 32u + r, c.used - 1u);
+    const float *ia = w.inv + (static_cast<size_t>(slot) * c.nm + q) * kMaxRank + kh;
+    const float *ba = md.a2 + static_cast<size_t>(kh) * c.used + sc;
+    const uint32_t steps = md.rank / 16u;
+    f32x16_t acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    for (uint32_t k0 = 16u * (wave * steps / 4u); k0 < 16u * ((wave + 1u) * steps / 4u); k0 += 16) {
+        float av[8], bv[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            av[u] = ia[k0 + 2u * u];
+            bv[u] = ba[static_cast<size_t>(k0 + 2u * u) * c.used];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], bv[u], acc, 0, 0, 0);
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) part[wave - 1][i][lane] = acc[i];
     }
     __syncthreads();
-    const uint32_t sc = chunk * 256u + threadIdx.x;
-    if (sc >= c.used) return;
-    float acc[kHvSlots] = {};
-    // rank is a multiple of 16: eight independent loads per step.
-    for (uint32_t k0 = 0; k0 < md.rank; k0 += 8) {
-        float a2[8];
+    if (wave > 0) return;
 #pragma unroll
-        for (uint32_t j = 0; j < 8; ++j) a2[j] = md.a2[static_cast<size_t>(k0 + j) * c.used + sc];
+    for (int i = 0; i < 16; ++i) acc[i] = ((acc[i] + part[0][i][lane]) + part[1][i][lane]) + part[2][i][lane];
+    const uint32_t col = ct * 32u + r;
+    if (col >= c.used) return;
 #pragma unroll
-        for (uint32_t j = 0; j < 8; ++j)
-#pragma unroll
-            for (uint32_t i = 0; i < kHvSlots / 4; ++i) {
-                const float4 v = sinv[k0 + j][i];
-                acc[4 * i] += a2[j] * v.x;
-                acc[4 * i + 1] += a2[j] * v.y;
-                acc[4 * i + 2] += a2[j] * v.z;
-                acc[4 * i + 3] += a2[j] * v.w;
-            }
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t s = st * 32u + 8u * (i / 4) + 4u * kh + (i & 3);
+        if (s < slots) w.hv[(static_cast<size_t>(s) * c.nm + q) * c.used + col] = acc[i];
     }
-#pragma unroll
-    for (uint32_t i = 0; i < kHvSlots; ++i)
-        if (s0 + i < slots) w.hv[(static_cast<size_t>(s0 + i) * c.nm + q) * c.used + sc] = acc[i];
 }
 
 // tb[q] = t[slot, q][col][k] / (lam_k + n_q) in 2-part fragment planes, row
@@ -1033,9 +1038,9 @@ int32_t LaunchCeApply(const LmmseFilter *f, const __half *hr, const __half *hi, 
     hipLaunchKernelGGL(CeProjReduceKernel, dim3(CeilDiv(t_elems, 256)), dim3(256), 0, s, w, slots, c);
     hipLaunchKernelGGL(CeEnergyKernel, dim3(slots * f->nm * ncol), dim3(256), 0, s, hr, hi, w, c);
     hipLaunchKernelGGL(CeStatsKernel, dim3(slots), dim3(256), 0, s, noise_in, noise_out, w, c);
-    const uint32_t hv_tiles = CeilDiv(slots, kHvSlots) * CeilDiv(g.used, 256) * f->nm;
-    hipLaunchKernelGGL(CeHvKernel, dim3(8 * CeilDiv(hv_tiles, 8)), dim3(256), 0, s, w, slots, c,
-                       CeilDiv(hv_tiles, 8));
+    const uint32_t hv_slot_tiles = CeilDiv(slots, 32), hv_sc_tiles = CeilDiv(g.used, 32);
+    hipLaunchKernelGGL(CeHvMfmaKernel, dim3(hv_slot_tiles * hv_sc_tiles * f->nm), dim3(256), 0, s, w, slots, c,
+                       hv_slot_tiles, hv_sc_tiles);
     hipLaunchKernelGGL(CeScaleSplitKernel, dim3(CeilDiv(static_cast<uint64_t>(w.npad) * kMaxRank / 8, 256), f->nm),
                        dim3(256), 0, s, w, slots, c);
     const uint32_t sc_tiles = AlignUp(g.used, kInterpTile) / kInterpTile;

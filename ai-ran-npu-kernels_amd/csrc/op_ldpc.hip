@@ -631,13 +631,15 @@ __device__ inline void PairEmit(const uint32_t *lam2, uint32_t h, uint32_t cb, u
     __syncthreads();
 }
 
-// Persistent decoder: workgroup b of G = gridDim.x decodes blocks b, b + G,
-// b + 2 G, ... two at a time, one per half of the posterior words.  When a
-// half's block is done (syndrome clear under early stop, or the iteration
-// limit) its outputs are written and the next block of the list is loaded
-// into that half (its messages reset) while the other half runs on, so
-// neither half idles while the list lasts; a half past the list holds a
-// zero block and is never written.  Blocks are independent of the pairing
+// Persistent decoder: the G = gridDim.x workgroups decode the blocks two at
+// a time, one per half of the posterior words.  When a half's block is done
+// (syndrome clear under early stop, or the iteration limit) its outputs are
+// written and the next block is loaded into that half (its messages reset)
+// while the other half runs on; a half past the last block holds a zero
+// block and is never written.  Workgroup b starts with blocks b and b + G;
+// after that it takes blocks 2 G, 2 G + 1, ... from the counter `queue`
+// (zero at launch) as halves free up, or, with queue null, its static list
+// b + 2 G, b + 3 G, ...  Blocks are independent of the pairing
 // (the halves never interact), so every block's outputs equal a decode of
 // that block alone.  src: int16 decoder LLRs [cb][68 Z] (!FUSED) or
 // bit-plane codeword LLRs with per-block descriptors (FUSED).  LDS holds the
@@ -647,20 +649,27 @@ template <bool FUSED, int ROWS>
 __global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
     LdpcPairKernel(const int16_t *__restrict__ src, const RateMatchDescriptor *__restrict__ desc,
                    int8_t *__restrict__ bits, int16_t *__restrict__ lam_out, uint8_t *__restrict__ iters_out,
-                   LdpcFastDev d, airan::CrcFuse crc)
+                   LdpcFastDev d, airan::CrcFuse crc, uint32_t *__restrict__ queue)
 {
+    __shared__ uint32_t taken_cb;
     uint32_t *lam2 = reinterpret_cast<uint32_t *>(airan_ldpc_lam);
     const uint32_t Z = d.z, i = threadIdx.x;
     const uint32_t nl = (22u + ROWS) * Z;
     constexpr int kEdges = FastRowStart(ROWS);
     uint32_t msg[kEdges];
-    uint32_t next = blockIdx.x;
+    uint32_t taken = 0;
     uint32_t cur[2], it[2];
     airan::RateCb rc[2];
-    // Loads the next block of the list into half h (a zero block past it).
+    // Loads the workgroup's next block into half h (a zero block past the end).
     auto start = [&](uint32_t h) {
-        cur[h] = next;
-        next += gridDim.x;
+        if (queue == nullptr || taken < 2) {
+            cur[h] = blockIdx.x + taken * gridDim.x;
+        } else {
+            if (i == 0) taken_cb = 2u * gridDim.x + atomicAdd(queue, 1u);
+            __syncthreads();
+            cur[h] = __builtin_amdgcn_readfirstlane(taken_cb);
+        }
+        ++taken;
         rc[h] = airan::RateCb{};
         if (FUSED && cur[h] < d.num_cb) rc[h] = airan::RateCbSetup(desc[cur[h]], d.rate);
         PairLoad<FUSED>(airan_ldpc_lam, h, nl, src, rc[h], d, cur[h]);
@@ -713,10 +722,10 @@ uint32_t PairCapacity(const void *kernel, uint32_t threads, size_t lds)
 template <bool FUSED>
 void LaunchPair(uint32_t rows, uint32_t threads, size_t lds, hipStream_t s, const int16_t *src,
                 const RateMatchDescriptor *desc, int8_t *bits, int16_t *lam_out, uint8_t *iters_out,
-                const LdpcFastDev &d, const airan::CrcFuse &crc)
+                const LdpcFastDev &d, const airan::CrcFuse &crc, uint32_t *queue)
 {
     using Kern = void (*)(const int16_t *, const RateMatchDescriptor *, int8_t *, int16_t *, uint8_t *, LdpcFastDev,
-                          airan::CrcFuse);
+                          airan::CrcFuse, uint32_t *);
     Kern k = rows <= 4 ? LdpcPairKernel<FUSED, 4>
            : rows == 5 ? LdpcPairKernel<FUSED, 5>
            : rows == 6 ? LdpcPairKernel<FUSED, 6>
@@ -726,7 +735,7 @@ void LaunchPair(uint32_t rows, uint32_t threads, size_t lds, hipStream_t s, cons
     uint32_t grid = (d.num_cb + 1u) / 2u;
     const uint32_t cap = PairCapacity(reinterpret_cast<const void *>(k), threads, lds);
     if (cap > 0) grid = std::min(grid, cap);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, s, src, desc, bits, lam_out, iters_out, d, crc);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, s, src, desc, bits, lam_out, iters_out, d, crc, queue);
 }
 
 bool FastEligible(uint32_t mode, uint32_t z, uint32_t rows)
@@ -790,7 +799,7 @@ int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, 
         const uint32_t rows = a->active_rows;  // validated >= 4
         LdpcFastDev fd{a->z, LiftIndex(a->z), a->k, a->max_iterations, rows, a->early_stop ? 1u : 0u, a->num_cb, {}};
         const size_t fast_lds = static_cast<size_t>(22 + rows) * a->z * sizeof(uint32_t);
-        LaunchPair<false>(rows, threads, fast_lds, s, in, nullptr, bits, lo, io, fd, airan::CrcFuse{});
+        LaunchPair<false>(rows, threads, fast_lds, s, in, nullptr, bits, lo, io, fd, airan::CrcFuse{}, nullptr);
         return airan::LaunchStatus();
     }
     const bool reg = a->active_rows <= static_cast<uint32_t>(kRegRows);
@@ -814,7 +823,8 @@ namespace airan {
 
 int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const RateMatchDescriptor *desc,
                                const int16_t *cw, uint32_t num_cb, uint32_t rows, uint32_t iters, bool early_stop,
-                               int8_t *bits, uint8_t *iters_out, const CrcFuse *crc, hipStream_t s)
+                               int8_t *bits, uint8_t *iters_out, const CrcFuse *crc, uint32_t *queue,
+                               hipStream_t s)
 {
     if (num_cb == 0 || !FastEligible(AIRAN_LDPC_MODE_FP16, p.Z, rows) || rows < 4 || iters == 0 || iters > 100 ||
         (p.Z % 24) != 0 || p.K == 0 || p.K > 22 * p.Z || p.Ncb <= p.filler || (reinterpret_cast<uintptr_t>(bits) & 3u) ||
@@ -826,7 +836,7 @@ int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const Rate
                    {g.qm, g.cw_symbols, g.cw_stride, p.Z, p.Kprime, p.filler, p.Ncb, num_cb}};
     const size_t lds = static_cast<size_t>(22 + rows) * p.Z * sizeof(uint32_t);
     LaunchPair<true>(rows, AlignUp(p.Z, 64), lds, s, cw, desc, bits, nullptr, iters_out, fd,
-                     crc != nullptr ? *crc : CrcFuse{});
+                     crc != nullptr ? *crc : CrcFuse{}, queue);
     return LaunchStatus();
 }
 
