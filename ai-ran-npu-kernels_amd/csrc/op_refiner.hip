@@ -230,6 +230,28 @@ __global__ void __launch_bounds__(256) InputKernel(InputArgs a)
         }
 }
 
+// Network input from a channel estimate grid h fp16 [slots][NR][L][14][pad]
+// (the LMMSE estimator's output): x[s][k][u][2 (r L + l) + {re, im}] =
+// bf16(h).  Grid (used / 256, slots).
+__global__ void __launch_bounds__(256) EstimateInputKernel(const __half *__restrict__ h_re,
+                                                           const __half *__restrict__ h_im, bf16 *__restrict__ x,
+                                                           uint32_t used, uint32_t pad, uint32_t nr, uint32_t nl)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = blockIdx.y;
+    if (u >= used) return;
+    bf16 *xo = x + (static_cast<size_t>(s) * kSym * used + u) * kCh;
+    for (uint32_t r = 0; r < nr; ++r)
+        for (uint32_t l = 0; l < nl; ++l) {
+            const size_t base = ((static_cast<size_t>(s) * nr + r) * nl + l) * kSym * pad + u;
+            const uint32_t c = 2 * (r * nl + l);
+            for (uint32_t k = 0; k < kSym; ++k) {
+                xo[static_cast<size_t>(k) * used * kCh + c] = static_cast<bf16>(__half2float(h_re[base + k * pad]));
+                xo[static_cast<size_t>(k) * used * kCh + c + 1] = static_cast<bf16>(__half2float(h_im[base + k * pad]));
+            }
+        }
+}
+
 // ----------------------------------------------------------------- conv
 enum { kEpiRelu = 0, kEpiResid = 1, kEpiMask = 2 };
 
@@ -515,6 +537,13 @@ struct AiranCeRefiner {
     __half *dmrs_re = nullptr, *dmrs_im = nullptr;
     __half *rx_re = nullptr, *rx_im = nullptr;
     __half *hls_re = nullptr, *hls_im = nullptr, *noise = nullptr;
+    // Input mode (airan_ce_refiner_SetInput): 0 LS interpolation of h_ls, 1
+    // the LMMSE estimate (SynthBatch computes it with `filter` into est_*).
+    uint32_t input_mode = 0;
+    const LmmseFilter *filter = nullptr;
+    void *ce_ws = nullptr;
+    size_t ce_ws_bytes = 0;
+    __half *est_re = nullptr, *est_im = nullptr;  // [slots][NR][L][14][padded]
 };
 
 namespace {
@@ -755,7 +784,7 @@ void airan_ce_refiner_Destroy(AiranCeRefiner *r)
     void *bufs[] = {r->params, r->adam_m, r->adam_v, r->grad, r->wf, r->wb, r->x, r->target, r->out, r->dz[0],
                     r->dz[1], r->part, r->loss_part, r->loss_dev, r->gains, r->tau, r->snr, r->slot_w, r->dmrs_re,
                     r->dmrs_im,
-                    r->rx_re, r->rx_im, r->hls_re, r->hls_im, r->noise};
+                    r->rx_re, r->rx_im, r->hls_re, r->hls_im, r->noise, r->ce_ws, r->est_re, r->est_im};
     for (void *b : bufs) FreeDev(b);
     for (bf16 *a : r->act) FreeDev(a);
     delete r;
@@ -784,19 +813,45 @@ int32_t airan_ce_refiner_Enqueue(AiranCeRefiner *r, const void *h_ls_re, const v
         return AIRAN_INVALID_ARGUMENT;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Geom &g = r->g;
-    const InputArgs ia = MakeInput(r, static_cast<const __half *>(h_ls_re), static_cast<const __half *>(h_ls_im), r->x);
-    hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
+    if (r->input_mode == 1) {
+        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s,
+                           static_cast<const __half *>(h_re), static_cast<const __half *>(h_im), r->x, g.used, g.pad,
+                           g.nr, g.nl);
+    } else {
+        const InputArgs ia =
+            MakeInput(r, static_cast<const __half *>(h_ls_re), static_cast<const __half *>(h_ls_im), r->x);
+        hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
+    }
     Forward(r, r->x, slots, g.used, s);
     hipLaunchKernelGGL(OutToGridKernel, dim3(airan::CeilDiv(g.pad, 256), kSym, slots), dim3(256), 0, s, r->out, keep,
                        static_cast<__half *>(h_re), static_cast<__half *>(h_im), g.used, g.pad, g.nr, g.nl);
     return airan::LaunchStatus();
 }
 
+int32_t airan_ce_refiner_SetInput(AiranCeRefiner *r, uint32_t mode, const struct LmmseFilter *filter)
+{
+    if (r == nullptr || mode > 1 || (filter != nullptr && !airan::CeFilterMatches(filter, r->g)))
+        return AIRAN_INVALID_ARGUMENT;
+    if (mode == 1 && filter != nullptr && r->ce_ws == nullptr) {
+        const Geom &g = r->g;
+        const size_t grid = static_cast<size_t>(r->max_slots) * g.nr * g.nl * kSym * g.pad * sizeof(__half);
+        r->ce_ws_bytes = airan::CeWorkspaceBytes(g, r->max_slots);
+        if (hipMalloc(&r->ce_ws, r->ce_ws_bytes) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&r->est_re), grid) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&r->est_im), grid) != hipSuccess)
+            return AIRAN_RESOURCE_TOO_SMALL;
+    }
+    r->input_mode = mode;
+    r->filter = mode == 1 ? filter : nullptr;
+    return AIRAN_OK;
+}
+
 int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *r, const float *gains, const float *tap_delay_s, uint32_t taps,
                                     const float *snr_db, uint32_t slots, float scs_hz, uint64_t seed, void *stream)
 {
     if (r == nullptr || gains == nullptr || tap_delay_s == nullptr || snr_db == nullptr || stream == nullptr ||
-        slots == 0 || slots > r->max_slots || taps == 0 || taps > r->max_taps || !(scs_hz > 0.0f))
+        slots == 0 || slots > r->max_slots || taps == 0 || taps > r->max_taps || !(scs_hz > 0.0f) ||
+        (r->input_mode == 1 && r->filter == nullptr))
         return AIRAN_INVALID_ARGUMENT;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Geom &g = r->g;
@@ -839,8 +894,16 @@ int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *r, const float *gains, const
     int32_t st = airan::LaunchDmrsLs(g, r->rx_re, r->rx_im, r->dmrs_re, r->dmrs_im, r->hls_re, r->hls_im, r->noise,
                                      slots, s);
     if (st != AIRAN_OK) return st;
-    const InputArgs ia = MakeInput(r, r->hls_re, r->hls_im, r->x);
-    hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
+    if (r->input_mode == 1) {
+        st = airan::LaunchCeApply(r->filter, r->hls_re, r->hls_im, r->noise, nullptr, r->ce_ws, r->ce_ws_bytes,
+                                  r->est_re, r->est_im, g.nl, slots, s);
+        if (st != AIRAN_OK) return st;
+        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, r->est_re,
+                           r->est_im, r->x, g.used, g.pad, g.nr, g.nl);
+    } else {
+        const InputArgs ia = MakeInput(r, r->hls_re, r->hls_im, r->x);
+        hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
+    }
     return airan::LaunchStatus();
 }
 

@@ -378,6 +378,7 @@ PROTOTYPES = {
     "airan_ce_refiner_GetParams": (i32, [vp, vp]),
     "airan_ce_refiner_SetParams": (i32, [vp, vp]),
     "airan_ce_refiner_Enqueue": (i32, [vp, vp, vp, u32, vp, vp, vp, vp]),
+    "airan_ce_refiner_SetInput": (i32, [vp, u32, vp]),
     "airan_ce_refiner_SynthBatch": (i32, [vp, vp, vp, u32, vp, u32, C.c_float, C.c_uint64, vp]),
     "airan_ce_refiner_Batch": (i32, [vp, P(AiranCeRefinerBatchV1)]),
     "airan_ce_refiner_TrainStep": (i32, [vp, u32, C.c_float, vp, vp]),

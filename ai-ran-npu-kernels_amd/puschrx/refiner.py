@@ -11,6 +11,7 @@ Jakes fading per OFDM symbol, exponential Kronecker Rx correlation).
 from __future__ import annotations
 
 import ctypes as C
+import json
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -65,18 +66,38 @@ def nmse_db(est: torch.Tensor, ref: torch.Tensor) -> float:
     return float(10.0 * torch.log10(e / ref.double().pow(2).sum()).item())
 
 
+# First parameter of the last convolution (32 <- 64): layer i holds
+# 9 cout cin weights then cout biases.
+LAST_LAYER_OFFSET = (9 * 64 * 32 + 64) + 4 * (9 * 64 * 64 + 64)
+
+
 class Refiner:
-    def __init__(self, cfg, max_slots: int, params: np.ndarray | None = None, seed: int = 1):
+    """input_mode 0: the network refines the LS interpolation of the DMRS
+    estimate; 1: it learns a residual correction of the LMMSE estimate
+    (training batches then need `estimator`, an LmmseFilter of the same grant
+    shape; in the chain the estimate is the chain's own CE output).  Fresh
+    mode-1 weights start with a zero last layer, i.e. at the LMMSE estimate."""
+
+    def __init__(self, cfg, max_slots: int, params: np.ndarray | None = None, seed: int = 1,
+                 input_mode: int = 0, estimator: ops.LmmseFilter | None = None):
         self.lib = abi.load_library()
         self.cfg = cfg
         self.g = ops.Geometry(cfg)
         self.max_slots = max_slots
+        self.input_mode = input_mode
         self.handle = C.c_void_p()
         p = None if params is None else np.ascontiguousarray(params, np.float32)
         ops._check("refiner Create", self.lib.airan_ce_refiner_Create(
             C.byref(cfg), max_slots, None if p is None else p.ctypes.data_as(C.c_void_p), seed,
             C.c_void_p(ops.stream_ptr()), C.byref(self.handle)))
         self._loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        if input_mode:
+            ops._check("refiner SetInput", self.lib.airan_ce_refiner_SetInput(
+                self.handle, input_mode, estimator.handle if estimator is not None else None))
+            if params is None:
+                w = self.params()
+                w[LAST_LAYER_OFFSET:] = 0.0
+                self.set_params(w)
 
     # ---------------------------------------------------------------- params
     def params(self) -> np.ndarray:
@@ -92,8 +113,15 @@ class Refiner:
                                                                           p.ctypes.data_as(C.c_void_p)))
 
     def save(self, path: Path = WEIGHTS) -> None:
+        """Weights (fp32 raw) and a JSON sidecar with the input mode."""
         path.parent.mkdir(parents=True, exist_ok=True)
         self.params().tofile(path)
+        Path(str(path) + ".json").write_text(json.dumps({"input_mode": self.input_mode}) + "\n")
+
+    @staticmethod
+    def load_meta(path: Path = WEIGHTS) -> dict:
+        meta = Path(str(path) + ".json")
+        return json.loads(meta.read_text()) if meta.exists() else {"input_mode": 0}
 
     @staticmethod
     def load_params(path: Path = WEIGHTS) -> np.ndarray:
@@ -174,7 +202,9 @@ class Refiner:
         x = self.fetch("x", torch.bfloat16, n).float()
         t = self.fetch("target", torch.float32, n)
         o = self.fetch("out", torch.float32, n)
-        res = {"ls_interp": nmse_db(x, t), "refiner": nmse_db(o, t)}
+        res = {"input": nmse_db(x, t), "refiner": nmse_db(o, t)}
+        if self.input_mode == 0:
+            res["ls_interp"] = res["input"]
         if lmmse is not None:
             res["lmmse"] = nmse_db(self.lmmse_grid(slots, lmmse), t)
         return res

@@ -17,6 +17,9 @@
  * tapped-delay-line gains, the DMRS symbols of the received grid with
  * AWGN, the library's DMRS least-squares kernels, then the interpolation
  * that forms X; the loss is the mean squared error against the true channel.
+ * In input mode 1 (airan_ce_refiner_SetInput) X is instead the LMMSE
+ * estimate (the channel_est_lmmse output), so the network learns a residual
+ * correction of it.
  */
 #ifndef AIRAN_CE_REFINER_H
 #define AIRAN_CE_REFINER_H
@@ -52,6 +55,15 @@ int32_t airan_ce_refiner_SetParams(AiranCeRefiner *refiner, const float *params)
  * slot whose flag keep[slot] is non-zero (keep may be NULL: all slots). */
 int32_t airan_ce_refiner_Enqueue(AiranCeRefiner *refiner, const void *h_ls_re, const void *h_ls_im,
                                  uint32_t slots, const uint8_t *keep, void *h_re, void *h_im, void *stream);
+
+/* Network input.  mode 0 (the default): the LS interpolation of h_ls.
+ * mode 1: a residual correction of the LMMSE estimate -- Enqueue reads the
+ * estimate from h_re / h_im (the channel_est_lmmse output grid, which it then
+ * overwrites for the kept slots; h_ls is not read), and SynthBatch computes
+ * it from the batch's DMRS LS with `filter` (an LMMSE filter of the same
+ * grant shape; required for SynthBatch in mode 1, not owned).  Weights are
+ * trained for one mode. */
+int32_t airan_ce_refiner_SetInput(AiranCeRefiner *refiner, uint32_t mode, const struct LmmseFilter *filter);
 
 /* Synthesises a training or evaluation batch on the device into the
  * refiner's batch buffers: gains host complex64 [slots][NR][L][taps][14]

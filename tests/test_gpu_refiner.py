@@ -76,3 +76,28 @@ def test_refiner_training_beats_ls_interpolation(gpu_stream):
     assert after["refiner"] < before["refiner"] - 1.0
     assert after["refiner"] < after["ls_interp"] - 0.4
     ref.close()
+
+
+def test_refiner_lmmse_input_mode(gpu_stream):
+    """Input mode 1: the batch input is the LMMSE estimate of the synthesised
+    slots (the chain's filter: 300 ns, 100 Hz, rank 384) rounded to bf16,
+    and a fresh network (zero last layer) returns it unchanged."""
+    from puschrx import ops
+
+    cfg = P.FULL_CARRIER_4RX.config()
+    spec = TrainSpec(snr_db=(34.0, 34.0), delay_spread_s=(300e-9, 300e-9), doppler_hz=(100.0, 100.0),
+                     rx_corr=(0.5, 0.5))
+    filt = ops.LmmseFilter(cfg, tau_s=300e-9, fd_hz=100.0, scs_hz=30e3, max_rank=384)
+    ref = Refiner(cfg, max_slots=4, seed=5, input_mode=1, estimator=filt)
+    ref.synth(np.random.default_rng(21), 4, spec, seed=21)
+    res = ref.evaluate(4, lmmse=filt)
+    print(res, flush=True)
+    n = 4 * 14 * ref.g.used * 32
+    x = ref.fetch("x", torch.bfloat16, n).float()
+    out = ref.fetch("out", torch.float32, n)
+    assert torch.equal(out, x)
+    # bf16 rounding of the estimate costs well under a dB at 34 dB.
+    assert res["lmmse"] - 0.05 <= res["input"] <= res["lmmse"] + 1.0
+    assert res["input"] < -38.0
+    ref.close()
+    filt.close()
