@@ -103,6 +103,24 @@ __device__ inline CFrag<S> LoadFrag(const uint16_t *__restrict__ block, uint32_t
     return f;
 }
 
+// Copies one 1 KB plane of a fragment block (64 lanes x 16 B at src) to LDS
+// dst in the same order (lane l -> dst + 16 l) by the direct global -> LDS
+// path (global_load_lds_dwordx4: no VGPRs, counted by vmcnt); the caller
+// waits (WaitVm) and synchronises before reading dst.
+__device__ inline void PlaneToLds(const uint16_t *__restrict__ src, uint4 *dst, uint32_t lane)
+{
+    __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4 *>(src) + lane,
+                                     (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(N), N < 16, other counters not waited on.
+template <int N>
+__device__ inline void WaitVm()
+{
+    static_assert(N >= 0 && N < 16, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt(0xF70 | N);
+}
+
 __device__ inline uint4 NegBf16x8(uint4 v)
 {
     return make_uint4(v.x ^ 0x80008000u, v.y ^ 0x80008000u, v.z ^ 0x80008000u, v.w ^ 0x80008000u);

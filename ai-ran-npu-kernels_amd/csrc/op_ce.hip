@@ -614,16 +614,20 @@ __global__ void __launch_bounds__(256) CeScaleSplitKernel(CeWs w, uint32_t slots
 
 // hf[slot, q][col][sc] = sum_k (t[slot, q][col][k] / (lam_k + n_q)) A[k][sc]
 // on split bf16 MFMA with 2 parts: C = TB A^T with rows n (slot, col) and
-// columns sc.
-// Workgroup: a kInterpTile x kInterpTile tile of one model, four waves of
-// 64 x 64 (2 x 2 blocks of 32 x 32), operands loaded straight from the
-// fragment planes.  Workgroups are handed out so that each XCD works through
-// a contiguous range of tiles with the row tiles of one subcarrier tile
-// adjacent: the subcarrier-side operand is read once per XCD, the small
-// row-side operand stays in every L2.  Linear grid of 8 * per_xcd.
+// columns sc.  Workgroup: a kInterpTile x kInterpTile tile of one model, four
+// waves of 64 x 64 (2 x 2 blocks of 32 x 32).  Per K-step of 16 the tile's
+// 4 row blocks and 4 subcarrier blocks (32 planes of 1 KB) are copied to
+// LDS by the direct global -> LDS path, double-buffered: the next step's
+// copies are in flight while this step's MFMAs run, and every fragment is
+// fetched from HBM/L2 once per workgroup instead of once per wave.
+// Workgroups are handed out so that each XCD works through a contiguous
+// range of tiles with the row tiles of one subcarrier tile adjacent: the
+// subcarrier-side operand is read once per XCD, the small row-side operand
+// stays in every L2.  Linear grid of 8 * per_xcd.
 __global__ void __launch_bounds__(256, 2) CeInterpMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t sc_tiles,
                                                           uint32_t per_xcd)
 {
+    __shared__ uint4 stage[2][8][4][64];  // [buffer][block: 4 row, 4 subcarrier][plane][lane]
     const uint32_t xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
     const uint32_t n_tiles = w.npad / kInterpTile;
     const uint32_t tile = xcd * per_xcd + j;
@@ -632,8 +636,8 @@ __global__ void __launch_bounds__(256, 2) CeInterpMfmaKernel(CeWs w, uint32_t sl
     const uint32_t st = rest % sc_tiles, q = rest / sc_tiles;
     const CeModelDev &md = c.m[q];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t nb0 = (nt * kInterpTile) / 32u + (wave & 1u) * 2u;
-    const uint32_t sb0 = (st * kInterpTile) / 32u + (wave >> 1) * 2u;
+    const uint32_t nrow0 = (nt * kInterpTile) / 32u, nsc0 = (st * kInterpTile) / 32u;
+    const uint32_t nb0 = nrow0 + (wave & 1u) * 2u, sb0 = nsc0 + (wave >> 1) * 2u;
     const uint32_t kb_count = md.rank / 16u;
     const uint16_t *ta = w.tb + q * TbModelElems(w.npad);
     f32x16_t acc_r[2][2], acc_i[2][2];
@@ -643,25 +647,44 @@ __global__ void __launch_bounds__(256, 2) CeInterpMfmaKernel(CeWs w, uint32_t sl
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int i = 0; i < 16; ++i) acc_r[a][b][i] = acc_i[a][b][i] = 0.0f;
-    auto load = [&](uint32_t kb, airan::CFrag<2> *fa, airan::CFrag<2> *fb) {
+    // Wave w copies blocks 2w and 2w + 1 of K-step kb (blocks 0..3: row
+    // blocks nrow0 + g, 4..7: subcarrier blocks nsc0 + g - 4): 8 planes.
+    auto issue = [&](uint32_t kb, uint32_t buf) {
+#pragma unroll
+        for (uint32_t jb = 0; jb < 2; ++jb) {
+            const uint32_t g = 2u * wave + jb;
+            const uint16_t *src =
+                g < 4u ? ta + (static_cast<size_t>(nrow0 + g) * kb_count + kb) * airan::FragBlock<2>()
+                       : md.a16 + (static_cast<size_t>(nsc0 + g - 4u) * kb_count + kb) * airan::FragBlock<2>();
+#pragma unroll
+            for (uint32_t p = 0; p < 4; ++p) airan::PlaneToLds(src + p * airan::kFragPlane, &stage[buf][g][p][0], lane);
+        }
+    };
+    issue(0, 0);
+    for (uint32_t kb = 0; kb < kb_count; ++kb) {
+        const uint32_t cur = kb & 1u;
+        if (kb + 1u < kb_count) {
+            issue(kb + 1u, cur ^ 1u);
+            airan::WaitVm<8>();  // this wave's copies of step kb have landed
+        } else {
+            airan::WaitVm<0>();
+        }
+        __syncthreads();
+        airan::CFrag<2> fa[2], fb[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
-            fa[a] = airan::LoadFrag<2>(ta + (static_cast<size_t>(nb0 + a) * kb_count + kb) * airan::FragBlock<2>(),
-                                       lane);
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
-            fb[b] = airan::LoadFrag<2>(md.a16 + (static_cast<size_t>(sb0 + b) * kb_count + kb) * airan::FragBlock<2>(),
-                                       lane);
-    };
-    for (uint32_t kb = 0; kb < kb_count; ++kb) {
-        airan::CFrag<2> fa[2], fb[2];
-        load(kb, fa, fb);
+            for (int p = 0; p < 4; ++p) {
+                fa[a].p[p] = stage[cur][(wave & 1u) * 2u + a][p][lane];
+                fb[a].p[p] = stage[cur][4u + (wave >> 1) * 2u + a][p][lane];
+            }
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             const uint4 nai[2] = {airan::NegBf16x8(fa[a].p[2]), airan::NegBf16x8(fa[a].p[3])};
 #pragma unroll
             for (int b = 0; b < 2; ++b) airan::CMfmaSplit<2>(acc_r[a][b], acc_i[a][b], fa[a], nai, fb[b]);
         }
+        __syncthreads();  // buffer cur is refilled by the next step's copies
     }
     const uint32_t ncol = c.nr * c.ndmrs, N = slots * ncol;
 #pragma unroll
