@@ -357,23 +357,50 @@ __device__ __forceinline__ uint32_t NegBit2(uint32_t x)
     return PkMinU16(PkSubSatU16(0x7c007c00u, PkSubU16(x, 0x80018001u)), 0x00010001u);
 }
 
+// Byte offset in LDS of edge e's posterior word for check i: column base +
+// 4 ((i + shift) mod Z), i4 = 4 i.  With ZT != 0, Z = ZT and the edge's
+// column and shift are compile-time constants (callers unroll over e): the
+// wrap is min(4 (i + s), 4 (i + s - Z)) (unsigned) in three VALU
+// instructions and the column base goes to the LDS instruction's offset;
+// otherwise the edge word comes from the per-lifting-size table.
+template <uint32_t ZT>
+__device__ __forceinline__ uint32_t EdgeByte(const uint32_t *lw, int e, uint32_t i, uint32_t i4, uint32_t Z)
+{
+    if constexpr (ZT != 0) {
+        const uint32_t s4 = 4u * (static_cast<uint32_t>(AIRAN_BG1_SHIFT[e]) % ZT);
+        return 4u * static_cast<uint32_t>(AIRAN_BG1_COL[e]) * ZT + min(i4 + s4, i4 + s4 - 4u * ZT);
+    } else {
+        return 4u * LiftAddr(lw[e], i, Z);
+    }
+}
+
+__device__ __forceinline__ uint32_t &LdsWord(uint32_t *lam2, uint32_t byte)
+{
+    return *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(lam2) + byte);
+}
+
 // Check i of row R for both blocks.  min1 / min2 are tracked on the fp16
 // magnitudes (with duplicates: two edges at the minimum give min2 == min1),
 // starting from two virtual edges of magnitude 0x7c00, which is what the
 // keys (|x| << 5 | k) from (0x7c00 << 5) give for the magnitudes.
-template <int R>
+template <int R, uint32_t ZT>
 __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i, uint32_t Z, uint32_t *msg)
 {
     constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
     const uint32_t *lw = kLift.w[zi];
+    // Opaque per row: with compile-time edges the addresses depend on i
+    // alone and would otherwise be hoisted out of the iteration loop, every
+    // row's at once, in registers.
+    uint32_t i4 = 4u * i;
+    asm volatile("" : "+v"(i4));
     uint32_t addr[DEG], ext[DEG];
     uint32_t m1 = 0x7c007c00u, m2 = 0x7c007c00u;
     uint32_t sgl = 0, sgh = 0;  // sign bits of edges 0..15 and 16.. (A low, B high half)
     // Every gather of the row is issued before the first is consumed.
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
-        addr[k] = LiftAddr(lw[E0 + k], i, Z);
-        ext[k] = lam2[addr[k]];
+        addr[k] = EdgeByte<ZT>(lw, E0 + k, i, i4, Z);
+        ext[k] = LdsWord(lam2, addr[k]);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -400,36 +427,38 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
         const uint32_t e = PkSubSatU16(0x00010001u, (ext[k] & 0x7fff7fffu) ^ m1);
         const uint32_t m = PkMadU16(e, d12, s1p) ^ (ext[k] & 0x80008000u);
         msg[E0 + k] = m;
-        lam2[addr[k]] = WH(H2(ext[k]) + H2(m));
+        LdsWord(lam2, addr[k]) = WH(H2(ext[k]) + H2(m));
     }
 }
 
-template <int R, int ROWS>
+template <int R, int ROWS, uint32_t ZT>
 __device__ __forceinline__ void PairIteration(uint32_t *lam2, uint32_t zi, uint32_t i, uint32_t Z, uint32_t *msg)
 {
     if constexpr (R < ROWS) {
-        if (i < Z) PairRow<R>(lam2, zi, i, Z, msg);
+        if (i < Z) PairRow<R, ZT>(lam2, zi, i, Z, msg);
         __syncthreads();
-        PairIteration<R + 1, ROWS>(lam2, zi, i, Z, msg);
+        PairIteration<R + 1, ROWS, ZT>(lam2, zi, i, Z, msg);
     }
 }
 
 // Bit 0: A's check i fails a row's parity; bit 16: B's.
-template <int R, int ROWS>
-__device__ __forceinline__ uint32_t PairSyndrome(const uint32_t *lam2, const uint32_t *lw, uint32_t i, uint32_t Z)
+template <int R, int ROWS, uint32_t ZT>
+__device__ __forceinline__ uint32_t PairSyndrome(uint32_t *lam2, const uint32_t *lw, uint32_t i, uint32_t Z)
 {
     if constexpr (R < ROWS) {
         constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
+        uint32_t i4 = 4u * i;
+        asm volatile("" : "+v"(i4));
         uint32_t v[DEG];
 #pragma unroll
-        for (int k = 0; k < DEG; ++k) v[k] = lam2[LiftAddr(lw[E0 + k], i, Z)];
+        for (int k = 0; k < DEG; ++k) v[k] = LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, Z));
         __builtin_amdgcn_sched_barrier(0);
         uint32_t par = 0;
 #pragma unroll
         for (int k = 0; k < DEG; ++k) par ^= NegBit2(v[k]);
         // One row's loads at a time (keeps the register count down).
         asm volatile("" : "+v"(par) : : "memory");
-        return par | PairSyndrome<R + 1, ROWS>(lam2, lw, i, Z);
+        return par | PairSyndrome<R + 1, ROWS, ZT>(lam2, lw, i, Z);
     }
     return 0;
 }
@@ -645,7 +674,7 @@ __device__ inline void PairEmit(const uint32_t *lam2, uint32_t h, uint32_t cb, u
 // bit-plane codeword LLRs with per-block descriptors (FUSED).  LDS holds the
 // (22 + rows) Z posterior pairs the active rows touch.  With crc.cb_ok set,
 // the epilogue also checks each block's CRC (crc.h).
-template <bool FUSED, int ROWS>
+template <bool FUSED, int ROWS, uint32_t ZT>
 __global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
     LdpcPairKernel(const int16_t *__restrict__ src, const RateMatchDescriptor *__restrict__ desc,
                    int8_t *__restrict__ bits, int16_t *__restrict__ lam_out, uint8_t *__restrict__ iters_out,
@@ -682,12 +711,12 @@ __global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
     __syncthreads();
     const uint32_t *lw = kLift.w[d.zi];
     while (cur[0] < d.num_cb || cur[1] < d.num_cb) {
-        PairIteration<0, ROWS>(lam2, d.zi, i, Z, msg);
+        PairIteration<0, ROWS, ZT>(lam2, d.zi, i, Z, msg);
         ++it[0];
         ++it[1];
         uint32_t ok = 0;  // bit h: half h's syndrome is clear
         if (d.early_stop) {
-            const uint32_t bad = i < Z ? PairSyndrome<0, ROWS>(lam2, lw, i, Z) : 0u;
+            const uint32_t bad = i < Z ? PairSyndrome<0, ROWS, ZT>(lam2, lw, i, Z) : 0u;
             ok = (__syncthreads_or((bad & 1u) != 0) ? 0u : 1u) | (__syncthreads_or((bad >> 16) != 0) ? 0u : 2u);
         }
         bool refilled = false;
@@ -718,7 +747,7 @@ uint32_t PairCapacity(const void *kernel, uint32_t threads, size_t lds)
     return static_cast<uint32_t>(std::max(cus, 0)) * static_cast<uint32_t>(std::max(per_cu, 0));
 }
 
-// Launches LdpcPairKernel<FUSED, rows> (rows 4 .. kFastRows).
+// Launches LdpcPairKernel<FUSED, rows, ZT> (rows 4 .. kFastRows).
 template <bool FUSED>
 void LaunchPair(uint32_t rows, uint32_t threads, size_t lds, hipStream_t s, const int16_t *src,
                 const RateMatchDescriptor *desc, int8_t *bits, int16_t *lam_out, uint8_t *iters_out,
@@ -726,11 +755,13 @@ void LaunchPair(uint32_t rows, uint32_t threads, size_t lds, hipStream_t s, cons
 {
     using Kern = void (*)(const int16_t *, const RateMatchDescriptor *, int8_t *, int16_t *, uint8_t *, LdpcFastDev,
                           airan::CrcFuse, uint32_t *);
-    Kern k = rows <= 4 ? LdpcPairKernel<FUSED, 4>
-           : rows == 5 ? LdpcPairKernel<FUSED, 5>
-           : rows == 6 ? LdpcPairKernel<FUSED, 6>
-           : rows == 7 ? LdpcPairKernel<FUSED, 7>
-                       : LdpcPairKernel<FUSED, 8>;
+    // Z = 384 (every PUSCH profile here): edges folded at compile time.
+    const bool z384 = d.z == 384;
+    Kern k = rows <= 4 ? (z384 ? LdpcPairKernel<FUSED, 4, 384> : LdpcPairKernel<FUSED, 4, 0>)
+           : rows == 5 ? (z384 ? LdpcPairKernel<FUSED, 5, 384> : LdpcPairKernel<FUSED, 5, 0>)
+           : rows == 6 ? (z384 ? LdpcPairKernel<FUSED, 6, 384> : LdpcPairKernel<FUSED, 6, 0>)
+           : rows == 7 ? (z384 ? LdpcPairKernel<FUSED, 7, 384> : LdpcPairKernel<FUSED, 7, 0>)
+                       : (z384 ? LdpcPairKernel<FUSED, 8, 384> : LdpcPairKernel<FUSED, 8, 0>);
     // One resident wave of workgroups, each walking its list of blocks.
     uint32_t grid = (d.num_cb + 1u) / 2u;
     const uint32_t cap = PairCapacity(reinterpret_cast<const void *>(k), threads, lds);

@@ -416,7 +416,7 @@ def test_bg1_table_structure():
     text = (Path(__file__).resolve().parents[1] / "include" / "airan_ldpc_bg1.h").read_text()
 
     def arr(name):
-        body = re.search(r"static const \w+ " + name + r"\[[^\]]*\] = \{(.*?)\};", text, re.S).group(1)
+        body = re.search(r"AIRAN_BG1_TABLE \w+ " + name + r"\[[^\]]*\] = \{(.*?)\};", text, re.S).group(1)
         body = re.sub(r"/\*.*?\*/", "", body)
         return [int(v) for v in body.replace("\n", " ").split(",") if v.strip()]
 
