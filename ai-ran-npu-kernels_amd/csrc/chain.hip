@@ -205,7 +205,7 @@ struct AiranPuschChain {
     float refiner_snr_min = 0.0f, refiner_snr_max = 0.0f;
     uint8_t *refiner_keep = nullptr;
     uint32_t *crc_xpow = nullptr;  // [C] code-block shifts of the TB CRC remainder
-    uint32_t *ldpc_queue = nullptr;  // decoder work counter, zeroed before each decode
+    uint32_t *ldpc_queue = nullptr;  // decoder work counter (2 words, left zero by each decode)
     // Stage timing (events before the first stage and after each stage).
     bool timing = false;
     hipEvent_t ev[AIRAN_CHAIN_STAGES + 1] = {};
@@ -218,6 +218,13 @@ struct AiranPuschChain {
 };
 
 namespace {
+
+// The CRC check's masks: the part of crc_ws after the remainder parts
+// (airan_crc_check_WorkspaceBytes lays it out so).
+uint32_t *CrcMasks(AiranPuschChain *c)
+{
+    return reinterpret_cast<uint32_t *>(c->crc_ws + airan::CrcMaskOffset(c->opts.max_slots));
+}
 
 void DropGraph(AiranPuschChain *c)
 {
@@ -291,21 +298,18 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
     mark();
     const uint32_t ncb = slots * c->plan.C;
     if (c->fused_ldpc) {
-        // Rate dematch runs inside the decoder's load (the stage is empty)
-        // and the CRC checks inside its epilogue (the CRC stage only turns
-        // the transport-block remainders into tb_ok).
+        // Rate dematch runs inside the decoder's load (the stage is empty).
         mark();
-        airan::CrcFuse crc{};
-        if ((st = airan::CrcFuseSetup(c->plan, c->cb_ok, reinterpret_cast<uint32_t *>(c->crc_ws), &crc)) != AIRAN_OK)
-            return st;
-        crc.xpow = c->crc_xpow;
-        airan::LaunchCrcZero(reinterpret_cast<uint32_t *>(c->crc_ws), slots, s);
-        airan::LaunchCrcZero(c->ldpc_queue, 1, s);
         st = airan::LaunchLdpcFromCodeword(g, c->plan, c->desc, c->cw, ncb, c->active_rows, c->opts.ldpc_iterations,
-                                           c->opts.early_stop != 0, c->bits, c->iters, &crc, c->ldpc_queue, s);
+                                           c->opts.early_stop != 0, c->bits, c->iters, c->ldpc_queue, s);
         if (st != AIRAN_OK) return st;
         mark();
-        airan::LaunchCrcFinish(reinterpret_cast<const uint32_t *>(c->crc_ws), c->tb_ok, slots, s);
+        airan::CrcJob crc{};
+        if ((st = airan::CrcJobSetup(c->plan, c->cb_ok, reinterpret_cast<uint32_t *>(c->crc_ws), &crc)) != AIRAN_OK)
+            return st;
+        crc.xpow = c->crc_xpow;
+        airan::LaunchCbCrc(c->bits, crc, c->plan.K, ncb, CrcMasks(c), s);
+        airan::LaunchCrcFinish(reinterpret_cast<const uint32_t *>(c->crc_ws), c->tb_ok, slots, c->plan.C, s);
         if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
         mark();
         return AIRAN_OK;
@@ -362,7 +366,7 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     if (st != AIRAN_OK) return st;
     if (config->flags != 0 || g.ndmrs > 2 || (g.pad & 1u) != 0) return AIRAN_UNSUPPORTED_PROFILE;
     const AiranTbPlanV1 &p = *plan;
-    if (p.C == 0 || p.Z < 24 || p.K != 22 * p.Z || p.G != g.cw_symbols * g.qm ||
+    if (p.C == 0 || p.C > airan::kCrcMaxCb || p.Z < 24 || p.K != 22 * p.Z || p.G != g.cw_symbols * g.qm ||
         static_cast<uint64_t>(p.n_low) * p.E_low + static_cast<uint64_t>(p.C - p.n_low) * p.E_high != p.G ||
         p.E_low % g.qm != 0 || p.E_high % g.qm != 0 || p.Ncb <= p.filler ||
         static_cast<uint64_t>(options->max_slots) * p.C > 65535)
@@ -411,7 +415,7 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         !Alloc(&c->lam, c->fused_ldpc ? 0 : ncb * 68 * p.Z, &b) || !Alloc(&c->ldpc_ws, c->ldpc_ws_bytes, &b) ||
         !Alloc(&c->bits, ncb * p.K, &b) || !Alloc(&c->iters, ncb, &b) || !Alloc(&c->cb_ok, ncb, &b) ||
         !Alloc(&c->tb_ok, S, &b) || !Alloc(&c->refiner_keep, S, &b) || !Alloc(&c->crc_xpow, p.C, &b) || !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
-        !Alloc(&c->ldpc_queue, 1, &b) ||
+        !Alloc(&c->ldpc_queue, 2, &b) ||
         !Alloc(reinterpret_cast<uint8_t **>(&c->zero_iq), g.nr * g.samples * iq_elem, &b))
         return fail(AIRAN_RESOURCE_TOO_SMALL);
     // Per-code-block shifts of the transport-block CRC remainder.
@@ -419,6 +423,14 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         std::vector<uint32_t> xp(p.C);
         for (uint32_t r = 0; r < p.C; ++r) xp[r] = airan::CrcXpowHost(p, r);
         if (hipMemcpy(c->crc_xpow, xp.data(), p.C * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(AIRAN_LAUNCH_FAILED);
+        // The decoder's block queue starts (and stays) zero.
+        if (hipMemset(c->ldpc_queue, 0, 2 * sizeof(uint32_t)) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
+        // The CRC check's masks, after the TB remainder parts in crc_ws.
+        airan::CrcJob job{};
+        if ((st = airan::CrcJobSetup(p, nullptr, nullptr, &job)) != AIRAN_OK) return fail(st);
+        airan::LaunchCrcMasks(job, CrcMasks(c), nullptr);
+        if (hipDeviceSynchronize() != hipSuccess || airan::LaunchStatus() != AIRAN_OK)
             return fail(AIRAN_LAUNCH_FAILED);
     }
     // DMRS reference of this slot configuration.

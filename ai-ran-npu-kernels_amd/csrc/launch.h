@@ -8,7 +8,7 @@ struct LmmseFilter;
 
 namespace airan {
 
-struct CrcFuse;
+struct CrcJob;
 
 // DMRS least squares + per-Rx noise for `slots` stacked grids
 // (rx [slots,NR,14,pad] -> h_ls [slots,NR,L,D,pilot_pad], noise [slots,NR]).
@@ -36,23 +36,40 @@ int32_t LaunchRateDematch(const Geom &g, const AiranTbPlanV1 &plan, const RateMa
 // bit-plane codeword LLRs through the rate-match descriptors: the rate
 // dematch and decoder stages in one launch.  AIRAN_UNSUPPORTED_PROFILE when
 // the active rows exceed the fused decoder's limit (8) or Z is not 24 * 2^j.
-// With crc non-null the decoder also checks each code block's CRC24B
-// (cb_ok) and accumulates the transport blocks' CRC remainders (tb_acc,
-// zeroed by LaunchCrcZero before, turned into tb_ok by LaunchCrcFinish after).
-// queue: null, or one device word zeroed before the launch (LaunchCrcZero
-// with n = 1): the decoder's workgroups then take code blocks from it as
-// they free up instead of walking fixed lists (same outputs, better balance).
+// queue: null, or two device words, zero before the first launch (each
+// launch leaves them zero): the decoder's workgroups then take code blocks
+// from it as they free up instead of walking fixed lists (same outputs,
+// better balance).
 int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &plan, const RateMatchDescriptor *desc,
                                const int16_t *cw, uint32_t num_cb, uint32_t rows, uint32_t iters, bool early_stop,
-                               int8_t *bits, uint8_t *iters_out, const CrcFuse *crc, uint32_t *queue,
+                               int8_t *bits, uint8_t *iters_out, uint32_t *queue,
                                hipStream_t s);
 
-// CRC check of decoded bits inside the decoder (crc.h).
-int32_t CrcFuseSetup(const AiranTbPlanV1 &plan, uint8_t *cb_ok, uint32_t *tb_acc, CrcFuse *out);
+// CRC checks of decoded bits (crc.h): LaunchCbCrc checks each code block's
+// CRC24B (cb_ok) and stores its part of its transport block's CRC remainder
+// (tb_part, one word per block), which LaunchCrcFinish turns into tb_ok.
+// bits: int8 0/1, [num_cb][K].
+int32_t CrcJobSetup(const AiranTbPlanV1 &plan, uint8_t *cb_ok, uint32_t *tb_part, CrcJob *out);
 // x^((C - 1 - r) L) mod g_TB: the shift of code block r's remainder into its
-// transport block's (a per-plan table for CrcFuse::xpow).
+// transport block's (a per-plan table for CrcJob::xpow).
 uint32_t CrcXpowHost(const AiranTbPlanV1 &plan, uint32_t r);
-void LaunchCrcZero(uint32_t *tb_acc, uint32_t num_tbs, hipStream_t s);
-void LaunchCrcFinish(const uint32_t *tb_acc, uint8_t *tb_ok, uint32_t num_tbs, hipStream_t s);
+// Zeroes n device words.
+void LaunchCrcZero(uint32_t *words, uint32_t n, hipStream_t s);
+// The check's per-plan masks (kCrcMaskBytes of device memory, built by
+// LaunchCrcMasks before LaunchCbCrc reads them).
+constexpr size_t kCrcMaskBytes = 2 * 24 * 384 * sizeof(uint32_t);
+// Code blocks per TB the CRC workspace is sized for (TS 38.214 TB sizes
+// need at most 152).
+constexpr uint32_t kCrcMaxCb = 256;
+// Byte offset of the masks in airan_crc_check_WorkspaceBytes(num_tbs): after
+// the num_tbs * kCrcMaxCb remainder parts.
+inline size_t CrcMaskOffset(uint32_t num_tbs)
+{
+    return (static_cast<size_t>(num_tbs) * kCrcMaxCb * sizeof(uint32_t) + 255) / 256 * 256;
+}
+void LaunchCrcMasks(const CrcJob &c, uint32_t *masks, hipStream_t s);
+void LaunchCbCrc(const int8_t *bits, const CrcJob &c, uint32_t K, uint32_t num_cb, const uint32_t *masks,
+                 hipStream_t s);
+void LaunchCrcFinish(const uint32_t *tb_part, uint8_t *tb_ok, uint32_t num_tbs, uint32_t C, hipStream_t s);
 
 }  // namespace airan

@@ -3,10 +3,15 @@
 //
 // The CRC remainder is linear in the message bits: for an n-bit message
 // (data followed by its parity) the remainder is XOR over set bits j of
-// x^(L + n - 1 - j) mod g.  Each workgroup owns one code block, each thread a
-// strided subset of its bits; the partial remainders are XOR-reduced.  The TB
-// remainder of code block r is its local remainder times x^(bits after r)
-// mod g, accumulated with atomicXor (order independent) per TB.
+// x^(L + n - 1 - j) mod g, a GF(2) matrix times the bit vector.  Thread t
+// of the checking kernel owns the 24 positions [24 t, 24 t + 24) of every
+// code block and holds that slice of the matrix as masks m[k] (bit 23 - jj of
+// m[k] = bit k of position 24 t + jj's term): its share of remainder bit k is
+// the parity of (its 24 bits & m[k]); the shares are XOR-reduced over the
+// workgroup.  The TB remainder of code block r is its local remainder times
+// x^(bits after r) mod g, accumulated with atomicXor (order independent) per
+// TB.
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -88,42 +93,157 @@ __global__ void __launch_bounds__(256) ZeroKernel(uint32_t *p, uint32_t n)
     if (t < n) p[t] = 0;
 }
 
-__global__ void __launch_bounds__(256) CbCrcKernel(const int8_t *__restrict__ bits, airan::CrcFuse c, uint32_t K)
+constexpr uint32_t kCrcW = 24;           // bits per thread and block
+constexpr uint32_t kCrcThreads = 384;     // 24 * 384 >= 8448, the largest K
+constexpr uint32_t kCrcMaxLen = 24;
+static_assert(airan::kCrcMaskBytes == 2 * kCrcMaxLen * kCrcThreads * sizeof(uint32_t), "mask table size");
+
+// The four 0/1 bytes of w as 4 bits, the first byte highest.
+__device__ inline uint32_t Pack4(uint32_t w)
 {
-    const uint32_t cbg = blockIdx.x;            // global CB index over all TBs
-    const int8_t *b = bits + static_cast<size_t>(cbg) * K;
-    uint32_t rb = 0, ra = 0;
-    for (uint32_t j = threadIdx.x; j < c.kprime; j += blockDim.x)
-        if (b[j] & 1) airan::CrcAddBit(c, j, rb, ra);
-    // Wavefront then block XOR reduction.
-    for (int off = 32; off > 0; off >>= 1) {
-        rb ^= __shfl_xor(rb, off);
-        ra ^= __shfl_xor(ra, off);
+    return (((w & 0x01010101u) * 0x08040201u) >> 24) & 0xfu;
+}
+
+// Masks of one CRC for positions [w0, w0 + 24) of an n-bit message;
+// pw[e] = x^(len + e) mod g.  Positions >= n get no bits.
+__device__ inline void CrcMasks(const uint32_t *pw, uint32_t poly, uint32_t len, uint32_t n, uint32_t w0,
+                                uint32_t (&m)[kCrcMaxLen])
+{
+#pragma unroll
+    for (uint32_t k = 0; k < kCrcMaxLen; ++k) m[k] = 0;
+    if (w0 >= n) return;
+    const uint32_t last = min(n - w0, kCrcW) - 1;  // last position that counts
+    uint32_t p = pw[n - 1 - (w0 + last)];          // its term; earlier ones are times x
+    const uint32_t top = 1u << (len - 1), mask = (top << 1) - 1u;
+#pragma unroll
+    for (int jj = kCrcW - 1; jj >= 0; --jj) {
+        if (static_cast<uint32_t>(jj) > last) continue;
+#pragma unroll
+        for (uint32_t k = 0; k < kCrcMaxLen; ++k) m[k] |= ((p >> k) & 1u) << (kCrcW - 1 - jj);
+        p = ((p << 1) & mask) ^ ((p & top) ? poly : 0u);
     }
-    __shared__ uint32_t sb[8], sa[8];
-    const uint32_t w = threadIdx.x / 64;
-    if ((threadIdx.x & 63) == 0) {
-        sb[w] = rb;
-        sa[w] = ra;
+}
+
+// Remainder share of the 24 bits v (first position highest) under masks m.
+__device__ inline uint32_t CrcApply(uint32_t v, const uint32_t (&m)[kCrcMaxLen])
+{
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kCrcMaxLen; ++k) r |= (__popc(v & m[k]) & 1u) << k;
+    return r;
+}
+
+// The masks of both CRCs for every thread slot: masks[k][t] (CRC24B rows
+// k < 24, then the TB CRC's), built once per plan.
+__global__ void __launch_bounds__(kCrcThreads) CrcMaskKernel(airan::CrcJob c, uint32_t *__restrict__ masks)
+{
+    const uint32_t t = threadIdx.x, w0 = kCrcW * t;
+    uint32_t mb[kCrcMaxLen], ma[kCrcMaxLen];
+    CrcMasks(c.pow_b, airan::kCrcPolyB, 24, c.crc_cb ? c.kprime : 0u, w0, mb);
+    CrcMasks(c.pow_tb, c.poly_tb, c.len_tb, c.L, w0, ma);
+#pragma unroll
+    for (uint32_t k = 0; k < kCrcMaxLen; ++k) {
+        masks[k * kCrcThreads + t] = mb[k];
+        masks[(kCrcMaxLen + k) * kCrcThreads + t] = ma[k];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t xb = 0, xa = 0;
-        for (uint32_t k = 0; k < blockDim.x / 64; ++k) {
-            xb ^= sb[k];
-            xa ^= sa[k];
+}
+
+constexpr uint32_t kCrcBatch = 4;  // blocks a workgroup checks together
+
+// bits: int8 0/1 per bit, [cb][K].  Persistent: workgroup b checks blocks
+// kCrcBatch at a time (b kCrcBatch, ..., then G kCrcBatch further on) with
+// kCrcThreads threads, thread t owning positions [24 t, 24 t + 24) and
+// holding their masks (CrcMaskKernel) in registers; the batch's loads are in
+// flight together and threads 0 .. kCrcBatch - 1 finish its blocks.
+__global__ void __launch_bounds__(kCrcThreads) CbCrcKernel(const int8_t *__restrict__ bits, airan::CrcJob c,
+                                                          uint32_t K, uint32_t num_cb,
+                                                          const uint32_t *__restrict__ masks)
+{
+    const uint32_t t = threadIdx.x, w0 = kCrcW * t;
+    uint32_t mb[kCrcMaxLen], ma[kCrcMaxLen];
+#pragma unroll
+    for (uint32_t k = 0; k < kCrcMaxLen; ++k) {
+        mb[k] = masks[k * kCrcThreads + t];
+        ma[k] = masks[(kCrcMaxLen + k) * kCrcThreads + t];
+    }
+    // Three 8-byte loads when the block base and w0 are 8-aligned and the
+    // slice lies inside the block.
+    const bool vec = (K % 8u) == 0 && (reinterpret_cast<uintptr_t>(bits) & 7u) == 0 && w0 + kCrcW <= K;
+    const bool any = w0 < c.kprime;
+    __shared__ uint32_t red[kCrcBatch][2][kCrcThreads / 64];
+    for (uint32_t base = blockIdx.x * kCrcBatch; base < num_cb; base += gridDim.x * kCrcBatch) {
+        uint32_t v[kCrcBatch];
+        if (vec) {
+            uint2 q[kCrcBatch][3];
+#pragma unroll
+            for (uint32_t u = 0; u < kCrcBatch; ++u) {
+                q[u][0] = q[u][1] = q[u][2] = make_uint2(0, 0);
+                if (any && base + u < num_cb) {
+                    const uint2 *p = reinterpret_cast<const uint2 *>(bits + static_cast<size_t>(base + u) * K + w0);
+                    q[u][0] = p[0];
+                    q[u][1] = p[1];
+                    q[u][2] = p[2];
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kCrcBatch; ++u)
+                v[u] = Pack4(q[u][0].x) << 20 | Pack4(q[u][0].y) << 16 | Pack4(q[u][1].x) << 12 |
+                       Pack4(q[u][1].y) << 8 | Pack4(q[u][2].x) << 4 | Pack4(q[u][2].y);
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < kCrcBatch; ++u) {
+                v[u] = 0;
+                if (!any || base + u >= num_cb) continue;
+                const int8_t *b = bits + static_cast<size_t>(base + u) * K + w0;
+                for (uint32_t j = 0; j < kCrcW && w0 + j < K; ++j)
+                    v[u] |= static_cast<uint32_t>(b[j] & 1) << (kCrcW - 1 - j);
+            }
         }
-        airan::CrcFinishBlock(c, cbg, xb, xa);
+        uint32_t rb[kCrcBatch], ra[kCrcBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kCrcBatch; ++u) {
+            rb[u] = CrcApply(v[u], mb);
+            ra[u] = CrcApply(v[u], ma);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (uint32_t u = 0; u < kCrcBatch; ++u) {
+                rb[u] ^= __shfl_xor(rb[u], off);
+                ra[u] ^= __shfl_xor(ra[u], off);
+            }
+        }
+        if ((t & 63u) == 0) {
+#pragma unroll
+            for (uint32_t u = 0; u < kCrcBatch; ++u) {
+                red[u][0][t >> 6] = rb[u];
+                red[u][1][t >> 6] = ra[u];
+            }
+        }
+        __syncthreads();
+        if (t < kCrcBatch && base + t < num_cb) {
+            uint32_t xb = 0, xa = 0;
+            for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
+                xb ^= red[t][0][w];
+                xa ^= red[t][1][w];
+            }
+            airan::CrcFinishBlock(c, base + t, xb, xa);
+        }
+        __syncthreads();
     }
 }
 
-__global__ void __launch_bounds__(256) TbFinishKernel(const uint32_t *__restrict__ tb_acc,
-                                                      uint8_t *__restrict__ tb_ok, uint32_t n)
+// tb_ok of TB blockIdx.x (one wave): the XOR of its C blocks' shifted
+// remainder parts is zero.
+__global__ void __launch_bounds__(64) TbFinishKernel(const uint32_t *__restrict__ tb_part,
+                                                     uint8_t *__restrict__ tb_ok, uint32_t C)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) tb_ok[t] = tb_acc[t] == 0;
+    const uint32_t tb = blockIdx.x, l = threadIdx.x;
+    const uint32_t *p = tb_part + static_cast<size_t>(tb) * C;
+    uint32_t x = 0;
+    for (uint32_t r = l; r < C; r += 64) x ^= p[r];
+    for (int off = 32; off > 0; off >>= 1) x ^= __shfl_xor(x, off);
+    if (l == 0) tb_ok[tb] = x == 0;
 }
-
 // Payload byte b of TB tb: bits 8b..8b+7 (MSB first) of the TB bit stream.
 __global__ void __launch_bounds__(256) PackKernel(const int8_t *__restrict__ bits,
                                                   uint8_t *__restrict__ out, CrcDev d)
@@ -175,35 +295,58 @@ uint32_t CrcXpowHost(const AiranTbPlanV1 &p, uint32_t r)
     return x;
 }
 
-int32_t CrcFuseSetup(const AiranTbPlanV1 &p, uint8_t *cb_ok, uint32_t *tb_acc, CrcFuse *out)
+int32_t CrcJobSetup(const AiranTbPlanV1 &p, uint8_t *cb_ok, uint32_t *tb_part, CrcJob *out)
 {
-    if (p.C == 0 || p.Kprime > p.K || p.K > 8448 || p.Kprime < p.crc_cb || (p.crc_tb != 24 && p.crc_tb != 16) ||
+    if (p.C == 0 || p.C > kCrcMaxCb || p.Kprime > p.K || p.K > 8448 || p.Kprime < p.crc_cb ||
+        (p.crc_tb != 24 && p.crc_tb != 16) ||
         (p.crc_cb != 0 && p.crc_cb != 24) ||
         static_cast<uint64_t>(p.C) * (p.Kprime - p.crc_cb) != static_cast<uint64_t>(p.tbs) + p.crc_tb)
         return AIRAN_INVALID_ARGUMENT;
     CrcTables *t = nullptr;
     const int32_t st = EnsureTables(&t);
     if (st != AIRAN_OK) return st;
-    *out = CrcFuse{t->pow_b, p.crc_tb == 24 ? t->pow_a : t->pow_16, cb_ok, tb_acc, nullptr, p.C, p.Kprime, p.crc_cb,
+    *out = CrcJob{t->pow_b, p.crc_tb == 24 ? t->pow_a : t->pow_16, cb_ok, tb_part, nullptr, p.C, p.Kprime, p.crc_cb,
                    p.Kprime - p.crc_cb, p.crc_tb == 24 ? kPolyA : kPoly16, p.crc_tb};
     return AIRAN_OK;
 }
 
-void LaunchCrcZero(uint32_t *tb_acc, uint32_t num_tbs, hipStream_t s)
+void LaunchCrcZero(uint32_t *words, uint32_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(ZeroKernel, dim3(CeilDiv(num_tbs, 256)), dim3(256), 0, s, tb_acc, num_tbs);
+    hipLaunchKernelGGL(ZeroKernel, dim3(CeilDiv(n, 256)), dim3(256), 0, s, words, n);
 }
 
-void LaunchCrcFinish(const uint32_t *tb_acc, uint8_t *tb_ok, uint32_t num_tbs, hipStream_t s)
+void LaunchCrcMasks(const CrcJob &c, uint32_t *masks, hipStream_t s)
 {
-    hipLaunchKernelGGL(TbFinishKernel, dim3(CeilDiv(num_tbs, 256)), dim3(256), 0, s, tb_acc, tb_ok, num_tbs);
+    hipLaunchKernelGGL(CrcMaskKernel, dim3(1), dim3(kCrcThreads), 0, s, c, masks);
+}
+
+void LaunchCbCrc(const int8_t *bits, const CrcJob &c, uint32_t K, uint32_t num_cb, const uint32_t *masks,
+                 hipStream_t s)
+{
+    // One resident wave of workgroups (the masks are built once per workgroup).
+    uint32_t cap = 1024;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(CbCrcKernel),
+                                                     static_cast<int>(kCrcThreads), 0) == hipSuccess &&
+        cus > 0 && per_cu > 0)
+        cap = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu);
+    const uint32_t grid = std::max(1u, std::min(CeilDiv(num_cb, kCrcBatch), cap));
+    hipLaunchKernelGGL(CbCrcKernel, dim3(grid), dim3(kCrcThreads), 0, s, bits, c, K, num_cb, masks);
+}
+
+void LaunchCrcFinish(const uint32_t *tb_part, uint8_t *tb_ok, uint32_t num_tbs, uint32_t C, hipStream_t s)
+{
+    hipLaunchKernelGGL(TbFinishKernel, dim3(num_tbs), dim3(64), 0, s, tb_part, tb_ok, C);
 }
 
 }  // namespace airan
 
 extern "C" size_t airan_crc_check_WorkspaceBytes(uint32_t num_tbs)
 {
-    return static_cast<size_t>(num_tbs) * sizeof(uint32_t);
+    // Per-block TB remainder parts, then the check masks.
+    return airan::CrcMaskOffset(num_tbs) + airan::kCrcMaskBytes;
 }
 
 extern "C" int32_t airan_crc_check_Enqueue(const AiranTbPlanV1 *p, uint32_t num_tbs, const void *bits,
@@ -211,8 +354,8 @@ extern "C" int32_t airan_crc_check_Enqueue(const AiranTbPlanV1 *p, uint32_t num_
                                            size_t workspace_bytes, void *stream)
 {
     if (p == nullptr || bits == nullptr || cb_ok == nullptr || tb_ok == nullptr || stream == nullptr ||
-        num_tbs == 0 || p->C == 0 || p->Kprime > p->K || p->K > 8448 || p->Kprime < p->crc_cb ||
-        (p->crc_tb != 24 && p->crc_tb != 16) || (p->crc_cb != 0 && p->crc_cb != 24) ||
+        num_tbs == 0 || p->C == 0 || p->C > airan::kCrcMaxCb || p->Kprime > p->K || p->K > 8448 ||
+        p->Kprime < p->crc_cb || (p->crc_tb != 24 && p->crc_tb != 16) || (p->crc_cb != 0 && p->crc_cb != 24) ||
         static_cast<uint64_t>(p->C) * (p->Kprime - p->crc_cb) != static_cast<uint64_t>(p->tbs) + p->crc_tb)
         return AIRAN_INVALID_ARGUMENT;
     if (workspace == nullptr || workspace_bytes < airan_crc_check_WorkspaceBytes(num_tbs))
@@ -222,14 +365,14 @@ extern "C" int32_t airan_crc_check_Enqueue(const AiranTbPlanV1 *p, uint32_t num_
     if (st != AIRAN_OK) return st;
     CrcDev d{p->C, p->K, p->Kprime, p->crc_cb, p->crc_tb, p->tbs,
              p->crc_tb == 24 ? kPolyA : kPoly16, p->crc_tb};
-    airan::CrcFuse c{t->pow_b, p->crc_tb == 24 ? t->pow_a : t->pow_16, static_cast<uint8_t *>(cb_ok),
+    airan::CrcJob c{t->pow_b, p->crc_tb == 24 ? t->pow_a : t->pow_16, static_cast<uint8_t *>(cb_ok),
                      static_cast<uint32_t *>(workspace), nullptr, p->C, p->Kprime, p->crc_cb,
                      p->Kprime - p->crc_cb, d.poly_tb, d.len_tb};
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(ZeroKernel, dim3(airan::CeilDiv(num_tbs, 256)), dim3(256), 0, s, static_cast<uint32_t *>(workspace), num_tbs);
-    hipLaunchKernelGGL(CbCrcKernel, dim3(num_tbs * p->C), dim3(256), 0, s, static_cast<const int8_t *>(bits), c, p->K);
-    hipLaunchKernelGGL(TbFinishKernel, dim3(airan::CeilDiv(num_tbs, 256)), dim3(256), 0, s, static_cast<uint32_t *>(workspace),
-                       static_cast<uint8_t *>(tb_ok), num_tbs);
+    uint32_t *masks = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(workspace) + airan::CrcMaskOffset(num_tbs));
+    airan::LaunchCrcMasks(c, masks, s);
+    airan::LaunchCbCrc(static_cast<const int8_t *>(bits), c, p->K, num_tbs * p->C, masks, s);
+    airan::LaunchCrcFinish(static_cast<const uint32_t *>(workspace), static_cast<uint8_t *>(tb_ok), num_tbs, p->C, s);
     if (tb_bytes != nullptr) {
         dim3 grid(airan::CeilDiv((p->tbs + 7) / 8, 256), num_tbs);
         hipLaunchKernelGGL(PackKernel, grid, dim3(256), 0, s, static_cast<const int8_t *>(bits),

@@ -15,7 +15,6 @@
 
 #include "common.h"
 #include "airan_ldpc_bg1.h"
-#include "crc.h"
 #include "launch.h"
 #include "ratematch.h"
 
@@ -475,71 +474,85 @@ __device__ inline uint32_t InputH(int32_t v)
     return BitsH(static_cast<h16>(static_cast<float>(v) * (1.0f / 256.0f)));
 }
 
-// Loads one block's posterior start values into half h of lam2 (nl
-// positions).  FUSED: de-rate-matched from the bit-plane codeword LLRs.
+// Loads block cb's posterior start values into half h of lam2 (nl
+// positions; cb >= num_cb: a zero block).  FUSED: de-rate-matched from the
+// bit-plane codeword LLRs with descriptor desc[cb].
 template <bool FUSED>
 __device__ inline void PairLoad(uint16_t *lam16, uint32_t h, uint32_t nl, const int16_t *__restrict__ src,
-                                const airan::RateCb &rc, const LdpcFastDev &d, uint32_t cb)
+                                const RateMatchDescriptor *__restrict__ desc, const LdpcFastDev &d, uint32_t cb)
 {
     const uint32_t i = threadIdx.x;
     if (cb >= d.num_cb) {
         for (uint32_t t = i; t < nl; t += blockDim.x) lam16[2 * t + h] = 0;
         return;
     }
-    if (FUSED && airan::RateSingle(rc, d.rate)) {
-        // Every transmitted bit kk < E lands on its own position (the
-        // inverse of RateSingleIndex): q = (q0 + kk) mod (Ncb - filler),
-        // position 2Z + (q < fstart ? q : q + filler).  So the block's
-        // codeword LLRs -- Qm planes of E/Qm consecutive symbols -- are read
-        // in aligned 16-byte chunks and scattered into LDS, after the other
-        // positions are set (zero, or +5120 for filler bits).
-        const uint32_t two_z = 2u * d.z, fstart = d.rate.kprime - two_z;
-        const uint32_t nf = d.rate.ncb - d.rate.filler;
-        const uint32_t fill = InputH(5120);
-        for (uint32_t t = i; t < nl; t += blockDim.x)
-            lam16[2 * t + h] = static_cast<uint16_t>(
-                t >= two_z + fstart && t < two_z + fstart + d.rate.filler ? fill : 0u);
-        __syncthreads();
-        // Symbols rem0 .. rem0 + eq - 1 of each plane of the slot, in chunks
-        // of 8 aligned to the plane start (cw_stride is a multiple of 8).
-        const int16_t *slot = src + static_cast<size_t>(rc.slot0) * d.rate.qm * d.rate.cw_stride;
-        const uint32_t c0 = rc.rem0 / 8u, chunks = (rc.rem0 + rc.eq + 7u) / 8u - c0;
-        const uint32_t total = chunks * d.rate.qm;
-        constexpr uint32_t kBatch = 4;  // chunk loads of a thread in flight together
-        for (uint32_t e0 = i; e0 < total; e0 += kBatch * blockDim.x) {
+    if constexpr (FUSED) {
+        const airan::RateCb rc = airan::RateCbSetup(desc[cb], d.rate);
+        if (airan::RateSingle(rc, d.rate)) {
+            // Every transmitted bit kk < E lands on its own position (the
+            // inverse of RateSingleIndex): q = (q0 + kk) mod (Ncb - filler),
+            // position 2Z + (q < fstart ? q : q + filler).  So the block's
+            // codeword LLRs -- Qm planes of E/Qm consecutive symbols -- are
+            // read in aligned 16-byte chunks and scattered into LDS, after the
+            // other positions are set (zero, or +5120 for filler bits).
+            const uint32_t two_z = 2u * d.z, fstart = d.rate.kprime - two_z;
+            const uint32_t nf = d.rate.ncb - d.rate.filler;
+            // Symbols rem0 .. rem0 + eq - 1 of each plane of the slot, in
+            // chunks of 8 aligned to the plane start (cw_stride is a multiple
+            // of 8).
+            const int16_t *slot = src + static_cast<size_t>(rc.slot0) * d.rate.qm * d.rate.cw_stride;
+            const uint32_t c0 = rc.rem0 / 8u, chunks = (rc.rem0 + rc.eq + 7u) / 8u - c0;
+            const uint32_t total = chunks * d.rate.qm;
+            constexpr uint32_t kBatch = 4;  // chunk loads of a thread in flight together
             uint4 raw[kBatch];
+            auto fetch = [&](uint32_t e0) {
 #pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                const uint32_t e = e0 + u * blockDim.x;
-                if (e < total) {
+                for (uint32_t u = 0; u < kBatch; ++u) {
+                    const uint32_t e = e0 + u * blockDim.x;
+                    if (e < total) {
+                        const uint32_t b = e / chunks, c = c0 + (e - b * chunks);
+                        raw[u] = *reinterpret_cast<const uint4 *>(slot + static_cast<size_t>(b) * d.rate.cw_stride +
+                                                                  8u * c);
+                    }
+                }
+            };
+            // The first batch is in flight while the other positions are set.
+            uint32_t e0 = i;
+            fetch(e0);
+            const uint32_t fill = InputH(5120);
+            for (uint32_t t = i; t < nl; t += blockDim.x)
+                lam16[2 * t + h] = static_cast<uint16_t>(
+                    t >= two_z + fstart && t < two_z + fstart + d.rate.filler ? fill : 0u);
+            __syncthreads();
+            for (;;) {
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) {
+                    const uint32_t e = e0 + u * blockDim.x;
+                    if (e >= total) continue;
                     const uint32_t b = e / chunks, c = c0 + (e - b * chunks);
-                    raw[u] = *reinterpret_cast<const uint4 *>(slot + static_cast<size_t>(b) * d.rate.cw_stride + 8u * c);
-                }
-            }
+                    const uint32_t w[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
 #pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                const uint32_t e = e0 + u * blockDim.x;
-                if (e >= total) continue;
-                const uint32_t b = e / chunks, c = c0 + (e - b * chunks);
-                const uint32_t w[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
-#pragma unroll
-                for (uint32_t x = 0; x < 8; ++x) {
-                    const int32_t j = static_cast<int32_t>(8u * c + x) - static_cast<int32_t>(rc.rem0);
-                    if (j < 0 || j >= static_cast<int32_t>(rc.eq)) continue;
-                    const uint32_t kk = b * rc.eq + static_cast<uint32_t>(j);
-                    uint32_t q = rc.q0 + kk;
-                    q = q >= nf ? q - nf : q;
-                    const uint32_t t = two_z + (q < fstart ? q : q + d.rate.filler);
-                    if (t >= nl) continue;
-                    int32_t v = static_cast<int16_t>(w[x >> 1] >> (16u * (x & 1u))) * 8;
-                    v = v > 5120 ? 5120 : (v < -5120 ? -5120 : v);
-                    lam16[2 * t + h] = static_cast<uint16_t>(InputH(v));
+                    for (uint32_t x = 0; x < 8; ++x) {
+                        const int32_t j = static_cast<int32_t>(8u * c + x) - static_cast<int32_t>(rc.rem0);
+                        if (j < 0 || j >= static_cast<int32_t>(rc.eq)) continue;
+                        const uint32_t kk = b * rc.eq + static_cast<uint32_t>(j);
+                        uint32_t q = rc.q0 + kk;
+                        q = q >= nf ? q - nf : q;
+                        const uint32_t t = two_z + (q < fstart ? q : q + d.rate.filler);
+                        if (t >= nl) continue;
+                        int32_t v = static_cast<int16_t>(w[x >> 1] >> (16u * (x & 1u))) * 8;
+                        v = v > 5120 ? 5120 : (v < -5120 ? -5120 : v);
+                        lam16[2 * t + h] = static_cast<uint16_t>(InputH(v));
+                    }
                 }
+                e0 += kBatch * blockDim.x;
+                if (e0 >= total) break;
+                fetch(e0);
             }
+        } else {
+            for (uint32_t t = i; t < nl; t += blockDim.x)
+                lam16[2 * t + h] = static_cast<uint16_t>(InputH(airan::RateDematchValue(t, rc, d.rate, src)));
         }
-    } else if (FUSED) {
-        for (uint32_t t = i; t < nl; t += blockDim.x)
-            lam16[2 * t + h] = static_cast<uint16_t>(InputH(airan::RateDematchValue(t, rc, d.rate, src)));
     } else {
         // nl is a multiple of 4 (Z is a multiple of 24) and the block 8-byte aligned.
         const uint2 *s4 = reinterpret_cast<const uint2 *>(src + static_cast<size_t>(cb) * 68u * d.z);
@@ -553,96 +566,34 @@ __device__ inline void PairLoad(uint16_t *lam16, uint32_t h, uint32_t nl, const 
     }
 }
 
-// Writes block cb (half h of lam2): iteration count, hard bits (with the CRC
-// epilogue when crc.cb_ok is set) and, when lam_out is set, the int16
-// posterior (positions beyond the LDS part keep their input).  Called by the
-// whole workgroup; ends with a barrier.
+// Writes block cb (half h of lam2): iteration count, hard bits and, when
+// lam_out is set, the int16 posterior (positions beyond the LDS part keep
+// their input).  Called by the whole workgroup; ends with a barrier, after
+// which half h may be overwritten.
 template <bool FUSED>
 __device__ inline void PairEmit(const uint32_t *lam2, uint32_t h, uint32_t cb, uint32_t it, uint32_t nl,
-                                const int16_t *__restrict__ src, const airan::RateCb &rc, int8_t *__restrict__ bits,
-                                int16_t *__restrict__ lam_out, uint8_t *__restrict__ iters_out, const LdpcFastDev &d,
-                                const airan::CrcFuse &crc)
+                                const int16_t *__restrict__ src, const RateMatchDescriptor *__restrict__ desc,
+                                int8_t *__restrict__ bits, int16_t *__restrict__ lam_out,
+                                uint8_t *__restrict__ iters_out, const LdpcFastDev &d)
 {
     const uint32_t i = threadIdx.x;
     const uint16_t *lam16 = reinterpret_cast<const uint16_t *>(lam2);
     if (iters_out != nullptr && i == 0) iters_out[cb] = static_cast<uint8_t>(it);
     int8_t *bo = bits + static_cast<size_t>(cb) * d.k;
-    const bool check = crc.cb_ok != nullptr;
-    uint32_t rb = 0, ra = 0;
+    const uint32_t sh = 16u * h;
     if ((d.k & 3u) == 0) {
         const uint4 *l4 = reinterpret_cast<const uint4 *>(lam2);
-        // With kprime and L multiples of 4, the power-table entries of bits
-        // 4t .. 4t+3 are four consecutive words (in reverse order): one
-        // 16-byte load per table instead of a branch and a load per bit.
-        const bool vec = check && (crc.kprime & 3u) == 0 && (crc.L & 3u) == 0;
-        const uint32_t sh = 16u * h;
-        auto sel = [](uint32_t b4, uint4 p) {
-            return ((b4 & 1u) ? p.w : 0u) ^ ((b4 & 0x100u) ? p.z : 0u) ^ ((b4 & 0x10000u) ? p.y : 0u) ^
-                   ((b4 & 0x1000000u) ? p.x : 0u);
-        };
-        // kBatch groups of 4 bits per thread at a time, their power-table
-        // loads in flight together.
-        constexpr uint32_t kBatch = 3;
-        for (uint32_t t0 = i; t0 < d.k / 4u; t0 += kBatch * blockDim.x) {
-            uint32_t b4[kBatch];
-            uint4 pb[kBatch], pt[kBatch];
-#pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                const uint32_t t = t0 + u * blockDim.x;
-                b4[u] = 0;
-                pb[u] = pt[u] = make_uint4(0, 0, 0, 0);
-                if (t < d.k / 4u) {
-                    const uint4 v = l4[t];
-                    b4[u] = NegBit(v.x >> sh) | NegBit(v.y >> sh) << 8 | NegBit(v.z >> sh) << 16 |
-                            NegBit(v.w >> sh) << 24;
-                    reinterpret_cast<uint32_t *>(bo)[t] = b4[u];
-                    const uint32_t j = 4 * t;
-                    if (vec && crc.crc_cb && j < crc.kprime)
-                        pb[u] = *reinterpret_cast<const uint4 *>(crc.pow_b + (crc.kprime - 4 - j));
-                    if (vec && j < crc.L) pt[u] = *reinterpret_cast<const uint4 *>(crc.pow_tb + (crc.L - 4 - j));
-                }
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kBatch; ++u) {
-                const uint32_t t = t0 + u * blockDim.x;
-                if (vec) {
-                    rb ^= sel(b4[u], pb[u]);
-                    ra ^= sel(b4[u], pt[u]);
-                } else if (check && t < d.k / 4u) {
-                    for (uint32_t q = 0; q < 4; ++q)
-                        if ((b4[u] >> (8 * q)) & 1u) airan::CrcAddBit(crc, 4 * t + q, rb, ra);
-                }
-            }
+        for (uint32_t t = i; t < d.k / 4u; t += blockDim.x) {
+            const uint4 v = l4[t];
+            reinterpret_cast<uint32_t *>(bo)[t] =
+                NegBit(v.x >> sh) | NegBit(v.y >> sh) << 8 | NegBit(v.z >> sh) << 16 | NegBit(v.w >> sh) << 24;
         }
     } else {
-        for (uint32_t t = i; t < d.k; t += blockDim.x) {
-            const uint32_t b = NegBit(lam16[2 * t + h]);
-            bo[t] = static_cast<int8_t>(b);
-            if (check && b) airan::CrcAddBit(crc, t, rb, ra);
-        }
-    }
-    if (check) {
-        __shared__ uint32_t red[2][8];
-        for (int off = 32; off > 0; off >>= 1) {
-            rb ^= __shfl_xor(rb, off);
-            ra ^= __shfl_xor(ra, off);
-        }
-        if ((i & 63u) == 0) {
-            red[0][i >> 6] = rb;
-            red[1][i >> 6] = ra;
-        }
-        __syncthreads();
-        if (i == 0) {
-            uint32_t xb = 0, xa = 0;
-            for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
-                xb ^= red[0][w];
-                xa ^= red[1][w];
-            }
-            airan::CrcFinishBlock(crc, cb, xb, xa);
-        }
+        for (uint32_t t = i; t < d.k; t += blockDim.x) bo[t] = static_cast<int8_t>(NegBit(lam16[2 * t + h]));
     }
     if (lam_out != nullptr) {
         const uint32_t N = 68u * d.z;
+        const airan::RateCb rc = FUSED ? airan::RateCbSetup(desc[cb], d.rate) : airan::RateCb{};
         int16_t *lo = lam_out + static_cast<size_t>(cb) * N;
         for (uint32_t t = i; t < N; t += blockDim.x) {
             float v;
@@ -666,19 +617,23 @@ __device__ inline void PairEmit(const uint32_t *lam2, uint32_t h, uint32_t cb, u
 // written and the next block is loaded into that half (its messages reset)
 // while the other half runs on; a half past the last block holds a zero
 // block and is never written.  Workgroup b starts with blocks b and b + G;
-// after that it takes blocks 2 G, 2 G + 1, ... from the counter `queue`
-// (zero at launch) as halves free up, or, with queue null, its static list
+// after that it takes blocks 2 G, 2 G + 1, ... from the counter queue[0]
+// (two words, zero at launch and left zero) as halves free up, or, with
+// queue null, its static list
 // b + 2 G, b + 3 G, ...  Blocks are independent of the pairing
 // (the halves never interact), so every block's outputs equal a decode of
 // that block alone.  src: int16 decoder LLRs [cb][68 Z] (!FUSED) or
 // bit-plane codeword LLRs with per-block descriptors (FUSED).  LDS holds the
-// (22 + rows) Z posterior pairs the active rows touch.  With crc.cb_ok set,
-// the epilogue also checks each block's CRC (crc.h).
+// (22 + rows) Z posterior pairs the active rows touch.
+//
+// The refill code (emit, take, load) exists once, in a rolled loop over the
+// two halves that also does the first loads: the kernel is large, and each
+// inlined copy would be a separate cold instruction fetch on every CU.
 template <bool FUSED, int ROWS, uint32_t ZT>
 __global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
     LdpcPairKernel(const int16_t *__restrict__ src, const RateMatchDescriptor *__restrict__ desc,
                    int8_t *__restrict__ bits, int16_t *__restrict__ lam_out, uint8_t *__restrict__ iters_out,
-                   LdpcFastDev d, airan::CrcFuse crc, uint32_t *__restrict__ queue)
+                   LdpcFastDev d, uint32_t *__restrict__ queue)
 {
     __shared__ uint32_t taken_cb;
     uint32_t *lam2 = reinterpret_cast<uint32_t *>(airan_ldpc_lam);
@@ -686,52 +641,64 @@ __global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
     const uint32_t nl = (22u + ROWS) * Z;
     constexpr int kEdges = FastRowStart(ROWS);
     uint32_t msg[kEdges];
-    uint32_t taken = 0;
-    uint32_t cur[2], it[2];
-    airan::RateCb rc[2];
-    // Loads the workgroup's next block into half h (a zero block past the end).
-    auto start = [&](uint32_t h) {
-        if (queue == nullptr || taken < 2) {
-            cur[h] = blockIdx.x + taken * gridDim.x;
-        } else {
-            if (i == 0) taken_cb = 2u * gridDim.x + atomicAdd(queue, 1u);
-            __syncthreads();
-            cur[h] = __builtin_amdgcn_readfirstlane(taken_cb);
-        }
-        ++taken;
-        rc[h] = airan::RateCb{};
-        if (FUSED && cur[h] < d.num_cb) rc[h] = airan::RateCbSetup(desc[cur[h]], d.rate);
-        PairLoad<FUSED>(airan_ldpc_lam, h, nl, src, rc[h], d, cur[h]);
-        it[h] = 0;
-    };
-    start(0);
-    start(1);
 #pragma unroll
     for (int e = 0; e < kEdges; ++e) msg[e] = 0;
-    __syncthreads();
+    uint32_t taken = 0;
+    // Per half: its block (>= num_cb: none) and the iterations run on it.
+    uint32_t cur0 = ~0u, cur1 = ~0u, it0 = 0, it1 = 0;
+    uint32_t refill = 3u;  // bit h: half h is written (if it holds a block) and refilled
     const uint32_t *lw = kLift.w[d.zi];
-    while (cur[0] < d.num_cb || cur[1] < d.num_cb) {
+    for (;;) {
+        if (refill != 0) {
+#pragma nounroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                if (((refill >> h) & 1u) == 0) continue;
+                const uint32_t cb = h ? cur1 : cur0;
+                if (cb < d.num_cb) PairEmit<FUSED>(lam2, h, cb, h ? it1 : it0, nl, src, desc, bits, lam_out, iters_out, d);
+                uint32_t next;
+                if (queue == nullptr || taken < 2) {
+                    next = blockIdx.x + taken * gridDim.x;
+                } else {
+                    if (i == 0) taken_cb = 2u * gridDim.x + atomicAdd(queue, 1u);
+                    __syncthreads();
+                    next = __builtin_amdgcn_readfirstlane(taken_cb);
+                }
+                ++taken;
+                PairLoad<FUSED>(airan_ldpc_lam, h, nl, src, desc, d, next);
+                if (h) {
+                    cur1 = next;
+                    it1 = 0;
+                } else {
+                    cur0 = next;
+                    it0 = 0;
+                }
+                const uint32_t keep = h ? 0x0000ffffu : 0xffff0000u;
+#pragma unroll
+                for (int e = 0; e < kEdges; ++e) msg[e] &= keep;
+            }
+            __syncthreads();
+            if (cur0 >= d.num_cb && cur1 >= d.num_cb) break;
+        }
         PairIteration<0, ROWS, ZT>(lam2, d.zi, i, Z, msg);
-        ++it[0];
-        ++it[1];
+        ++it0;
+        ++it1;
         uint32_t ok = 0;  // bit h: half h's syndrome is clear
         if (d.early_stop) {
             const uint32_t bad = i < Z ? PairSyndrome<0, ROWS, ZT>(lam2, lw, i, Z) : 0u;
             ok = (__syncthreads_or((bad & 1u) != 0) ? 0u : 1u) | (__syncthreads_or((bad >> 16) != 0) ? 0u : 2u);
         }
-        bool refilled = false;
-#pragma unroll
-        for (uint32_t h = 0; h < 2; ++h) {
-            if (cur[h] < d.num_cb && (((ok >> h) & 1u) || it[h] >= d.iters)) {
-                PairEmit<FUSED>(lam2, h, cur[h], it[h], nl, src, rc[h], bits, lam_out, iters_out, d, crc);
-                start(h);
-                const uint32_t keep = h == 0 ? 0xffff0000u : 0x0000ffffu;
-#pragma unroll
-                for (int e = 0; e < kEdges; ++e) msg[e] &= keep;
-                refilled = true;
-            }
+        refill = 0;
+        if (cur0 < d.num_cb && ((ok & 1u) || it0 >= d.iters)) refill |= 1u;
+        if (cur1 < d.num_cb && ((ok & 2u) || it1 >= d.iters)) refill |= 2u;
+    }
+    // The last workgroup to finish leaves the queue zeroed for the next
+    // launch: every take (queue[0]) precedes its workgroup's count in queue[1].
+    if (queue != nullptr && i == 0) {
+        __threadfence();
+        if (atomicAdd(queue + 1, 1u) == gridDim.x - 1) {
+            atomicExch(queue, 0u);
+            atomicExch(queue + 1, 0u);
         }
-        if (refilled) __syncthreads();
     }
 }
 
@@ -751,10 +718,10 @@ uint32_t PairCapacity(const void *kernel, uint32_t threads, size_t lds)
 template <bool FUSED>
 void LaunchPair(uint32_t rows, uint32_t threads, size_t lds, hipStream_t s, const int16_t *src,
                 const RateMatchDescriptor *desc, int8_t *bits, int16_t *lam_out, uint8_t *iters_out,
-                const LdpcFastDev &d, const airan::CrcFuse &crc, uint32_t *queue)
+                const LdpcFastDev &d, uint32_t *queue)
 {
     using Kern = void (*)(const int16_t *, const RateMatchDescriptor *, int8_t *, int16_t *, uint8_t *, LdpcFastDev,
-                          airan::CrcFuse, uint32_t *);
+                          uint32_t *);
     // Z = 384 (every PUSCH profile here): edges folded at compile time.
     const bool z384 = d.z == 384;
     Kern k = rows <= 4 ? (z384 ? LdpcPairKernel<FUSED, 4, 384> : LdpcPairKernel<FUSED, 4, 0>)
@@ -766,7 +733,7 @@ void LaunchPair(uint32_t rows, uint32_t threads, size_t lds, hipStream_t s, cons
     uint32_t grid = (d.num_cb + 1u) / 2u;
     const uint32_t cap = PairCapacity(reinterpret_cast<const void *>(k), threads, lds);
     if (cap > 0) grid = std::min(grid, cap);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, s, src, desc, bits, lam_out, iters_out, d, crc, queue);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, s, src, desc, bits, lam_out, iters_out, d, queue);
 }
 
 bool FastEligible(uint32_t mode, uint32_t z, uint32_t rows)
@@ -830,7 +797,7 @@ int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, 
         const uint32_t rows = a->active_rows;  // validated >= 4
         LdpcFastDev fd{a->z, LiftIndex(a->z), a->k, a->max_iterations, rows, a->early_stop ? 1u : 0u, a->num_cb, {}};
         const size_t fast_lds = static_cast<size_t>(22 + rows) * a->z * sizeof(uint32_t);
-        LaunchPair<false>(rows, threads, fast_lds, s, in, nullptr, bits, lo, io, fd, airan::CrcFuse{}, nullptr);
+        LaunchPair<false>(rows, threads, fast_lds, s, in, nullptr, bits, lo, io, fd, nullptr);
         return airan::LaunchStatus();
     }
     const bool reg = a->active_rows <= static_cast<uint32_t>(kRegRows);
@@ -854,7 +821,7 @@ namespace airan {
 
 int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const RateMatchDescriptor *desc,
                                const int16_t *cw, uint32_t num_cb, uint32_t rows, uint32_t iters, bool early_stop,
-                               int8_t *bits, uint8_t *iters_out, const CrcFuse *crc, uint32_t *queue,
+                               int8_t *bits, uint8_t *iters_out, uint32_t *queue,
                                hipStream_t s)
 {
     if (num_cb == 0 || !FastEligible(AIRAN_LDPC_MODE_FP16, p.Z, rows) || rows < 4 || iters == 0 || iters > 100 ||
@@ -866,8 +833,7 @@ int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const Rate
     LdpcFastDev fd{p.Z, LiftIndex(p.Z), p.K, iters, rows, early_stop ? 1u : 0u, num_cb,
                    {g.qm, g.cw_symbols, g.cw_stride, p.Z, p.Kprime, p.filler, p.Ncb, num_cb}};
     const size_t lds = static_cast<size_t>(22 + rows) * p.Z * sizeof(uint32_t);
-    LaunchPair<true>(rows, AlignUp(p.Z, 64), lds, s, cw, desc, bits, nullptr, iters_out, fd,
-                     crc != nullptr ? *crc : CrcFuse{}, queue);
+    LaunchPair<true>(rows, AlignUp(p.Z, 64), lds, s, cw, desc, bits, nullptr, iters_out, fd, queue);
     return LaunchStatus();
 }
 

@@ -148,3 +148,45 @@ def test_native_chain_matches_op_chain(gpu_stream, prof, snr):
             assert np.array_equal(tb_bits(plan, torch.from_numpy(part("bits", plan.C * plan.K))), payload[s])
     nat.close()
     op.close()
+
+
+CRC_CASES = [
+    (P.SISO_52PRB, (6.0, 12.0, 16.0, 20.0, 40.0)),
+    (P.FULL_CARRIER_4RX, (24.0, 30.0, 33.0, 36.0, 46.0)),
+]
+
+
+@pytest.mark.parametrize("prof,snrs", CRC_CASES, ids=[c[0].name for c in CRC_CASES])
+def test_native_chain_crc_flags_match_crc_operator(gpu_stream, prof, snrs):
+    """The decoder's CRC epilogue (code-block CRC24B and the transport block
+    CRC from each block's hard bits) gives the same flags as the standalone
+    CRC operator on the same decoded bits, on slots at SNRs where some code
+    blocks fail and some pass."""
+    from puschrx.chain import NativeChain
+    cfg, g, plan = plan_for(prof)
+    rng = np.random.default_rng(41)
+    S = len(snrs)
+    spec = synth.ChannelSpec(taps=8, delay_spread_s=100e-9, doppler_hz=10.0, rx_corr=0.3)
+    payload = synth.random_payload(rng, S, plan.tbs)
+    grid = synth.tx_grid(cfg, plan, payload)
+    H = synth.channel(rng, cfg, spec, S)
+    iq = np.concatenate([synth.receive(rng, cfg, grid[s:s + 1], H[s:s + 1], snr, iq_format=abi.IQ_COMPLEX64)
+                         for s, snr in enumerate(snrs)])
+    opts = ChainOptions(tau_s=spec.delay_spread_s, fd_hz=spec.doppler_hz, iq_format=abi.IQ_COMPLEX64)
+    nat = NativeChain(cfg, plan, max_slots=S, opts=opts, use_graph=False)
+    print(f"{prof.name} snrs {snrs}", flush=True)
+    nat.enqueue(torch.from_numpy(iq).cuda(), S)
+    torch.cuda.synchronize()
+    bits = nat.fetch("bits", torch.int8, S * plan.C * plan.K)
+    cb_nat = nat.fetch("cb_ok", torch.uint8, S * plan.C).cpu().numpy()
+    tb_nat = nat.fetch("tb_ok", torch.uint8, S).cpu().numpy()
+    cb_op, tb_op, _ = ops.crc_check(plan, S, bits)
+    torch.cuda.synchronize()
+    cb_op = cb_op.cpu().numpy().reshape(-1)
+    tb_op = tb_op.cpu().numpy().reshape(-1)
+    assert np.array_equal(cb_nat, cb_op)
+    assert np.array_equal(tb_nat, tb_op)
+    # Both outcomes occur, so the test sees failing and passing checks.
+    assert 0 < int(cb_nat.sum()) < cb_nat.size
+    assert 0 < int(tb_nat.sum()) < S
+    nat.close()
