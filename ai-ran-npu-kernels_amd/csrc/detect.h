@@ -32,7 +32,7 @@ __device__ inline void HermSolve(Cf (&A)[4][4], Cf (&m)[4], Cf (&x)[4], float (&
 {
 #pragma clang fp contract(on)
     Cf Lm[4][4];
-    float D[4];
+    float D[4], Dinv[4];  // pivots and their reciprocals (one division each)
 #pragma unroll
     for (int j = 0; j < NL; ++j) {
         float dj = A[j][j].r;
@@ -40,6 +40,7 @@ __device__ inline void HermSolve(Cf (&A)[4][4], Cf (&m)[4], Cf (&x)[4], float (&
         for (int k = 0; k < j; ++k) dj -= D[k] * (Lm[j][k].r * Lm[j][k].r + Lm[j][k].i * Lm[j][k].i);
         D[j] = dj;
         const float inv = 1.0f / dj;
+        Dinv[j] = inv;
 #pragma unroll
         for (int i = j + 1; i < NL; ++i) {
             Cf s = A[i][j];
@@ -64,7 +65,7 @@ __device__ inline void HermSolve(Cf (&A)[4][4], Cf (&m)[4], Cf (&x)[4], float (&
     // diagonal: w = D^-1 z, backward L^H x = w
 #pragma unroll
     for (int i = NL - 1; i >= 0; --i) {
-        Cf s = {z[i].r / D[i], z[i].i / D[i]};
+        Cf s = {z[i].r * Dinv[i], z[i].i * Dinv[i]};
 #pragma unroll
         for (int k = i + 1; k < NL; ++k) s = csub(s, cmulc(Lm[k][i], x[k]));
         x[i] = s;
@@ -85,7 +86,7 @@ __device__ inline void HermSolve(Cf (&A)[4][4], Cf (&m)[4], Cf (&x)[4], float (&
             for (int k = 0; k < i; ++k)
                 if (k >= c) s = csub(s, cmul(Lm[i][k], col[k]));
             col[i] = s;
-            acc += (s.r * s.r + s.i * s.i) / D[i];
+            acc += (s.r * s.r + s.i * s.i) * Dinv[i];
         }
         dinv[c] = acc;
     }
