@@ -219,11 +219,11 @@ struct AiranPuschChain {
 
 namespace {
 
-// The CRC check's masks: the part of crc_ws after the remainder parts
+// The CRC check's tables: the part of crc_ws after the remainder parts
 // (airan_crc_check_WorkspaceBytes lays it out so).
-uint32_t *CrcMasks(AiranPuschChain *c)
+void *CrcTables(AiranPuschChain *c)
 {
-    return reinterpret_cast<uint32_t *>(c->crc_ws + airan::CrcMaskOffset(c->opts.max_slots));
+    return c->crc_ws + airan::CrcTableOffset(c->opts.max_slots);
 }
 
 void DropGraph(AiranPuschChain *c)
@@ -308,7 +308,7 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
         if ((st = airan::CrcJobSetup(c->plan, c->cb_ok, reinterpret_cast<uint32_t *>(c->crc_ws), &crc)) != AIRAN_OK)
             return st;
         crc.xpow = c->crc_xpow;
-        airan::LaunchCbCrc(c->bits, crc, c->plan.K, ncb, CrcMasks(c), s);
+        airan::LaunchCbCrc(c->bits, crc, c->plan.K, ncb, CrcTables(c), s);
         airan::LaunchCrcFinish(reinterpret_cast<const uint32_t *>(c->crc_ws), c->tb_ok, slots, c->plan.C, s);
         if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
         mark();
@@ -426,10 +426,10 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
             return fail(AIRAN_LAUNCH_FAILED);
         // The decoder's block queue starts (and stays) zero.
         if (hipMemset(c->ldpc_queue, 0, 2 * sizeof(uint32_t)) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
-        // The CRC check's masks, after the TB remainder parts in crc_ws.
+        // The CRC check's tables, after the TB remainder parts in crc_ws.
         airan::CrcJob job{};
         if ((st = airan::CrcJobSetup(p, nullptr, nullptr, &job)) != AIRAN_OK) return fail(st);
-        airan::LaunchCrcMasks(job, CrcMasks(c), nullptr);
+        airan::LaunchCrcTables(job, p.K, CrcTables(c), nullptr);
         if (hipDeviceSynchronize() != hipSuccess || airan::LaunchStatus() != AIRAN_OK)
             return fail(AIRAN_LAUNCH_FAILED);
     }

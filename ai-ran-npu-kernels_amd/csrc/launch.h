@@ -55,20 +55,22 @@ int32_t CrcJobSetup(const AiranTbPlanV1 &plan, uint8_t *cb_ok, uint32_t *tb_part
 uint32_t CrcXpowHost(const AiranTbPlanV1 &plan, uint32_t r);
 // Zeroes n device words.
 void LaunchCrcZero(uint32_t *words, uint32_t n, hipStream_t s);
-// The check's per-plan masks (kCrcMaskBytes of device memory, built by
-// LaunchCrcMasks before LaunchCbCrc reads them).
-constexpr size_t kCrcMaskBytes = 2 * 24 * 384 * sizeof(uint32_t);
+// The check's per-plan tables (kCrcTableBytes of 16-byte aligned device
+// memory, built by LaunchCrcTables before LaunchCbCrc reads them): the 0/1
+// term matrix in MFMA fragment order, then the VALU path's masks.
+constexpr size_t kCrcFragBytes = 2 * (8448 / 32) * 64 * 16;
+constexpr size_t kCrcTableBytes = kCrcFragBytes + 2 * 24 * 384 * sizeof(uint32_t);
 // Code blocks per TB the CRC workspace is sized for (TS 38.214 TB sizes
 // need at most 152).
 constexpr uint32_t kCrcMaxCb = 256;
-// Byte offset of the masks in airan_crc_check_WorkspaceBytes(num_tbs): after
-// the num_tbs * kCrcMaxCb remainder parts.
-inline size_t CrcMaskOffset(uint32_t num_tbs)
+// Byte offset of the tables in airan_crc_check_WorkspaceBytes(num_tbs):
+// after the num_tbs * kCrcMaxCb remainder parts.
+inline size_t CrcTableOffset(uint32_t num_tbs)
 {
     return (static_cast<size_t>(num_tbs) * kCrcMaxCb * sizeof(uint32_t) + 255) / 256 * 256;
 }
-void LaunchCrcMasks(const CrcJob &c, uint32_t *masks, hipStream_t s);
-void LaunchCbCrc(const int8_t *bits, const CrcJob &c, uint32_t K, uint32_t num_cb, const uint32_t *masks,
+void LaunchCrcTables(const CrcJob &c, uint32_t K, void *tables, hipStream_t s);
+void LaunchCbCrc(const int8_t *bits, const CrcJob &c, uint32_t K, uint32_t num_cb, const void *tables,
                  hipStream_t s);
 void LaunchCrcFinish(const uint32_t *tb_part, uint8_t *tb_ok, uint32_t num_tbs, uint32_t C, hipStream_t s);
 

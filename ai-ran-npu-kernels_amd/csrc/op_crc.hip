@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) ZeroKernel(uint32_t *p, uint32_t n)
 constexpr uint32_t kCrcW = 24;           // bits per thread and block
 constexpr uint32_t kCrcThreads = 384;     // 24 * 384 >= 8448, the largest K
 constexpr uint32_t kCrcMaxLen = 24;
-static_assert(airan::kCrcMaskBytes == 2 * kCrcMaxLen * kCrcThreads * sizeof(uint32_t), "mask table size");
+static_assert(airan::kCrcTableBytes >= 2 * kCrcMaxLen * kCrcThreads * sizeof(uint32_t), "mask table size");
 
 // The four 0/1 bytes of w as 4 bits, the first byte highest.
 __device__ inline uint32_t Pack4(uint32_t w)
@@ -174,21 +174,23 @@ __global__ void __launch_bounds__(kCrcThreads) CbCrcKernel(const int8_t *__restr
     for (uint32_t base = blockIdx.x * kCrcBatch; base < num_cb; base += gridDim.x * kCrcBatch) {
         uint32_t v[kCrcBatch];
         if (vec) {
+            // Unconditional loads (clamped block, own slice or the block's
+            // first), masked at use, so the batch is in flight together.
             uint2 q[kCrcBatch][3];
 #pragma unroll
             for (uint32_t u = 0; u < kCrcBatch; ++u) {
-                q[u][0] = q[u][1] = q[u][2] = make_uint2(0, 0);
-                if (any && base + u < num_cb) {
-                    const uint2 *p = reinterpret_cast<const uint2 *>(bits + static_cast<size_t>(base + u) * K + w0);
-                    q[u][0] = p[0];
-                    q[u][1] = p[1];
-                    q[u][2] = p[2];
-                }
+                const uint32_t cbu = min(base + u, num_cb - 1);
+                const uint2 *p = reinterpret_cast<const uint2 *>(bits + static_cast<size_t>(cbu) * K + (any ? w0 : 0u));
+                q[u][0] = p[0];
+                q[u][1] = p[1];
+                q[u][2] = p[2];
             }
 #pragma unroll
-            for (uint32_t u = 0; u < kCrcBatch; ++u)
+            for (uint32_t u = 0; u < kCrcBatch; ++u) {
                 v[u] = Pack4(q[u][0].x) << 20 | Pack4(q[u][0].y) << 16 | Pack4(q[u][1].x) << 12 |
                        Pack4(q[u][1].y) << 8 | Pack4(q[u][2].x) << 4 | Pack4(q[u][2].y);
+                if (!any || base + u >= num_cb) v[u] = 0;
+            }
         } else {
 #pragma unroll
             for (uint32_t u = 0; u < kCrcBatch; ++u) {
@@ -229,6 +231,106 @@ __global__ void __launch_bounds__(kCrcThreads) CbCrcKernel(const int8_t *__restr
             airan::CrcFinishBlock(c, base + t, xb, xa);
         }
         __syncthreads();
+    }
+}
+
+// The same products on the matrix cores when K is a multiple of 32: the
+// remainders of all blocks are R = M B over the integers, then mod 2, with
+// M [64][K] the 0/1 term bits (rows 0..23 CRC24B, rows 32.. the TB CRC) and
+// B [K][blocks] the 0/1 decoded bytes as they lie in memory (block-major).
+// One v_mfma_i32_32x32x16_i8 step takes 16 positions; positions are taken
+// in pairs of steps, lane l (half h = l / 32) supplying bytes
+// [32 p + 16 h, 32 p + 16 h + 16) of its row (A) or block (B): the first 8
+// for step 2p, the next 8 for step 2p+1 (any order of the positions serves,
+// A and B agreeing).  The fragment table holds M in that order,
+// frag[(tile P + p) 64 + l], so a wave loads 1 KB contiguous per tile and
+// pair.
+typedef int i32x16_t __attribute__((ext_vector_type(16)));
+constexpr uint32_t kCrcCols = 32;   // blocks per workgroup (MFMA columns)
+constexpr uint32_t kCrcWaves = 8;   // waves splitting the positions
+constexpr uint32_t kCrcUnroll = 11; // position pairs of a wave in flight together
+
+__global__ void __launch_bounds__(256) CrcFragKernel(airan::CrcJob c, uint32_t P, uint4 *__restrict__ frag)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * P * 64) return;
+    const uint32_t l = e % 64, p = (e / 64) % P, tile = e / (64 * P);
+    const uint32_t row = l % 32, pos0 = 32 * p + 16 * (l / 32);
+    const uint32_t *pw = tile ? c.pow_tb : c.pow_b;
+    const uint32_t n = tile ? c.L : (c.crc_cb ? c.kprime : 0u);
+    const uint32_t len = tile ? c.len_tb : 24u;
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (row < len) {
+        for (uint32_t j = 0; j < 16; ++j) {
+            const uint32_t pos = pos0 + j;
+            if (pos < n) w[j / 4] |= ((pw[n - 1 - pos] >> row) & 1u) << (8 * (j % 4));
+        }
+    }
+    frag[e] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__device__ inline long Lo8(uint4 v) { return static_cast<long>(static_cast<uint64_t>(v.y) << 32 | v.x); }
+__device__ inline long Hi8(uint4 v) { return static_cast<long>(static_cast<uint64_t>(v.w) << 32 | v.z); }
+
+// bits: int8 0/1, [cb][K], K a multiple of 32 and bits 16-byte aligned.
+// Workgroup b checks blocks 32 b .. 32 b + 31; wave w takes a contiguous
+// eighth of the position pairs (consecutive loads of a lane share cache
+// lines); the parities of the waves' partial sums are XOR-combined.
+__global__ void __launch_bounds__(kCrcWaves * 64) CrcMfmaKernel(const int8_t *__restrict__ bits, airan::CrcJob c,
+                                                               uint32_t K, uint32_t num_cb,
+                                                               const uint4 *__restrict__ frag)
+{
+    const uint32_t l = threadIdx.x & 63u, w = threadIdx.x >> 6, h = l >> 5, col = l & 31u;
+    const uint32_t P = K / 32;
+    const uint32_t cb = blockIdx.x * kCrcCols + col;
+    const bool live = cb < num_cb;
+    const uint4 *src = reinterpret_cast<const uint4 *>(bits + static_cast<size_t>(live ? cb : 0u) * K) + h;
+    i32x16_t acc_b = {}, acc_a = {};
+    const uint32_t span = (P + kCrcWaves - 1) / kCrcWaves, pe = min(P, (w + 1) * span);
+    for (uint32_t p0 = w * span; p0 < pe; p0 += kCrcUnroll) {
+        // Every load is unconditional (clamped to a valid pair, masked at
+        // use): a guarded load would make the compiler wait for it before
+        // the next one.
+        uint4 b[kCrcUnroll], a0[kCrcUnroll], a1[kCrcUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kCrcUnroll; ++u) {
+            const uint32_t p = min(p0 + u, pe - 1);
+            b[u] = src[2 * p];
+            a0[u] = frag[p * 64 + l];
+            a1[u] = frag[(P + p) * 64 + l];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kCrcUnroll; ++u) {
+            if (!live || p0 + u >= pe) b[u] = make_uint4(0, 0, 0, 0);
+            acc_b = __builtin_amdgcn_mfma_i32_32x32x16_i8(Lo8(a0[u]), Lo8(b[u]), acc_b, 0, 0, 0);
+            acc_b = __builtin_amdgcn_mfma_i32_32x32x16_i8(Hi8(a0[u]), Hi8(b[u]), acc_b, 0, 0, 0);
+            acc_a = __builtin_amdgcn_mfma_i32_32x32x16_i8(Lo8(a1[u]), Lo8(b[u]), acc_a, 0, 0, 0);
+            acc_a = __builtin_amdgcn_mfma_i32_32x32x16_i8(Hi8(a1[u]), Hi8(b[u]), acc_a, 0, 0, 0);
+        }
+    }
+    // Accumulator r of lane l is row 8 (r / 4) + 4 h + r % 4 of column col.
+    uint32_t pb = 0, pa = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < 16; ++r) {
+        const uint32_t row = 8 * (r / 4) + 4 * h + r % 4;
+        pb |= (static_cast<uint32_t>(acc_b[r]) & 1u) << row;
+        pa |= (static_cast<uint32_t>(acc_a[r]) & 1u) << row;
+    }
+    pb |= __shfl_xor(pb, 32);
+    pa |= __shfl_xor(pa, 32);
+    __shared__ uint32_t red[kCrcWaves][2][kCrcCols];
+    if (h == 0) {
+        red[w][0][col] = pb;
+        red[w][1][col] = pa;
+    }
+    __syncthreads();
+    if (threadIdx.x < kCrcCols && live) {
+        uint32_t xb = 0, xa = 0;
+        for (uint32_t v = 0; v < kCrcWaves; ++v) {
+            xb ^= red[v][0][col];
+            xa ^= red[v][1][col];
+        }
+        airan::CrcFinishBlock(c, cb, xb, xa);
     }
 }
 
@@ -315,14 +417,35 @@ void LaunchCrcZero(uint32_t *words, uint32_t n, hipStream_t s)
     hipLaunchKernelGGL(ZeroKernel, dim3(CeilDiv(n, 256)), dim3(256), 0, s, words, n);
 }
 
-void LaunchCrcMasks(const CrcJob &c, uint32_t *masks, hipStream_t s)
+// The matrix-core path needs whole position pairs and aligned 16-byte loads.
+bool CrcMfmaEligible(const void *bits, uint32_t K, const void *tables)
 {
-    hipLaunchKernelGGL(CrcMaskKernel, dim3(1), dim3(kCrcThreads), 0, s, c, masks);
+    return K % 32 == 0 && ((reinterpret_cast<uintptr_t>(bits) | reinterpret_cast<uintptr_t>(tables)) & 15u) == 0;
 }
 
-void LaunchCbCrc(const int8_t *bits, const CrcJob &c, uint32_t K, uint32_t num_cb, const uint32_t *masks,
+void LaunchCrcTables(const CrcJob &c, uint32_t K, void *tables, hipStream_t s)
+{
+    if (K % 32 == 0) {
+        const uint32_t P = K / 32;
+        static_assert(kCrcTableBytes >= 2 * (8448 / 32) * 64 * sizeof(uint4), "fragment table size");
+        hipLaunchKernelGGL(CrcFragKernel, dim3(CeilDiv(2 * P * 64, 256)), dim3(256), 0, s, c, P,
+                           static_cast<uint4 *>(tables));
+    }
+    // The masks of the VALU path follow the fragment table (used when the
+    // bits are not 16-byte aligned).
+    hipLaunchKernelGGL(CrcMaskKernel, dim3(1), dim3(kCrcThreads), 0, s, c,
+                       reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(tables) + kCrcFragBytes));
+}
+
+void LaunchCbCrc(const int8_t *bits, const CrcJob &c, uint32_t K, uint32_t num_cb, const void *tables,
                  hipStream_t s)
 {
+    if (CrcMfmaEligible(bits, K, tables)) {
+        hipLaunchKernelGGL(CrcMfmaKernel, dim3(CeilDiv(num_cb, kCrcCols)), dim3(kCrcWaves * 64), 0, s, bits, c, K,
+                           num_cb, static_cast<const uint4 *>(tables));
+        return;
+    }
+    const uint32_t *masks = reinterpret_cast<const uint32_t *>(static_cast<const uint8_t *>(tables) + kCrcFragBytes);
     // One resident wave of workgroups (the masks are built once per workgroup).
     uint32_t cap = 1024;
     int dev = 0, cus = 0, per_cu = 0;
@@ -346,7 +469,7 @@ void LaunchCrcFinish(const uint32_t *tb_part, uint8_t *tb_ok, uint32_t num_tbs, 
 extern "C" size_t airan_crc_check_WorkspaceBytes(uint32_t num_tbs)
 {
     // Per-block TB remainder parts, then the check masks.
-    return airan::CrcMaskOffset(num_tbs) + airan::kCrcMaskBytes;
+    return airan::CrcTableOffset(num_tbs) + airan::kCrcTableBytes;
 }
 
 extern "C" int32_t airan_crc_check_Enqueue(const AiranTbPlanV1 *p, uint32_t num_tbs, const void *bits,
@@ -369,9 +492,9 @@ extern "C" int32_t airan_crc_check_Enqueue(const AiranTbPlanV1 *p, uint32_t num_
                      static_cast<uint32_t *>(workspace), nullptr, p->C, p->Kprime, p->crc_cb,
                      p->Kprime - p->crc_cb, d.poly_tb, d.len_tb};
     hipStream_t s = static_cast<hipStream_t>(stream);
-    uint32_t *masks = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(workspace) + airan::CrcMaskOffset(num_tbs));
-    airan::LaunchCrcMasks(c, masks, s);
-    airan::LaunchCbCrc(static_cast<const int8_t *>(bits), c, p->K, num_tbs * p->C, masks, s);
+    void *tables = static_cast<uint8_t *>(workspace) + airan::CrcTableOffset(num_tbs);
+    airan::LaunchCrcTables(c, p->K, tables, s);
+    airan::LaunchCbCrc(static_cast<const int8_t *>(bits), c, p->K, num_tbs * p->C, tables, s);
     airan::LaunchCrcFinish(static_cast<const uint32_t *>(workspace), static_cast<uint8_t *>(tb_ok), num_tbs, p->C, s);
     if (tb_bytes != nullptr) {
         dim3 grid(airan::CeilDiv((p->tbs + 7) / 8, 256), num_tbs);
