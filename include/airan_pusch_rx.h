@@ -609,7 +609,9 @@ uint32_t airan_ldpc_ActiveRows(const AiranTbPlanV1 *plan);
  * reference receiver has no CRC (fec/ldpc_decode/operator.yaml:49).
  * bits: int8 [num_tbs*C, K]; cb_ok uint8 [num_tbs*C] (1 when C == 1);
  * tb_ok uint8 [num_tbs]; tb_bytes (optional) uint8 packed payload bytes
- * [num_tbs, ceil(tbs/8)], MSB first; workspace >= WorkspaceBytes(num_tbs). */
+ * [num_tbs, ceil(tbs/8)], MSB first; workspace >= WorkspaceBytes(num_tbs),
+ * 16-byte aligned for the matrix-core path.  plan->C <= 256 (every TS 38.214
+ * TB size needs at most 152 code blocks). */
 size_t airan_crc_check_WorkspaceBytes(uint32_t num_tbs);
 int32_t airan_crc_check_Enqueue(const AiranTbPlanV1 *plan, uint32_t num_tbs, const void *bits,
                                 void *cb_ok, void *tb_ok, void *tb_bytes, void *workspace,
