@@ -440,24 +440,32 @@ __device__ __forceinline__ void PairIteration(uint32_t *lam2, uint32_t zi, uint3
     }
 }
 
-// Bit 0: A's check i fails a row's parity; bit 16: B's.
+// Bit 0: A's check i fails a row's parity; bit 16: B's.  Rows are taken
+// three at a time, their loads in flight together (all six at once would
+// spill).
 template <int R, int ROWS, uint32_t ZT>
 __device__ __forceinline__ uint32_t PairSyndrome(uint32_t *lam2, const uint32_t *lw, uint32_t i, uint32_t Z)
 {
+    constexpr int G = 3;
     if constexpr (R < ROWS) {
-        constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
+        constexpr int RE = R + G < ROWS ? R + G : ROWS;
+        constexpr int E0 = FastRowStart(R), N = FastRowStart(RE) - E0;
         uint32_t i4 = 4u * i;
         asm volatile("" : "+v"(i4));
-        uint32_t v[DEG];
+        uint32_t v[N];
 #pragma unroll
-        for (int k = 0; k < DEG; ++k) v[k] = LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, Z));
+        for (int k = 0; k < N; ++k) v[k] = LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, Z));
         __builtin_amdgcn_sched_barrier(0);
         uint32_t par = 0;
 #pragma unroll
-        for (int k = 0; k < DEG; ++k) par ^= NegBit2(v[k]);
-        // One row's loads at a time (keeps the register count down).
+        for (int r = R; r < RE; ++r) {
+            uint32_t pr = 0;
+#pragma unroll
+            for (int k = FastRowStart(r) - E0; k < FastRowStart(r + 1) - E0; ++k) pr ^= NegBit2(v[k]);
+            par |= pr;
+        }
         asm volatile("" : "+v"(par) : : "memory");
-        return par | PairSyndrome<R + 1, ROWS, ZT>(lam2, lw, i, Z);
+        return par | PairSyndrome<RE, ROWS, ZT>(lam2, lw, i, Z);
     }
     return 0;
 }
@@ -636,6 +644,7 @@ __global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
                    LdpcFastDev d, uint32_t *__restrict__ queue)
 {
     __shared__ uint32_t taken_cb;
+    __shared__ uint32_t syn_bad[8];  // per wave: bit h set when half h fails a check
     uint32_t *lam2 = reinterpret_cast<uint32_t *>(airan_ldpc_lam);
     const uint32_t Z = d.z, i = threadIdx.x;
     const uint32_t nl = (22u + ROWS) * Z;
@@ -684,8 +693,15 @@ __global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
         ++it1;
         uint32_t ok = 0;  // bit h: half h's syndrome is clear
         if (d.early_stop) {
+            // Both halves' flags in one barrier: each wave's ballots into
+            // syn_bad[wave] (rewritten only after further barriers).
             const uint32_t bad = i < Z ? PairSyndrome<0, ROWS, ZT>(lam2, lw, i, Z) : 0u;
-            ok = (__syncthreads_or((bad & 1u) != 0) ? 0u : 1u) | (__syncthreads_or((bad >> 16) != 0) ? 0u : 2u);
+            const uint32_t wb = (__ballot((bad & 1u) != 0) != 0 ? 1u : 0u) | (__ballot((bad >> 16) != 0) != 0 ? 2u : 0u);
+            if ((i & 63u) == 0) syn_bad[i >> 6] = wb;
+            __syncthreads();
+            uint32_t any = 0;
+            for (uint32_t w = 0; w < blockDim.x / 64u; ++w) any |= syn_bad[w];
+            ok = ~any & 3u;
         }
         refill = 0;
         if (cur0 < d.num_cb && ((ok & 1u) || it0 >= d.iters)) refill |= 1u;
