@@ -421,30 +421,13 @@ __global__ void __launch_bounds__(256, 2) CeProjectMfmaKernel(CeWs w, uint32_t s
     }
 }
 
-// t = ((tp[0] + tp[1]) + tp[2]) + tp[3] ... in split order.  Grid over
-// slots * models * NR * D * kMaxRank elements (entries k >= rank skipped).
-__global__ void __launch_bounds__(256) CeProjReduceKernel(CeWs w, uint32_t slots, CeApply c)
-{
-    const uint32_t ncol = c.nr * c.ndmrs;
-    const size_t total = static_cast<size_t>(slots) * c.nm * kMaxRank * ncol;
-    const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= total) return;
-    const uint32_t k = static_cast<uint32_t>(e % kMaxRank);
-    const uint32_t q = static_cast<uint32_t>((e / (static_cast<size_t>(ncol) * kMaxRank)) % c.nm);
-    if (k >= c.m[q].rank) return;
-    float2 acc = w.tp[e];
-    for (uint32_t part = 1; part < kProjSplits; ++part) {
-        const float2 v = w.tp[static_cast<size_t>(part) * total + e];
-        acc.x += v.x;
-        acc.y += v.y;
-    }
-    w.t[e] = acc;
-}
-
 // Column energies: |z|^2 over the observations and |u|^2 over the kept
-// eigenvectors, fp64 with a fixed reduction tree.  Grid slots * models * NR*D.
+// eigenvectors, fp64 with a fixed reduction tree.  The projection's split-K
+// partials are summed here first, t = ((tp[0] + tp[1]) + tp[2]) + tp[3] ...
+// in split order, and t is stored for the kernels after.  Grid
+// slots * models * NR*D.
 __global__ void __launch_bounds__(256) CeEnergyKernel(const __half *__restrict__ hr, const __half *__restrict__ hi,
-                                                      CeWs w, CeApply c)
+                                                      CeWs w, CeApply c, uint32_t slots)
 {
     __shared__ double red[2][4];
     const uint32_t ncol = c.nr * c.ndmrs;
@@ -457,9 +440,16 @@ __global__ void __launch_bounds__(256) CeEnergyKernel(const __half *__restrict__
         const float2 v = LoadZ(hr, hi, c, md, slot, col, p);
         ez += static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
     }
-    const float2 *tc = w.t + (static_cast<size_t>(sq) * ncol + col) * kMaxRank;
+    const size_t total = static_cast<size_t>(slots) * c.nm * kMaxRank * ncol;
+    const size_t e0 = (static_cast<size_t>(sq) * ncol + col) * kMaxRank;
     for (uint32_t k = threadIdx.x; k < md.rank; k += blockDim.x) {
-        const float2 u = tc[k];
+        float2 u = w.tp[e0 + k];
+        for (uint32_t part = 1; part < kProjSplits; ++part) {
+            const float2 v = w.tp[static_cast<size_t>(part) * total + e0 + k];
+            u.x += v.x;
+            u.y += v.y;
+        }
+        w.t[e0 + k] = u;
         eu += static_cast<double>(u.x) * u.x + static_cast<double>(u.y) * u.y;
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -492,10 +482,20 @@ __global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ 
                                                      __half *__restrict__ noise_out, CeWs w, CeApply c)
 {
     __shared__ float nq[kMaxModels];
+    // The slot's energies staged in LDS by all threads, so thread 0's serial
+    // sums read LDS instead of waiting on one global load at a time.
+    constexpr uint32_t kStage = 2 * kMaxModels * 32;
+    __shared__ double stage[kStage];
     const uint32_t slot = blockIdx.x;
     const uint32_t ncol = c.nr * c.ndmrs;
+    const uint32_t n_e = 2 * c.nm * ncol;
+    const double *eg = w.energy + 2 * static_cast<size_t>(slot) * c.nm * ncol;
+    if (n_e <= kStage) {
+        for (uint32_t t = threadIdx.x; t < n_e; t += blockDim.x) stage[t] = eg[t];
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        const double *e = w.energy + 2 * static_cast<size_t>(slot) * c.nm * ncol;
+        const double *e = n_e <= kStage ? stage : eg;
         double dims = 0.0;
         for (uint32_t q = 0; q < c.nm; ++q) dims += static_cast<double>(c.m[q].n - c.m[q].kept) * c.ndmrs;
         double s2_sum = 0.0;
@@ -1057,9 +1057,7 @@ int32_t LaunchCeApply(const LmmseFilter *f, const __half *hr, const __half *hi, 
     const uint32_t proj_per_xcd = CeilDiv(f->nm * kProjSplits * (w.npad / 64) * k_tiles, 8);
     hipLaunchKernelGGL(CeProjectMfmaKernel, dim3(8 * proj_per_xcd), dim3(256), 0, s, w, slots, c, k_tiles,
                        proj_per_xcd);
-    const uint64_t t_elems = static_cast<uint64_t>(slots) * f->nm * kMaxRank * ncol;
-    hipLaunchKernelGGL(CeProjReduceKernel, dim3(CeilDiv(t_elems, 256)), dim3(256), 0, s, w, slots, c);
-    hipLaunchKernelGGL(CeEnergyKernel, dim3(slots * f->nm * ncol), dim3(256), 0, s, hr, hi, w, c);
+    hipLaunchKernelGGL(CeEnergyKernel, dim3(slots * f->nm * ncol), dim3(256), 0, s, hr, hi, w, c, slots);
     hipLaunchKernelGGL(CeStatsKernel, dim3(slots), dim3(256), 0, s, noise_in, noise_out, w, c);
     const uint32_t hv_slot_tiles = CeilDiv(slots, 32), hv_sc_tiles = CeilDiv(g.used, 32);
     hipLaunchKernelGGL(CeHvMfmaKernel, dim3(hv_slot_tiles * hv_sc_tiles * f->nm), dim3(256), 0, s, w, slots, c,
