@@ -163,8 +163,17 @@ __device__ inline void LdsPass(float *lre, float *lim, uint32_t j, uint32_t ns, 
     constexpr uint32_t N = 1u << LOGN;
     constexpr uint32_t TPS = N / 16;
     constexpr int B = 16 / R;
-    float2 v[B][R];
+    float2 v[B][R], tw[B][R];
     if (active) {
+        // Twiddle loads first: their latency overlaps the LDS reads and the
+        // barrier instead of following them.
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            const uint32_t k = (j + t * TPS) & (ns - 1);  // ns is a power of two
+            const uint32_t step = (kMaxN / R) / ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) tw[t][r] = airan::kTwiddle4096[k * r * step];
+        }
 #pragma unroll
         for (int t = 0; t < B; ++t) {
             const uint32_t jj = j + t * TPS;
@@ -180,10 +189,9 @@ __device__ inline void LdsPass(float *lre, float *lim, uint32_t j, uint32_t ns, 
 #pragma unroll
         for (int t = 0; t < B; ++t) {
             const uint32_t jj = j + t * TPS;
-            const uint32_t k = jj & (ns - 1);  // ns is a power of two
-            const uint32_t step = (kMaxN / R) / ns;
+            const uint32_t k = jj & (ns - 1);
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[t][r] = CMul(v[t][r], airan::kTwiddle4096[k * r * step]);
+            for (int r = 1; r < R; ++r) v[t][r] = CMul(v[t][r], tw[t][r]);
             Dft<R>(v[t]);
             const uint32_t base = (jj - k) * R + k;
 #pragma unroll
