@@ -394,7 +394,9 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
     asm volatile("" : "+v"(i4));
     uint32_t addr[DEG], ext[DEG];
     uint32_t m1 = 0x7c007c00u, m2 = 0x7c007c00u;
-    uint32_t sgl = 0, sgh = 0;  // sign bits of edges 0..15 and 16.. (A low, B high half)
+    // XOR of the extrinsics (bits 15 / 31: the sign parity of each half) and
+    // the largest magnitude per half (a NaN shows as one above 0x7c00).
+    uint32_t sx = 0, mx = 0;
     // Every gather of the row is issued before the first is consumed.
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
@@ -409,15 +411,21 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
         const uint32_t mg = x & 0x7fff7fffu;
         m2 = PkMinU16(m2, PkMaxU16(m1, mg));
         m1 = PkMinU16(m1, mg);
-        if (k < 16)
-            sgl |= AboveInf2(x) << k;
-        else
-            sgh |= AboveInf2(x) << (k - 16);
+        sx ^= x;
+        mx = PkMaxU16(mx, mg);
     }
     const h2_t q = {static_cast<_Float16>(0.75f), static_cast<_Float16>(0.75f)};
-    const uint32_t pa = (__popc(sgl & 0xffffu) + __popc(sgh & 0xffffu)) & 1u;
-    const uint32_t pb = (__popc(sgl >> 16) + __popc(sgh >> 16)) & 1u;
-    const uint32_t par = pa << 15 | pb << 31;
+    // The parity counts the edges whose bits exceed 0x7c00 (negative, or a
+    // NaN of either sign).  Without a NaN in the row that is the sign bit,
+    // so the parity is sx's; a row with a NaN (rare, and sticky once a
+    // posterior is NaN) counts the extrinsics again.
+    uint32_t par = sx & 0x80008000u;
+    if (PkSubSatU16(mx, 0x7c007c00u) != 0) {
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < DEG; ++k) cnt += AboveInf2(ext[k]);
+        par = (cnt & 1u) << 15 | ((cnt >> 16) & 1u) << 31;
+    }
     const uint32_t s1p = WH(q * H2(m1)) | par, s2p = WH(q * H2(m2)) | par;
     const uint32_t d12 = PkSubU16(s2p, s1p);
 #pragma unroll
