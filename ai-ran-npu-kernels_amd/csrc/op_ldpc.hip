@@ -392,22 +392,20 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
     // row's at once, in registers.
     uint32_t i4 = 4u * i;
     asm volatile("" : "+v"(i4));
-    uint32_t addr[DEG], ext[DEG];
     uint32_t m1 = 0x7c007c00u, m2 = 0x7c007c00u;
     // XOR of the extrinsics (bits 15 / 31: the sign parity of each half) and
     // the largest magnitude per half (a NaN shows as one above 0x7c00).
     uint32_t sx = 0, mx = 0;
-    // Every gather of the row is issued before the first is consumed.
+    // The row's extrinsics and addresses are not held: the message pass
+    // reads each posterior again (no other thread writes it within the row)
+    // and recomputes its address, and the loads go kChunk at a time, so that
+    // the kernel fits 128 VGPRs and a CU holds two workgroups (12 waves)
+    // whatever SIMDs their waves land on.
+    constexpr int kChunk = 5;
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
-        addr[k] = EdgeByte<ZT>(lw, E0 + k, i, i4, Z);
-        ext[k] = LdsWord(lam2, addr[k]);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 0; k < DEG; ++k) {
-        const uint32_t x = WH(H2(ext[k]) - H2(msg[E0 + k]));
-        ext[k] = x;
+        if (k % kChunk == 0 && k > 0) __builtin_amdgcn_sched_barrier(0);
+        const uint32_t x = WH(H2(LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, Z))) - H2(msg[E0 + k]));
         const uint32_t mg = x & 0x7fff7fffu;
         m2 = PkMinU16(m2, PkMaxU16(m1, mg));
         m1 = PkMinU16(m1, mg);
@@ -423,18 +421,22 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
     if (PkSubSatU16(mx, 0x7c007c00u) != 0) {
         uint32_t cnt = 0;
 #pragma unroll
-        for (int k = 0; k < DEG; ++k) cnt += AboveInf2(ext[k]);
+        for (int k = 0; k < DEG; ++k)
+            cnt += AboveInf2(WH(H2(LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, Z))) - H2(msg[E0 + k])));
         par = (cnt & 1u) << 15 | ((cnt >> 16) & 1u) << 31;
     }
     const uint32_t s1p = WH(q * H2(m1)) | par, s2p = WH(q * H2(m2)) | par;
     const uint32_t d12 = PkSubU16(s2p, s1p);
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
+        if (k % kChunk == 0 && k > 0) __builtin_amdgcn_sched_barrier(0);
+        const uint32_t a = EdgeByte<ZT>(lw, E0 + k, i, i4, Z);
+        const uint32_t x = WH(H2(LdsWord(lam2, a)) - H2(msg[E0 + k]));
         // 1 per half where |x_k| == min1, then s1p + that * (s2p - s1p).
-        const uint32_t e = PkSubSatU16(0x00010001u, (ext[k] & 0x7fff7fffu) ^ m1);
-        const uint32_t m = PkMadU16(e, d12, s1p) ^ (ext[k] & 0x80008000u);
+        const uint32_t e = PkSubSatU16(0x00010001u, (x & 0x7fff7fffu) ^ m1);
+        const uint32_t m = PkMadU16(e, d12, s1p) ^ (x & 0x80008000u);
         msg[E0 + k] = m;
-        LdsWord(lam2, addr[k]) = WH(H2(ext[k]) + H2(m));
+        LdsWord(lam2, a) = WH(H2(x) + H2(m));
     }
 }
 
@@ -449,12 +451,12 @@ __device__ __forceinline__ void PairIteration(uint32_t *lam2, uint32_t zi, uint3
 }
 
 // Bit 0: A's check i fails a row's parity; bit 16: B's.  Rows are taken
-// three at a time, their loads in flight together (all six at once would
-// spill).
+// one at a time within the 128-VGPR budget of up to 6 active rows, three at
+// a time above (their loads in flight together).
 template <int R, int ROWS, uint32_t ZT>
 __device__ __forceinline__ uint32_t PairSyndrome(uint32_t *lam2, const uint32_t *lw, uint32_t i, uint32_t Z)
 {
-    constexpr int G = 3;
+    constexpr int G = ROWS <= 6 ? 1 : 3;
     if constexpr (R < ROWS) {
         constexpr int RE = R + G < ROWS ? R + G : ROWS;
         constexpr int E0 = FastRowStart(R), N = FastRowStart(RE) - E0;
@@ -632,21 +634,24 @@ __device__ inline void PairEmit(const uint32_t *lam2, uint32_t h, uint32_t cb, u
 // (syndrome clear under early stop, or the iteration limit) its outputs are
 // written and the next block is loaded into that half (its messages reset)
 // while the other half runs on; a half past the last block holds a zero
-// block and is never written.  Workgroup b starts with blocks b and b + G;
-// after that it takes blocks 2 G, 2 G + 1, ... from the counter queue[0]
-// (two words, zero at launch and left zero) as halves free up, or, with
-// queue null, its static list
-// b + 2 G, b + 3 G, ...  Blocks are independent of the pairing
+// block and is never written.  The workgroups take blocks 0, 1, ... from
+// the counter queue[0] (two words, zero at launch and left zero) as halves
+// free up -- workgroups that start late hold no block meanwhile -- or, with
+// queue null, workgroup b walks its static list
+// b, b + G, b + 2 G, ...  Blocks are independent of the pairing
 // (the halves never interact), so every block's outputs equal a decode of
 // that block alone.  src: int16 decoder LLRs [cb][68 Z] (!FUSED) or
 // bit-plane codeword LLRs with per-block descriptors (FUSED).  LDS holds the
 // (22 + rows) Z posterior pairs the active rows touch.
 //
 // The refill code (emit, take, load) exists once, in a rolled loop over the
-// two halves that also does the first loads: the kernel is large, and each
-// inlined copy would be a separate cold instruction fetch on every CU.
+// two halves that also does the first loads.  Up to 6 active rows the kernel
+// is held to 4 waves per SIMD (128 VGPRs): a 6-wave workgroup's waves land
+// 2, 2, 1, 1 on the SIMDs, and only at 4 waves per SIMD does a second
+// workgroup always fit beside it (at 3 the hardware kept one per CU).
 template <bool FUSED, int ROWS, uint32_t ZT>
-__global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
+__global__ void __launch_bounds__(384)
+    __attribute__((amdgpu_waves_per_eu(ROWS <= 6 ? 4 : 1, ROWS <= 6 ? 4 : 8)))
     LdpcPairKernel(const int16_t *__restrict__ src, const RateMatchDescriptor *__restrict__ desc,
                    int8_t *__restrict__ bits, int16_t *__restrict__ lam_out, uint8_t *__restrict__ iters_out,
                    LdpcFastDev d, uint32_t *__restrict__ queue)
@@ -673,10 +678,10 @@ __global__ void __launch_bounds__(384, ROWS <= 6 ? 3 : 2)
                 const uint32_t cb = h ? cur1 : cur0;
                 if (cb < d.num_cb) PairEmit<FUSED>(lam2, h, cb, h ? it1 : it0, nl, src, desc, bits, lam_out, iters_out, d);
                 uint32_t next;
-                if (queue == nullptr || taken < 2) {
+                if (queue == nullptr) {
                     next = blockIdx.x + taken * gridDim.x;
                 } else {
-                    if (i == 0) taken_cb = 2u * gridDim.x + atomicAdd(queue, 1u);
+                    if (i == 0) taken_cb = atomicAdd(queue, 1u);
                     __syncthreads();
                     next = __builtin_amdgcn_readfirstlane(taken_cb);
                 }
