@@ -399,9 +399,9 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
     // The row's extrinsics and addresses are not held: the message pass
     // reads each posterior again (no other thread writes it within the row)
     // and recomputes its address, and the loads go kChunk at a time, so that
-    // the kernel fits 128 VGPRs and a CU holds two workgroups (12 waves)
-    // whatever SIMDs their waves land on.
-    constexpr int kChunk = 5;
+    // the kernel fits 128 VGPRs without spills and a CU holds two workgroups
+    // (12 waves) whatever SIMDs their waves land on.
+    constexpr int kChunk = 6;
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
         if (k % kChunk == 0 && k > 0) __builtin_amdgcn_sched_barrier(0);
