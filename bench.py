@@ -8,7 +8,12 @@ slot.  Synthetic slots from the library's transmitter through a 16-tap TDL
 channel (300 ns RMS, exponential PDP, Jakes 100 Hz, Kronecker Rx
 correlation 0.5) with complex64 IQ; a step is one pass of the chain over a
 batch of slots (IQ already in HBM -> decoded bits and CRC outcomes in HBM),
-replayed from a HIP graph.
+replayed from a HIP graph.  As a receiver pipelines slots, two batches are in
+flight by default (--inflight): two chain instances, each with its own
+workspace and stream, take the steps in turn, so that one batch's decoder
+tail overlaps the next batch's front end.  The first pipelined batch must
+decode to the same bits as the single-stream run, whose time is reported
+beside the throughput.
 
 The operating SNR is the lowest point of a 20..46 dB sweep (step 2) at which
 the chain's BLER is <= 10 %, measured first on a smaller batch (4 Rx carrying
@@ -80,6 +85,9 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=64, help="slots per step")
     ap.add_argument("--snr", type=float, default=None, help="skip the sweep and use this SNR (dB)")
     ap.add_argument("--sweep-slots", type=int, default=16)
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="slot batches in flight: chain instances (own workspace, own stream) that take the "
+                         "steps in turn, so one batch's decoder tail overlaps the next batch's front end")
     ap.add_argument("--cpu-slots", type=int, default=32, help="CPU baseline sample size (slots)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -195,7 +203,7 @@ def main() -> int:
             mism = sum(int(not np.array_equal(b[s, :, :per_cb].reshape(-1)[: plan.tbs], payload[s]))
                        for s in range(S))
             r = {"snr_db": snr_op, "ms_per_step": ms, "tb_ok": tb_ok, "mean_iters": float(iters.float().mean()),
-                 "wall": wall, "mismatches": mism, "iq_dev": iq_dev, "iq": iq}
+                 "wall": wall, "mismatches": mism, "iq_dev": iq_dev, "iq": iq, "bits": b, "payload": payload}
             log(f"{label}: {args.steps} steps at {snr_op} dB: {ms:.3f} ms/step, wall {wall:.3f} s, "
                 f"tb_ok {int(tb_ok.sum())}/{S}, payload mismatches {mism}, mean LDPC iterations "
                 f"{r['mean_iters']:.2f}")
@@ -211,6 +219,61 @@ def main() -> int:
         plain = measure(snr_op, "lmmse")
         ms_per_step, tb_ok, mean_iters = plain["ms_per_step"], plain["tb_ok"], plain["mean_iters"]
         iq_dev, iq = plain["iq_dev"], plain["iq"]
+
+        # Throughput with `inflight` batches in flight: chain j (its own
+        # workspace and stream) takes steps j, j + n, ...; batch 0 is the
+        # input just measured and must decode to the same bits.
+        pipe = None
+        if args.inflight > 1:
+            n_in = args.inflight
+            streams = [stream] + [torch.cuda.Stream() for _ in range(n_in - 1)]
+            chains = [chain]
+            for j in range(1, n_in):
+                with torch.cuda.stream(streams[j]):
+                    chains.append(NativeChain(cfg, plan, max_slots=S, opts=opts, use_graph=True))
+            batches = [(plain["payload"], plain["iq_dev"])]
+            for j in range(1, n_in):
+                pl, _, _, iqj = make_slots(S, snr_op)
+                batches.append((pl, torch.from_numpy(iqj).cuda()))
+            torch.cuda.synchronize()
+            for _ in range(max(args.warmup, 1)):
+                for j in range(n_in):
+                    with torch.cuda.stream(streams[j]):
+                        chains[j].enqueue(batches[j][1], S)
+            torch.cuda.synchronize()
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record(stream)
+            for j in range(1, n_in):
+                streams[j].wait_event(start)
+            for k in range(args.steps):
+                j = k % n_in
+                with torch.cuda.stream(streams[j]):
+                    chains[j].enqueue(batches[j][1], S)
+            for j in range(1, n_in):
+                done = torch.cuda.Event()
+                done.record(streams[j])
+                stream.wait_event(done)
+            end.record(stream)
+            torch.cuda.synchronize()
+            pipe_ms = start.elapsed_time(end) / args.steps
+            per_cb = plan.Kprime - plan.crc_cb
+            pipe_ok, pipe_mism = 0, 0
+            for j in range(n_in):
+                with torch.cuda.stream(streams[j]):
+                    okj = chains[j].fetch("tb_ok", torch.uint8, S)
+                    bj = chains[j].fetch("bits", torch.int8, S * plan.C * plan.K)
+                torch.cuda.synchronize()
+                bj = bj.cpu().numpy().reshape(S, plan.C, plan.K).astype(np.uint8)
+                if j == 0 and not np.array_equal(bj, plain["bits"]):
+                    raise RuntimeError("pipelined batch 0 decoded differently from the single-stream run")
+                pipe_ok += int(okj.cpu().numpy().sum())
+                pipe_mism += sum(int(not np.array_equal(bj[s, :, :per_cb].reshape(-1)[: plan.tbs],
+                                                        batches[j][0][s])) for s in range(S))
+            for j in range(1, n_in):
+                chains[j].close()
+            pipe = {"ms_per_step": pipe_ms, "tb_ok": pipe_ok, "slots": n_in * S, "mismatches": pipe_mism}
+            log(f"lmmse, {n_in} batches in flight: {pipe_ms:.3f} ms/step, tb_ok {pipe_ok}/{n_in * S}, "
+                f"payload mismatches {pipe_mism}, batch 0 bit-identical to the single-stream run")
 
         # Per-stage times (HIP events on the chain's stream, direct launches).
         chain.set_timing(True)
@@ -257,6 +320,9 @@ def main() -> int:
         chain.close()
 
     tbs_bits = S * plan.tbs
+    single_ms = ms_per_step
+    if pipe is not None:
+        ms_per_step = pipe["ms_per_step"]
     value = tbs_bits / (ms_per_step * 1e-3) / 1e9
     goodput = float(tb_ok.sum()) * plan.tbs / (ms_per_step * 1e-3) / 1e9
     bytes_per = stage_bytes(g, plan, S, fused_ldpc)
@@ -284,6 +350,9 @@ def main() -> int:
             "bler": round(1.0 - float(tb_ok.mean()), 4),
             "goodput_gbps": round(goodput, 4),
             "slots_per_s": round(S / (ms_per_step * 1e-3), 1),
+            "inflight": args.inflight if pipe is not None else 1,
+            "single_stream": {"ms_per_step": round(single_ms, 4),
+                              "gbps": round(tbs_bits / (single_ms * 1e-3) / 1e9, 4)},
             "ldpc_max_iterations": opts.ldpc_iterations,
             "ldpc_mean_iterations": round(mean_iters, 3),
             "ce_rank": opts.ce_rank,
