@@ -190,3 +190,57 @@ def test_native_chain_crc_flags_match_crc_operator(gpu_stream, prof, snrs):
     assert 0 < int(cb_nat.sum()) < cb_nat.size
     assert 0 < int(tb_nat.sum()) < S
     nat.close()
+
+
+def test_native_chains_on_two_streams_match_serial(gpu_stream):
+    """Two chain instances replaying their graphs on two streams at once (the
+    decoder's work queues, CRC tables and scratch are per instance) decode
+    every batch exactly as one instance decoding the batches in turn."""
+    from puschrx.chain import NativeChain
+    prof = P.FULL_CARRIER_4RX
+    cfg, g, plan = plan_for(prof)
+    rng = np.random.default_rng(43)
+    S = 4
+    spec = synth.ChannelSpec(taps=16, delay_spread_s=300e-9, doppler_hz=100.0, rx_corr=0.5)
+    opts = ChainOptions(tau_s=spec.delay_spread_s, fd_hz=spec.doppler_hz, ce_rank=384,
+                        iq_format=abi.IQ_COMPLEX64)
+    batches = []
+    for snr in (36.0, 40.0):
+        payload = synth.random_payload(rng, S, plan.tbs)
+        grid = synth.tx_grid(cfg, plan, payload)
+        H = synth.channel(rng, cfg, spec, S)
+        batches.append(torch.from_numpy(synth.receive(rng, cfg, grid, H, snr, iq_format=abi.IQ_COMPLEX64)).cuda())
+    names = (("bits", torch.int8, S * plan.C * plan.K), ("cb_ok", torch.uint8, S * plan.C),
+             ("tb_ok", torch.uint8, S), ("iterations", torch.uint8, S * plan.C))
+
+    def outputs(chain):
+        out = [chain.fetch(n, dt, cnt) for n, dt, cnt in names]
+        torch.cuda.synchronize()
+        return [o.cpu().numpy() for o in out]
+
+    ref_chain = NativeChain(cfg, plan, max_slots=S, opts=opts, use_graph=True)
+    want = []
+    for iq in batches:
+        ref_chain.enqueue(iq, S)
+        want.append(outputs(ref_chain))
+    ref_chain.close()
+
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    chains = []
+    for st in streams:
+        with torch.cuda.stream(st):
+            chains.append(NativeChain(cfg, plan, max_slots=S, opts=opts, use_graph=True))
+    torch.cuda.synchronize()
+    for rep in range(3):
+        print(f"two streams, replay {rep}", flush=True)
+        for j in range(2):
+            with torch.cuda.stream(streams[j]):
+                chains[j].enqueue(batches[j], S)
+        torch.cuda.synchronize()
+        for j in range(2):
+            with torch.cuda.stream(streams[j]):
+                got = outputs(chains[j])
+            for n, a, b in zip(names, got, want[j]):
+                assert np.array_equal(a, b), (rep, j, n[0])
+    for c in chains:
+        c.close()
