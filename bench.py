@@ -298,6 +298,7 @@ def main() -> int:
             chain.set_refiner(ref, args.refiner_snr_min, args.refiner_snr_max)
             l_snr, l_sweep = (operating_point("learned") if args.snr is None else (args.snr, {}))
             learned = {"weights": str(wpath.relative_to(ROOT)) if wpath.is_relative_to(ROOT) else str(wpath),
+                       "inflight": 1,
                        "snr_range_db": [args.refiner_snr_min, args.refiner_snr_max],
                        "sweep_bler": {str(k): round(v, 4) for k, v in l_sweep.items()}}
             if l_snr is not None:
