@@ -1,8 +1,9 @@
 // chain.hip -- the batched PUSCH receive chain (include/airan_pusch_chain.h).
 //
-// One pipeline of batched launches over `slots` slots (one TB each); the only
-// kernel specific to the chain is the fused detector/demapper, the rest are
-// the per-operator kernels launched over all slots or code blocks at once.
+// One pipeline of batched launches over `slots` slots (transport blocks of
+// one or more consecutive slots); the only kernel specific to the chain is
+// the fused detector/demapper, the rest are the per-operator kernels launched
+// over all slots or code blocks at once.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -23,12 +24,13 @@ namespace {
 
 struct DemapArgs {
     uint32_t nr, qm, used, pad, nsym, cw_symbols, cw_stride;
+    uint32_t slots_per_tb;  // the Gold sequence runs on across a TB's slots
     uint32_t data_sym[14];
     uint16_t floor16;  // fp16 bits of the no_eff floor
 };
 
-// Packed Gold bits c(0 .. 32 words) of the data scrambling sequence: word w
-// holds c(32 w .. 32 w + 31), bit j = c(32 w + j).
+// Packed Gold bits of the data scrambling sequence of one transport block:
+// word w holds c(32 w .. 32 w + 31), bit j = c(32 w + j).
 __global__ void __launch_bounds__(256) GoldBitsKernel(uint32_t *__restrict__ bits, uint32_t words, uint32_t cinit)
 {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -47,8 +49,11 @@ __device__ inline int16_t SatNeg16(int v) { return static_cast<int16_t>(v == -32
 //   x, ne  = Detect<L>(...) rounded to fp16              (detect_lmmse)
 //   ne     = max(ne, fp16(floor))                        (no_eff floor)
 //   llr    = QamSymbolLlrs(x, ne)                        (qam256_demod)
-//   cw[b][re L + l] = +-llr[stream(b)] by c((re L + l) Qm + b)
+//   cw[b][re L + l] = +-llr[stream(b)] by c(n0 + (re L + l) Qm + b)
 //                                                        (layer_demap, descramble)
+// where n0 = (slot mod slots_per_tb) * codeword_symbols * Qm: the Gold index
+// continues across the slots of a transport block over transmitted symbols
+// only (descramble_mimo.cpp:176-184).
 // QM = 8 fixes the modulation at compile time (the bit -> stream mapping then
 // indexes registers); QM = 0 takes it from the arguments.
 template <int NL, int QM>
@@ -85,7 +90,8 @@ __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restric
     airan::Detect<NL>(static_cast<int>(nr), no, loadh, loady, xh, ne);
     const uint32_t qm = QM > 0 ? static_cast<uint32_t>(QM) : a.qm, m = qm / 2;
     const uint32_t re = ds * a.used + sc;
-    const uint64_t bit0 = static_cast<uint64_t>(re) * NL * qm;
+    const uint32_t slot_in_tb = slot - (slot / a.slots_per_tb) * a.slots_per_tb;
+    const uint64_t bit0 = (static_cast<uint64_t>(slot_in_tb) * a.cw_symbols + static_cast<uint64_t>(re) * NL) * qm;
     const uint32_t w0 = static_cast<uint32_t>(bit0 >> 5);
     const uint64_t gw = (static_cast<uint64_t>(gold[w0]) | (static_cast<uint64_t>(gold[w0 + 1]) << 32)) >> (bit0 & 31u);
     const __half floor16 = __ushort_as_half(a.floor16);
@@ -176,6 +182,9 @@ struct AiranPuschChain {
     AiranTbPlanV1 plan{};
     AiranPuschChainOptionsV1 opts{};
     Geom g{};
+    uint32_t slots_per_tb = 1;   // plan.G / (codeword_symbols * Qm)
+    bool crc = true;             // the plan carries a TB CRC (the reference's fixed plan has none)
+    uint16_t *slot_default = nullptr;  // [max_slots] config->slot_number (airan_pusch_chain_Enqueue)
     uint32_t active_rows = 0;
     bool fused_ldpc = false;  // decoder reads the codeword LLRs (no lam buffer)
     uint32_t last_slots = 0;
@@ -213,6 +222,7 @@ struct AiranPuschChain {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void *graph_iq = nullptr;
+    const void *graph_slot_no = nullptr;
     uint32_t graph_slots = 0;
     void *graph_stream = nullptr;
 };
@@ -233,11 +243,13 @@ void DropGraph(AiranPuschChain *c)
     c->exec = nullptr;
     c->graph = nullptr;
     c->graph_iq = nullptr;
+    c->graph_slot_no = nullptr;
     c->graph_slots = 0;
     c->graph_stream = nullptr;
 }
 
-int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStream_t s, bool timed)
+int32_t EnqueueStages(AiranPuschChain *c, const void *iq, const uint16_t *slot_no, uint32_t slots, hipStream_t s,
+                      bool timed)
 {
     const Geom &g = c->g;
     int stage = 0;
@@ -257,8 +269,11 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
     int32_t st = airan_ofdm_demod_redemap_Enqueue(&oa, c->rx_re, c->rx_im);
     if (st != AIRAN_OK) return st;
     mark();
+    // Each slot's DMRS from its own slot number, then the LS observations.
+    st = airan::LaunchDmrsGenSlots(c->cfg, g, slot_no, slots, c->dmrs_re, c->dmrs_im, s);
+    if (st != AIRAN_OK) return st;
     st = airan::LaunchDmrsLs(g, c->rx_re, c->rx_im, c->dmrs_re, c->dmrs_im, c->hls_re, c->hls_im, c->noise_ls,
-                             slots, s);
+                             slots, s, true);
     if (st != AIRAN_OK) return st;
     mark();
     st = airan::LaunchCeApply(c->filter, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce, c->ce_ws, c->ce_ws_bytes,
@@ -279,6 +294,7 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
     da.nsym = g.nsym;
     da.cw_symbols = g.cw_symbols;
     da.cw_stride = g.cw_stride;
+    da.slots_per_tb = c->slots_per_tb;
     for (uint32_t i = 0; i < g.ndata; ++i) da.data_sym[i] = g.data_sym[i];
     da.floor16 = HostF2H(airan_pusch_NoEffFloor(g.qm));
     const dim3 grid(airan::CeilDiv(g.used, 256), g.ndata, slots);
@@ -296,7 +312,8 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
                        da);
     if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
     mark();
-    const uint32_t ncb = slots * c->plan.C;
+    const uint32_t tbs = slots / c->slots_per_tb;
+    const uint32_t ncb = tbs * c->plan.C;
     if (c->fused_ldpc) {
         // Rate dematch runs inside the decoder's load (the stage is empty).
         mark();
@@ -304,13 +321,16 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
                                            c->opts.early_stop != 0, c->bits, c->iters, c->ldpc_queue, s);
         if (st != AIRAN_OK) return st;
         mark();
-        airan::CrcJob crc{};
-        if ((st = airan::CrcJobSetup(c->plan, c->cb_ok, reinterpret_cast<uint32_t *>(c->crc_ws), &crc)) != AIRAN_OK)
-            return st;
-        crc.xpow = c->crc_xpow;
-        airan::LaunchCbCrc(c->bits, crc, c->plan.K, ncb, CrcTables(c), s);
-        airan::LaunchCrcFinish(reinterpret_cast<const uint32_t *>(c->crc_ws), c->tb_ok, slots, c->plan.C, s);
-        if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
+        if (c->crc) {
+            airan::CrcJob crc{};
+            if ((st = airan::CrcJobSetup(c->plan, c->cb_ok, reinterpret_cast<uint32_t *>(c->crc_ws), &crc)) !=
+                AIRAN_OK)
+                return st;
+            crc.xpow = c->crc_xpow;
+            airan::LaunchCbCrc(c->bits, crc, c->plan.K, ncb, CrcTables(c), s);
+            airan::LaunchCrcFinish(reinterpret_cast<const uint32_t *>(c->crc_ws), c->tb_ok, tbs, c->plan.C, s);
+            if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
+        }
         mark();
         return AIRAN_OK;
     }
@@ -334,9 +354,11 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, uint32_t slots, hipStr
     st = airan_ldpc_decode_Enqueue(&la, c->ldpc_ws, c->ldpc_ws_bytes);
     if (st != AIRAN_OK) return st;
     mark();
-    st = airan_crc_check_Enqueue(&c->plan, slots, c->bits, c->cb_ok, c->tb_ok, nullptr, c->crc_ws,
-                                 c->crc_ws_bytes, s);
-    if (st != AIRAN_OK) return st;
+    if (c->crc) {
+        st = airan_crc_check_Enqueue(&c->plan, tbs, c->bits, c->cb_ok, c->tb_ok, nullptr, c->crc_ws,
+                                     c->crc_ws_bytes, s);
+        if (st != AIRAN_OK) return st;
+    }
     mark();
     return AIRAN_OK;
 }
@@ -366,10 +388,14 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     if (st != AIRAN_OK) return st;
     if (config->flags != 0 || g.ndmrs > 2 || (g.pad & 1u) != 0) return AIRAN_UNSUPPORTED_PROFILE;
     const AiranTbPlanV1 &p = *plan;
-    if (p.C == 0 || p.C > airan::kCrcMaxCb || p.Z < 24 || p.K != 22 * p.Z || p.G != g.cw_symbols * g.qm ||
+    const uint64_t slot_bits = static_cast<uint64_t>(g.cw_symbols) * g.qm;
+    const uint32_t T = slot_bits > 0 && p.G % slot_bits == 0 ? static_cast<uint32_t>(p.G / slot_bits) : 0;
+    if (p.C == 0 || p.C > airan::kCrcMaxCb || p.Z < 24 || p.K != 22 * p.Z || T == 0 ||
+        options->max_slots % T != 0 ||
         static_cast<uint64_t>(p.n_low) * p.E_low + static_cast<uint64_t>(p.C - p.n_low) * p.E_high != p.G ||
         p.E_low % g.qm != 0 || p.E_high % g.qm != 0 || p.Ncb <= p.filler ||
-        static_cast<uint64_t>(options->max_slots) * p.C > 65535)
+        static_cast<uint64_t>(options->max_slots / T) * p.C > 65535 ||
+        static_cast<uint64_t>(options->max_slots) * g.cw_symbols > 0xffffffffull)
         return AIRAN_INVALID_ARGUMENT;
     hipStream_t s = static_cast<hipStream_t>(stream);
     auto *c = new AiranPuschChain{};
@@ -377,6 +403,8 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     c->plan = p;
     c->opts = *options;
     c->g = g;
+    c->slots_per_tb = T;
+    c->crc = p.crc_tb != 0;
     c->active_rows = airan_ldpc_ActiveRows(plan);
     c->fused_ldpc = options->ldpc_mode == AIRAN_LDPC_MODE_FP16 && c->active_rows <= 8 && p.Z % 24 == 0;
     auto fail = [&](int32_t code) {
@@ -394,17 +422,19 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     fp.eig_floor = options->eig_floor;
     if ((st = airan_channel_est_lmmse_CreateFilter(config, &fp, stream, &c->filter)) != AIRAN_OK) return fail(st);
     const size_t S = options->max_slots;
+    const size_t NT = S / T;  // transport blocks per Enqueue at most
     const size_t plane = static_cast<size_t>(g.nsym) * g.pad;
-    const size_t ncb = S * p.C;
-    const size_t gold_words = (static_cast<size_t>(g.cw_symbols) * g.qm + 31) / 32 + 2;
+    const size_t ncb = NT * p.C;
+    const size_t gold_words = (static_cast<size_t>(T) * g.cw_symbols * g.qm + 31) / 32 + 2;
+    const size_t dmrs_elems = S * g.nl * g.ndmrs * g.ref_pad;  // one DMRS reference per slot
     c->ce_ws_bytes = airan::CeWorkspaceBytes(g, static_cast<uint32_t>(S));
     c->ldpc_ws_bytes =
         c->fused_ldpc ? 0 : airan_ldpc_decode_WorkspaceBytes(static_cast<uint32_t>(ncb), p.Z, c->active_rows);
-    c->crc_ws_bytes = airan_crc_check_WorkspaceBytes(static_cast<uint32_t>(S));
+    c->crc_ws_bytes = c->crc ? airan_crc_check_WorkspaceBytes(static_cast<uint32_t>(NT)) : 0;
     const size_t iq_elem = options->iq_format == AIRAN_IQ_COMPLEX64 ? 8 : 4;
     size_t &b = c->bytes;
-    if (!Alloc(&c->dmrs_re, static_cast<size_t>(g.nl) * g.ndmrs * g.ref_pad, &b) ||
-        !Alloc(&c->dmrs_im, static_cast<size_t>(g.nl) * g.ndmrs * g.ref_pad, &b) ||
+    if (!Alloc(&c->dmrs_re, dmrs_elems, &b) || !Alloc(&c->dmrs_im, dmrs_elems, &b) ||
+        !Alloc(&c->slot_default, S, &b) ||
         !Alloc(&c->rx_re, S * g.nr * plane, &b) || !Alloc(&c->rx_im, S * g.nr * plane, &b) ||
         !Alloc(&c->hls_re, S * g.nr * g.nl * g.ndmrs * g.pilot_pad, &b) ||
         !Alloc(&c->hls_im, S * g.nr * g.nl * g.ndmrs * g.pilot_pad, &b) ||
@@ -414,18 +444,19 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         !Alloc(&c->cw, S * g.qm * g.cw_stride, &b) || !Alloc(&c->desc, ncb, &b) ||
         !Alloc(&c->lam, c->fused_ldpc ? 0 : ncb * 68 * p.Z, &b) || !Alloc(&c->ldpc_ws, c->ldpc_ws_bytes, &b) ||
         !Alloc(&c->bits, ncb * p.K, &b) || !Alloc(&c->iters, ncb, &b) || !Alloc(&c->cb_ok, ncb, &b) ||
-        !Alloc(&c->tb_ok, S, &b) || !Alloc(&c->refiner_keep, S, &b) || !Alloc(&c->crc_xpow, p.C, &b) || !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
+        !Alloc(&c->tb_ok, NT, &b) || !Alloc(&c->refiner_keep, S, &b) || !Alloc(&c->crc_xpow, p.C, &b) ||
+        !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
         !Alloc(&c->ldpc_queue, 2, &b) ||
         !Alloc(reinterpret_cast<uint8_t **>(&c->zero_iq), g.nr * g.samples * iq_elem, &b))
         return fail(AIRAN_RESOURCE_TOO_SMALL);
-    // Per-code-block shifts of the transport-block CRC remainder.
-    {
+    // The decoder's block queue starts (and stays) zero.
+    if (hipMemset(c->ldpc_queue, 0, 2 * sizeof(uint32_t)) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
+    if (c->crc) {
+        // Per-code-block shifts of the transport-block CRC remainder.
         std::vector<uint32_t> xp(p.C);
         for (uint32_t r = 0; r < p.C; ++r) xp[r] = airan::CrcXpowHost(p, r);
         if (hipMemcpy(c->crc_xpow, xp.data(), p.C * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
             return fail(AIRAN_LAUNCH_FAILED);
-        // The decoder's block queue starts (and stays) zero.
-        if (hipMemset(c->ldpc_queue, 0, 2 * sizeof(uint32_t)) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
         // The CRC check's tables, after the TB remainder parts in crc_ws.
         airan::CrcJob job{};
         if ((st = airan::CrcJobSetup(p, nullptr, nullptr, &job)) != AIRAN_OK) return fail(st);
@@ -433,39 +464,30 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         if (hipDeviceSynchronize() != hipSuccess || airan::LaunchStatus() != AIRAN_OK)
             return fail(AIRAN_LAUNCH_FAILED);
     }
-    // DMRS reference of this slot configuration.
-    MimoDmrsGenRuntimeV1 *rt = nullptr;
-    if ((st = airan_mimo_dmrs_gen_CreateRuntime(&rt)) != AIRAN_OK) return fail(st);
-    PuschMimoLayout lay{};
-    airan_DerivePuschMimoLayout(config, &lay);
-    MimoDmrsGenOpArgsV1 ga{};
-    ga.abi_version = AIRAN_PUSCH_MIMO_ABI_VERSION;
-    ga.struct_size = sizeof(ga);
-    ga.dmrs_re = c->dmrs_re;
-    ga.dmrs_im = c->dmrs_im;
-    ga.config = config;
-    ga.layout = &lay;
-    ga.stream = stream;
-    st = airan_mimo_dmrs_gen_Enqueue(rt, &ga);
-    const bool sync_ok = hipStreamSynchronize(s) == hipSuccess;
-    airan_mimo_dmrs_gen_DestroyRuntime(rt);
-    if (st != AIRAN_OK || !sync_ok) return fail(st != AIRAN_OK ? st : AIRAN_LAUNCH_FAILED);
-    // Gold bits of one slot's codeword (scrambling restarts every slot).
+    // config->slot_number for every slot of a plain Enqueue.
+    {
+        const std::vector<uint16_t> sn(S, config->slot_number);
+        if (hipMemcpy(c->slot_default, sn.data(), S * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(AIRAN_LAUNCH_FAILED);
+    }
+    // Gold bits of one transport block's codeword (the sequence restarts with
+    // every TB and runs on across its slots).
     const uint32_t cinit = (static_cast<uint32_t>(config->rnti) << 15) |
                            ((static_cast<uint32_t>(config->codeword_index) & 1u) << 14) | config->data_scrambling_id;
     hipLaunchKernelGGL(GoldBitsKernel, dim3(airan::CeilDiv(gold_words, 256)), dim3(256), 0, s, c->gold,
                        static_cast<uint32_t>(gold_words), cinit);
-    // Rate-match descriptors of every code block of every slot.
+    // Rate-match descriptors of every code block of every transport block
+    // (symbol offsets into the stacked codeword buffer; a block may span a
+    // slot boundary).
     std::vector<RateMatchDescriptor> desc(ncb);
-    for (size_t slot = 0; slot < S; ++slot) {
-        uint64_t off = slot * static_cast<uint64_t>(g.cw_symbols);
+    for (size_t tb = 0; tb < NT; ++tb) {
+        uint64_t off = tb * T * static_cast<uint64_t>(g.cw_symbols);
         for (uint32_t r = 0; r < p.C; ++r) {
             const uint32_t e = r < p.n_low ? p.E_low : p.E_high;
-            desc[slot * p.C + r] = {e, p.k0, p.Ncb, static_cast<uint32_t>(off)};
+            desc[tb * p.C + r] = {e, p.k0, p.Ncb, static_cast<uint32_t>(off)};
             off += e / g.qm;
         }
     }
-    if (S * g.cw_symbols > 0xffffffffull) return fail(AIRAN_INVALID_ARGUMENT);
     if (hipMemcpyAsync(c->desc, desc.data(), ncb * sizeof(RateMatchDescriptor), hipMemcpyHostToDevice, s) !=
             hipSuccess ||
         hipMemsetAsync(c->cw, 0, S * g.qm * g.cw_stride * sizeof(int16_t), s) != hipSuccess ||
@@ -474,9 +496,9 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     // One eager slot loads every per-device table before any graph capture.
     for (hipEvent_t &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
-    if ((st = EnqueueStages(c, c->zero_iq, 1, s, false)) != AIRAN_OK) return fail(st);
+    if ((st = EnqueueStages(c, c->zero_iq, c->slot_default, T, s, false)) != AIRAN_OK) return fail(st);
     if (hipStreamSynchronize(s) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
-    c->last_slots = 1;
+    c->last_slots = T;
     *chain = c;
     return AIRAN_OK;
 }
@@ -488,7 +510,7 @@ void airan_pusch_chain_Destroy(AiranPuschChain *c)
     for (hipEvent_t e : c->ev)
         if (e != nullptr) (void)hipEventDestroy(e);
     airan_channel_est_lmmse_DestroyFilter(c->filter);
-    void *bufs[] = {c->dmrs_re, c->dmrs_im, c->rx_re, c->rx_im, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce,
+    void *bufs[] = {c->dmrs_re, c->dmrs_im, c->slot_default, c->rx_re, c->rx_im, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce,
                     c->ce_ws, c->h_re, c->h_im, c->gold, c->cw, c->desc, c->lam, c->ldpc_ws, c->bits,
                     c->iters, c->cb_ok, c->tb_ok, c->crc_ws, c->zero_iq, c->refiner_keep, c->crc_xpow,
                     c->ldpc_queue};
@@ -496,17 +518,20 @@ void airan_pusch_chain_Destroy(AiranPuschChain *c)
     delete c;
 }
 
-int32_t airan_pusch_chain_Enqueue(AiranPuschChain *c, const void *iq, uint32_t num_slots, void *stream)
+int32_t airan_pusch_chain_EnqueueSlots(AiranPuschChain *c, const void *iq, uint32_t num_slots,
+                                       const uint16_t *slot_numbers, void *stream)
 {
-    if (c == nullptr || iq == nullptr || stream == nullptr || num_slots == 0 || num_slots > c->opts.max_slots)
+    if (c == nullptr || iq == nullptr || slot_numbers == nullptr || stream == nullptr || num_slots == 0 ||
+        num_slots > c->opts.max_slots || num_slots % c->slots_per_tb != 0)
         return AIRAN_INVALID_ARGUMENT;
     hipStream_t s = static_cast<hipStream_t>(stream);
     c->last_slots = num_slots;
-    if (c->timing || !c->opts.use_graph) return EnqueueStages(c, iq, num_slots, s, c->timing);
-    if (c->exec == nullptr || c->graph_iq != iq || c->graph_slots != num_slots || c->graph_stream != stream) {
+    if (c->timing || !c->opts.use_graph) return EnqueueStages(c, iq, slot_numbers, num_slots, s, c->timing);
+    if (c->exec == nullptr || c->graph_iq != iq || c->graph_slot_no != slot_numbers || c->graph_slots != num_slots ||
+        c->graph_stream != stream) {
         DropGraph(c);
         if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return AIRAN_LAUNCH_FAILED;
-        const int32_t st = EnqueueStages(c, iq, num_slots, s, false);
+        const int32_t st = EnqueueStages(c, iq, slot_numbers, num_slots, s, false);
         hipGraph_t graph = nullptr;
         const hipError_t end = hipStreamEndCapture(s, &graph);
         if (st != AIRAN_OK || end != hipSuccess || graph == nullptr) {
@@ -519,11 +544,20 @@ int32_t airan_pusch_chain_Enqueue(AiranPuschChain *c, const void *iq, uint32_t n
             return AIRAN_LAUNCH_FAILED;
         }
         c->graph_iq = iq;
+        c->graph_slot_no = slot_numbers;
         c->graph_slots = num_slots;
         c->graph_stream = stream;
     }
     return hipGraphLaunch(c->exec, s) == hipSuccess ? AIRAN_OK : AIRAN_LAUNCH_FAILED;
 }
+
+int32_t airan_pusch_chain_Enqueue(AiranPuschChain *c, const void *iq, uint32_t num_slots, void *stream)
+{
+    if (c == nullptr) return AIRAN_INVALID_ARGUMENT;
+    return airan_pusch_chain_EnqueueSlots(c, iq, num_slots, c->slot_default, stream);
+}
+
+uint32_t airan_pusch_chain_SlotsPerTb(const AiranPuschChain *c) { return c == nullptr ? 0 : c->slots_per_tb; }
 
 int32_t airan_pusch_chain_Outputs(const AiranPuschChain *c, AiranPuschChainOutputsV1 *o)
 {
@@ -532,8 +566,8 @@ int32_t airan_pusch_chain_Outputs(const AiranPuschChain *c, AiranPuschChainOutpu
         return AIRAN_INVALID_ARGUMENT;
     o->num_slots = c->last_slots;
     o->bits = c->bits;
-    o->tb_ok = c->tb_ok;
-    o->cb_ok = c->cb_ok;
+    o->tb_ok = c->crc ? c->tb_ok : nullptr;
+    o->cb_ok = c->crc ? c->cb_ok : nullptr;
     o->iterations = c->iters;
     o->rx_grid_re = c->rx_re;
     o->rx_grid_im = c->rx_im;

@@ -12,9 +12,15 @@ struct CrcJob;
 
 // DMRS least squares + per-Rx noise for `slots` stacked grids
 // (rx [slots,NR,14,pad] -> h_ls [slots,NR,L,D,pilot_pad], noise [slots,NR]).
+// ref_re/ref_im: DMRS references [L][D][ref_pad], one per slot
+// (ref_per_slot) or one shared by every slot.
 int32_t LaunchDmrsLs(const Geom &g, const __half *rx_re, const __half *rx_im, const __half *ref_re,
                      const __half *ref_im, __half *h_re, __half *h_im, __half *noise, uint32_t slots,
-                     hipStream_t s);
+                     hipStream_t s, bool ref_per_slot = false);
+// DMRS references of `slots` slots [slots][L][D][ref_pad], slot s with the
+// c_init of slot number slot_no[s] (device uint16) and the rest of cfg.
+int32_t LaunchDmrsGenSlots(const PuschMimoConfig &cfg, const Geom &g, const uint16_t *slot_no, uint32_t slots,
+                           __half *ref_re, __half *ref_im, hipStream_t s);
 
 // LMMSE channel estimate for `slots` stacked observations; output
 // [slots,NR,out_layers,14,pad] (layers >= L untouched).

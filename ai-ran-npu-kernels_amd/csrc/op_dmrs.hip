@@ -25,19 +25,32 @@ struct DmrsGenDev {
     uint32_t nl, ndmrs, n_dmrs_re, ref_pad, off;
     uint32_t cinit[4];
     uint32_t wf_odd[4];
+    uint32_t sym[4];      // DMRS symbol of each d (c_init from a slot number)
+    uint32_t nid, nscid;  // dmrs_scrambling_id, n_SCID
 };
+
+// DmrsCinit on the device.
+__device__ inline uint32_t DevDmrsCinit(uint32_t slot_number, uint32_t symbol, uint32_t nid, uint32_t nscid)
+{
+    const uint64_t term = (1ull << 17) * (14ull * slot_number + symbol + 1ull) * (2ull * nid + 1ull);
+    return static_cast<uint32_t>((term + 2ull * nid + nscid) & 0x7fffffffu);
+}
 
 // r(m) = ((1-2c(2m)) + j(1-2c(2m+1)))/sqrt(2) (mimo_dmrs_gen_kernel.cpp:163),
 // times w_f = (-1)^m on ports 1001/1003.  A thread owns 16 consecutive m.
+// Grid (m blocks, D, slots * L).  slot_no == nullptr: one reference from
+// d.cinit; else one reference per slot ([slots][L][D][ref_pad]), its c_init
+// from the slot's number slot_no[slot].
 __global__ void __launch_bounds__(256) DmrsGenKernel(uint16_t *__restrict__ re, uint16_t *__restrict__ im,
-                                                     DmrsGenDev d)
+                                                     const uint16_t *__restrict__ slot_no, DmrsGenDev d)
 {
     const uint32_t m0 = (blockIdx.x * blockDim.x + threadIdx.x) * 16;
     if (m0 >= d.ref_pad) return;
-    const uint32_t dd = blockIdx.y, l = blockIdx.z;
+    const uint32_t dd = blockIdx.y, slot = blockIdx.z / d.nl, l = blockIdx.z - slot * d.nl;
     const uint16_t pos = 0x39a8, neg = 0xb9a8;  // fp16(+-0.70710678)
-    const uint32_t bits = airan::GoldWordAt(d.cinit[dd], 2ull * (m0 + d.off));
-    const size_t base = (static_cast<size_t>(l) * d.ndmrs + dd) * d.ref_pad;
+    const uint32_t cinit = slot_no == nullptr ? d.cinit[dd] : DevDmrsCinit(slot_no[slot], d.sym[dd], d.nid, d.nscid);
+    const uint32_t bits = airan::GoldWordAt(cinit, 2ull * (m0 + d.off));
+    const size_t base = ((static_cast<size_t>(slot) * d.nl + l) * d.ndmrs + dd) * d.ref_pad;
 #pragma unroll
     for (uint32_t j = 0; j < 16; ++j) {
         const uint32_t m = m0 + j;
@@ -76,11 +89,13 @@ __device__ inline __half HSub(__half a, __half b)
 
 // channel_est/mimo_dmrs_ls/mimo_dmrs_ls_kernel.cpp:146-177,301-331: h = y x*
 // in the reference fp16 operation order, FD-OCC2 pair mean when two layers
-// share the comb.  Grid (pilot blocks, D, NR * L).
+// share the comb.  Grid (pilot blocks, D, slots * NR * L).  The DMRS
+// references are [L][D][ref_pad], one for every slot (ref_per_slot, their
+// c_init depends on the slot number, mimo_dmrs_gen.cpp:38-45) or one shared.
 __global__ void __launch_bounds__(256) DmrsLsKernel(const __half *__restrict__ yr,
                                                     const __half *__restrict__ yi,
                                                     const __half *__restrict__ xr,
-                                                    const __half *__restrict__ xi,
+                                                    const __half *__restrict__ xi, uint32_t ref_per_slot,
                                                     __half *__restrict__ hr, __half *__restrict__ hi,
                                                     DevGeom g)
 {
@@ -92,8 +107,10 @@ __global__ void __launch_bounds__(256) DmrsLsKernel(const __half *__restrict__ y
     const size_t out = (static_cast<size_t>(blockIdx.z) * g.ndmrs + d) * g.pilot_pad + p;
     const __half *y_r = yr + (static_cast<size_t>(srx) * g.nsym + g.dmrs_sym[d]) * g.pad;
     const __half *y_i = yi + (static_cast<size_t>(srx) * g.nsym + g.dmrs_sym[d]) * g.pad;
-    const __half *x_r = xr + (static_cast<size_t>(l) * g.ndmrs + d) * g.ref_pad;
-    const __half *x_i = xi + (static_cast<size_t>(l) * g.ndmrs + d) * g.ref_pad;
+    const uint32_t ref_slot = ref_per_slot ? srx / g.nr : 0u;
+    const size_t ref = (static_cast<size_t>(ref_slot) * g.nl + l) * g.ndmrs + d;
+    const __half *x_r = xr + ref * g.ref_pad;
+    const __half *x_i = xi + ref * g.ref_pad;
     const uint32_t comb = g.comb[l];
     auto ls = [&](uint32_t m, __half *or_, __half *oi) {
         const uint32_t sc = comb + 2 * m;
@@ -177,12 +194,46 @@ namespace airan {
 
 int32_t LaunchDmrsLs(const Geom &g, const __half *rx_re, const __half *rx_im, const __half *ref_re,
                      const __half *ref_im, __half *h_re, __half *h_im, __half *noise, uint32_t slots,
-                     hipStream_t s)
+                     hipStream_t s, bool ref_per_slot)
 {
     const DevGeom dg = ToDev(g);
     const dim3 grid(CeilDiv(g.pilot_pad, 256), g.ndmrs, slots * g.nr * g.nl);
-    hipLaunchKernelGGL(DmrsLsKernel, grid, dim3(256), 0, s, rx_re, rx_im, ref_re, ref_im, h_re, h_im, dg);
+    hipLaunchKernelGGL(DmrsLsKernel, grid, dim3(256), 0, s, rx_re, rx_im, ref_re, ref_im, ref_per_slot ? 1u : 0u,
+                       h_re, h_im, dg);
     hipLaunchKernelGGL(DmrsNoiseKernel, dim3(slots * g.nr), dim3(256), 0, s, h_re, h_im, noise, dg);
+    return LaunchStatus();
+}
+
+namespace {
+
+DmrsGenDev GenDev(const PuschMimoConfig &cfg, const Geom &g)
+{
+    DmrsGenDev d{};
+    d.nl = g.nl;
+    d.ndmrs = g.ndmrs;
+    d.n_dmrs_re = g.n_dmrs_re;
+    d.ref_pad = g.ref_pad;
+    d.off = 6u * cfg.rb_start;
+    for (uint32_t i = 0; i < g.ndmrs; ++i) {
+        d.cinit[i] = DmrsCinit(cfg, g.dmrs_sym[i]);
+        d.sym[i] = g.dmrs_sym[i];
+    }
+    for (uint32_t l = 0; l < g.nl; ++l) d.wf_odd[l] = (cfg.dmrs_ports[l] - 1000u) & 1u;
+    d.nid = cfg.dmrs_scrambling_id;
+    d.nscid = cfg.n_scid;
+    return d;
+}
+
+}  // namespace
+
+int32_t LaunchDmrsGenSlots(const PuschMimoConfig &cfg, const Geom &g, const uint16_t *slot_no, uint32_t slots,
+                           __half *ref_re, __half *ref_im, hipStream_t s)
+{
+    if (slot_no == nullptr || slots == 0) return AIRAN_INVALID_ARGUMENT;
+    const DmrsGenDev d = GenDev(cfg, g);
+    const dim3 grid(CeilDiv(CeilDiv(g.ref_pad, 16), 256), g.ndmrs, slots * g.nl);
+    hipLaunchKernelGGL(DmrsGenKernel, grid, dim3(256), 0, s, reinterpret_cast<uint16_t *>(ref_re),
+                       reinterpret_cast<uint16_t *>(ref_im), slot_no, d);
     return LaunchStatus();
 }
 
@@ -269,17 +320,10 @@ int32_t airan_mimo_dmrs_gen_Enqueue(MimoDmrsGenRuntimeV1 *runtime, const MimoDmr
     if (st != AIRAN_OK) return st;
     Geom g;
     airan::DeriveGeom(*a->config, &g);
-    DmrsGenDev d{};
-    d.nl = g.nl;
-    d.ndmrs = g.ndmrs;
-    d.n_dmrs_re = g.n_dmrs_re;
-    d.ref_pad = g.ref_pad;
-    d.off = 6u * a->config->rb_start;
-    for (uint32_t i = 0; i < g.ndmrs; ++i) d.cinit[i] = DmrsCinit(*a->config, g.dmrs_sym[i]);
-    for (uint32_t l = 0; l < g.nl; ++l) d.wf_odd[l] = (a->config->dmrs_ports[l] - 1000u) & 1u;
+    const DmrsGenDev d = airan::GenDev(*a->config, g);
     dim3 grid(airan::CeilDiv(airan::CeilDiv(g.ref_pad, 16), 256), g.ndmrs, g.nl);
     hipLaunchKernelGGL(DmrsGenKernel, grid, dim3(256), 0, static_cast<hipStream_t>(a->stream),
-                       static_cast<uint16_t *>(a->dmrs_re), static_cast<uint16_t *>(a->dmrs_im), d);
+                       static_cast<uint16_t *>(a->dmrs_re), static_cast<uint16_t *>(a->dmrs_im), nullptr, d);
     return airan::LaunchStatus();
 }
 

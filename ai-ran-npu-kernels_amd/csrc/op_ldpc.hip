@@ -3,7 +3,9 @@
 // Schedule and arithmetic follow fec/ldpc_decode/ldpc_decode_kernel.cpp:
 // rows in order (:533-713), per row gather -> subtract old message ->
 // min1/min2/sign product -> alpha = 0.75 (:248-300) -> posterior update and
-// scatter; final clamp +-5120 and bit = (lam < 0) (:719-780).
+// scatter; final clamp +-5120, int16 conversion with CAST_RINT and bit =
+// (int16 value < 0) (:719-780), i.e. a REF-mode posterior in [-0.5, 0)
+// decodes to 0.
 //
 // Mapping: one workgroup per code block; the posterior [68][Z] lives in LDS
 // as fp16 (52 KB at Z = 384); thread i owns check i of every row, so within a row the
@@ -12,6 +14,7 @@
 // (min1, min2, index of min1, sign bits): 8 bytes per (row, check) in
 // registers when the active rows fit REG_ROWS, otherwise in a global workspace.
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "airan_ldpc_bg1.h"
@@ -31,12 +34,16 @@ struct Bg1Dev {
 
 __constant__ Bg1Dev kBg1;
 
+// Per-device "constant tables loaded" flags of kBg1 and kLift, guarded by
+// one mutex (callers on several host threads may load them at once).
+std::mutex g_table_mutex;
 bool g_table_loaded[64] = {};
 
 int32_t LoadTable()
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return AIRAN_LAUNCH_FAILED;
+    std::lock_guard<std::mutex> lock(g_table_mutex);
     if (g_table_loaded[dev]) return AIRAN_OK;
     Bg1Dev t{};
     for (int i = 0; i <= AIRAN_BG1_ROWS; ++i) t.row_start[i] = AIRAN_BG1_ROW_START[i];
@@ -118,10 +125,14 @@ __device__ inline void ProcessRow(uint16_t *lam, int r, uint32_t i, uint32_t Z, 
     }
 }
 
+// MODE_REF: the reference's rint(clamp(v, +-5120)) < 0, which for an fp16
+// posterior is v < -0.5 (a NaN gives 0).  MODE_FP16 (no reference
+// counterpart): v < 0.
 template <int MODE>
 __device__ inline bool HardBit(const uint16_t *lam, uint32_t a)
 {
-    return __half2float(__ushort_as_half(lam[a])) < 0.0f;
+    const float v = __half2float(__ushort_as_half(lam[a]));
+    return MODE == AIRAN_LDPC_MODE_REF ? v < -0.5f : v < 0.0f;
 }
 
 // True when check i of every active row is satisfied by the hard decisions.
@@ -258,6 +269,7 @@ int32_t LoadLiftTable()
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return AIRAN_LAUNCH_FAILED;
+    std::lock_guard<std::mutex> lock(g_table_mutex);
     if (g_lift_loaded[dev]) return AIRAN_OK;
     for (int r = 0; r < kFastRows; ++r)
         if (AIRAN_BG1_ROW_START[r] != FastRowStart(r) ||
@@ -656,7 +668,9 @@ __global__ void __launch_bounds__(384)
                    int8_t *__restrict__ bits, int16_t *__restrict__ lam_out, uint8_t *__restrict__ iters_out,
                    LdpcFastDev d, uint32_t *__restrict__ queue)
 {
-    __shared__ uint32_t taken_cb;
+    // One slot per half: a wave may still be reading half 0's take when
+    // thread 0 takes for half 1 (PairLoad need not end with a barrier).
+    __shared__ uint32_t taken_cb[2];
     __shared__ uint32_t syn_bad[8];  // per wave: bit h set when half h fails a check
     uint32_t *lam2 = reinterpret_cast<uint32_t *>(airan_ldpc_lam);
     const uint32_t Z = d.z, i = threadIdx.x;
@@ -681,9 +695,9 @@ __global__ void __launch_bounds__(384)
                 if (queue == nullptr) {
                     next = blockIdx.x + taken * gridDim.x;
                 } else {
-                    if (i == 0) taken_cb = atomicAdd(queue, 1u);
+                    if (i == 0) taken_cb[h] = atomicAdd(queue, 1u);
                     __syncthreads();
-                    next = __builtin_amdgcn_readfirstlane(taken_cb);
+                    next = __builtin_amdgcn_readfirstlane(taken_cb[h]);
                 }
                 ++taken;
                 PairLoad<FUSED>(airan_ldpc_lam, h, nl, src, desc, d, next);

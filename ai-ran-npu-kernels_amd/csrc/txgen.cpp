@@ -183,8 +183,11 @@ bool PlanOk(const AiranTbPlanV1 &p, const airan::Geom &g, uint32_t slots_per_tb)
     return p.Ncb > p.filler;
 }
 
+// slot_numbers: the slot number of each of the TB's slots (its DMRS), or
+// null for cfg.slot_number.
 int32_t EncodeOneTb(const PuschMimoConfig &cfg, const airan::Geom &g, const AiranTbPlanV1 &p,
-                    uint32_t slots_per_tb, const uint8_t *payload, float *grid, uint8_t *coded_out)
+                    uint32_t slots_per_tb, const uint16_t *slot_numbers, const uint8_t *payload, float *grid,
+                    uint8_t *coded_out)
 {
     const uint32_t Z = p.Z;
     // TB with CRC.
@@ -251,10 +254,11 @@ int32_t EncodeOneTb(const PuschMimoConfig &cfg, const airan::Geom &g, const Aira
             o[0] = AxisLevel(ib, m) * norm;
             o[1] = AxisLevel(qb, m) * norm;
         }
-        // DMRS.
+        // DMRS of this slot's number (mimo_dmrs_gen.cpp:38-45).
         const float h = 0.70710678118654752f;
+        const uint64_t slot_number = slot_numbers != nullptr ? slot_numbers[s] : cfg.slot_number;
         for (uint32_t d = 0; d < g.ndmrs; ++d) {
-            const uint64_t term = (1ull << 17) * (14ull * cfg.slot_number + g.dmrs_sym[d] + 1ull) *
+            const uint64_t term = (1ull << 17) * (14ull * slot_number + g.dmrs_sym[d] + 1ull) *
                                   (2ull * cfg.dmrs_scrambling_id + 1ull);
             const uint32_t c_dmrs =
                 static_cast<uint32_t>((term + 2ull * cfg.dmrs_scrambling_id + cfg.n_scid) & 0x7fffffffu);
@@ -299,6 +303,13 @@ int32_t airan_ldpc_bg1_Encode(uint32_t z, const uint8_t *info, uint8_t *codeword
 int32_t airan_pusch_tx_EncodeTbs(const PuschMimoConfig *config, const AiranTbPlanV1 *plan, uint32_t num_tbs,
                                  uint32_t slots_per_tb, const uint8_t *payload, float *grid, uint8_t *coded_bits)
 {
+    return airan_pusch_tx_EncodeTbsSlots(config, plan, num_tbs, slots_per_tb, nullptr, payload, grid, coded_bits);
+}
+
+int32_t airan_pusch_tx_EncodeTbsSlots(const PuschMimoConfig *config, const AiranTbPlanV1 *plan, uint32_t num_tbs,
+                                      uint32_t slots_per_tb, const uint16_t *slot_numbers, const uint8_t *payload,
+                                      float *grid, uint8_t *coded_bits)
+{
     if (config == nullptr || plan == nullptr || payload == nullptr || grid == nullptr || num_tbs == 0 ||
         slots_per_tb == 0)
         return AIRAN_INVALID_ARGUMENT;
@@ -312,7 +323,9 @@ int32_t airan_pusch_tx_EncodeTbs(const PuschMimoConfig *config, const AiranTbPla
     std::vector<int32_t> status(num_tbs, AIRAN_OK);
     auto work = [&](unsigned t) {
         for (uint32_t tb = t; tb < num_tbs; tb += nthreads)
-            status[tb] = EncodeOneTb(*config, g, *plan, slots_per_tb, payload + static_cast<size_t>(tb) * plan->tbs,
+            status[tb] = EncodeOneTb(*config, g, *plan, slots_per_tb,
+                                     slot_numbers ? slot_numbers + static_cast<size_t>(tb) * slots_per_tb : nullptr,
+                                     payload + static_cast<size_t>(tb) * plan->tbs,
                                      grid + slot_floats * slots_per_tb * tb,
                                      coded_bits ? coded_bits + static_cast<size_t>(tb) * plan->G : nullptr);
     };

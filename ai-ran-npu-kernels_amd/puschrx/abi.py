@@ -31,6 +31,8 @@ IQ_COMPLEX64 = 1
 DFT_FP32 = 0
 LDPC_MODE_REF = 0
 LDPC_MODE_FP16 = 1
+OBS_COMB2 = 0
+OBS_FD_OCC2 = 1
 DESCR_RESTART_PER_SLOT = 1
 
 
@@ -358,8 +360,21 @@ PROTOTYPES = {
     "airan_crc_check_WorkspaceBytes": (C.c_size_t, [u32]),
     "airan_crc_check_Enqueue": (i32, [P(AiranTbPlanV1), u32, vp, vp, vp, vp, vp, C.c_size_t, vp]),
     "airan_pusch_NoEffFloor": (C.c_float, [u32]),
+    # host references (bit-exact, no GPU)
+    "airan_re_demap_batch_Reference": (i32, [P(PuschMimoConfig), vp, vp, u32, vp, vp, vp]),
+    "airan_layer_demap_ReferenceDemap": (i32, [vp, P(PuschMimoConfig), P(PuschMimoLayout), vp]),
+    "airan_descramble_mimo_ReferenceDescramble": (i32, [vp, vp, P(PuschMimoConfig), P(PuschMimoLayout), u32, vp]),
+    "airan_rate_dematch_mimo_ReferenceRateDematch": (i32, [vp, P(PuschMimoConfig), P(PuschMimoLayout), u32,
+                                                           P(RateMatchDescriptor), C.c_size_t, vp]),
+    "airan_rate_dematch_mimo_ReferenceRateDematchForTb": (i32, [vp, P(PuschMimoConfig), P(PuschMimoLayout), u32,
+                                                                P(AiranTbPlanV1), P(RateMatchDescriptor),
+                                                                C.c_size_t, vp]),
+    "airan_mimo_dmrs_ls_DescribeObservationModels": (i32, [vp, vp, u32, u32, vp]),
+    "airan_mimo_dmrs_ls_ValidateNaturalLmmseContract": (i32, [vp, vp, vp, u32, u32]),
+    "airan_mimo_dmrs_ls_ValidateLmmse798Compatibility": (i32, [vp, vp, u32, u32]),
     # synthetic transmitter (include/airan_pusch_tx.h)
     "airan_pusch_tx_EncodeTbs": (i32, [P(PuschMimoConfig), P(AiranTbPlanV1), u32, u32, vp, vp, vp]),
+    "airan_pusch_tx_EncodeTbsSlots": (i32, [P(PuschMimoConfig), P(AiranTbPlanV1), u32, u32, vp, vp, vp, vp]),
     "airan_ldpc_bg1_Encode": (i32, [u32, vp, vp]),
     "airan_crc_Compute": (u32, [u32, vp, C.c_uint64]),
     # batched chain (include/airan_pusch_chain.h)
@@ -367,6 +382,8 @@ PROTOTYPES = {
                                        P(vp)]),
     "airan_pusch_chain_Destroy": (None, [vp]),
     "airan_pusch_chain_Enqueue": (i32, [vp, vp, u32, vp]),
+    "airan_pusch_chain_EnqueueSlots": (i32, [vp, vp, u32, vp, vp]),
+    "airan_pusch_chain_SlotsPerTb": (u32, [vp]),
     "airan_pusch_chain_Outputs": (i32, [vp, P(AiranPuschChainOutputsV1)]),
     "airan_pusch_chain_DeviceBytes": (C.c_size_t, [vp]),
     "airan_pusch_chain_SetTiming": (i32, [vp, u32]),

@@ -38,17 +38,27 @@ class OpChain:
         self.g = ops.Geometry(cfg)
         self.filter = ops.LmmseFilter(cfg, tau_s=self.opts.tau_s, fd_hz=self.opts.fd_hz, scs_hz=self.opts.scs_hz,
                                       max_rank=self.opts.ce_rank)
-        self.dmrs_re, self.dmrs_im = ops.dmrs_gen(cfg)
+        self._dmrs = {cfg.slot_number: ops.dmrs_gen(cfg)}
         self.active_rows = ops.ldpc_active_rows(plan)
         self.no_eff_floor = float(abi.load_library().airan_pusch_NoEffFloor(cfg.qm))
 
-    def run_slot(self, iq: torch.Tensor, keep: bool = False):
+    def dmrs(self, slot_number: int):
+        """DMRS reference of a slot number (the rest of the config fixed)."""
+        if slot_number not in self._dmrs:
+            c = type(self.cfg).from_buffer_copy(self.cfg)
+            c.slot_number = slot_number
+            self._dmrs[slot_number] = ops.dmrs_gen(c)
+        return self._dmrs[slot_number]
+
+    def run_slot(self, iq: torch.Tensor, keep: bool = False, slot_number: int | None = None):
         """iq: one slot, [NR, samples, 2] on the device.  Returns a dict with
-        bits [C*K] int8, cb_ok, tb_ok and (keep=True) the intermediates."""
+        bits [C*K] int8, cb_ok, tb_ok and (keep=True) the intermediates.
+        slot_number: the slot's number (its DMRS), default the config's."""
         cfg, g, plan, o = self.cfg, self.g, self.plan, self.opts
         out = {}
+        dmrs_re, dmrs_im = self.dmrs(cfg.slot_number if slot_number is None else slot_number)
         rx_re, rx_im = ops.ofdm_demod(cfg, iq, g.nr, o.iq_format, fused_redemap=True)
-        h_re, h_im, sc, cnt, noise = ops.dmrs_ls(cfg, rx_re, rx_im, self.dmrs_re, self.dmrs_im)
+        h_re, h_im, sc, cnt, noise = ops.dmrs_ls(cfg, rx_re, rx_im, dmrs_re, dmrs_im)
         hg_re, hg_im, _, noise_ce = ops.ce_lmmse(cfg, self.filter, h_re, h_im, sc, cnt, noise)
         hrm_r, hrm_i, yv_r, yv_i, no = ops.detect_io_pack(cfg, rx_re, rx_im, hg_re, hg_im, noise_ce)
         xr, xi, ne = ops.detect_lmmse(cfg, hrm_r, hrm_i, yv_r, yv_i, no)
@@ -113,10 +123,25 @@ class NativeChain:
                                                                     C.c_void_p(ops.stream_ptr()),
                                                                     C.byref(self.handle)))
 
-    def enqueue(self, iq: torch.Tensor, num_slots: int):
-        ops._check("chain Enqueue", self.lib.airan_pusch_chain_Enqueue(self.handle, self._C.c_void_p(iq.data_ptr()),
-                                                                      num_slots,
-                                                                      self._C.c_void_p(ops.stream_ptr())))
+    def enqueue(self, iq: torch.Tensor, num_slots: int, slot_numbers: torch.Tensor | None = None):
+        """slot_numbers: device int16 [num_slots] (uint16 values), the slot
+        number of each slot; None uses the config's for every slot.  With a
+        HIP graph the tensor's storage is captured: rewrite it in place."""
+        C = self._C
+        if slot_numbers is None:
+            rc = self.lib.airan_pusch_chain_Enqueue(self.handle, C.c_void_p(iq.data_ptr()), num_slots,
+                                                    C.c_void_p(ops.stream_ptr()))
+        else:
+            if slot_numbers.dtype != torch.int16 or not slot_numbers.is_cuda or slot_numbers.numel() < num_slots:
+                raise ValueError("slot_numbers: a device int16 tensor with one entry per slot")
+            rc = self.lib.airan_pusch_chain_EnqueueSlots(self.handle, C.c_void_p(iq.data_ptr()), num_slots,
+                                                         C.c_void_p(slot_numbers.data_ptr()),
+                                                         C.c_void_p(ops.stream_ptr()))
+        ops._check("chain Enqueue", rc)
+
+    @property
+    def slots_per_tb(self) -> int:
+        return int(self.lib.airan_pusch_chain_SlotsPerTb(self.handle))
 
     def outputs(self) -> abi.AiranPuschChainOutputsV1:
         o = abi.AiranPuschChainOutputsV1()
@@ -127,6 +152,8 @@ class NativeChain:
     def fetch(self, name: str, dtype, count: int) -> torch.Tensor:
         """Copy `count` elements of output buffer `name` into a new tensor."""
         src = getattr(self.outputs(), name)
+        if not src:
+            raise ValueError(f"chain output {name} is not produced for this plan")
         out = torch.empty(count, dtype=dtype, device="cuda")
         nbytes = count * out.element_size()
         rc = _hip().hipMemcpyAsync(out.data_ptr(), src, nbytes, 3, ops.stream_ptr())

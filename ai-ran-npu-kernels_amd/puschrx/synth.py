@@ -61,20 +61,28 @@ def random_payload(rng: np.random.Generator, num_tbs: int, tbs: int) -> np.ndarr
     return rng.integers(0, 2, size=(num_tbs, tbs), dtype=np.uint8)
 
 
-def tx_grid(cfg, plan, payload: np.ndarray, slots_per_tb: int = 1, want_coded: bool = False):
+def tx_grid(cfg, plan, payload: np.ndarray, slots_per_tb: int = 1, want_coded: bool = False,
+            slot_numbers=None):
     """Resource grid complex64 [slots, L, 14, used] (and the scrambled coded
-    bits [num_tbs, G] when asked)."""
+    bits [num_tbs, G] when asked).  slot_numbers: the slot number (DMRS) of
+    each slot, default cfg.slot_number for all."""
     lib = load_library()
     g = Geometry.of(cfg)
     num_tbs = payload.shape[0]
     payload = np.ascontiguousarray(payload, dtype=np.uint8)
     grid = np.zeros((num_tbs * slots_per_tb, g.nl, 14, g.used), np.complex64)
     coded = np.zeros((num_tbs, plan.G), np.uint8) if want_coded else None
-    st = lib.airan_pusch_tx_EncodeTbs(C.byref(cfg), C.byref(plan), num_tbs, slots_per_tb,
-                                      payload.ctypes.data_as(C.c_void_p), grid.ctypes.data_as(C.c_void_p),
-                                      None if coded is None else coded.ctypes.data_as(C.c_void_p))
+    sn = None
+    if slot_numbers is not None:
+        sn = np.ascontiguousarray(slot_numbers, dtype=np.uint16)
+        if sn.shape != (num_tbs * slots_per_tb,):
+            raise ValueError("slot_numbers needs one entry per slot")
+    st = lib.airan_pusch_tx_EncodeTbsSlots(C.byref(cfg), C.byref(plan), num_tbs, slots_per_tb,
+                                           None if sn is None else sn.ctypes.data_as(C.c_void_p),
+                                           payload.ctypes.data_as(C.c_void_p), grid.ctypes.data_as(C.c_void_p),
+                                           None if coded is None else coded.ctypes.data_as(C.c_void_p))
     if st != abi.OK:
-        raise RuntimeError(f"airan_pusch_tx_EncodeTbs failed with status {st}")
+        raise RuntimeError(f"airan_pusch_tx_EncodeTbsSlots failed with status {st}")
     return (grid, coded) if want_coded else grid
 
 

@@ -5,8 +5,9 @@
  * (Run: per slot OFDM demod -> RE demap -> DMRS gen -> DMRS LS -> LMMSE CE ->
  * detector I/O pack -> detection -> QAM demap -> layer demap, then per TB
  * descramble -> rate dematch -> LDPC) for grants of one shape: `num_slots`
- * slots, one transport block per slot, run as one pipeline of batched
- * kernels that a HIP graph replays:
+ * slots carrying transport blocks of T consecutive slots each (T = 1: one TB
+ * per slot; the reference's profile: T = 23, C = 143), run as one pipeline
+ * of batched kernels that a HIP graph replays:
  *
  *   OFDM demod + RE demap     airan_ofdm_demod_redemap_Enqueue, all slots
  *   DMRS LS + noise           the mimo_dmrs_ls kernels, all slots
@@ -21,6 +22,16 @@
  *                             the rate_dematch_mimo kernel then
  *                             airan_ldpc_decode_Enqueue, all code blocks
  *   CRC                       airan_crc_check_Enqueue, all transport blocks
+ *                             (skipped for a plan without CRC, such as the
+ *                             reference's fixed plan)
+ *
+ * Each slot's DMRS is that of its own slot number (c_init depends on it,
+ * mimo_dmrs_gen.cpp:38-45; the reference re-derives its config per slot,
+ * rx_chain.cpp:464): EnqueueSlots takes the slot number of each slot and the
+ * pipeline generates each slot's DMRS from it.  The descrambling
+ * sequence restarts with each TB and runs on across its slots
+ * (descramble_mimo.cpp:176-184); code blocks may span slot boundaries
+ * (rate_dematch_mimo.cpp:93-106).
  *
  * Every stage computes exactly what the per-operator entry points compute
  * (the fused stage is bit-identical to detect_io_pack + detect_lmmse +
@@ -57,17 +68,18 @@ typedef struct AiranPuschChainOptionsV1 {
 
 typedef struct AiranPuschChain AiranPuschChain;
 
-/* Device buffers of the last Enqueue (valid until the next one), slot-major:
- * bits int8 [slots][C][K], tb_ok / cb_ok / iterations uint8, and the
- * intermediates in the operator layouts for inspection. */
+/* Device buffers of the last Enqueue (valid until the next one): bits int8
+ * [TBs][C][K], tb_ok / cb_ok / iterations uint8 (TB-major; tb_ok and cb_ok
+ * are NULL when the plan has no CRC), and the slot-major intermediates in
+ * the operator layouts for inspection. */
 typedef struct AiranPuschChainOutputsV1 {
     uint16_t abi_version;
     uint16_t struct_size;
     uint32_t num_slots;
-    const void *bits;           /* int8 [slots][C][K] */
-    const void *tb_ok;          /* uint8 [slots] */
-    const void *cb_ok;          /* uint8 [slots][C] */
-    const void *iterations;     /* uint8 [slots][C] */
+    const void *bits;           /* int8 [TBs][C][K] */
+    const void *tb_ok;          /* uint8 [TBs], NULL without CRC */
+    const void *cb_ok;          /* uint8 [TBs][C], NULL without CRC */
+    const void *iterations;     /* uint8 [TBs][C] */
     const void *rx_grid_re;     /* fp16 [slots][NR][14][padded] */
     const void *rx_grid_im;
     const void *h_ls_re;        /* fp16 [slots][NR][L][D][pilot_pad] */
@@ -77,23 +89,32 @@ typedef struct AiranPuschChainOutputsV1 {
     const void *h_re;           /* fp16 [slots][NR][L][14][padded] */
     const void *h_im;
     const void *cw_llr;         /* int16 [slots][Qm][codeword_stride], descrambled */
-    const void *ldpc_llr;       /* int16 [slots][C][68 Z]; NULL when the decoder reads cw_llr directly
+    const void *ldpc_llr;       /* int16 [TBs][C][68 Z]; NULL when the decoder reads cw_llr directly
                                    (FP16 messages and at most 8 active rows) */
 } AiranPuschChainOutputsV1;
 
-/* Builds the CE filter, the DMRS reference, the Gold bits and every buffer
- * for max_slots slots, then runs the pipeline once on a zero slot (loading
- * the per-device tables) and synchronises `stream`.  plan->G must be one
- * slot's coded bits (codeword_symbols * Qm). */
+/* Builds the CE filter, the Gold bits and every buffer for max_slots slots,
+ * then runs the pipeline once on zero slots (loading the per-device tables)
+ * and synchronises `stream`.  plan->G must be T slots' coded bits
+ * (T * codeword_symbols * Qm, T >= 1) and max_slots a multiple of T. */
 int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPlanV1 *plan,
                                  const AiranPuschChainOptionsV1 *options, void *stream,
                                  AiranPuschChain **chain);
 void airan_pusch_chain_Destroy(AiranPuschChain *chain);
-/* iq: device [num_slots][NR][samples][2] (int16 Q8.8 or float per iq_format).
- * Asynchronous on `stream`; with use_graph, the first call for a given
- * (iq, num_slots, stream) captures the graph. */
+/* iq: device [num_slots][NR][samples][2] (int16 Q8.8 or float per iq_format);
+ * num_slots a multiple of T.  slot_numbers: device uint16 [num_slots], the
+ * slot number of each slot (the DMRS c_init term of mimo_dmrs_gen.cpp:38-45),
+ * read by the DMRS stage when the pipeline runs; the caller owns the buffer
+ * and may rewrite it between calls.  Asynchronous on `stream`;
+ * with use_graph, the first call for a given (iq, slot_numbers, num_slots,
+ * stream) captures the graph. */
+int32_t airan_pusch_chain_EnqueueSlots(AiranPuschChain *chain, const void *iq, uint32_t num_slots,
+                                       const uint16_t *slot_numbers, void *stream);
+/* EnqueueSlots with config->slot_number for every slot. */
 int32_t airan_pusch_chain_Enqueue(AiranPuschChain *chain, const void *iq, uint32_t num_slots,
                                   void *stream);
+/* T, the slots per transport block (0 on a bad argument). */
+uint32_t airan_pusch_chain_SlotsPerTb(const AiranPuschChain *chain);
 int32_t airan_pusch_chain_Outputs(const AiranPuschChain *chain, AiranPuschChainOutputsV1 *out);
 /* Device bytes the chain holds for max_slots (0 on a bad argument). */
 size_t airan_pusch_chain_DeviceBytes(const AiranPuschChain *chain);
@@ -101,7 +122,10 @@ size_t airan_pusch_chain_DeviceBytes(const AiranPuschChain *chain);
  * after the LMMSE CE, every slot whose SNR estimate 10 log10(1 / mean noise
  * of the CE) lies in [snr_min_db, snr_max_db] gets the refiner's channel
  * estimate instead; the others keep LMMSE.  The refiner must hold at least
- * as many slots as each Enqueue; NULL detaches it.  Drops the cached graph. */
+ * as many slots as each Enqueue; NULL detaches it.  Drops the cached graph.
+ * The refiner's activations live in the refiner handle: a refiner attached
+ * to two chains must not be used by both at once (one refiner per chain for
+ * chains on concurrent streams). */
 int32_t airan_pusch_chain_SetRefiner(AiranPuschChain *chain, AiranCeRefiner *refiner, float snr_min_db,
                                      float snr_max_db);
 

@@ -123,6 +123,14 @@ int32_t airan_re_demap_batch_ValidateOpArgs(const ReDemapBatchOpArgsV1 *args);
  * fixed.  index_elems >= padded_subcarriers; entries are fp16 byte offsets. */
 int32_t airan_re_demap_batch_BuildGatherIndex(const PuschMimoConfig *config, uint32_t *index,
                                               size_t index_elems);
+/* mapping/re_demap_batch/re_demap_batch.h:69-76 (Reference): bit-exact host
+ * gather of the used subcarriers through the byte-offset table, zero tail.
+ * fft grids [NR][14][fft], rx grids [NR][14][padded] (fp16 bits); the
+ * config argument supplies the geometry the reference fixes at compile time. */
+int32_t airan_re_demap_batch_Reference(const PuschMimoConfig *config, const uint16_t *fft_grid_re,
+                                       const uint16_t *fft_grid_im, uint32_t num_rx_antennas,
+                                       const uint32_t *gather_index, uint16_t *rx_grid_re,
+                                       uint16_t *rx_grid_im);
 int32_t airan_re_demap_batch_Enqueue(const ReDemapBatchOpArgsV1 *args, const void *gather_index,
                                      size_t gather_index_bytes, void *workspace,
                                      size_t workspace_bytes, const void *tiling,
@@ -215,6 +223,26 @@ int32_t airan_mimo_dmrs_ls_BuildCurrentProfile(const PuschMimoConfig *config,
                                                uint16_t *pilot_sc /* [L,D,pilot_pad] */);
 int32_t airan_mimo_dmrs_ls_ValidateOpArgs(const MimoDmrsLsOpArgsV1 *args);
 int32_t airan_mimo_dmrs_ls_Enqueue(const MimoDmrsLsOpArgsV1 *args);
+/* channel_est/mimo_dmrs_ls/mimo_dmrs_ls.h:118-142: the per-layer observation
+ * contract of the DMRS-LS outputs.  pilot_count [L][D], pilot_sc
+ * [L][D][pilot_pad] as BuildCurrentProfile writes them.  The reference's
+ * pilot counts are those of its 133-PRB profile (798 comb / 399 FD-OCC2
+ * observations, stride 832); here they follow `config` (used / 2 and
+ * used / 4, stride pilot_pad), or the reference's values for config NULL.
+ * Models: AIRAN_OBS_COMB2 / AIRAN_OBS_FD_OCC2 (the reference's COMB2_798 /
+ * FD_OCC2_399, same values). */
+#define AIRAN_OBS_COMB2 0u
+#define AIRAN_OBS_FD_OCC2 1u
+int32_t airan_mimo_dmrs_ls_DescribeObservationModels(const PuschMimoConfig *config, const uint16_t *pilot_count,
+                                                     uint32_t num_layers, uint32_t num_dmrs_symbols,
+                                                     uint32_t *models);
+int32_t airan_mimo_dmrs_ls_ValidateNaturalLmmseContract(const PuschMimoConfig *config, const uint16_t *pilot_count,
+                                                        const uint16_t *pilot_sc, uint32_t num_layers,
+                                                        uint32_t num_dmrs_symbols);
+/* AIRAN_LAYOUT_MISMATCH (-3, the reference's LMMSE_OBSERVATION_MODEL_MISMATCH)
+ * unless every layer is on the comb model. */
+int32_t airan_mimo_dmrs_ls_ValidateLmmse798Compatibility(const PuschMimoConfig *config, const uint16_t *pilot_count,
+                                                         uint32_t num_layers, uint32_t num_dmrs_symbols);
 
 /* ============================================================= CE LMMSE */
 /* channel_est/channel_est_lmmse_mimo/channel_est_lmmse.h:127-171.  The
@@ -422,6 +450,12 @@ int32_t airan_layer_demap_BuildCurrentProfile(const PuschMimoConfig *config,
                                               LayerDemapKernelMetadata *metadata);
 int32_t airan_layer_demap_ValidateOpArgs(const LayerDemapOpArgsV1 *args);
 int32_t airan_layer_demap_Enqueue(const LayerDemapOpArgsV1 *args);
+/* mapping/layer_demap_mimo/layer_demap.h:104-109 (ReferenceDemap): bit-exact
+ * host layer demap, layer LLRs [L][Qm][14-D][qam_row_stride] ->
+ * codeword [Qm][codeword_stride], tails zero; AIRAN_LAYOUT_MISMATCH when
+ * `layout` is not the config's. */
+int32_t airan_layer_demap_ReferenceDemap(const int16_t *layer_llr, const PuschMimoConfig *config,
+                                         const PuschMimoLayout *layout, int16_t *cw_llr);
 
 /* ============================================================ descramble */
 /* fec/descramble_mimo/descramble_mimo.h:48-113.  sign may be NULL: the Gold
@@ -467,6 +501,14 @@ int32_t airan_descramble_mimo_Enqueue(const DescrambleMimoOpArgsV1 *args, const 
                                       size_t sign_bytes, void *workspace,
                                       size_t workspace_bytes, const void *tiling,
                                       size_t tiling_bytes);
+/* fec/descramble_mimo/descramble_mimo.h:92-98 (ReferenceDescramble): bit-exact
+ * host descramble of num_slots slots [slots][Qm][codeword_stride], one Gold
+ * sequence continuing across the slots; sign is BuildGoldSign's table, or
+ * NULL to generate the same sequence here.  Tails zero. */
+int32_t airan_descramble_mimo_ReferenceDescramble(const int16_t *llr_qam, const int16_t *sign,
+                                                  const PuschMimoConfig *config,
+                                                  const PuschMimoLayout *layout, uint32_t num_slots,
+                                                  int16_t *llr_nr);
 
 /* ========================================================= TB / CB plan */
 /* TS 38.214 5.1.3.2 and TS 38.212 5.1-5.4.  The reference has no TB layer:
@@ -571,6 +613,21 @@ int32_t airan_rate_dematch_mimo_Enqueue(const RateDematchMimoOpArgsV1 *args,
                                         const void *descriptors, size_t descriptor_bytes,
                                         void *workspace, size_t workspace_bytes,
                                         const void *tiling, size_t tiling_bytes);
+/* fec/rate_dematch_mimo/rate_dematch_mimo.h:112-120 (ReferenceRateDematch):
+ * bit-exact host rate dematch of the reference's fixed plan (descriptors as
+ * BuildCurrentProfile gives them, else AIRAN_LAYOUT_MISMATCH) into
+ * ldpc_llr [C][68 Z]. */
+int32_t airan_rate_dematch_mimo_ReferenceRateDematch(const int16_t *cw_llr, const PuschMimoConfig *config,
+                                                     const PuschMimoLayout *layout, uint32_t num_slots,
+                                                     const RateMatchDescriptor *descriptors,
+                                                     size_t descriptor_count, int16_t *ldpc_llr);
+/* The same for a TB plan (filler bits +5120, puncturing, the descriptors of
+ * BuildProfileForTb); no counterpart in the reference, whose plan is fixed. */
+int32_t airan_rate_dematch_mimo_ReferenceRateDematchForTb(const int16_t *cw_llr, const PuschMimoConfig *config,
+                                                          const PuschMimoLayout *layout, uint32_t num_slots,
+                                                          const AiranTbPlanV1 *plan,
+                                                          const RateMatchDescriptor *descriptors,
+                                                          size_t descriptor_count, int16_t *ldpc_llr);
 
 /* ================================================================= LDPC */
 /* Replaces the raw launch ldpc_decode_kernel(lam_in, packed_bc, packed_s,

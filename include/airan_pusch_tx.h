@@ -36,6 +36,13 @@ extern "C" {
 int32_t airan_pusch_tx_EncodeTbs(const PuschMimoConfig *config, const AiranTbPlanV1 *plan,
                                  uint32_t num_tbs, uint32_t slots_per_tb, const uint8_t *payload,
                                  float *grid, uint8_t *coded_bits);
+/* As EncodeTbs, slot s of the batch (TB s / slots_per_tb) carrying the DMRS
+ * of slot number slot_numbers[s] (host uint16 [num_tbs * slots_per_tb];
+ * NULL: config->slot_number for every slot). */
+int32_t airan_pusch_tx_EncodeTbsSlots(const PuschMimoConfig *config, const AiranTbPlanV1 *plan,
+                                      uint32_t num_tbs, uint32_t slots_per_tb,
+                                      const uint16_t *slot_numbers, const uint8_t *payload,
+                                      float *grid, uint8_t *coded_bits);
 
 /* LDPC base graph 1 systematic encoder: info [22Z] bits -> codeword [68Z]
  * bits (columns 0..67; filler positions must be 0 in info). */

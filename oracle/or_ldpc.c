@@ -9,13 +9,17 @@
  *   min1/min2 over |ext_k|, sign product over sgn(ext_k) (sgn(0) = +1)
  *   msg_k      = alpha * (prod / sgn_k) * (|ext_k| == min1 ? min2 : min1)
  *   lam[col_k][(i + sh_k) mod Z] = ext_k + msg_k
- * then the posterior is clamped to +-5120 and bit = (lam < 0)
- * (:719-780).  alpha = 0.75 (ldpc_decode.h:50).
+ * then the posterior is clamped to +-5120, converted to int16 with CAST_RINT
+ * (:735, the exported posterior), converted back and bit = (value < 0)
+ * (:753-767), so a posterior in [-0.5, 0) decodes to 0.  alpha = 0.75
+ * (ldpc_decode.h:50).
  *
  * MODE_REF restates the reference arithmetic: fp16 posterior without
- * per-row saturation (:659-661), messages stored as int8 (:666, the
- * conversion is taken as round-half-even with saturation to +-127, which the
- * reference source does not pin).
+ * per-row saturation (:659-661), messages stored as int8 (:666).  Two
+ * roundings of the reference are not fixed by its source and are choices
+ * here ("parity unpinned"): MulCast<int8_t, half> (:666) is taken as
+ * round-half-even with saturation to +-127 (PREV_CLIP_FX, ldpc_decode.h:59),
+ * and MulAddDst (:657) as two fp16 roundings (the product, then the sum).
  * MODE_FP16 is the build's decoder: the same fp16 arithmetic in LLR units
  * (input / 256) with fp16 messages.  An integer posterior saturated at int16
  * range drifts on high-degree columns (increases are clipped, decreases are
@@ -109,12 +113,24 @@ int or_ldpc_active_rows(uint32_t Z, uint32_t Kprime, uint32_t E_max, uint32_t k0
 /* ---------------------------------------------------------------- decode */
 static inline float h16(float v) { return or_h2f(or_f2h(v)); }
 
+/* Hard decision.  MODE_REF: rint(clamp(v, +-5120)) < 0, the reference's
+ * int16 round trip (ldpc_decode_kernel.cpp:735-762); a NaN gives 0.
+ * MODE_FP16 (no reference counterpart): v < 0. */
+static inline int hard_bit(float v, int mode)
+{
+    if (mode == OR_LDPC_MODE_REF) {
+        float c = v > 5120.0f ? 5120.0f : (v < -5120.0f ? -5120.0f : v);
+        return nearbyintf(c) < 0.0f;
+    }
+    return v < 0.0f;
+}
+
 /* Both modes keep an fp16 posterior and fp16 arithmetic, one rounding per
  * operation.  MODE_REF: posterior in the input Q8.8 units, messages stored as
  * int8 (round-half-even, saturated to +-127).  MODE_FP16: posterior in LLR
  * units (input / 256, exact scaling), messages stored as fp16. */
 static void decode_one(uint32_t Z, int16_t *lam_io, int mode, int iters, int rows, uint32_t K,
-                       int early_stop, int8_t *bits, uint8_t *iters_out)
+                       int early_stop, int8_t *bits, uint8_t *iters_out, float *post_out)
 {
     const uint32_t NV = 68 * Z;
     const int E = AIRAN_BG1_ROW_START[rows];
@@ -165,7 +181,7 @@ static void decode_one(uint32_t Z, int16_t *lam_io, int mode, int iters, int row
                 for (uint32_t i = 0; i < Z && ok; ++i) {
                     int par = 0;
                     for (int e = AIRAN_BG1_ROW_START[r]; e < AIRAN_BG1_ROW_START[r + 1]; ++e)
-                        par ^= lam[AIRAN_BG1_COL[e] * Z + (i + edge_shift(e, Z)) % Z] < 0.0f;
+                        par ^= hard_bit(lam[AIRAN_BG1_COL[e] * Z + (i + edge_shift(e, Z)) % Z], mode);
                     ok &= par == 0;
                 }
             if (ok) break;
@@ -175,20 +191,29 @@ static void decode_one(uint32_t Z, int16_t *lam_io, int mode, int iters, int row
     const float clip = mode == OR_LDPC_MODE_REF ? 5120.0f : 20.0f;
     for (uint32_t k = 0; k < NV; ++k) {
         float v = lam[k];
+        if (post_out) post_out[k] = v;
+        if (k < K) bits[k] = (int8_t)hard_bit(v, mode);
         if (v > clip) v = clip;
         if (v < -clip) v = -clip;
         lam_io[k] = (int16_t)nearbyintf(v / in_scale);
-        if (k < K) bits[k] = v < 0.0f ? 1 : 0;
     }
     free(lam);
     free(msg);
 }
 
-void or_ldpc_decode(uint32_t Z, uint32_t n_cb, int16_t *lam, int mode, int iterations,
-                    int active_rows, uint32_t K, int early_stop, int8_t *bits, uint8_t *iters_out)
+void or_ldpc_decode_posterior(uint32_t Z, uint32_t n_cb, int16_t *lam, int mode, int iterations,
+                              int active_rows, uint32_t K, int early_stop, int8_t *bits, uint8_t *iters_out,
+                              float *posterior)
 {
 #pragma omp parallel for schedule(dynamic, 1)
     for (int cb = 0; cb < (int)n_cb; ++cb)
         decode_one(Z, lam + (size_t)cb * 68 * Z, mode, iterations, active_rows, K, early_stop,
-                   bits + (size_t)cb * K, iters_out ? iters_out + cb : NULL);
+                   bits + (size_t)cb * K, iters_out ? iters_out + cb : NULL,
+                   posterior ? posterior + (size_t)cb * 68 * Z : NULL);
+}
+
+void or_ldpc_decode(uint32_t Z, uint32_t n_cb, int16_t *lam, int mode, int iterations,
+                    int active_rows, uint32_t K, int early_stop, int8_t *bits, uint8_t *iters_out)
+{
+    or_ldpc_decode_posterior(Z, n_cb, lam, mode, iterations, active_rows, K, early_stop, bits, iters_out, NULL);
 }

@@ -24,7 +24,11 @@
  *   or_ldpc_decode             pinned at outcome level only (encode -> decode
  *                              round trip, syndrome); the reference's fp16/int8
  *                              message trajectory has no host oracle anywhere
- *                              ("parity unpinned" below the outcome level)
+ *                              ("parity unpinned" below the outcome level:
+ *                              the int8 message rounding of MulCast and the
+ *                              rounding count of MulAddDst are choices).  The
+ *                              REF hard decision follows the reference's
+ *                              int16 CAST_RINT round trip.
  */
 #ifndef AIRAN_ORACLE_H
 #define AIRAN_ORACLE_H
@@ -146,6 +150,11 @@ int or_ldpc_active_rows(uint32_t Z, uint32_t Kprime, uint32_t E_max, uint32_t k0
 void or_ldpc_decode(uint32_t Z, uint32_t n_cb, int16_t *lam /* in/out [n_cb,68Z] */,
                     int mode, int iterations, int active_rows, uint32_t K, int early_stop,
                     int8_t *bits /* [n_cb, K] */, uint8_t *iters_out /* [n_cb] or NULL */);
+/* As or_ldpc_decode, also writing the final fp16 posterior (before the
+ * clamp and the int16 conversion) as float [n_cb, 68Z]. */
+void or_ldpc_decode_posterior(uint32_t Z, uint32_t n_cb, int16_t *lam, int mode, int iterations,
+                              int active_rows, uint32_t K, int early_stop, int8_t *bits,
+                              uint8_t *iters_out, float *posterior);
 void or_ldpc_encode(uint32_t Z, const uint8_t *info /* 22Z */, uint8_t *cw /* 68Z */);
 int or_ldpc_syndrome_ok(uint32_t Z, const uint8_t *cw);
 

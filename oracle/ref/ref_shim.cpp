@@ -77,6 +77,24 @@ int ref_dmrs_ls_profile(const void *cfg, void *layout, uint16_t *count, uint16_t
                                                     static_cast<airan::PuschMimoLayout *>(layout), &m, count, sc);
 }
 
+int ref_dmrs_ls_describe(const uint16_t *count, unsigned layers, unsigned dmrs, uint32_t *models)
+{
+    airan::mimo_dmrs_ls::ObservationModel m[airan::mimo_dmrs_ls::MAX_LAYERS] = {};
+    const int st = airan::mimo_dmrs_ls::DescribeObservationModels(count, layers, dmrs, m);
+    for (unsigned l = 0; l < layers && l < airan::mimo_dmrs_ls::MAX_LAYERS; ++l) models[l] = m[l];
+    return st;
+}
+
+int ref_dmrs_ls_validate_natural(const uint16_t *count, const uint16_t *sc, unsigned layers, unsigned dmrs)
+{
+    return airan::mimo_dmrs_ls::ValidateNaturalLmmseContract(count, sc, layers, dmrs);
+}
+
+int ref_dmrs_ls_validate_798(const uint16_t *count, unsigned layers, unsigned dmrs)
+{
+    return airan::mimo_dmrs_ls::ValidateLmmse798Compatibility(count, layers, dmrs);
+}
+
 unsigned long long ref_pilot_sc_hash(const uint16_t *sc, unsigned count)
 {
     return airan::channel_est_lmmse::PilotScHash(sc, count);

@@ -58,6 +58,7 @@ def load_oracle():
     lib.or_rate_dematch.argtypes = [C.POINTER(OrTbPlan), u32, u32, u32, u32, u32, vp, vp]
     lib.or_ldpc_active_rows.argtypes = [u32, u32, u32, u32, u32]
     lib.or_ldpc_decode.argtypes = [u32, u32, vp, C.c_int, C.c_int, C.c_int, u32, C.c_int, vp, vp]
+    lib.or_ldpc_decode_posterior.argtypes = [u32, u32, vp, C.c_int, C.c_int, C.c_int, u32, C.c_int, vp, vp, vp]
     lib.or_ldpc_encode.argtypes = [u32, vp, vp]
     lib.or_ldpc_syndrome_ok.argtypes = [u32, vp]
     lib.or_crc.restype = u32
@@ -90,6 +91,9 @@ def load_ref():
     lib.ref_dmrs_gen_profile.argtypes = [vp, vp, vp]
     lib.ref_dmrs_gold_basis.argtypes = [vp, vp]
     lib.ref_dmrs_ls_profile.argtypes = [vp, vp, vp, vp]
+    lib.ref_dmrs_ls_describe.argtypes = [vp, C.c_uint, C.c_uint, vp]
+    lib.ref_dmrs_ls_validate_natural.argtypes = [vp, vp, C.c_uint, C.c_uint]
+    lib.ref_dmrs_ls_validate_798.argtypes = [vp, C.c_uint, C.c_uint]
     lib.ref_pilot_sc_hash.restype = C.c_ulonglong
     lib.ref_pilot_sc_hash.argtypes = [vp, C.c_uint]
     return lib

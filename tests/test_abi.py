@@ -141,3 +141,36 @@ def test_ops_reject_bad_arguments(lib):
     cfg2.num_rb = 300
     a.abi_version, a.config = 1, C.pointer(cfg2)
     assert lib.airan_layer_demap_ValidateOpArgs(C.byref(a)) == abi.UNSUPPORTED_PROFILE
+
+
+@pytest.mark.parametrize("prof", [P.FULL_CARRIER_4RX, P.REFERENCE_FD16X64], ids=lambda p: p.name)
+def test_compiled_cpp_caller(lib, tmp_path, prof):
+    """tests/abi_caller.cpp, compiled with g++ against include/*.h and linked
+    to the library (as a host C++ chain would bind it), drives the host-side
+    contract functions; its numbers equal the ctypes reading of the same
+    library."""
+    exe = tmp_path / "abi_caller"
+    libdir = ROOT / "ai-ran-npu-kernels_amd" / "lib"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", f"-I{ROOT / 'include'}",
+                    str(ROOT / "tests" / "abi_caller.cpp"), f"-L{libdir}", "-lpuschrx_hip",
+                    f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    rt = prof.runtime_config()
+    path = tmp_path / "runtime.bin"
+    path.write_bytes(bytes(rt))
+    out = subprocess.run([str(exe), str(path), str(prof.rx)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    vals = {}
+    for line in out.stdout.splitlines():
+        k, _, v = line.partition(" ")
+        vals[k] = v
+    assert vals["result"] == "OK"
+    cfg = prof.config()
+    lay = ops.layout(cfg)
+    assert int(vals["data_stride"]) == lay.data_stride and int(vals["codeword_stride"]) == lay.codeword_stride
+    g = ops.Geometry(cfg)
+    tbs = ops.tbs(cfg.num_rb, 12 * g.ndata, g.qm, g.nl, 873 / 1024)
+    plan = ops.tb_plan(tbs, g.cw_symbols * g.qm, g.qm, g.nl)
+    assert (int(vals["tbs"]), int(vals["C"]), int(vals["Z"]), int(vals["E_low"]), int(vals["E_high"])) == \
+        (plan.tbs, plan.C, plan.Z, plan.E_low, plan.E_high)
+    assert int(vals["active_rows"]) == lib.airan_ldpc_ActiveRows(C.byref(plan))
+    assert int(vals["bad_rnti"].split()[0]) != 0

@@ -244,3 +244,63 @@ def test_native_chains_on_two_streams_match_serial(gpu_stream):
                 assert np.array_equal(a, b), (rep, j, n[0])
     for c in chains:
         c.close()
+
+
+def test_native_chain_per_slot_dmrs(gpu_stream):
+    """20 consecutive slots with slot numbers 0..19, each carrying the DMRS
+    of its own number (c_init depends on it, mimo_dmrs_gen.cpp:38-45): the
+    batched chain decodes every slot and matches the op chain run per slot
+    with that slot's number.  Negative control: enqueued with the config's
+    slot number for all, every slot but slot 0 fails its CRC.  The slot
+    numbers are read when the graph replays: rewriting the buffer in place
+    switches between the two outcomes."""
+    from puschrx.chain import NativeChain
+    prof = P.MID_4RX_RANK2
+    cfg, g, plan = plan_for(prof)
+    assert cfg.slot_number == 0
+    rng = np.random.default_rng(51)
+    S = 20
+    sn = np.arange(S, dtype=np.uint16)
+    spec = synth.ChannelSpec(taps=8, delay_spread_s=100e-9, doppler_hz=10.0, rx_corr=0.3, snr_db=30.0)
+    payload = synth.random_payload(rng, S, plan.tbs)
+    grid = synth.tx_grid(cfg, plan, payload, slot_numbers=sn)
+    H = synth.channel(rng, cfg, spec, S)
+    iq = synth.receive(rng, cfg, grid, H, spec.snr_db, iq_format=abi.IQ_COMPLEX64)
+    opts = ChainOptions(tau_s=spec.delay_spread_s, fd_hz=spec.doppler_hz, iq_format=abi.IQ_COMPLEX64)
+    nat = NativeChain(cfg, plan, max_slots=S, opts=opts, use_graph=True)
+    iq_dev = torch.from_numpy(iq).cuda()
+    sn_dev = torch.from_numpy(sn.astype(np.int16)).cuda()
+
+    def run():
+        nat.enqueue(iq_dev, S, sn_dev)
+        out = (nat.fetch("bits", torch.int8, S * plan.C * plan.K), nat.fetch("tb_ok", torch.uint8, S),
+               nat.fetch("cb_ok", torch.uint8, S * plan.C), nat.fetch("iterations", torch.uint8, S * plan.C))
+        torch.cuda.synchronize()
+        return [o.cpu().numpy() for o in out]
+
+    print("per-slot DMRS: slot numbers 0..19", flush=True)
+    bits, tb_ok, cb_ok, iters = run()
+    assert tb_ok.tolist() == [1] * S
+    op = OpChain(cfg, plan, opts)
+    for s in range(S):
+        r = op.run_slot(iq_dev[s].contiguous(), slot_number=s)
+        torch.cuda.synchronize()
+        assert np.array_equal(bits.reshape(S, -1)[s], r["bits"].cpu().numpy()), s
+        assert np.array_equal(iters.reshape(S, -1)[s], r["iterations"].cpu().numpy()), s
+        assert np.array_equal(cb_ok.reshape(S, -1)[s], r["cb_ok"].cpu().numpy()), s
+        assert np.array_equal(tb_bits(plan, torch.from_numpy(bits.reshape(S, -1)[s])), payload[s]), s
+    op.close()
+    print("per-slot DMRS: wrong slot numbers (all 0), same graph", flush=True)
+    sn_dev.zero_()
+    _, tb_bad, _, _ = run()
+    assert tb_bad[0] == 1 and not tb_bad[1:].any()
+    print("per-slot DMRS: slot numbers restored in place", flush=True)
+    sn_dev.copy_(torch.from_numpy(sn.astype(np.int16)).cuda())
+    bits2, tb2, _, _ = run()
+    assert tb2.tolist() == [1] * S and np.array_equal(bits2, bits)
+    # The plain Enqueue uses the config's slot number: the same as the control.
+    nat.enqueue(iq_dev, S)
+    tb3 = nat.fetch("tb_ok", torch.uint8, S)
+    torch.cuda.synchronize()
+    assert np.array_equal(tb3.cpu().numpy(), tb_bad)
+    nat.close()

@@ -220,6 +220,34 @@ def test_ldpc_bit_exact(oracle, gpu_stream, mode, rows, iters, early):
         assert np.array_equal(want_bits.reshape(ncb, -1), infos.astype(np.int8))
 
 
+def test_ldpc_ref_hard_decision_near_zero(oracle, gpu_stream):
+    """REF mode decides a bit on rint(clamp(posterior)) < 0, as the reference
+    does (ldpc_decode_kernel.cpp:735-767): inputs of a few Q8.8 units drive
+    thousands of posteriors into [-0.5, 0), which must decode to 0 on the
+    device exactly as in the oracle."""
+    Z, ncb = 384, 4
+    rng = np.random.default_rng(12)
+    lam = rng.integers(-3, 4, (ncb, 68 * Z)).astype(np.int16)
+    lam[:, :2 * Z] = 0
+    lam = lam.reshape(-1)
+    for rows, iters, early in ((46, 1, False), (46, 2, False), (8, 2, False), (46, 3, True)):
+        print(f"ref near-zero rows={rows} iters={iters} early={early}", flush=True)
+        want_lam = lam.copy()
+        want_bits = np.zeros(ncb * 22 * Z, np.int8)
+        want_it = np.zeros(ncb, np.uint8)
+        post = np.zeros(ncb * 68 * Z, np.float32)
+        oracle.or_ldpc_decode_posterior(Z, ncb, ptr(want_lam), abi.LDPC_MODE_REF, iters, rows, 22 * Z,
+                                        1 if early else 0, ptr(want_bits), ptr(want_it), ptr(post))
+        p = post.reshape(ncb, 68 * Z)[:, :22 * Z].reshape(-1)
+        near = (p >= -0.5) & (p < 0)
+        assert near.sum() > 1000 and not want_bits[near].any()
+        bits, lam_out, it = ops.ldpc_decode(dev(lam), ncb, Z, 22 * Z, iters, rows, abi.LDPC_MODE_REF, early,
+                                            want_lam=True)
+        assert np.array_equal(host(bits), want_bits)
+        assert np.array_equal(host(lam_out), want_lam)
+        assert np.array_equal(host(it), want_it)
+
+
 @pytest.mark.parametrize("rows,early", [(46, False), (6, False), (8, True), (4, True)])
 def test_ldpc_small_lifting_sizes(oracle, gpu_stream, rows, early):
     for Z in (24, 48, 96, 192):

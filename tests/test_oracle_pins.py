@@ -705,3 +705,34 @@ def test_ce_occ_pair_does_not_leak(oracle):
     e0 = np.sum(np.abs(H[0, 0, 0]) ** 2)
     e1 = np.sum(np.abs(H[0, 1, 0]) ** 2)
     assert e0 < 1e-3 * e1
+
+
+def _small_ref_llrs(Z, ncb, seed):
+    """Q8.8 inputs of magnitude <= 3 (a few thousandths of an LLR): after one
+    REF-mode row pass many posteriors are fractions such as -1 + 0.75."""
+    rng = np.random.default_rng(seed)
+    lam = rng.integers(-3, 4, (ncb, 68 * Z)).astype(np.int16)
+    lam[:, :2 * Z] = 0
+    return lam.reshape(-1)
+
+
+def test_ldpc_ref_bits_follow_the_int16_posterior(oracle):
+    """fec/ldpc_decode/ldpc_decode_kernel.cpp:735-767: the posterior is
+    clamped, converted to int16 with CAST_RINT (the exported lam_out),
+    converted back, and the bit is (value < 0).  So in REF mode every bit
+    equals lam_out < 0, and a posterior in [-0.5, 0) decodes to 0; the
+    inputs here put thousands of posteriors there."""
+    Z, ncb = 384, 3
+    lam = _small_ref_llrs(Z, ncb, 11)
+    for iters in (1, 2):
+        out = lam.copy()
+        bits = np.zeros(ncb * 22 * Z, np.int8)
+        post = np.zeros(ncb * 68 * Z, np.float32)
+        oracle.or_ldpc_decode_posterior(Z, ncb, ptr(out), 0, iters, 46, 22 * Z, 0, ptr(bits), None, ptr(post))
+        info_out = out.reshape(ncb, 68 * Z)[:, :22 * Z].reshape(-1)
+        assert np.array_equal(bits, (info_out < 0).astype(np.int8))
+        p = post.reshape(ncb, 68 * Z)[:, :22 * Z].reshape(-1)
+        near = (p >= -0.5) & (p < 0)
+        assert near.sum() > 1000, near.sum()
+        assert not bits[near].any()
+        assert np.array_equal(bits[p < -0.5], np.ones(int((p < -0.5).sum()), np.int8))

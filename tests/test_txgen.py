@@ -125,3 +125,34 @@ def test_oracle_chain_decodes_a_fading_slot(lib, oracle, prof):
     b = bits.reshape(op.C, op.K).astype(np.uint8)
     tb = np.concatenate([b[r, : op.Kprime - op.crc_cb] for r in range(op.C)])[: plan.tbs]
     assert np.array_equal(tb, payload[0])
+
+
+def test_tx_dmrs_follows_slot_numbers(lib, oracle):
+    """A two-slot transport block sent with slot numbers (5, 6): the data
+    resource elements are those of the same TB sent with the config's slot
+    number, and each slot's DMRS is the oracle's DMRS of its own slot number
+    (c_init depends on it, mimo_dmrs_gen.cpp:38-45)."""
+    prof = P.MID_4RX_RANK2
+    cfg = prof.config()
+    g = OracleGeom(cfg)
+    tbs = ops.tbs(prof.num_rb, 12 * g.ndata * 2, g.qm, g.nl, prof.code_rate)
+    plan = ops.tb_plan(tbs, 2 * g.cw_symbols * g.qm, g.qm, g.nl)
+    payload = synth.random_payload(np.random.default_rng(3), 1, plan.tbs)
+    base = synth.tx_grid(cfg, plan, payload, slots_per_tb=2)
+    moved = synth.tx_grid(cfg, plan, payload, slots_per_tb=2, slot_numbers=[5, 6])
+    data = [s for s in range(14) if s not in g.dmrs_sym]
+    assert np.array_equal(base[:, :, data], moved[:, :, data])
+    for k, sn in enumerate((5, 6)):
+        c = type(cfg).from_buffer_copy(cfg)
+        c.slot_number = sn
+        ref_re = np.zeros(g.nl * g.ndmrs * g.ref_pad, np.uint16)
+        ref_im = np.zeros_like(ref_re)
+        oracle.or_dmrs_gen(cref(c), ptr(ref_re), ptr(ref_im))
+        ref = (ref_re.view(np.float16).astype(np.float32) + 1j * ref_im.view(np.float16).astype(np.float32))
+        ref = ref.reshape(g.nl, g.ndmrs, g.ref_pad)
+        for l in range(g.nl):
+            comb = (cfg.dmrs_ports[l] - 1000) // 2
+            for d, sym in enumerate(g.dmrs_sym):
+                got = moved[k, l, sym, comb::2][: g.n_dmrs_re]
+                assert np.allclose(got, ref[l, d, : g.n_dmrs_re], atol=1e-3)
+                assert not np.allclose(got, base[k, l, sym, comb::2][: g.n_dmrs_re], atol=1e-3)
