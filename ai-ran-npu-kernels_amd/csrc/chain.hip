@@ -55,8 +55,10 @@ __device__ inline int16_t SatNeg16(int v) { return static_cast<int16_t>(v == -32
 // continues across the slots of a transport block over transmitted symbols
 // only (descramble_mimo.cpp:176-184).
 // QM = 8 fixes the modulation at compile time (the bit -> stream mapping then
-// indexes registers); QM = 0 takes it from the arguments.
-template <int NL, int QM>
+// indexes registers); QM = 0 takes it from the arguments.  ZF (one Rx, one
+// layer): single-tap zero forcing with n0 = the slot's noise (the
+// equalize_zf_siso arithmetic) in place of the LMMSE solve.
+template <int NL, int QM, bool ZF = false>
 __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restrict__ rx_re,
                                                          const __half *__restrict__ rx_im,
                                                          const __half *__restrict__ h_re,
@@ -87,7 +89,13 @@ __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restric
     };
     Cf xh[4];
     float ne[4];
-    airan::Detect<NL>(static_cast<int>(nr), no, loadh, loady, xh, ne);
+    if constexpr (ZF) {
+        static_assert(NL == 1, "zero forcing is the SISO equaliser");
+        const Cf h = loadh(0, 0), y = loady(0);
+        airan::ZfSolve(h.r, h.i, y.r, y.i, no, &xh[0].r, &xh[0].i, &ne[0]);
+    } else {
+        airan::Detect<NL>(static_cast<int>(nr), no, loadh, loady, xh, ne);
+    }
     const uint32_t qm = QM > 0 ? static_cast<uint32_t>(QM) : a.qm, m = qm / 2;
     const uint32_t re = ds * a.used + sc;
     const uint32_t slot_in_tb = slot - (slot / a.slots_per_tb) * a.slots_per_tb;
@@ -184,6 +192,7 @@ struct AiranPuschChain {
     Geom g{};
     uint32_t slots_per_tb = 1;   // plan.G / (codeword_symbols * Qm)
     bool crc = true;             // the plan carries a TB CRC (the reference's fixed plan has none)
+    bool ls_zf = false;          // AIRAN_CHAIN_RX_LS_ZF: LS + linear interpolation, zero forcing
     uint16_t *slot_default = nullptr;  // [max_slots] config->slot_number (airan_pusch_chain_Enqueue)
     uint32_t active_rows = 0;
     bool fused_ldpc = false;  // decoder reads the codeword LLRs (no lam buffer)
@@ -276,8 +285,12 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, const uint16_t *slot_n
                              slots, s, true);
     if (st != AIRAN_OK) return st;
     mark();
-    st = airan::LaunchCeApply(c->filter, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce, c->ce_ws, c->ce_ws_bytes,
-                              c->h_re, c->h_im, g.nl, slots, s);
+    if (c->ls_zf)
+        st = airan::LaunchLsInterpSiso(c->cfg, g, c->rx_re, c->rx_im, c->dmrs_re, c->dmrs_im, true, c->h_re, c->h_im,
+                                       slots, s);
+    else
+        st = airan::LaunchCeApply(c->filter, c->hls_re, c->hls_im, c->noise_ls, c->noise_ce, c->ce_ws,
+                                  c->ce_ws_bytes, c->h_re, c->h_im, g.nl, slots, s);
     if (st != AIRAN_OK) return st;
     if (c->refiner != nullptr) {
         hipLaunchKernelGGL(RefinerKeepKernel, dim3(airan::CeilDiv(slots, 64)), dim3(64), 0, s, c->noise_ce,
@@ -302,14 +315,15 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, const uint16_t *slot_n
                               const uint32_t *, int16_t *, DemapArgs);
     const bool q8 = g.qm == 8;
     DdKernel dd;
-    switch (g.nl) {
+    switch (c->ls_zf ? 0u : g.nl) {
+    case 0: dd = q8 ? DetectDemapKernel<1, 8, true> : DetectDemapKernel<1, 0, true>; break;
     case 1: dd = q8 ? DetectDemapKernel<1, 8> : DetectDemapKernel<1, 0>; break;
     case 2: dd = q8 ? DetectDemapKernel<2, 8> : DetectDemapKernel<2, 0>; break;
     case 3: dd = q8 ? DetectDemapKernel<3, 8> : DetectDemapKernel<3, 0>; break;
     default: dd = q8 ? DetectDemapKernel<4, 8> : DetectDemapKernel<4, 0>; break;
     }
-    hipLaunchKernelGGL(dd, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im, c->noise_ce, c->gold, c->cw,
-                       da);
+    hipLaunchKernelGGL(dd, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im,
+                       c->ls_zf ? c->noise_ls : c->noise_ce, c->gold, c->cw, da);
     if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
     mark();
     const uint32_t tbs = slots / c->slots_per_tb;
@@ -370,7 +384,7 @@ bool OptionsOk(const AiranPuschChainOptionsV1 &o)
            (o.iq_format == AIRAN_IQ_INT16_Q8_8 || o.iq_format == AIRAN_IQ_COMPLEX64) && o.ldpc_iterations >= 1 &&
            o.ldpc_iterations <= 64 && (o.ldpc_mode == AIRAN_LDPC_MODE_REF || o.ldpc_mode == AIRAN_LDPC_MODE_FP16) &&
            o.ce_max_rank >= 1 && o.ce_max_rank <= 512 && o.tau_s >= 0.0 && o.fd_hz >= 0.0 && o.scs_hz > 0.0 &&
-           o.eig_floor >= 0.0;
+           o.eig_floor >= 0.0 && (o.receiver == AIRAN_CHAIN_RX_LMMSE || o.receiver == AIRAN_CHAIN_RX_LS_ZF);
 }
 
 }  // namespace
@@ -407,10 +421,12 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     c->crc = p.crc_tb != 0;
     c->active_rows = airan_ldpc_ActiveRows(plan);
     c->fused_ldpc = options->ldpc_mode == AIRAN_LDPC_MODE_FP16 && c->active_rows <= 8 && p.Z % 24 == 0;
+    c->ls_zf = options->receiver == AIRAN_CHAIN_RX_LS_ZF;
     auto fail = [&](int32_t code) {
         airan_pusch_chain_Destroy(c);
         return code;
     };
+    if (c->ls_zf && (g.nr != 1 || g.nl != 1 || config->dmrs_ports[0] != 1000)) return fail(AIRAN_UNSUPPORTED_PROFILE);
     if (hipGetDevice(&c->device) != hipSuccess || airan_InitDevice() != AIRAN_OK) return fail(AIRAN_LAUNCH_FAILED);
     LmmseFilterParamsV1 fp{};
     fp.abi_version = AIRAN_PUSCH_MIMO_ABI_VERSION;
@@ -420,14 +436,15 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     fp.fd_hz = options->fd_hz;
     fp.scs_hz = options->scs_hz;
     fp.eig_floor = options->eig_floor;
-    if ((st = airan_channel_est_lmmse_CreateFilter(config, &fp, stream, &c->filter)) != AIRAN_OK) return fail(st);
+    if (!c->ls_zf && (st = airan_channel_est_lmmse_CreateFilter(config, &fp, stream, &c->filter)) != AIRAN_OK)
+        return fail(st);
     const size_t S = options->max_slots;
     const size_t NT = S / T;  // transport blocks per Enqueue at most
     const size_t plane = static_cast<size_t>(g.nsym) * g.pad;
     const size_t ncb = NT * p.C;
     const size_t gold_words = (static_cast<size_t>(T) * g.cw_symbols * g.qm + 31) / 32 + 2;
     const size_t dmrs_elems = S * g.nl * g.ndmrs * g.ref_pad;  // one DMRS reference per slot
-    c->ce_ws_bytes = airan::CeWorkspaceBytes(g, static_cast<uint32_t>(S));
+    c->ce_ws_bytes = c->ls_zf ? 0 : airan::CeWorkspaceBytes(g, static_cast<uint32_t>(S));
     c->ldpc_ws_bytes =
         c->fused_ldpc ? 0 : airan_ldpc_decode_WorkspaceBytes(static_cast<uint32_t>(ncb), p.Z, c->active_rows);
     c->crc_ws_bytes = c->crc ? airan_crc_check_WorkspaceBytes(static_cast<uint32_t>(NT)) : 0;
@@ -447,7 +464,7 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         !Alloc(&c->tb_ok, NT, &b) || !Alloc(&c->refiner_keep, S, &b) || !Alloc(&c->crc_xpow, p.C, &b) ||
         !Alloc(&c->crc_ws, c->crc_ws_bytes, &b) ||
         !Alloc(&c->ldpc_queue, 2, &b) ||
-        !Alloc(reinterpret_cast<uint8_t **>(&c->zero_iq), g.nr * g.samples * iq_elem, &b))
+        !Alloc(reinterpret_cast<uint8_t **>(&c->zero_iq), T * g.nr * g.samples * iq_elem, &b))
         return fail(AIRAN_RESOURCE_TOO_SMALL);
     // The decoder's block queue starts (and stays) zero.
     if (hipMemset(c->ldpc_queue, 0, 2 * sizeof(uint32_t)) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
@@ -491,9 +508,10 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
     if (hipMemcpyAsync(c->desc, desc.data(), ncb * sizeof(RateMatchDescriptor), hipMemcpyHostToDevice, s) !=
             hipSuccess ||
         hipMemsetAsync(c->cw, 0, S * g.qm * g.cw_stride * sizeof(int16_t), s) != hipSuccess ||
-        hipMemsetAsync(c->zero_iq, 0, g.nr * g.samples * iq_elem, s) != hipSuccess)
+        hipMemsetAsync(c->zero_iq, 0, T * g.nr * g.samples * iq_elem, s) != hipSuccess)
         return fail(AIRAN_LAUNCH_FAILED);
-    // One eager slot loads every per-device table before any graph capture.
+    // One eager transport block (T slots of zero IQ: zero_iq holds T slots)
+    // loads every per-device table before any graph capture.
     for (hipEvent_t &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
     if ((st = EnqueueStages(c, c->zero_iq, c->slot_default, T, s, false)) != AIRAN_OK) return fail(st);
@@ -574,7 +592,7 @@ int32_t airan_pusch_chain_Outputs(const AiranPuschChain *c, AiranPuschChainOutpu
     o->h_ls_re = c->hls_re;
     o->h_ls_im = c->hls_im;
     o->noise_ls = c->noise_ls;
-    o->noise_ce = c->noise_ce;
+    o->noise_ce = c->ls_zf ? nullptr : c->noise_ce;
     o->h_re = c->h_re;
     o->h_im = c->h_im;
     o->cw_llr = c->cw;
@@ -588,6 +606,7 @@ int32_t airan_pusch_chain_SetRefiner(AiranPuschChain *c, AiranCeRefiner *refiner
                                      float snr_max_db)
 {
     if (c == nullptr || (refiner != nullptr && !(snr_min_db <= snr_max_db))) return AIRAN_INVALID_ARGUMENT;
+    if (refiner != nullptr && c->ls_zf) return AIRAN_UNSUPPORTED_PROFILE;
     DropGraph(c);
     c->refiner = refiner;
     c->refiner_snr_min = snr_min_db;

@@ -139,6 +139,25 @@ inline int32_t LaunchStatus()
 
 inline uint32_t CeilDiv(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 
+// fp16 arithmetic with one rounding per operation, as the reference's fp16
+// vector units compute: contraction into v_fma_f16 would skip a product's
+// rounding and differ by one ulp.
+__device__ inline __half HMul(__half a, __half b)
+{
+#pragma clang fp contract(off)
+    return __float2half_rn(__half2float(a) * __half2float(b));
+}
+__device__ inline __half HAdd(__half a, __half b)
+{
+#pragma clang fp contract(off)
+    return __float2half_rn(__half2float(a) + __half2float(b));
+}
+__device__ inline __half HSub(__half a, __half b)
+{
+#pragma clang fp contract(off)
+    return __float2half_rn(__half2float(a) - __half2float(b));
+}
+
 // Kernel-argument copy of the geometry (trivially copyable).
 struct DevGeom {
     uint32_t nr, nl, qm, nsym, fft, used, pad;

@@ -131,4 +131,18 @@ __device__ inline void Detect(int nr, float no, LoadH loadh, LoadY loady, Cf (&x
     }
 }
 
+// Single-tap zero forcing (equalization/equalize_zf_siso/equalize_kernel.cpp:
+// Process; operator.yaml:72-84) on fp16 inputs in fp32: g = max(hr^2 + hi^2,
+// floor), x = conj(h) y / g, no_eff = n0 / g, IEEE divisions.  Products of
+// fp16 values are exact in fp32, so every sum rounds once whether or not it
+// is contracted (the reference's MulAddDst gives the same value).
+__device__ inline void ZfSolve(float hr, float hi, float yr, float yi, float n0, float *xr, float *xi, float *ne)
+{
+#pragma clang fp contract(off)
+    const float g = fmaxf(hr * hr + hi * hi, AIRAN_ZF_G_FLOOR);
+    *xr = (hr * yr + hi * yi) / g;
+    *xi = (hr * yi - hi * yr) / g;
+    *ne = n0 / g;
+}
+
 }  // namespace airan

@@ -22,6 +22,14 @@ int32_t LaunchDmrsLs(const Geom &g, const __half *rx_re, const __half *rx_im, co
 int32_t LaunchDmrsGenSlots(const PuschMimoConfig &cfg, const Geom &g, const uint16_t *slot_no, uint32_t slots,
                            __half *ref_re, __half *ref_im, hipStream_t s);
 
+// SISO LS + linear interpolation (op_siso.hip) for `slots` stacked grids
+// rx [slots,1,14,pad] -> H [slots,1,1,14,pad]; ref [1 or slots][D][ref_pad].
+// AIRAN_UNSUPPORTED_PROFILE unless the config is 1 Rx, rank 1 on port 1000
+// with one or two DMRS symbols.
+int32_t LaunchLsInterpSiso(const PuschMimoConfig &cfg, const Geom &g, const __half *rx_re, const __half *rx_im,
+                           const __half *ref_re, const __half *ref_im, bool ref_per_slot, __half *h_re,
+                           __half *h_im, uint32_t slots, hipStream_t s);
+
 // LMMSE channel estimate for `slots` stacked observations; output
 // [slots,NR,out_layers,14,pad] (layers >= L untouched).
 size_t CeWorkspaceBytes(const Geom &g, uint32_t slots);

@@ -6,6 +6,9 @@
 
 using airan::DevGeom;
 using airan::Geom;
+using airan::HAdd;
+using airan::HMul;
+using airan::HSub;
 
 namespace {
 
@@ -67,24 +70,6 @@ __global__ void __launch_bounds__(256) DmrsGenKernel(uint16_t *__restrict__ re, 
         re[base + m] = vr;
         im[base + m] = vi;
     }
-}
-
-// Each fp16 operation rounds on its own: contraction into v_fma_f16 would
-// skip the product's rounding and differ from the reference by one ulp.
-__device__ inline __half HMul(__half a, __half b)
-{
-#pragma clang fp contract(off)
-    return __float2half_rn(__half2float(a) * __half2float(b));
-}
-__device__ inline __half HAdd(__half a, __half b)
-{
-#pragma clang fp contract(off)
-    return __float2half_rn(__half2float(a) + __half2float(b));
-}
-__device__ inline __half HSub(__half a, __half b)
-{
-#pragma clang fp contract(off)
-    return __float2half_rn(__half2float(a) - __half2float(b));
 }
 
 // channel_est/mimo_dmrs_ls/mimo_dmrs_ls_kernel.cpp:146-177,301-331: h = y x*

@@ -256,11 +256,26 @@ class LdpcDecodeOpArgsV1(C.Structure):
                      ("stream", vp))
 
 
+class ChannelEstLsSisoOpArgsV1(C.Structure):
+    _fields_ = _args(("rx_grid_re", vp), ("rx_grid_im", vp), ("dmrs_ref_re", vp), ("dmrs_ref_im", vp),
+                     ("h_re", vp), ("h_im", vp), ("config", C.POINTER(PuschMimoConfig)), ("batch", u32),
+                     ("ref_per_slot", u32), ("stream", vp))
+
+
+class EqualizeZfSisoOpArgsV1(C.Structure):
+    _fields_ = _args(("y_re", vp), ("y_im", vp), ("h_re", vp), ("h_im", vp), ("n0", vp), ("x_re", vp),
+                     ("x_im", vp), ("no_eff", vp), ("n", u32), ("reserved", u32), ("stream", vp))
+
+
 class AiranPuschChainOptionsV1(C.Structure):
     _fields_ = _args(("max_slots", u32), ("iq_format", u32), ("ldpc_iterations", u32), ("ldpc_mode", u32),
                      ("early_stop", u32), ("ce_max_rank", u32), ("use_graph", u32), ("tau_s", C.c_double),
                      ("fd_hz", C.c_double), ("scs_hz", C.c_double), ("eig_floor", C.c_double),
-                     ("reserved", u32 * 8))
+                     ("receiver", u32), ("reserved", u32 * 7))
+
+
+CHAIN_RX_LMMSE = 0
+CHAIN_RX_LS_ZF = 1
 
 
 class AiranPuschChainOutputsV1(C.Structure):
@@ -353,6 +368,10 @@ PROTOTYPES = {
     "airan_rate_dematch_mimo_ValidateOpArgs": (i32, [P(RateDematchMimoOpArgsV1)]),
     "airan_rate_dematch_mimo_Enqueue": (i32, [P(RateDematchMimoOpArgsV1), P(RateDematchKernelMetadata), vp,
                                               C.c_size_t, vp, C.c_size_t, vp, C.c_size_t]),
+    "airan_channel_est_ls_siso_ValidateOpArgs": (i32, [P(ChannelEstLsSisoOpArgsV1)]),
+    "airan_channel_est_ls_siso_Enqueue": (i32, [P(ChannelEstLsSisoOpArgsV1)]),
+    "airan_equalize_zf_siso_ValidateOpArgs": (i32, [P(EqualizeZfSisoOpArgsV1)]),
+    "airan_equalize_zf_siso_Enqueue": (i32, [P(EqualizeZfSisoOpArgsV1)]),
     "airan_ldpc_decode_ValidateOpArgs": (i32, [P(LdpcDecodeOpArgsV1)]),
     "airan_ldpc_decode_WorkspaceBytes": (C.c_size_t, [u32, u32, u32]),
     "airan_ldpc_decode_Enqueue": (i32, [P(LdpcDecodeOpArgsV1), vp, C.c_size_t]),

@@ -5,8 +5,9 @@ reference's chain/pusch_mimo_rx_chain/rx_chain.cpp:440-640:
   I/O pack -> LMMSE detection -> QAM demap -> layer demap -> descramble ->
   rate dematch -> LDPC decode -> CRC.
 
-One slot per transport block.  Every stage is a separate launch with its
-buffers allocated per call; the native batched chain (airan_pusch_chain_*)
+A transport block spans one slot (run_slot) or T consecutive slots
+(run_tb).  Every stage is a separate launch with its buffers allocated per
+call; the native batched chain (airan_pusch_chain_*)
 is the production path and must reproduce this composition bit for bit.
 """
 from __future__ import annotations
@@ -28,6 +29,7 @@ class ChainOptions:
     scs_hz: float = 30e3
     ce_rank: int = 384
     iq_format: int = abi.IQ_INT16_Q8_8
+    receiver: int = abi.CHAIN_RX_LMMSE   # CHAIN_RX_LS_ZF: SISO LS + linear interpolation, zero forcing
 
 
 class OpChain:
@@ -36,8 +38,9 @@ class OpChain:
         self.plan = plan
         self.opts = opts or ChainOptions()
         self.g = ops.Geometry(cfg)
-        self.filter = ops.LmmseFilter(cfg, tau_s=self.opts.tau_s, fd_hz=self.opts.fd_hz, scs_hz=self.opts.scs_hz,
-                                      max_rank=self.opts.ce_rank)
+        self.ls_zf = self.opts.receiver == abi.CHAIN_RX_LS_ZF
+        self.filter = None if self.ls_zf else ops.LmmseFilter(cfg, tau_s=self.opts.tau_s, fd_hz=self.opts.fd_hz,
+                                                              scs_hz=self.opts.scs_hz, max_rank=self.opts.ce_rank)
         self._dmrs = {cfg.slot_number: ops.dmrs_gen(cfg)}
         self.active_rows = ops.ldpc_active_rows(plan)
         self.no_eff_floor = float(abi.load_library().airan_pusch_NoEffFloor(cfg.qm))
@@ -50,36 +53,84 @@ class OpChain:
             self._dmrs[slot_number] = ops.dmrs_gen(c)
         return self._dmrs[slot_number]
 
-    def run_slot(self, iq: torch.Tensor, keep: bool = False, slot_number: int | None = None):
-        """iq: one slot, [NR, samples, 2] on the device.  Returns a dict with
-        bits [C*K] int8, cb_ok, tb_ok and (keep=True) the intermediates.
-        slot_number: the slot's number (its DMRS), default the config's."""
-        cfg, g, plan, o = self.cfg, self.g, self.plan, self.opts
-        out = {}
+    def front(self, iq: torch.Tensor, slot_number: int | None = None, out: dict | None = None):
+        """Slot-local stages of one slot: OFDM demod (+RE demap) -> DMRS LS ->
+        LMMSE CE -> detector I/O pack -> detection -> QAM demap -> layer
+        demap (receiver LS_ZF: SISO LS + linear interpolation and zero
+        forcing with n0 = the DMRS-LS noise estimate in place of CE and
+        detection).  Returns the codeword LLRs before descrambling
+        [Qm, codeword_stride] int16; `out` (when given) receives the
+        intermediates."""
+        cfg, g, o = self.cfg, self.g, self.opts
         dmrs_re, dmrs_im = self.dmrs(cfg.slot_number if slot_number is None else slot_number)
         rx_re, rx_im = ops.ofdm_demod(cfg, iq, g.nr, o.iq_format, fused_redemap=True)
         h_re, h_im, sc, cnt, noise = ops.dmrs_ls(cfg, rx_re, rx_im, dmrs_re, dmrs_im)
-        hg_re, hg_im, _, noise_ce = ops.ce_lmmse(cfg, self.filter, h_re, h_im, sc, cnt, noise)
-        hrm_r, hrm_i, yv_r, yv_i, no = ops.detect_io_pack(cfg, rx_re, rx_im, hg_re, hg_im, noise_ce)
-        xr, xi, ne = ops.detect_lmmse(cfg, hrm_r, hrm_i, yv_r, yv_i, no)
+        if self.ls_zf:
+            hg_re, hg_im = ops.ls_siso(cfg, rx_re, rx_im, dmrs_re, dmrs_im)
+            noise_ce = noise
+            n0 = noise[:1].expand(14 * g.pad).contiguous()
+            xr, xi, ne = ops.zf_siso(rx_re, rx_im, hg_re, hg_im, n0)
+        else:
+            hg_re, hg_im, _, noise_ce = ops.ce_lmmse(cfg, self.filter, h_re, h_im, sc, cnt, noise)
+            hrm_r, hrm_i, yv_r, yv_i, no = ops.detect_io_pack(cfg, rx_re, rx_im, hg_re, hg_im, noise_ce)
+            xr, xi, ne = ops.detect_lmmse(cfg, hrm_r, hrm_i, yv_r, yv_i, no)
         # Keep the demapper's integer arithmetic in range (airan_pusch_NoEffFloor).
         ne_q = torch.clamp_min(ne, self.no_eff_floor)
         llr = ops.qam_demod(cfg, xr, xi, ne_q)
         cw = ops.layer_demap(cfg, llr)
-        cw_nr = ops.descramble(cfg, cw, 1, restart_per_slot=True)
-        lam = ops.rate_dematch(cfg, cw_nr, 1, plan)
+        if out is not None:
+            out.update(rx_re=rx_re, rx_im=rx_im, h_ls_re=h_re, h_ls_im=h_im, noise=noise, noise_ce=noise_ce,
+                       h_re=hg_re, h_im=hg_im, xhat_re=xr, xhat_im=xi, no_eff=ne, layer_llr=llr, cw_llr=cw)
+        return cw
+
+    def tail(self, cw: torch.Tensor, num_slots: int, restart_per_slot: bool = False, out: dict | None = None):
+        """Per-TB stages over the codeword LLRs of the TB's `num_slots`
+        slots ([num_slots, Qm, codeword_stride], contiguous): descramble (the
+        Gold sequence runs on across the slots, descramble_mimo.cpp:176-184),
+        rate dematch (code blocks may span slots), LDPC decode and, when the
+        plan carries CRCs, the CRC check."""
+        plan, o = self.plan, self.opts
+        cw_nr = ops.descramble(self.cfg, cw, num_slots, restart_per_slot=restart_per_slot)
+        lam = ops.rate_dematch(self.cfg, cw_nr, num_slots, plan)
         bits, _, iters = ops.ldpc_decode(lam, plan.C, plan.Z, plan.K, o.ldpc_iterations, self.active_rows,
                                          o.ldpc_mode, o.early_stop)
-        cb_ok, tb_ok, _ = ops.crc_check(plan, 1, bits)
-        out.update(bits=bits, cb_ok=cb_ok, tb_ok=tb_ok, iterations=iters)
-        if keep:
-            out.update(rx_re=rx_re, rx_im=rx_im, h_ls_re=h_re, h_ls_im=h_im, noise=noise, noise_ce=noise_ce,
-                       h_re=hg_re, h_im=hg_im,
-                       xhat_re=xr, xhat_im=xi, no_eff=ne, layer_llr=llr, cw_llr=cw, cw_nr=cw_nr, ldpc_llr=lam)
+        res = dict(bits=bits, iterations=iters)
+        if plan.crc_tb:
+            cb_ok, tb_ok, _ = ops.crc_check(plan, 1, bits)
+            res.update(cb_ok=cb_ok, tb_ok=tb_ok)
+        if out is not None:
+            out.update(cw_nr=cw_nr, ldpc_llr=lam)
+        return res
+
+    def run_slot(self, iq: torch.Tensor, keep: bool = False, slot_number: int | None = None):
+        """iq: one slot, [NR, samples, 2] on the device.  Returns a dict with
+        bits [C*K] int8, cb_ok, tb_ok and (keep=True) the intermediates.
+        slot_number: the slot's number (its DMRS), default the config's."""
+        out = {}
+        cw = self.front(iq, slot_number, out if keep else None)
+        out.update(self.tail(cw, 1, restart_per_slot=True, out=out if keep else None))
+        if not keep:
+            out = {k: out[k] for k in ("bits", "cb_ok", "tb_ok", "iterations") if k in out}
+        return out
+
+    def run_tb(self, iq: torch.Tensor, slot_numbers=None, keep: bool = False):
+        """One transport block over T = iq.shape[0] consecutive slots
+        (iq [T, NR, samples, 2]): the slot-local stages per slot, then the
+        TB stages over the T slots' codeword LLRs.  slot_numbers: each
+        slot's number (default the config's)."""
+        T = iq.shape[0]
+        g = self.g
+        cw = torch.empty((T, g.qm, g.cw_stride), dtype=torch.int16, device=iq.device)
+        for t in range(T):
+            sn = None if slot_numbers is None else int(slot_numbers[t])
+            cw[t].copy_(self.front(iq[t].contiguous(), sn).view(g.qm, g.cw_stride))
+        out = {"cw_llr": cw} if keep else {}
+        out.update(self.tail(cw, T, out=out if keep else None))
         return out
 
     def close(self):
-        self.filter.close()
+        if self.filter is not None:
+            self.filter.close()
 
 
 _HIP = None
@@ -118,6 +169,7 @@ class NativeChain:
         o.ce_max_rank = self.opts.ce_rank
         o.use_graph = 1 if use_graph else 0
         o.tau_s, o.fd_hz, o.scs_hz, o.eig_floor = self.opts.tau_s, self.opts.fd_hz, self.opts.scs_hz, 1e-12
+        o.receiver = self.opts.receiver
         self.handle = C.c_void_p()
         ops._check("chain Create", self.lib.airan_pusch_chain_Create(C.byref(cfg), C.byref(plan), C.byref(o),
                                                                     C.c_void_p(ops.stream_ptr()),

@@ -391,6 +391,33 @@ def detect_io_pack(cfg, rx_re, rx_im, h_re, h_im, noise_var_rx):
     return hrm_r, hrm_i, yv_r, yv_i, no
 
 
+def ls_siso(cfg, rx_re, rx_im, ref_re, ref_im, batch: int = 1, ref_per_slot: bool = False):
+    """SISO LS + linear interpolation: rx [batch,14,padded] fp16 planes ->
+    H [batch,14,padded] fp16 planes (airan_channel_est_ls_siso_Enqueue)."""
+    lib = load_library()
+    g = Geometry(cfg)
+    dev = rx_re.device
+    n = batch * 14 * g.pad
+    hr, hi = _dev(n, torch.float16, dev), _dev(n, torch.float16, dev)
+    a = _hdr(abi.ChannelEstLsSisoOpArgsV1, rx_grid_re=_dp(rx_re), rx_grid_im=_dp(rx_im), dmrs_ref_re=_dp(ref_re),
+             dmrs_ref_im=_dp(ref_im), h_re=_dp(hr), h_im=_dp(hi), config=C.pointer(cfg), batch=batch,
+             ref_per_slot=1 if ref_per_slot else 0, stream=C.c_void_p(stream_ptr()))
+    _check("ls_siso Enqueue", lib.airan_channel_est_ls_siso_Enqueue(C.byref(a)))
+    return hr, hi
+
+
+def zf_siso(y_re, y_im, h_re, h_im, n0):
+    """Single-tap ZF over n = y_re.numel() REs (airan_equalize_zf_siso_Enqueue)."""
+    lib = load_library()
+    n = y_re.numel()
+    dev = y_re.device
+    xr, xi, ne = _dev(n, torch.float16, dev), _dev(n, torch.float16, dev), _dev(n, torch.float16, dev)
+    a = _hdr(abi.EqualizeZfSisoOpArgsV1, y_re=_dp(y_re), y_im=_dp(y_im), h_re=_dp(h_re), h_im=_dp(h_im),
+             n0=_dp(n0), x_re=_dp(xr), x_im=_dp(xi), no_eff=_dp(ne), n=n, stream=C.c_void_p(stream_ptr()))
+    _check("zf_siso Enqueue", lib.airan_equalize_zf_siso_Enqueue(C.byref(a)))
+    return xr, xi, ne
+
+
 def detect_lmmse(cfg, hrm_r, hrm_i, yv_r, yv_i, no):
     lib = load_library()
     g = Geometry(cfg)

@@ -162,3 +162,14 @@ MID_4RX_RANK2 = Profile(name="fd4x2_106prb", rx=4, rank=2, num_rb=106, fft=2048,
 # Small SISO profile: 1 Rx, rank 1, 52 PRB, FFT 1024, 256QAM, one DMRS symbol.
 SISO_52PRB = Profile(name="fd1x1_52prb", rx=1, rank=1, num_rb=52, fft=1024, qm=8, dmrs_symbols=(2,),
                      padded_subcarriers=640, code_rate=0.75)
+
+# The plain SISO workload: 1 Tx / 1 Rx, 52 PRB in a 1024-point FFT, one DMRS
+# symbol (2), 256QAM at R ~ 0.75, data scrambling c_init = 0x1234 * 2^15 + 0
+# (rnti 0x1234, n_ID 0); received by LS + linear interpolation and ZF.
+SISO_CONFIG1 = Profile(name="fd1x1_52prb_ls_zf", rx=1, rank=1, num_rb=52, fft=1024, qm=8, dmrs_symbols=(2,),
+                       padded_subcarriers=640, code_rate=0.75, rnti=0x1234, data_scrambling_id=0)
+
+# SISO on the reference's grid (133 PRB, FFT 2048, DMRS {2, 11}): the geometry
+# of channel_est_ls_siso / equalize_zf_siso (1596 used, 1664 padded).
+SISO_133PRB = Profile(name="fd1x1_133prb", rx=1, rank=1, num_rb=133, fft=2048, qm=8,
+                      padded_subcarriers=1664, code_rate=0.75)

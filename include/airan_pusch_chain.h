@@ -12,7 +12,9 @@
  *   OFDM demod + RE demap     airan_ofdm_demod_redemap_Enqueue, all slots
  *   DMRS LS + noise           the mimo_dmrs_ls kernels, all slots
  *   LMMSE CE                  the channel_est_lmmse kernels, all slots
- *   detect+demap+descramble   one fused kernel: exact per-RE LMMSE, the
+ *                             (receiver LS_ZF: LS + linear interpolation)
+ *   detect+demap+descramble   one fused kernel: exact per-RE LMMSE (or
+ *                             single-tap zero forcing), the
  *                             QAM demapper, layer demap and Gold descramble,
  *                             writing the descrambled codeword LLRs
  *   rate dematch + LDPC       FP16 messages with at most 8 active rows: one
@@ -63,8 +65,17 @@ typedef struct AiranPuschChainOptionsV1 {
     double fd_hz;               /* CE time prior: maximum Doppler */
     double scs_hz;              /* subcarrier spacing */
     double eig_floor;           /* CE: drop eigenvalues below eig_floor * max */
-    uint32_t reserved[8];
+    uint32_t receiver;          /* AIRAN_CHAIN_RX_* */
+    uint32_t reserved[7];
 } AiranPuschChainOptionsV1;
+/* Channel estimate and equaliser of the chain.  LMMSE: LMMSE CE (filter
+ * built from tau_s, fd_hz) and per-RE LMMSE detection, any geometry.  LS_ZF:
+ * the SISO receiver (one Rx, one layer on port 1000): DMRS least squares with
+ * linear frequency/time interpolation (airan_channel_est_ls_siso) and
+ * single-tap zero forcing (airan_equalize_zf_siso) with n0 = the slot's
+ * DMRS-LS noise estimate; no CE filter is built. */
+#define AIRAN_CHAIN_RX_LMMSE 0u
+#define AIRAN_CHAIN_RX_LS_ZF 1u
 
 typedef struct AiranPuschChain AiranPuschChain;
 
@@ -85,7 +96,7 @@ typedef struct AiranPuschChainOutputsV1 {
     const void *h_ls_re;        /* fp16 [slots][NR][L][D][pilot_pad] */
     const void *h_ls_im;
     const void *noise_ls;       /* fp16 [slots][NR] (DMRS LS) */
-    const void *noise_ce;       /* fp16 [slots][NR] (CE subspace estimate) */
+    const void *noise_ce;       /* fp16 [slots][NR] (CE subspace estimate); NULL for receiver LS_ZF */
     const void *h_re;           /* fp16 [slots][NR][L][14][padded] */
     const void *h_im;
     const void *cw_llr;         /* int16 [slots][Qm][codeword_stride], descrambled */

@@ -629,6 +629,66 @@ int32_t airan_rate_dematch_mimo_ReferenceRateDematchForTb(const int16_t *cw_llr,
                                                           const RateMatchDescriptor *descriptors,
                                                           size_t descriptor_count, int16_t *ldpc_llr);
 
+/* ============================================ SISO LS + linear interpolation */
+/* Replaces the raw launch channel_est_ls_kernel(y_re, y_im, x_ref_re,
+ * x_ref_im, out_h_re, out_h_im, weave_idx, ws, tiling)
+ * (channel_est/channel_est_ls_siso/channel_est_ls_kernel.cpp; algorithm
+ * operator.yaml:82-93, golden scripts/channel_est_ref.py:97-121).  One Rx,
+ * one layer on port 1000 (comb 0), per slot, all arithmetic in fp16 in the
+ * reference's order:
+ *   h_ls[d][p]   = y[d_sym][2p] * conj(x[d][p])   (re: yr xr + yi xi,
+ *                  im: yi xr - yr xi, every product and sum rounded),
+ *                  h_ls[d][p] = 0 for p >= used / 2 (zero-extended grid)
+ *   H_d[2p]      = h_ls[d][p] + 0
+ *   H_d[2p + 1]  = (h_ls[d][p] + h_ls[d][p + 1]) * 0.5
+ *   H[sym]       = H_0 * fp16((b - sym) / (b - a)) + H_1 * fp16((sym - a) / (b - a))
+ *                  for DMRS symbols a < b (the reference's {2, 11}: (11 - sym) / 9
+ *                  and (sym - 2) / 9, extrapolated outside [a, b]); H_0 on every
+ *                  symbol when the slot has one DMRS symbol.
+ * Subcarriers >= used are written as zero.  dmrs_ref: one reference
+ * [D][dmrs_ref_pad] for every slot, or one per slot (ref_per_slot). */
+typedef struct ChannelEstLsSisoOpArgsV1 {
+    uint16_t abi_version;
+    uint16_t struct_size;
+    const void *rx_grid_re;     /* fp16 [batch,14,padded] */
+    const void *rx_grid_im;
+    const void *dmrs_ref_re;    /* fp16 [1 or batch][D,dmrs_ref_pad] (airan_mimo_dmrs_gen layout, L = 1) */
+    const void *dmrs_ref_im;
+    void *h_re;                 /* fp16 [batch,14,padded] */
+    void *h_im;
+    const PuschMimoConfig *config;
+    uint32_t batch;             /* slots */
+    uint32_t ref_per_slot;      /* 0: one DMRS reference for every slot */
+    void *stream;
+} ChannelEstLsSisoOpArgsV1;
+int32_t airan_channel_est_ls_siso_ValidateOpArgs(const ChannelEstLsSisoOpArgsV1 *args);
+int32_t airan_channel_est_ls_siso_Enqueue(const ChannelEstLsSisoOpArgsV1 *args);
+
+/* ============================================================= SISO ZF */
+/* Replaces the raw launch equalize_kernel(y_re, y_im, h_re, h_im, n0, x_re,
+ * x_im, no_eff, ws, tiling) (equalization/equalize_zf_siso/
+ * equalize_kernel.cpp; operator.yaml:72-84): per RE, from fp16 inputs in fp32,
+ *   g = max(hr^2 + hi^2, 1e-4), x = ((hr yr + hi yi) / g, (hr yi - hi yr) / g),
+ *   no_eff = n0 / g, outputs rounded to fp16 (IEEE division). */
+#define AIRAN_ZF_G_FLOOR 1.0e-4f
+typedef struct EqualizeZfSisoOpArgsV1 {
+    uint16_t abi_version;
+    uint16_t struct_size;
+    const void *y_re;           /* fp16 [n] */
+    const void *y_im;
+    const void *h_re;           /* fp16 [n] */
+    const void *h_im;
+    const void *n0;             /* fp16 [n], pre-equalisation noise variance */
+    void *x_re;                 /* fp16 [n] */
+    void *x_im;
+    void *no_eff;               /* fp16 [n] */
+    uint32_t n;                 /* resource elements (e.g. batch * 14 * padded) */
+    uint32_t reserved;
+    void *stream;
+} EqualizeZfSisoOpArgsV1;
+int32_t airan_equalize_zf_siso_ValidateOpArgs(const EqualizeZfSisoOpArgsV1 *args);
+int32_t airan_equalize_zf_siso_Enqueue(const EqualizeZfSisoOpArgsV1 *args);
+
 /* ================================================================= LDPC */
 /* Replaces the raw launch ldpc_decode_kernel(lam_in, packed_bc, packed_s,
  * degrees, edge_offsets, prev_msg, bits, lam_out, scratch)

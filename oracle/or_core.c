@@ -800,3 +800,83 @@ uint32_t or_crc(int kind, const uint8_t *bits, uint64_t nbits)
     }
     return reg;
 }
+
+/* ----------------------------------------------------- SISO LS + ZF */
+/* channel_est/channel_est_ls_siso/channel_est_ls_kernel.cpp Phase1_Ls (h_ls
+ * = y x* at the even subcarriers, fp16 op order of ComplexMulConj),
+ * Phase2_FreqInterp (even: h_ls + 0, odd: (h_ls[p] + h_ls[p+1]) * 0.5, the
+ * pilot past the last one zero: "zero_extended_padded_grid",
+ * operator.yaml:82-93) and Phase3_TimeInterpAndNaturalWrite (H_0 * fp16((b -
+ * sym) / (b - a)) + H_1 * fp16((sym - a) / (b - a)), the reference's k / 9
+ * weights for DMRS symbols {2, 11}); one DMRS symbol: H_0 on every symbol.
+ * rx [14][pad], ref [D][ref_pad] (airan_mimo_dmrs_gen layout) -> H
+ * [14][pad] fp16 bits; subcarriers >= used are zero. */
+void or_ls_siso(const PuschMimoConfig *cfg, const uint16_t *rx_re, const uint16_t *rx_im, const uint16_t *ref_re,
+                const uint16_t *ref_im, uint16_t *h_re, uint16_t *h_im)
+{
+    OrGeom g;
+    or_geom(cfg, &g);
+    const int n = g.n_dmrs_re;
+    const uint16_t zero = 0, half_one = or_f2h(0.5f);
+    uint16_t *er = (uint16_t *)calloc((size_t)2 * g.pad, sizeof(uint16_t));
+    uint16_t *ei = (uint16_t *)calloc((size_t)2 * g.pad, sizeof(uint16_t));
+    uint16_t *lr = (uint16_t *)calloc((size_t)n + 1, sizeof(uint16_t));
+    uint16_t *li = (uint16_t *)calloc((size_t)n + 1, sizeof(uint16_t));
+    for (int d = 0; d < g.ndmrs && d < 2; ++d) {
+        const uint16_t *yr = rx_re + (size_t)g.dmrs_sym[d] * g.pad, *yi = rx_im + (size_t)g.dmrs_sym[d] * g.pad;
+        const uint16_t *xr = ref_re + (size_t)d * g.ref_pad, *xi = ref_im + (size_t)d * g.ref_pad;
+        for (int m = 0; m < n; ++m) {
+            lr[m] = hadd(hmul(yr[2 * m], xr[m]), hmul(yi[2 * m], xi[m]));
+            li[m] = hsub(hmul(yi[2 * m], xr[m]), hmul(yr[2 * m], xi[m]));
+        }
+        lr[n] = li[n] = 0;
+        for (int m = 0; m < n; ++m) {
+            er[(size_t)d * g.pad + 2 * m] = hadd(lr[m], zero);
+            ei[(size_t)d * g.pad + 2 * m] = hadd(li[m], zero);
+            er[(size_t)d * g.pad + 2 * m + 1] = hmul(hadd(lr[m], lr[m + 1]), half_one);
+            ei[(size_t)d * g.pad + 2 * m + 1] = hmul(hadd(li[m], li[m + 1]), half_one);
+        }
+    }
+    for (int sym = 0; sym < 14; ++sym) {
+        uint16_t al = 0, om = 0;
+        if (g.ndmrs == 2) {
+            const float span = (float)g.dmrs_sym[1] - (float)g.dmrs_sym[0];
+            al = or_f2h(((float)sym - (float)g.dmrs_sym[0]) / span);
+            om = or_f2h(((float)g.dmrs_sym[1] - (float)sym) / span);
+        }
+        for (int sc = 0; sc < g.pad; ++sc) {
+            uint16_t vr = 0, vi = 0;
+            if (sc < g.used) {
+                if (g.ndmrs == 1) {
+                    vr = er[sc];
+                    vi = ei[sc];
+                } else {
+                    vr = hadd(hmul(er[sc], om), hmul(er[(size_t)g.pad + sc], al));
+                    vi = hadd(hmul(ei[sc], om), hmul(ei[(size_t)g.pad + sc], al));
+                }
+            }
+            h_re[(size_t)sym * g.pad + sc] = vr;
+            h_im[(size_t)sym * g.pad + sc] = vi;
+        }
+    }
+    free(er);
+    free(ei);
+    free(lr);
+    free(li);
+}
+
+/* equalization/equalize_zf_siso/equalize_kernel.cpp Process (operator.yaml:
+ * 72-84): fp32 from fp16 inputs, g = max(hr^2 + hi^2, 1e-4), x = conj(h) y /
+ * g, no_eff = n0 / g, each rounded to fp16.  n elements. */
+void or_zf_siso(size_t n, const uint16_t *y_re, const uint16_t *y_im, const uint16_t *h_re, const uint16_t *h_im,
+                const uint16_t *n0, uint16_t *x_re, uint16_t *x_im, uint16_t *no_eff)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const float hr = or_h2f(h_re[i]), hi = or_h2f(h_im[i]), yr = or_h2f(y_re[i]), yi = or_h2f(y_im[i]);
+        const float g = fmaxf(hr * hr + hi * hi, 1.0e-4f);
+        const float a = hr * yr + hi * yi, b = hr * yi - hi * yr;
+        x_re[i] = or_f2h(a / g);
+        x_im[i] = or_f2h(b / g);
+        no_eff[i] = or_f2h(or_h2f(n0[i]) / g);
+    }
+}

@@ -19,6 +19,10 @@
  *   or_ce_lmmse                pinned (noiseless/flat closed forms, full-rank
  *                              identity against a brute-force solve)
  *   or_detect_lmmse            pinned (closed form vs brute-force inverse)
+ *   or_ls_siso, or_zf_siso     pinned (pilots equal the DMRS-LS h_ls of port
+ *                              1000; interpolation exact on channels linear
+ *                              in frequency and time; ZF x = y / h closed
+ *                              form, the fp32 arithmetic against numpy)
  *   or_refiner_grad            pinned (exact mode against central finite
  *                              differences of its own loss)
  *   or_ldpc_decode             pinned at outcome level only (encode -> decode
@@ -108,6 +112,14 @@ typedef struct OrCeParams {
 int or_ce_lmmse(const PuschMimoConfig *cfg, const OrCeParams *prm, const double *noise_rel,
                 const double *h_ls, double *h_out);
 int or_ce_noise_rel(const PuschMimoConfig *cfg, const double *h_ls, double sigma2, double *noise_rel);
+
+/* ----------------------------------------------------------- SISO LS / ZF */
+/* channel_est_ls_siso (LS + linear frequency/time interpolation, fp16 op
+ * order) and equalize_zf_siso (single-tap ZF): or_core.c. */
+void or_ls_siso(const PuschMimoConfig *cfg, const uint16_t *rx_re, const uint16_t *rx_im, const uint16_t *ref_re,
+                const uint16_t *ref_im, uint16_t *h_re, uint16_t *h_im);
+void or_zf_siso(size_t n, const uint16_t *y_re, const uint16_t *y_im, const uint16_t *h_re, const uint16_t *h_im,
+                const uint16_t *n0, uint16_t *x_re, uint16_t *x_im, uint16_t *no_eff);
 
 /* ------------------------------------------------------------- detection */
 /* H [NR,L,14,used], y [NR,14,used] -> xhat [L,14,used] complex, no_eff

@@ -47,6 +47,8 @@ def load_oracle():
     lib.or_ce_lmmse.argtypes = [vp, vp, vp, vp, vp]
     lib.or_ce_noise_rel.argtypes = [vp, vp, C.c_double, vp]
     lib.or_detect_lmmse.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.or_ls_siso.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.or_zf_siso.argtypes = [C.c_size_t, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.or_qam_axis_llr.argtypes = [C.c_int, C.c_float, C.c_float, vp]
     lib.or_qam_axis_llr_bruteforce.argtypes = [C.c_int, C.c_float, C.c_float, vp]
     lib.or_qam_demod.argtypes = [vp, vp, vp, vp, vp]

@@ -292,8 +292,18 @@ def test_native_chain_per_slot_dmrs(gpu_stream):
     op.close()
     print("per-slot DMRS: wrong slot numbers (all 0), same graph", flush=True)
     sn_dev.zero_()
-    _, tb_bad, _, _ = run()
-    assert tb_bad[0] == 1 and not tb_bad[1:].any()
+    bits_bad, tb_bad, _, _ = run()
+    assert tb_bad[0] == 1 and np.array_equal(bits_bad.reshape(S, -1)[0], bits.reshape(S, -1)[0])
+    for s in range(1, S):
+        # Against the wrong pilots the estimate is noise-like, and no slot
+        # decodes its payload.  A slot whose estimate collapses to ~0 gets
+        # all-zero LLRs and decodes to the all-zero codeword, whose CRC24A
+        # remainder is zero (no initial inversion in TS 38.212 5.1): the only
+        # way such a slot can pass its CRC.
+        b = bits_bad.reshape(S, -1)[s]
+        assert not np.array_equal(tb_bits(plan, torch.from_numpy(b)), payload[s]), s
+        if tb_bad[s]:
+            assert not b.any(), s
     print("per-slot DMRS: slot numbers restored in place", flush=True)
     sn_dev.copy_(torch.from_numpy(sn.astype(np.int16)).cuda())
     bits2, tb2, _, _ = run()
@@ -304,3 +314,89 @@ def test_native_chain_per_slot_dmrs(gpu_stream):
     torch.cuda.synchronize()
     assert np.array_equal(tb3.cpu().numpy(), tb_bad)
     nat.close()
+
+
+def _or_plan(plan):
+    from oracle_lib import OrTbPlan
+    p = OrTbPlan()
+    for name, _ in OrTbPlan._fields_:
+        if hasattr(plan, name):
+            setattr(p, name, getattr(plan, name))
+    return p
+
+
+def test_reference_profile_23_slot_tb(oracle, gpu_stream):
+    """The reference's own workload end to end (chain/pusch_mimo_rx_chain,
+    rx_chain.cpp:463-664): the fd16x64 profile (64 Rx, rank 4, 133 PRB, FFT
+    2048, 256QAM, int16 IQ), one transport block over 23 slots with the
+    reference's fixed plan (C = 143 code blocks of K = 8448, no CRC), the
+    reference LDPC arithmetic (int8 messages, 3 iterations, no early stop),
+    through the reference's flat channel H[rx, l] = exp(j 2 pi rx l / 64)
+    (chain/pusch_mimo_loopback/rank2_23slot.py:93-103) with AWGN, slot numbers
+    0..22 (each slot's own DMRS).
+
+    Checked: the batched chain equals the op-by-op chain at the descrambled
+    codeword LLRs, the rate-dematched decoder input and the decoded bits; the
+    integer tail equals the oracle (rate dematch of the 23 slots' LLRs, then
+    the REF-mode decoder) bit for bit; and the bits are the payload (BER 0,
+    as the reference's loopback evidence)."""
+    from puschrx.chain import NativeChain
+    from oracle_lib import cref, ptr  # noqa: F401
+    import ctypes as C
+    prof = P.REFERENCE_FD16X64
+    cfg = prof.config()
+    g = ops.Geometry(cfg)
+    T = 23
+    plan = ops.reference_tb_plan(cfg, T)
+    assert (plan.C, plan.K, plan.Z, plan.crc_tb, plan.G) == (143, 8448, 384, 0, T * g.cw_symbols * g.qm)
+    rng = np.random.default_rng(0x523236)
+    sn = np.arange(T, dtype=np.uint16)
+    payload = synth.random_payload(rng, 1, plan.tbs)
+    grid = synth.tx_grid(cfg, plan, payload, slots_per_tb=T, slot_numbers=sn)
+    rx = np.arange(g.nr)[:, None]
+    lay = np.arange(g.nl)[None, :]
+    hflat = np.exp(2j * np.pi * rx * lay / g.nr).astype(np.complex64)
+    H = np.broadcast_to(hflat[None, :, :, None, None], (T, g.nr, g.nl, 14, g.used))
+    snr = 10.0
+    iq = synth.receive(rng, cfg, grid, H, snr)  # int16 Q8.8
+    opts = ChainOptions(ldpc_iterations=3, ldpc_mode=abi.LDPC_MODE_REF, early_stop=False, tau_s=100e-9,
+                        fd_hz=0.0, iq_format=abi.IQ_INT16_Q8_8)
+    iq_dev = torch.from_numpy(iq).cuda()
+    sn_dev = torch.from_numpy(sn.astype(np.int16)).cuda()
+    print("reference profile: native chain, 23 slots", flush=True)
+    nat = NativeChain(cfg, plan, max_slots=T, opts=opts, use_graph=True)
+    assert nat.slots_per_tb == T
+    nat.enqueue(iq_dev, T, sn_dev)
+    assert not nat.outputs().tb_ok  # the reference plan carries no CRC
+    cw_nat = nat.fetch("cw_llr", torch.int16, T * g.qm * g.cw_stride)
+    lam_nat = nat.fetch("ldpc_llr", torch.int16, plan.C * 68 * plan.Z)
+    bits_nat = nat.fetch("bits", torch.int8, plan.C * plan.K)
+    it_nat = nat.fetch("iterations", torch.uint8, plan.C)
+    torch.cuda.synchronize()
+    cw_nat, lam_nat = cw_nat.cpu().numpy(), lam_nat.cpu().numpy()
+    bits_nat, it_nat = bits_nat.cpu().numpy(), it_nat.cpu().numpy()
+    nat.close()
+    assert np.all(it_nat == 3)
+    assert np.array_equal(bits_nat.reshape(plan.C, plan.K).reshape(-1), payload[0].astype(np.int8))
+
+    print("reference profile: op-by-op chain, 23 slots", flush=True)
+    op = OpChain(cfg, plan, opts)
+    r = op.run_tb(iq_dev, slot_numbers=sn, keep=True)
+    torch.cuda.synchronize()
+    op.close()
+    assert np.array_equal(cw_nat, r["cw_nr"].cpu().numpy())
+    assert np.array_equal(lam_nat, r["ldpc_llr"].cpu().numpy())
+    assert np.array_equal(bits_nat, r["bits"].cpu().numpy())
+
+    print("reference profile: oracle rate dematch + REF decoder", flush=True)
+    op_plan = _or_plan(plan)
+    want_lam = np.zeros(plan.C * 68 * plan.Z, np.int16)
+    oracle.or_rate_dematch(C.byref(op_plan), g.nl, g.qm, g.cw_symbols, g.cw_stride, T, ptr(cw_nat), ptr(want_lam))
+    assert np.array_equal(lam_nat, want_lam)
+    want_bits = np.zeros(plan.C * plan.K, np.int8)
+    want_it = np.zeros(plan.C, np.uint8)
+    lam = want_lam.copy()
+    oracle.or_ldpc_decode(plan.Z, plan.C, ptr(lam), abi.LDPC_MODE_REF, 3, ops.ldpc_active_rows(plan), plan.K, 0,
+                          ptr(want_bits), ptr(want_it))
+    assert np.array_equal(bits_nat, want_bits)
+    assert np.array_equal(it_nat, want_it)

@@ -111,6 +111,7 @@ CE_PROFILES = {
     "mid_4rx_rank2_occ": P.MID_4RX_RANK2,
     "rank2_two_combs": dataclasses.replace(P.MID_4RX_RANK2, name="r2comb", ports=(1000, 1002)),
     "full_carrier_4rx": P.FULL_CARRIER_4RX,
+    "reference_fd16x64": P.REFERENCE_FD16X64,
 }
 
 
@@ -243,7 +244,7 @@ def test_ce_lmmse_matches_fp64(oracle, gpu_stream, name, max_rank):
 
 # ------------------------------------------------------------------ detection
 DET_PROFILES = {"siso_52prb": P.SISO_52PRB, "mid_4rx_rank2": P.MID_4RX_RANK2,
-                "full_carrier_4rx": P.FULL_CARRIER_4RX}
+                "full_carrier_4rx": P.FULL_CARRIER_4RX, "reference_fd16x64": P.REFERENCE_FD16X64}
 
 
 @pytest.mark.parametrize("name", list(DET_PROFILES))

@@ -736,3 +736,117 @@ def test_ldpc_ref_bits_follow_the_int16_posterior(oracle):
         assert near.sum() > 1000, near.sum()
         assert not bits[near].any()
         assert np.array_equal(bits[p < -0.5], np.ones(int((p < -0.5).sum()), np.int8))
+
+
+# ------------------------------------------------------------ SISO LS / ZF
+def _siso_ls(oracle, cfg, rx_re, rx_im, ref_re, ref_im):
+    g = OracleGeom(cfg)
+    hr = np.zeros(14 * g.pad, np.uint16)
+    hi = np.zeros_like(hr)
+    oracle.or_ls_siso(cref(cfg), ptr(rx_re), ptr(rx_im), ptr(ref_re), ptr(ref_im), ptr(hr), ptr(hi))
+    return (hr.view(np.float16).astype(np.float64) + 1j * hi.view(np.float16).astype(np.float64)).reshape(14, g.pad)
+
+
+def _siso_grid(oracle, cfg, h, rng):
+    """Received fp16 grid [14, pad] of data symbols and the DMRS through the
+    per-RE channel h [14, used], with the DMRS reference of airan_mimo_dmrs_gen."""
+    g = OracleGeom(cfg)
+    ref_re = np.zeros(g.ndmrs * g.ref_pad, np.uint16)
+    ref_im = np.zeros_like(ref_re)
+    oracle.or_dmrs_gen(cref(cfg), ptr(ref_re), ptr(ref_im))
+    x = (ref_re.view(np.float16).astype(np.float64) + 1j * ref_im.view(np.float16).astype(np.float64))
+    x = x.reshape(g.ndmrs, g.ref_pad)
+    s = (rng.choice([-1.0, 1.0], (14, g.used)) + 1j * rng.choice([-1.0, 1.0], (14, g.used))) / np.sqrt(2)
+    for d in range(g.ndmrs):
+        s[g.dmrs_sym[d], 0::2] = x[d, : g.n_dmrs_re]
+    y = np.zeros((14, g.pad), np.complex128)
+    y[:, : g.used] = h * s
+    return (y.real.astype(np.float16).view(np.uint16).ravel(), y.imag.astype(np.float16).view(np.uint16).ravel(),
+            ref_re, ref_im)
+
+
+@pytest.mark.parametrize("prof", [P.SISO_CONFIG1, P.SISO_133PRB], ids=lambda p: p.name)
+def test_ls_siso_pilots_are_dmrs_ls(oracle, prof):
+    """The SISO estimate at the DMRS symbols' even subcarriers is the DMRS-LS
+    observation h_ls of port 1000 (the same y x* in fp16, pinned against the
+    reference's DMRS-LS profile), and its odd subcarriers the midpoints."""
+    cfg = cfg_of(prof)
+    g = OracleGeom(cfg)
+    rng = np.random.default_rng(11)
+    h = (rng.normal(size=(14, g.used)) + 1j * rng.normal(size=(14, g.used))) * 0.7
+    yr, yi, xr, xi = _siso_grid(oracle, cfg, h, rng)
+    H = _siso_ls(oracle, cfg, yr, yi, xr, xi)
+    hls_r = np.zeros(g.ndmrs * g.pilot_pad, np.uint16)
+    hls_i = np.zeros_like(hls_r)
+    sc = np.zeros(g.ndmrs * g.pilot_pad, np.uint16)
+    cnt = np.zeros(16, np.uint16)
+    noise = np.zeros(1)
+    oracle.or_dmrs_ls(cref(cfg), ptr(yr), ptr(yi), ptr(xr), ptr(xi), ptr(hls_r), ptr(hls_i), ptr(sc), ptr(cnt),
+                      ptr(noise))
+    hls = (hls_r.view(np.float16).astype(np.float64) + 1j * hls_i.view(np.float16).astype(np.float64))
+    hls = hls.reshape(g.ndmrs, g.pilot_pad)[:, : g.n_dmrs_re]
+    for d in range(g.ndmrs):
+        sym = g.dmrs_sym[d]
+        assert np.array_equal(H[sym, 0:g.used:2], hls[d])
+        mid = np.append(hls[d][1:], 0.0)  # the pilot past the last one is zero
+        want = ((hls[d] + mid).astype(np.complex64))
+        want = want.real.astype(np.float16).astype(np.float64) * 0.5 + 1j * want.imag.astype(np.float16) * 0.5
+        assert np.allclose(H[sym, 1:g.used:2], want, rtol=1e-3, atol=1e-3)
+    assert not H[:, g.used:].any()
+
+
+def test_ls_siso_exact_on_linear_channels(oracle):
+    """Linear interpolation reproduces a channel linear in frequency and in
+    time (DMRS symbols {2, 11}: interpolated between, extrapolated outside)
+    to fp16 accuracy; the orientation of the time weights is fixed by it."""
+    cfg = cfg_of(P.SISO_133PRB)
+    g = OracleGeom(cfg)
+    rng = np.random.default_rng(12)
+    sc = np.arange(g.used)
+    sym = np.arange(14)[:, None]
+    h = (0.6 + 0.2j) + (1e-4 - 2e-4j) * sc[None, :] + (0.03 + 0.02j) * (sym - 6)
+    yr, yi, xr, xi = _siso_grid(oracle, cfg, h, rng)
+    H = _siso_ls(oracle, cfg, yr, yi, xr, xi)
+    inner = slice(0, g.used - 2)  # the last odd subcarrier meets the zero-extended pilot
+    err = np.abs(H[:, inner] - h[:, inner]) / np.abs(h[:, inner])
+    assert err.max() < 6e-3
+    # One DMRS symbol: the same estimate on every symbol.
+    cfg1 = cfg_of(P.SISO_CONFIG1)
+    g1 = OracleGeom(cfg1)
+    h1 = np.broadcast_to((0.5 - 0.3j) + (2e-4 + 1e-4j) * np.arange(g1.used)[None, :], (14, g1.used))
+    yr, yi, xr, xi = _siso_grid(oracle, cfg1, h1, rng)
+    H1 = _siso_ls(oracle, cfg1, yr, yi, xr, xi)
+    assert all(np.array_equal(H1[s], H1[2]) for s in range(14))
+    assert np.abs(H1[:, : g1.used - 2] - h1[:, : g1.used - 2]).max() < 4e-3
+
+
+def test_zf_siso_closed_form(oracle):
+    """ZF recovers x = y / h and no_eff = n0 / |h|^2 (to fp16), with the
+    channel power floored at 1e-4; the fp32 arithmetic equals numpy's IEEE
+    fp32 on the same fp16 inputs."""
+    rng = np.random.default_rng(13)
+    n = 4096
+    h = (rng.normal(size=n) + 1j * rng.normal(size=n)) * 0.8
+    h[:64] *= 1e-3  # |h|^2 below the floor
+    x = (rng.choice([-3, -1, 1, 3], n) + 1j * rng.choice([-3, -1, 1, 3], n)) / np.sqrt(10)
+    y = h * x
+    f16 = lambda a: a.astype(np.float16)
+    hr, hi, yr, yi = f16(h.real), f16(h.imag), f16(y.real), f16(y.imag)
+    n0 = f16(rng.uniform(0.001, 0.1, n))
+    out = [np.zeros(n, np.uint16) for _ in range(3)]
+    oracle.or_zf_siso(n, ptr(yr.view(np.uint16)), ptr(yi.view(np.uint16)), ptr(hr.view(np.uint16)),
+                      ptr(hi.view(np.uint16)), ptr(n0.view(np.uint16)), *[ptr(o) for o in out])
+    xr, xi, ne = (o.view(np.float16).astype(np.float64) for o in out)
+    hq = hr.astype(np.float64) + 1j * hi.astype(np.float64)
+    g = np.maximum(np.abs(hq) ** 2, 1e-4)
+    ok = np.abs(hq) ** 2 > 1e-3
+    assert np.abs((xr + 1j * xi)[ok] - x[ok]).max() < 2e-2
+    assert np.allclose(ne, n0.astype(np.float64) / g, rtol=2e-3)
+    assert np.allclose(ne[:64], n0[:64].astype(np.float64) / 1e-4, rtol=2e-3)
+    # fp32 arithmetic (numpy float32 ops are IEEE single with one rounding each).
+    h32r, h32i, y32r, y32i = (a.astype(np.float32) for a in (hr, hi, yr, yi))
+    g32 = np.maximum(h32r * h32r + h32i * h32i, np.float32(1e-4))
+    want_r = ((h32r * y32r + h32i * y32i) / g32).astype(np.float16)
+    want_i = ((h32r * y32i - h32i * y32r) / g32).astype(np.float16)
+    assert np.array_equal(out[0], want_r.view(np.uint16))
+    assert np.array_equal(out[1], want_i.view(np.uint16))
