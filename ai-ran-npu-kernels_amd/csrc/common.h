@@ -21,6 +21,9 @@ constexpr uint32_t AlignUp(uint32_t v, uint32_t a) { return (v + a - 1) / a * a;
 
 // Per-device FFT twiddle table (op_ofdm.hip); idempotent.
 int InitOfdmTables();
+// Per-device address of the decoder's no-alloc message array (op_ldpc.hip);
+// idempotent.
+int32_t InitLdpcScratch();
 
 // Geometry derived from a PuschMimoConfig; one definition for every operator
 // so that SameLayout checks agree across the chain.

@@ -51,6 +51,7 @@ int InitGoldTables()
 
 extern "C" int32_t airan_InitDevice(void)
 {
-    const int32_t st = airan::InitGoldTables();
-    return st != AIRAN_OK ? st : airan::InitOfdmTables();
+    int32_t st = airan::InitGoldTables();
+    if (st == AIRAN_OK) st = airan::InitOfdmTables();
+    return st != AIRAN_OK ? st : airan::InitLdpcScratch();
 }

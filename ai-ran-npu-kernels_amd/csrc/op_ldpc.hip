@@ -24,6 +24,14 @@
 // Posterior of the code block owned by the workgroup ([68][Z] x 16 bit).
 extern __shared__ uint16_t airan_ldpc_lam[];
 
+// Message state of airan_ldpc_decode_EnqueueNoAlloc: a fixed device array
+// instead of a caller workspace, for callers that cannot allocate.  Each GPU
+// the library runs on holds its own copy until the process ends.  32 MiB
+// covers the reference's 23-slot transport block (143 code blocks at all 46
+// rows, Z = 384: 20.2 MB).
+constexpr size_t kNoAllocScratchBytes = size_t{32} << 20;
+static __device__ __align__(256) uint2 scratch[kNoAllocScratchBytes / sizeof(uint2)];
+
 namespace {
 
 struct Bg1Dev {
@@ -55,6 +63,11 @@ int32_t LoadTable()
     g_table_loaded[dev] = true;
     return AIRAN_OK;
 }
+
+// Device address of `scratch` per device (hipGetSymbolAddress with that
+// device current, at its first airan_InitDevice); the array's name is a host
+// stand-in and must not be passed to a kernel.
+void *g_scratch_ptr[64] = {};
 
 constexpr int kMaxDeg = AIRAN_BG1_MAX_DEG;
 
@@ -818,6 +831,27 @@ size_t airan_ldpc_decode_WorkspaceBytes(uint32_t num_cb, uint32_t z, uint32_t ac
     return static_cast<size_t>(num_cb) * active_rows * z * sizeof(uint2);
 }
 
+size_t airan_ldpc_decode_NoAllocMaxWorkspaceBytes(void) { return kNoAllocScratchBytes; }
+
+int32_t airan_ldpc_decode_EnqueueNoAlloc(const LdpcDecodeOpArgsV1 *a)
+{
+    int32_t st = airan_ldpc_decode_ValidateOpArgs(a);
+    if (st != AIRAN_OK) return st;
+    const size_t need = airan_ldpc_decode_WorkspaceBytes(a->num_cb, a->z, a->active_rows);
+    if (need > kNoAllocScratchBytes) return AIRAN_RESOURCE_TOO_SMALL;
+    if (need == 0) return airan_ldpc_decode_Enqueue(a, nullptr, 0);
+    if ((st = airan_InitDevice()) != AIRAN_OK) return st;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return AIRAN_LAUNCH_FAILED;
+    void *ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_table_mutex);
+        ws = g_scratch_ptr[dev];
+    }
+    if (ws == nullptr) return AIRAN_LAUNCH_FAILED;
+    return airan_ldpc_decode_Enqueue(a, ws, need);
+}
+
 int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, size_t workspace_bytes)
 {
     int32_t st = airan_ldpc_decode_ValidateOpArgs(a);
@@ -861,6 +895,23 @@ int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, 
 }  // extern "C"
 
 namespace airan {
+
+int32_t InitLdpcScratch()
+{
+    // The decoder's constant tables too, so that a caller who ran
+    // airan_InitDevice launches nothing but the decoder afterwards.
+    int32_t st = LoadTable();
+    if (st == AIRAN_OK) st = LoadLiftTable();
+    if (st != AIRAN_OK) return st;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return AIRAN_LAUNCH_FAILED;
+    std::lock_guard<std::mutex> lock(g_table_mutex);
+    if (g_scratch_ptr[dev] != nullptr) return AIRAN_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return AIRAN_LAUNCH_FAILED;
+    g_scratch_ptr[dev] = p;
+    return AIRAN_OK;
+}
 
 int32_t LaunchLdpcFromCodeword(const Geom &g, const AiranTbPlanV1 &p, const RateMatchDescriptor *desc,
                                const int16_t *cw, uint32_t num_cb, uint32_t rows, uint32_t iters, bool early_stop,

@@ -375,6 +375,8 @@ PROTOTYPES = {
     "airan_ldpc_decode_ValidateOpArgs": (i32, [P(LdpcDecodeOpArgsV1)]),
     "airan_ldpc_decode_WorkspaceBytes": (C.c_size_t, [u32, u32, u32]),
     "airan_ldpc_decode_Enqueue": (i32, [P(LdpcDecodeOpArgsV1), vp, C.c_size_t]),
+    "airan_ldpc_decode_EnqueueNoAlloc": (i32, [P(LdpcDecodeOpArgsV1)]),
+    "airan_ldpc_decode_NoAllocMaxWorkspaceBytes": (C.c_size_t, []),
     "airan_ldpc_ActiveRows": (u32, [P(AiranTbPlanV1)]),
     "airan_crc_check_WorkspaceBytes": (C.c_size_t, [u32]),
     "airan_crc_check_Enqueue": (i32, [P(AiranTbPlanV1), u32, vp, vp, vp, vp, vp, C.c_size_t, vp]),

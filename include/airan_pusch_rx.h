@@ -34,9 +34,11 @@ extern "C" {
 #endif
 
 /* ============================================================ device init */
-/* Loads the per-device constant tables (Gold jump-ahead matrices) for the
- * current device.  Called implicitly by the first Enqueue on a device; call
- * it explicitly before capturing a stream into a HIP graph. */
+/* Loads the per-device constant tables (Gold jump-ahead matrices, FFT
+ * twiddles, LDPC base graph) and takes the device address of the decoder's
+ * no-alloc message array for the current device.  Called implicitly by the
+ * first Enqueue on a device; call it explicitly before capturing a stream
+ * into a HIP graph. */
 int32_t airan_InitDevice(void);
 
 /* ======================================================= runtime config */
@@ -718,6 +720,22 @@ int32_t airan_ldpc_decode_ValidateOpArgs(const LdpcDecodeOpArgsV1 *args);
 size_t airan_ldpc_decode_WorkspaceBytes(uint32_t num_cb, uint32_t z, uint32_t active_rows);
 int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *args, void *workspace,
                                   size_t workspace_bytes);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Decodes exactly as airan_ldpc_decode_Enqueue, with the
+ * decoder's message state (WorkspaceBytes) in a fixed device array of the
+ * library instead of a caller workspace, at most
+ * NoAllocMaxWorkspaceBytes() = 32 MiB (the reference's 143-block transport
+ * block at 46 rows needs 20.2 MB); larger requests return
+ * AIRAN_RESOURCE_TOO_SMALL.  Allocates nothing: the array's device address
+ * is taken once per device by airan_InitDevice (called here if it has not
+ * run on the current device; call it first to keep this entry point to the
+ * decoder launch alone).  The array is shared by every call on one GPU and
+ * keeps the last call's contents (the decoder clears the state it reads),
+ * so calls that overlap on one GPU (concurrent streams or threads) race:
+ * serialise them. */
+size_t airan_ldpc_decode_NoAllocMaxWorkspaceBytes(void);
+int32_t airan_ldpc_decode_EnqueueNoAlloc(const LdpcDecodeOpArgsV1 *args);
 /* Active rows for a TB plan (see LdpcDecodeOpArgsV1.active_rows). */
 uint32_t airan_ldpc_ActiveRows(const AiranTbPlanV1 *plan);
 
