@@ -11,6 +11,7 @@
 #include "airan_ce_refiner.h"
 #include "airan_pusch_chain.h"
 #include "common.h"
+#include "cgemm.h"
 #include "crc.h"
 #include "detect.h"
 #include "gold.h"
@@ -58,7 +59,97 @@ __device__ inline int16_t SatNeg16(int v) { return static_cast<int16_t>(v == -32
 // indexes registers); QM = 0 takes it from the arguments.  ZF (one Rx, one
 // layer): single-tap zero forcing with n0 = the slot's noise (the
 // equalize_zf_siso arithmetic) in place of the LMMSE solve.
-template <int NL, int QM, bool ZF = false>
+typedef _Float16 h16x4_t __attribute__((ext_vector_type(4)));
+typedef short s16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <uint32_t GRAM>
+__device__ inline f32x4_t Mfma4(h16x4_t a, h16x4_t b, f32x4_t c)
+{
+    if constexpr (GRAM == AIRAN_GRAM_FP16) {
+        return __builtin_amdgcn_mfma_f32_4x4x4f16(a, b, c, 0, 0, 0);
+    } else {
+        // bf16 operands: the fp16 values rounded to 8 significand bits.
+        s16x4_t ab, bb;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ab[k] = static_cast<short>(airan::Bf16Bits(static_cast<float>(a[k])));
+            bb[k] = static_cast<short>(airan::Bf16Bits(static_cast<float>(b[k])));
+        }
+        return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(ab, bb, c, 0, 0, 0);
+    }
+}
+
+// Gram matrix and matched filter of the 64 REs (subcarriers) of this wave on
+// the matrix cores.  v_mfma_f32_4x4x4_16b: lane 4 b + i holds row i of block
+// b's A and column i of its B (K = the 4 items), D[r][i] of block b lands in
+// lane 4 b + i, register r.  Block b is one RE: lane 4 b + j loads H[rx][j]
+// for 4 Rx at a time, which serves as A (rows = layers: A = H^T) and as B,
+// so D = H^T H per part; G = H^H H = (Hr^T Hr + Hi^T Hi) + i (Hr^T Hi -
+// Hi^T Hr), m = H^H y likewise with y in every column of B.  Four groups of
+// 16 REs per wave; the columns go through LDS to the thread that owns the
+// RE (lane r = subcarrier base + r), which gets A (lower triangle) and m.
+template <int NL, uint32_t GRAM>
+__device__ inline void GramMfma(const __half *__restrict__ rx_re, const __half *__restrict__ rx_im,
+                                const __half *__restrict__ h_re, const __half *__restrict__ h_im, size_t rx_slot,
+                                size_t h_slot, size_t plane, uint32_t sym_off, uint32_t sc_base, uint32_t used,
+                                uint32_t nr, float (*gl)[40], Cf (&A)[4][4], Cf (&m)[4])
+{
+    const uint32_t lane = threadIdx.x & 63u, b = lane >> 2, j = lane & 3u;
+#pragma unroll
+    for (uint32_t g = 0; g < 4; ++g) {
+        const uint32_t scg = sc_base + 16 * g + b;
+        const bool re_ok = scg < used;
+        const bool h_ok = re_ok && j < static_cast<uint32_t>(NL);
+        f32x4_t gr = {0, 0, 0, 0}, gi = gr, mr = gr, mi = gr;
+        for (uint32_t r0 = 0; r0 < nr; r0 += 4) {
+            h16x4_t hr, hi, yr, yi;
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t r = r0 + k;
+                const bool rk = r < nr;
+                const size_t hidx = h_slot + (static_cast<size_t>(r) * NL + j) * plane + sym_off + scg;
+                const size_t yidx = rx_slot + static_cast<size_t>(r) * plane + sym_off + scg;
+                hr[k] = h_ok && rk ? __builtin_bit_cast(_Float16, h_re[hidx]) : static_cast<_Float16>(0.0f);
+                hi[k] = h_ok && rk ? __builtin_bit_cast(_Float16, h_im[hidx]) : static_cast<_Float16>(0.0f);
+                yr[k] = re_ok && rk ? __builtin_bit_cast(_Float16, rx_re[yidx]) : static_cast<_Float16>(0.0f);
+                yi[k] = re_ok && rk ? __builtin_bit_cast(_Float16, rx_im[yidx]) : static_cast<_Float16>(0.0f);
+            }
+            const h16x4_t nhi = -hi;
+            gr = Mfma4<GRAM>(hr, hr, gr);
+            gr = Mfma4<GRAM>(hi, hi, gr);
+            gi = Mfma4<GRAM>(hr, hi, gi);
+            gi = Mfma4<GRAM>(nhi, hr, gi);
+            mr = Mfma4<GRAM>(hr, yr, mr);
+            mr = Mfma4<GRAM>(hi, yi, mr);
+            mi = Mfma4<GRAM>(hr, yi, mi);
+            mi = Mfma4<GRAM>(nhi, yr, mi);
+        }
+        float *dst = gl[16 * g + b];
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            dst[(i * 4 + j) * 2] = gr[i];
+            dst[(i * 4 + j) * 2 + 1] = gi[i];
+        }
+        if (j == 0) {
+#pragma unroll
+            for (uint32_t i = 0; i < 4; ++i) {
+                dst[32 + 2 * i] = mr[i];
+                dst[33 + 2 * i] = mi[i];
+            }
+        }
+    }
+    __syncthreads();
+    const float *src = gl[lane];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        m[i] = {src[32 + 2 * i], src[33 + 2 * i]};
+#pragma unroll
+        for (int jj = 0; jj <= i; ++jj) A[i][jj] = {src[(i * 4 + jj) * 2], src[(i * 4 + jj) * 2 + 1]};
+    }
+}
+
+template <int NL, int QM, bool ZF = false, uint32_t GRAM = AIRAN_GRAM_FP32>
 __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restrict__ rx_re,
                                                          const __half *__restrict__ rx_im,
                                                          const __half *__restrict__ h_re,
@@ -68,7 +159,9 @@ __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restric
                                                          int16_t *__restrict__ cw, DemapArgs a)
 {
     const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sc >= a.used) return;
+    // The matrix-core Gram needs every lane of the workgroup (the MFMAs and
+    // the barrier); its lanes past the last subcarrier leave after it.
+    if (GRAM == AIRAN_GRAM_FP32 && sc >= a.used) return;
     const uint32_t ds = blockIdx.y, slot = blockIdx.z;
     const uint32_t sym = a.data_sym[ds];
     const uint32_t nr = a.nr;
@@ -93,6 +186,13 @@ __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restric
         static_assert(NL == 1, "zero forcing is the SISO equaliser");
         const Cf h = loadh(0, 0), y = loady(0);
         airan::ZfSolve(h.r, h.i, y.r, y.i, no, &xh[0].r, &xh[0].i, &ne[0]);
+    } else if constexpr (GRAM != AIRAN_GRAM_FP32) {
+        __shared__ float gl[4][64][40];  // per wave: 64 REs x (G 4x4 + m 4) complex
+        Cf A[4][4], m[4];
+        GramMfma<NL, GRAM>(rx_re, rx_im, h_re, h_im, rx_slot, h_slot, plane, static_cast<uint32_t>(sym) * a.pad,
+                           sc - (threadIdx.x & 63u), a.used, nr, gl[threadIdx.x >> 6], A, m);
+        if (sc >= a.used) return;
+        airan::DetectFromGram<NL>(no, A, m, xh, ne);
     } else {
         airan::Detect<NL>(static_cast<int>(nr), no, loadh, loady, xh, ne);
     }
@@ -136,6 +236,29 @@ __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restric
             for (int l = 0; l < NL; ++l) o[l] = v[b][l];
         }
     }
+}
+
+using DdKernel = void (*)(const __half *, const __half *, const __half *, const __half *, const __half *,
+                          const uint32_t *, int16_t *, DemapArgs);
+
+template <uint32_t GRAM>
+DdKernel PickDetectGram(uint32_t nl, bool q8)
+{
+    switch (nl) {
+    case 1: return q8 ? DetectDemapKernel<1, 8, false, GRAM> : DetectDemapKernel<1, 0, false, GRAM>;
+    case 2: return q8 ? DetectDemapKernel<2, 8, false, GRAM> : DetectDemapKernel<2, 0, false, GRAM>;
+    case 3: return q8 ? DetectDemapKernel<3, 8, false, GRAM> : DetectDemapKernel<3, 0, false, GRAM>;
+    default: return q8 ? DetectDemapKernel<4, 8, false, GRAM> : DetectDemapKernel<4, 0, false, GRAM>;
+    }
+}
+
+// nl = 0: the SISO zero-forcing equaliser.
+DdKernel PickDetect(uint32_t nl, bool q8, uint32_t gram)
+{
+    if (nl == 0) return q8 ? DetectDemapKernel<1, 8, true> : DetectDemapKernel<1, 0, true>;
+    if (gram == AIRAN_GRAM_FP16) return PickDetectGram<AIRAN_GRAM_FP16>(nl, q8);
+    if (gram == AIRAN_GRAM_BF16) return PickDetectGram<AIRAN_GRAM_BF16>(nl, q8);
+    return PickDetectGram<AIRAN_GRAM_FP32>(nl, q8);
 }
 
 // Host fp32 -> fp16 bits, round to nearest even (normal range only).
@@ -272,7 +395,7 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, const uint16_t *slot_n
     oa.iq = iq;
     oa.iq_format = c->opts.iq_format;
     oa.batch = slots * g.nr;
-    oa.precision = AIRAN_DFT_FP32;
+    oa.precision = c->opts.dft_precision;
     oa.config = &c->cfg;
     oa.stream = s;
     int32_t st = airan_ofdm_demod_redemap_Enqueue(&oa, c->rx_re, c->rx_im);
@@ -311,17 +434,7 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, const uint16_t *slot_n
     for (uint32_t i = 0; i < g.ndata; ++i) da.data_sym[i] = g.data_sym[i];
     da.floor16 = HostF2H(airan_pusch_NoEffFloor(g.qm));
     const dim3 grid(airan::CeilDiv(g.used, 256), g.ndata, slots);
-    using DdKernel = void (*)(const __half *, const __half *, const __half *, const __half *, const __half *,
-                              const uint32_t *, int16_t *, DemapArgs);
-    const bool q8 = g.qm == 8;
-    DdKernel dd;
-    switch (c->ls_zf ? 0u : g.nl) {
-    case 0: dd = q8 ? DetectDemapKernel<1, 8, true> : DetectDemapKernel<1, 0, true>; break;
-    case 1: dd = q8 ? DetectDemapKernel<1, 8> : DetectDemapKernel<1, 0>; break;
-    case 2: dd = q8 ? DetectDemapKernel<2, 8> : DetectDemapKernel<2, 0>; break;
-    case 3: dd = q8 ? DetectDemapKernel<3, 8> : DetectDemapKernel<3, 0>; break;
-    default: dd = q8 ? DetectDemapKernel<4, 8> : DetectDemapKernel<4, 0>; break;
-    }
+    const DdKernel dd = PickDetect(c->ls_zf ? 0u : g.nl, g.qm == 8, c->opts.gram_precision);
     hipLaunchKernelGGL(dd, grid, dim3(256), 0, s, c->rx_re, c->rx_im, c->h_re, c->h_im,
                        c->ls_zf ? c->noise_ls : c->noise_ce, c->gold, c->cw, da);
     if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
@@ -384,7 +497,8 @@ bool OptionsOk(const AiranPuschChainOptionsV1 &o)
            (o.iq_format == AIRAN_IQ_INT16_Q8_8 || o.iq_format == AIRAN_IQ_COMPLEX64) && o.ldpc_iterations >= 1 &&
            o.ldpc_iterations <= 64 && (o.ldpc_mode == AIRAN_LDPC_MODE_REF || o.ldpc_mode == AIRAN_LDPC_MODE_FP16) &&
            o.ce_max_rank >= 1 && o.ce_max_rank <= 512 && o.tau_s >= 0.0 && o.fd_hz >= 0.0 && o.scs_hz > 0.0 &&
-           o.eig_floor >= 0.0 && (o.receiver == AIRAN_CHAIN_RX_LMMSE || o.receiver == AIRAN_CHAIN_RX_LS_ZF);
+           o.eig_floor >= 0.0 && (o.receiver == AIRAN_CHAIN_RX_LMMSE || o.receiver == AIRAN_CHAIN_RX_LS_ZF) &&
+           o.dft_precision <= AIRAN_DFT_FP16 && o.gram_precision <= AIRAN_GRAM_FP16;
 }
 
 }  // namespace
@@ -427,6 +541,8 @@ int32_t airan_pusch_chain_Create(const PuschMimoConfig *config, const AiranTbPla
         return code;
     };
     if (c->ls_zf && (g.nr != 1 || g.nl != 1 || config->dmrs_ports[0] != 1000)) return fail(AIRAN_UNSUPPORTED_PROFILE);
+    if (options->dft_precision != AIRAN_DFT_FP32 && !airan::MfmaDftSupported(g.fft))
+        return fail(AIRAN_UNSUPPORTED_PROFILE);
     if (hipGetDevice(&c->device) != hipSuccess || airan_InitDevice() != AIRAN_OK) return fail(AIRAN_LAUNCH_FAILED);
     LmmseFilterParamsV1 fp{};
     fp.abi_version = AIRAN_PUSCH_MIMO_ABI_VERSION;

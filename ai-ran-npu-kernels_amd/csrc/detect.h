@@ -92,6 +92,27 @@ __device__ inline void HermSolve(Cf (&A)[4][4], Cf (&m)[4], Cf (&x)[4], float (&
     }
 }
 
+// Unbiased LMMSE from the Gram matrix A = H^H H (lower triangle) and the
+// matched filter m = H^H y: x = (A + no I)^-1 m scaled by 1 / d per layer,
+// d = max(1 - no diag((A + no I)^-1), 1e-3), no_eff = (1 - d) / d.
+template <int NL>
+__device__ inline void DetectFromGram(float no, Cf (&A)[4][4], Cf (&m)[4], Cf (&xh)[4], float (&ne)[4])
+{
+#pragma clang fp contract(on)
+#pragma unroll
+    for (int i = 0; i < NL; ++i) A[i][i].r += no;
+    Cf x[4];
+    float dinv[4];
+    HermSolve<NL>(A, m, x, dinv);
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        const float d = fmaxf(1.0f - no * dinv[l], 1e-3f);
+        const float inv = 1.0f / d;
+        xh[l] = {x[l].r * inv, x[l].i * inv};
+        ne[l] = (1.0f - d) * inv;
+    }
+}
+
 // One thread per RE.  H(r, l) and y(r) are fetched by the caller-specific
 // loader; the solve is shared.
 template <int NL, typename LoadH, typename LoadY>
@@ -117,18 +138,7 @@ __device__ inline void Detect(int nr, float no, LoadH loadh, LoadY loady, Cf (&x
             for (int j = 0; j <= i; ++j) A[i][j] = cadd(A[i][j], cmulc(h[i], h[j]));  // (H^H H)_ij
         }
     }
-#pragma unroll
-    for (int i = 0; i < NL; ++i) A[i][i].r += no;
-    Cf x[4];
-    float dinv[4];
-    HermSolve<NL>(A, m, x, dinv);
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-        const float d = fmaxf(1.0f - no * dinv[l], 1e-3f);
-        const float inv = 1.0f / d;
-        xh[l] = {x[l].r * inv, x[l].i * inv};
-        ne[l] = (1.0f - d) * inv;
-    }
+    DetectFromGram<NL>(no, A, m, xh, ne);
 }
 
 // Single-tap zero forcing (equalization/equalize_zf_siso/equalize_kernel.cpp:

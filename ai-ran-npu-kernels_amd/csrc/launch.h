@@ -22,6 +22,14 @@ int32_t LaunchDmrsLs(const Geom &g, const __half *rx_re, const __half *rx_im, co
 int32_t LaunchDmrsGenSlots(const PuschMimoConfig &cfg, const Geom &g, const uint16_t *slot_no, uint32_t slots,
                            __half *ref_re, __half *ref_im, hipStream_t s);
 
+// OFDM demod as two DFT products on the matrix cores (op_ofdm_mfma.hip):
+// precision AIRAN_DFT_BF16 or AIRAN_DFT_FP16, fft 1024 / 2048 / 4096;
+// out_mode 0 stage-4 order, 1 the used-subcarrier grid.
+bool MfmaDftSupported(uint32_t fft);
+int32_t InitMfmaDftTables();
+int32_t LaunchMfmaDft(const Geom &g, uint32_t precision, uint32_t iq_format, uint32_t batch, const void *iq,
+                      int out_mode, __half *out_re, __half *out_im, hipStream_t s);
+
 // SISO LS + linear interpolation (op_siso.hip) for `slots` stacked grids
 // rx [slots,1,14,pad] -> H [slots,1,1,14,pad]; ref [1 or slots][D][ref_pad].
 // AIRAN_UNSUPPORTED_PROFILE unless the config is 1 Rx, rank 1 on port 1000

@@ -10,11 +10,14 @@
 // from HBM; the last pass leaves natural order in LDS and the write-out
 // permutes into the reference's stage-4 order ([P,64], bin k at
 // (k % P) * 64 + k / P) or, fused, straight into the used-subcarrier grid
-// that re_demap_batch would produce.
+// that re_demap_batch would produce.  The BF16 / FP16 precisions compute the
+// DFT as the reference does, as two dense products on the matrix cores
+// (op_ofdm_mfma.hip).
 #include <cmath>
 #include <mutex>
 
 #include "common.h"
+#include "launch.h"
 
 using airan::Geom;
 
@@ -315,6 +318,10 @@ int32_t Launch(const OfdmDemodBatchOpArgsV1 *args, int out_mode, void *ore, void
 {
     Geom g;
     airan::DeriveGeom(*args->config, &g);
+    if (args->precision != AIRAN_DFT_FP32)
+        return airan::LaunchMfmaDft(g, args->precision, args->iq_format, args->batch, args->iq, out_mode,
+                                    static_cast<__half *>(ore), static_cast<__half *>(oim),
+                                    static_cast<hipStream_t>(args->stream));
     if (airan::InitOfdmTables() != AIRAN_OK) return AIRAN_LAUNCH_FAILED;
     OfdmArgs a{};
     a.total_symbols = args->batch * g.nsym;
@@ -350,10 +357,12 @@ int32_t airan_ofdm_demod_batch_ValidateOpArgs(const OfdmDemodBatchOpArgsV1 *a)
         a->stream == nullptr || a->batch == 0 || a->reserved != 0)
         return AIRAN_INVALID_ARGUMENT;
     if (a->iq_format != AIRAN_IQ_INT16_Q8_8 && a->iq_format != AIRAN_IQ_COMPLEX64) return AIRAN_INVALID_ARGUMENT;
-    if (a->precision != AIRAN_DFT_FP32) return AIRAN_UNSUPPORTED_PROFILE;
+    if (a->precision != AIRAN_DFT_FP32 && a->precision != AIRAN_DFT_BF16 && a->precision != AIRAN_DFT_FP16)
+        return AIRAN_UNSUPPORTED_PROFILE;
     Geom g;
     const int32_t st = airan::DeriveGeom(*a->config, &g);
     if (st != AIRAN_OK) return st;
+    if (a->precision != AIRAN_DFT_FP32 && !airan::MfmaDftSupported(g.fft)) return AIRAN_UNSUPPORTED_PROFILE;
     if (static_cast<uint64_t>(a->batch) * g.nsym > 0xffffffffull / kThreads) return AIRAN_INVALID_ARGUMENT;
     return AIRAN_OK;
 }

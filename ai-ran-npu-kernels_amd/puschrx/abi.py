@@ -29,6 +29,8 @@ LAUNCH_FAILED = -5
 IQ_INT16_Q8_8 = 0
 IQ_COMPLEX64 = 1
 DFT_FP32 = 0
+DFT_BF16 = 1
+DFT_FP16 = 2
 LDPC_MODE_REF = 0
 LDPC_MODE_FP16 = 1
 OBS_COMB2 = 0
@@ -271,11 +273,15 @@ class AiranPuschChainOptionsV1(C.Structure):
     _fields_ = _args(("max_slots", u32), ("iq_format", u32), ("ldpc_iterations", u32), ("ldpc_mode", u32),
                      ("early_stop", u32), ("ce_max_rank", u32), ("use_graph", u32), ("tau_s", C.c_double),
                      ("fd_hz", C.c_double), ("scs_hz", C.c_double), ("eig_floor", C.c_double),
-                     ("receiver", u32), ("reserved", u32 * 7))
+                     ("receiver", u32), ("dft_precision", u32), ("gram_precision", u32),
+                     ("reserved", u32 * 5))
 
 
 CHAIN_RX_LMMSE = 0
 CHAIN_RX_LS_ZF = 1
+GRAM_FP32 = 0
+GRAM_BF16 = 1
+GRAM_FP16 = 2
 
 
 class AiranPuschChainOutputsV1(C.Structure):

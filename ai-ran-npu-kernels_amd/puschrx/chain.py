@@ -30,6 +30,8 @@ class ChainOptions:
     ce_rank: int = 384
     iq_format: int = abi.IQ_INT16_Q8_8
     receiver: int = abi.CHAIN_RX_LMMSE   # CHAIN_RX_LS_ZF: SISO LS + linear interpolation, zero forcing
+    dft_precision: int = abi.DFT_FP32    # DFT_BF16 / DFT_FP16: the OFDM DFT on the matrix cores
+    gram_precision: int = abi.GRAM_FP32  # GRAM_BF16 / GRAM_FP16: the detector's Gram on the matrix cores
 
 
 class OpChain:
@@ -170,6 +172,8 @@ class NativeChain:
         o.use_graph = 1 if use_graph else 0
         o.tau_s, o.fd_hz, o.scs_hz, o.eig_floor = self.opts.tau_s, self.opts.fd_hz, self.opts.scs_hz, 1e-12
         o.receiver = self.opts.receiver
+        o.dft_precision = self.opts.dft_precision
+        o.gram_precision = self.opts.gram_precision
         self.handle = C.c_void_p()
         ops._check("chain Create", self.lib.airan_pusch_chain_Create(C.byref(cfg), C.byref(plan), C.byref(o),
                                                                     C.c_void_p(ops.stream_ptr()),

@@ -273,16 +273,18 @@ def crc_check(plan: abi.AiranTbPlanV1, num_tbs: int, bits, want_bytes: bool = Fa
 
 
 # ---------------------------------------------------------------- OFDM demod
-def ofdm_demod(cfg, iq, batch: int, iq_format: int = abi.IQ_INT16_Q8_8, fused_redemap: bool = True):
+def ofdm_demod(cfg, iq, batch: int, iq_format: int = abi.IQ_INT16_Q8_8, fused_redemap: bool = True,
+               precision: int = 0):
     """CP removal + unitary DFT of `batch` antenna streams (one slot each).
     fused_redemap: return the used-subcarrier grid [batch,14,padded];
-    otherwise the stage-4 planes [batch,14,fft]."""
+    otherwise the stage-4 planes [batch,14,fft].  precision: abi.DFT_FP32
+    (FFT) or DFT_BF16 / DFT_FP16 (two DFT products on the matrix cores)."""
     lib = load_library()
     g = Geometry(cfg)
     dev = iq.device
     n = batch * 14 * (g.pad if fused_redemap else g.fft)
     ore, oim = _dev(n, torch.float16, dev), _dev(n, torch.float16, dev)
-    a = _hdr(abi.OfdmDemodBatchOpArgsV1, iq=_dp(iq), iq_format=iq_format, batch=batch, precision=abi.DFT_FP32,
+    a = _hdr(abi.OfdmDemodBatchOpArgsV1, iq=_dp(iq), iq_format=iq_format, batch=batch, precision=precision,
              config=C.pointer(cfg), stream=C.c_void_p(stream_ptr()))
     if fused_redemap:
         _check("ofdm_demod_redemap Enqueue", lib.airan_ofdm_demod_redemap_Enqueue(C.byref(a), _dp(ore), _dp(oim)))

@@ -66,7 +66,9 @@ typedef struct AiranPuschChainOptionsV1 {
     double scs_hz;              /* subcarrier spacing */
     double eig_floor;           /* CE: drop eigenvalues below eig_floor * max */
     uint32_t receiver;          /* AIRAN_CHAIN_RX_* */
-    uint32_t reserved[7];
+    uint32_t dft_precision;     /* AIRAN_DFT_* of the OFDM stage (FFT 1024..4096 for BF16 / FP16) */
+    uint32_t gram_precision;    /* AIRAN_GRAM_* of the detector's Gram and matched filter */
+    uint32_t reserved[5];
 } AiranPuschChainOptionsV1;
 /* Channel estimate and equaliser of the chain.  LMMSE: LMMSE CE (filter
  * built from tau_s, fd_hz) and per-RE LMMSE detection, any geometry.  LS_ZF:
@@ -76,6 +78,14 @@ typedef struct AiranPuschChainOptionsV1 {
  * DMRS-LS noise estimate; no CE filter is built. */
 #define AIRAN_CHAIN_RX_LMMSE 0u
 #define AIRAN_CHAIN_RX_LS_ZF 1u
+/* Gram matrix H^H H and matched filter H^H y of the LMMSE detector.  FP32:
+ * per-RE fp32 multiply-adds on the vector units.  FP16 / BF16: on the matrix
+ * cores (v_mfma_f32_4x4x4 with 16 REs per instruction, the H and y samples
+ * as fp16 -- exactly the fp16 values the chain stores -- or rounded to bf16,
+ * fp32 accumulation), then the same fp32 LDL^H solve. */
+#define AIRAN_GRAM_FP32 0u
+#define AIRAN_GRAM_BF16 1u
+#define AIRAN_GRAM_FP16 2u
 
 typedef struct AiranPuschChain AiranPuschChain;
 

@@ -71,7 +71,13 @@ int32_t airan_DerivePuschMimoLayout(const PuschMimoConfig *config, PuschMimoLayo
  * [(k % P) * 64 + k / P], unitary DFT, no fftshift. */
 #define AIRAN_IQ_INT16_Q8_8 0   /* int16 [batch, samples, 2], value / 256 */
 #define AIRAN_IQ_COMPLEX64 1    /* float [batch, samples, 2] */
-#define AIRAN_DFT_FP32 0        /* fp32 FFT (the only precision; the op is HBM bound) */
+#define AIRAN_DFT_FP32 0        /* fp32 radix-16 FFT */
+/* The reference's form: two dense DFT products (W_P x then x W_64, N = 64 P)
+ * on the matrix cores with FP32 accumulation, samples block-scaled per OFDM
+ * symbol by a power of two and rounded to the 16-bit type, the twiddled
+ * intermediate rounded again; FFT 1024, 2048 and 4096 only. */
+#define AIRAN_DFT_BF16 1
+#define AIRAN_DFT_FP16 2
 typedef struct OfdmDemodBatchOpArgsV1 {
     uint16_t abi_version;
     uint16_t struct_size;
@@ -80,7 +86,7 @@ typedef struct OfdmDemodBatchOpArgsV1 {
     uint32_t batch;             /* number of antenna streams */
     void *out_re;
     void *out_im;
-    uint32_t precision;         /* AIRAN_DFT_FP32 */
+    uint32_t precision;         /* AIRAN_DFT_* */
     uint32_t reserved;
     const PuschMimoConfig *config;
     void *stream;

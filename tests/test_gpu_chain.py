@@ -400,3 +400,56 @@ def test_reference_profile_23_slot_tb(oracle, gpu_stream):
                           ptr(want_bits), ptr(want_it))
     assert np.array_equal(bits_nat, want_bits)
     assert np.array_equal(it_nat, want_it)
+
+
+PRECISIONS = [(abi.DFT_FP16, abi.GRAM_FP16), (abi.DFT_BF16, abi.GRAM_BF16), (abi.DFT_FP32, abi.GRAM_FP16)]
+
+
+@pytest.mark.parametrize("dft,gram", PRECISIONS, ids=["fp16", "bf16", "fp32dft_fp16gram"])
+def test_native_chain_mixed_precision(oracle, gpu_stream, dft, gram):
+    """The chain with the OFDM DFT and the detector's Gram/matched filter on
+    the matrix cores (BF16 / FP16 operands, FP32 accumulation) against the
+    FP32 chain on the same slots: the received grid within the DFT's stated
+    NMSE of the FP32 FFT's, the detector's outputs (through the LLRs) nearly
+    identical, and every slot decoded with a passing CRC at 42 dB; the FP16
+    Gram on the FP32 grid changes at most a few LLRs by one step."""
+    from puschrx.chain import NativeChain
+    prof = P.FULL_CARRIER_4RX
+    cfg, g, plan = plan_for(prof)
+    rng = np.random.default_rng(61)
+    S = 4
+    spec = synth.ChannelSpec(taps=16, delay_spread_s=300e-9, doppler_hz=100.0, rx_corr=0.5, snr_db=42.0)
+    payload = synth.random_payload(rng, S, plan.tbs)
+    grid = synth.tx_grid(cfg, plan, payload)
+    H = synth.channel(rng, cfg, spec, S)
+    iq = torch.from_numpy(synth.receive(rng, cfg, grid, H, spec.snr_db, iq_format=abi.IQ_COMPLEX64)).cuda()
+    out = {}
+    for name, (d, gm) in (("fp32", (abi.DFT_FP32, abi.GRAM_FP32)), ("mixed", (dft, gram))):
+        opts = ChainOptions(tau_s=spec.delay_spread_s, fd_hz=spec.doppler_hz, iq_format=abi.IQ_COMPLEX64,
+                            dft_precision=d, gram_precision=gm)
+        nat = NativeChain(cfg, plan, max_slots=S, opts=opts, use_graph=True)
+        nat.enqueue(iq, S)
+        plane = 14 * g.pad
+        r = {"rx_re": nat.fetch("rx_grid_re", torch.float16, S * g.nr * plane),
+             "rx_im": nat.fetch("rx_grid_im", torch.float16, S * g.nr * plane),
+             "cw": nat.fetch("cw_llr", torch.int16, S * g.qm * g.cw_stride),
+             "bits": nat.fetch("bits", torch.int8, S * plan.C * plan.K),
+             "tb_ok": nat.fetch("tb_ok", torch.uint8, S)}
+        torch.cuda.synchronize()
+        out[name] = {k: v.cpu().numpy() for k, v in r.items()}
+        nat.close()
+    a, b = out["fp32"], out["mixed"]
+    ya = a["rx_re"].astype(np.float64) + 1j * a["rx_im"].astype(np.float64)
+    yb = b["rx_re"].astype(np.float64) + 1j * b["rx_im"].astype(np.float64)
+    err = 10 * np.log10(np.sum(np.abs(yb - ya) ** 2) / np.sum(np.abs(ya) ** 2))
+    llr_diff = float(np.mean(a["cw"] != b["cw"]))
+    sign_flip = float(np.mean(np.sign(a["cw"]) != np.sign(b["cw"])))
+    print(f"dft {dft} gram {gram}: grid NMSE vs FP32 {err:.1f} dB, LLRs changed {llr_diff:.4f}, "
+          f"signs flipped {sign_flip:.5f}, tb_ok {b['tb_ok'].tolist()}")
+    if dft == abi.DFT_FP32:
+        assert np.array_equal(a["rx_re"].view(np.uint16), b["rx_re"].view(np.uint16))
+        assert llr_diff < 0.01
+    else:
+        assert err <= {abi.DFT_FP16: -60.0, abi.DFT_BF16: -40.0}[dft]
+    assert b["tb_ok"].tolist() == [1] * S
+    assert np.array_equal(a["bits"], b["bits"])

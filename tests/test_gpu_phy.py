@@ -90,6 +90,53 @@ def test_ofdm_demod_matches_dft(oracle, gpu_stream, name, fmt):
     assert np.array_equal(host(fim).view(np.uint16), host(rim).view(np.uint16))
 
 
+# Stated tolerances of the matrix-core DFT against the FP64 DFT (NMSE; the
+# per-rounding error of an 8-bit / 11-bit significand is about -53 / -71 dB,
+# and the input, the factors and the twiddled intermediate are each rounded).
+MFMA_DFT_NMSE_DB = {abi.DFT_FP16: -60.0, abi.DFT_BF16: -40.0}
+
+
+@pytest.mark.parametrize("name", ["fft4096_4rx", "fft2048_4rx", "fft1024_1rx"])
+@pytest.mark.parametrize("prec", [abi.DFT_FP16, abi.DFT_BF16], ids=["fp16", "bf16"])
+@pytest.mark.parametrize("fmt", [abi.IQ_INT16_Q8_8, abi.IQ_COMPLEX64], ids=["int16", "c64"])
+def test_ofdm_mfma_dft_precision(oracle, gpu_stream, name, prec, fmt):
+    """The BF16 / FP16 matrix-core DFT (block-scaled per symbol, FP32
+    accumulation) against the FP64 DFT: NMSE within the stated tolerance on
+    every antenna, with antenna levels spread over 40 dB (the block scaling
+    keeps weak streams as accurate as strong ones); the fused
+    used-subcarrier write-out equals the stage-4 output re-demapped."""
+    cfg = OFDM_PROFILES[name].config()
+    g = OracleGeom(cfg)
+    rng = np.random.default_rng(17)
+    n = g.nr * g.samples * 2
+    level = 10.0 ** rng.uniform(-1.0, 1.0, (g.nr, 1, 1))  # per-antenna level, +-20 dB
+    if fmt == abi.IQ_INT16_Q8_8:
+        iq_i = np.clip(np.rint(rng.normal(0, 300.0, (g.nr, g.samples, 2)) * level), -32768, 32767).astype(np.int16)
+        iq_i = iq_i.ravel()
+        iq_f = iq_i.astype(np.float32) / 256.0
+        iq_d = dev(iq_i)
+    else:
+        iq_f = (rng.normal(0, 1.0, (g.nr, g.samples, 2)) * level).astype(np.float32).ravel()
+        iq_d = dev(iq_f)
+    want = np.zeros(g.nr * 14 * g.fft * 2)
+    oracle.or_ofdm_demod(cref(cfg), ptr(iq_f), ptr(want))
+    want = (want[0::2] + 1j * want[1::2]).reshape(g.nr, 14, g.fft)
+    sre, sim = ops.ofdm_demod(cfg, iq_d, g.nr, fmt, fused_redemap=False, precision=prec)
+    sre = host(sre).astype(np.float64).reshape(g.nr, 14, g.fft)
+    sim = host(sim).astype(np.float64).reshape(g.nr, 14, g.fft)
+    P_ = g.fft // 64
+    k = np.arange(g.fft)
+    e = (k % P_) * 64 + k // P_
+    got = sre[:, :, e] + 1j * sim[:, :, e]
+    worst = max(nmse_db(got[r], want[r]) for r in range(g.nr))
+    print(f"{name} precision {prec}: NMSE {nmse_db(got, want):.1f} dB (worst antenna {worst:.1f} dB)")
+    assert worst <= MFMA_DFT_NMSE_DB[prec]
+    fre, fim = ops.ofdm_demod(cfg, iq_d, g.nr, fmt, fused_redemap=True, precision=prec)
+    rre, rim = ops.re_demap(cfg, *ops.ofdm_demod(cfg, iq_d, g.nr, fmt, fused_redemap=False, precision=prec))
+    assert np.array_equal(host(fre).view(np.uint16), host(rre).view(np.uint16))
+    assert np.array_equal(host(fim).view(np.uint16), host(rim).view(np.uint16))
+
+
 def test_ofdm_rejects_bad_args(lib, gpu_stream):
     cfg = P.SISO_52PRB.config()
     a = abi.OfdmDemodBatchOpArgsV1()
@@ -98,8 +145,14 @@ def test_ofdm_rejects_bad_args(lib, gpu_stream):
     a.iq_format = 7
     assert lib.airan_ofdm_demod_batch_ValidateOpArgs(C.byref(a)) == abi.INVALID_ARGUMENT
     a.iq_format = abi.IQ_INT16_Q8_8
-    a.precision = 1
+    a.precision = 3
     assert lib.airan_ofdm_demod_batch_ValidateOpArgs(C.byref(a)) == abi.UNSUPPORTED_PROFILE
+    a.precision = abi.DFT_BF16  # the matrix-core DFT serves FFT 1024 .. 4096; this profile is 1024
+    assert lib.airan_ofdm_demod_batch_ValidateOpArgs(C.byref(a)) == abi.OK
+    small = OFDM_PROFILES["fft512"].config()
+    a.config = C.pointer(small)
+    assert lib.airan_ofdm_demod_batch_ValidateOpArgs(C.byref(a)) == abi.UNSUPPORTED_PROFILE
+    a.config = C.pointer(cfg)
     a.precision = abi.DFT_FP32
     a.batch = 0
     assert lib.airan_ofdm_demod_batch_ValidateOpArgs(C.byref(a)) == abi.INVALID_ARGUMENT
