@@ -14,6 +14,7 @@
 #include "cgemm.h"
 #include "crc.h"
 #include "detect.h"
+#include "detect_demap.h"
 #include "gold.h"
 #include "launch.h"
 #include "qam.h"
@@ -40,8 +41,6 @@ __global__ void __launch_bounds__(256) GoldBitsKernel(uint32_t *__restrict__ bit
     airan::GoldStateAt(32ull * w + 1600ull, cinit, &s1, &s2);
     bits[w] = airan::GoldNext32(&s1, &s2);
 }
-
-__device__ inline int16_t SatNeg16(int v) { return static_cast<int16_t>(v == -32768 ? 32767 : -v); }
 
 // Detection + QAM demap + layer demap + descramble for one data RE of one
 // slot (grid (used / 256, ndata, slots)).  The arithmetic is that of the
@@ -202,40 +201,8 @@ __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restric
     const uint64_t bit0 = (static_cast<uint64_t>(slot_in_tb) * a.cw_symbols + static_cast<uint64_t>(re) * NL) * qm;
     const uint32_t w0 = static_cast<uint32_t>(bit0 >> 5);
     const uint64_t gw = (static_cast<uint64_t>(gold[w0]) | (static_cast<uint64_t>(gold[w0 + 1]) << 32)) >> (bit0 & 31u);
-    const __half floor16 = __ushort_as_half(a.floor16);
     int16_t *out = cw + static_cast<size_t>(slot) * qm * a.cw_stride + static_cast<size_t>(re) * NL;
-    // LLRs of all layers first, then one contiguous NL-element store per bit
-    // plane (the thread's REs are adjacent within a plane).
-    int16_t v[8][NL];
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-        const __half xr16 = __float2half_rn(xh[l].r), xi16 = __float2half_rn(xh[l].i);
-        __half ne16 = __float2half_rn(ne[l]);
-        if (__half2float(ne16) < __half2float(floor16)) ne16 = floor16;
-        int llr[8];
-        airan::QamSymbolLlrs(qm, xr16, xi16, ne16, llr);
-#pragma unroll
-        for (uint32_t b = 0; b < 8; ++b) {
-            const uint32_t stream = (b & 1u) ? m + b / 2 : b / 2;
-            const int x = b < qm ? llr[stream] : 0;
-            const bool flip = (gw >> (static_cast<uint32_t>(l) * qm + b)) & 1u;
-            v[b][l] = flip ? SatNeg16(x) : static_cast<int16_t>(x);
-        }
-    }
-    for (uint32_t b = 0; b < qm; ++b) {
-        int16_t *o = out + static_cast<size_t>(b) * a.cw_stride;
-        if constexpr (NL == 4) {
-            *reinterpret_cast<uint2 *>(o) =
-                make_uint2(static_cast<uint16_t>(v[b][0]) | static_cast<uint32_t>(static_cast<uint16_t>(v[b][1])) << 16,
-                           static_cast<uint16_t>(v[b][2]) | static_cast<uint32_t>(static_cast<uint16_t>(v[b][3])) << 16);
-        } else if constexpr (NL == 2) {
-            *reinterpret_cast<uint32_t *>(o) =
-                static_cast<uint16_t>(v[b][0]) | static_cast<uint32_t>(static_cast<uint16_t>(v[b][1])) << 16;
-        } else {
-#pragma unroll
-            for (int l = 0; l < NL; ++l) o[l] = v[b][l];
-        }
-    }
+    airan::DemapStore<NL, QM>(xh, ne, qm, __ushort_as_half(a.floor16), gw, out, a.cw_stride);
 }
 
 using DdKernel = void (*)(const __half *, const __half *, const __half *, const __half *, const __half *,

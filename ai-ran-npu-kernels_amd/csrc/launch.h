@@ -96,4 +96,38 @@ void LaunchCbCrc(const int8_t *bits, const CrcJob &c, uint32_t K, uint32_t num_c
                  hipStream_t s);
 void LaunchCrcFinish(const uint32_t *tb_part, uint8_t *tb_ok, uint32_t num_tbs, uint32_t C, hipStream_t s);
 
+// ---------------------------------------------------- ragged multi-grant batch
+// (ragged.h; every launch computes what the single-grant kernels compute.)
+struct RaggedGrant;
+struct RaggedStream;
+struct RaggedItem;
+struct RaggedCb;
+// OFDM demod + RE demap of num_streams streams of one FFT size.
+int32_t LaunchOfdmRagged(uint32_t fft, uint32_t iq_format, const RaggedStream *streams, uint32_t num_streams,
+                         const void *iq, __half *out_re, __half *out_im, hipStream_t s);
+// DMRS references (items (grant, layer)), DMRS LS (items (grant, rx, layer))
+// and noise (items (grant, rx)); the max_* bound the grids.
+int32_t LaunchDmrsRagged(const RaggedGrant *grants, const RaggedItem *gen_items, uint32_t n_gen,
+                         const RaggedItem *ls_items, uint32_t n_ls, const RaggedItem *noise_items, uint32_t n_noise,
+                         uint32_t max_ref_pad, uint32_t max_pilot_pad, uint32_t max_dmrs, const __half *rx_re,
+                         const __half *rx_im, __half *ref_re, __half *ref_im, __half *h_re, __half *h_im,
+                         __half *noise, hipStream_t s);
+// c_init of DMRS symbol `symbol` of cfg (mimo_dmrs_gen.cpp:38-45).
+uint32_t HostDmrsCinit(const PuschMimoConfig &cfg, uint32_t symbol);
+// Rate dematch of every code block into its decoder input (lam_off).
+int32_t LaunchRateDematchRagged(const RaggedGrant *grants, const RaggedCb *cbs, uint32_t num_cb, const int16_t *cw,
+                                int16_t *out, hipStream_t s);
+// LDPC (FP16 messages) of the listed code blocks: reg_list with <= 8 rows
+// (messages in registers), ws_list with more (messages in ws at ws_off).
+int32_t LaunchLdpcRagged(const RaggedCb *cbs, const uint32_t *reg_list, uint32_t n_reg, const uint32_t *ws_list,
+                         uint32_t n_ws, uint32_t iters, bool early_stop, const int16_t *lam, int8_t *bits,
+                         uint8_t *iters_out, uint2 *ws, hipStream_t s);
+// CRC24B of every code block and the CRC of every TB (tb_first / tb_count
+// index the blocks of each TB).
+int32_t LaunchCrcRagged(const RaggedCb *cbs, uint32_t num_cb, const int8_t *bits, uint8_t *cb_ok, uint32_t *tb_part,
+                        const uint32_t *tb_first, const uint32_t *tb_count, uint8_t *tb_ok, uint32_t num_tbs,
+                        hipStream_t s);
+// The TB CRC polynomial (without its top term) of a plan's crc_tb length.
+uint32_t CrcTbPoly(uint32_t crc_tb);
+
 }  // namespace airan

@@ -18,6 +18,7 @@
 #include "common.h"
 #include "crc.h"
 #include "launch.h"
+#include "ragged.h"
 
 namespace {
 
@@ -368,6 +369,61 @@ __global__ void __launch_bounds__(256) PackKernel(const int8_t *__restrict__ bit
     out[static_cast<size_t>(tb) * nbytes + byte] = static_cast<uint8_t>(v);
 }
 
+
+// Ragged batch: one workgroup per code block of any grant.  The block's
+// CRC24B remainder over its K' bits and the remainder of its part of the
+// TB (its first K' - crc_cb bits) as the last bits of a message, shifted by
+// x^(bits after it in its TB) mod g_TB (the host's per-block `xpow`).
+__global__ void __launch_bounds__(256) RaggedCrcKernel(const int8_t *__restrict__ bits,
+                                                       const airan::RaggedCb *__restrict__ cbs,
+                                                       const uint32_t *__restrict__ pow_b,
+                                                       const uint32_t *__restrict__ pow_a,
+                                                       const uint32_t *__restrict__ pow_16,
+                                                       uint8_t *__restrict__ cb_ok, uint32_t *__restrict__ tb_part)
+{
+    const airan::RaggedCb c = cbs[blockIdx.x];
+    const int8_t *b = bits + c.bits_off;
+    const uint32_t n = c.kprime, L = c.kprime - c.crc_cb;
+    const uint32_t *pw = c.len_tb == 24 ? pow_a : pow_16;
+    uint32_t rb = 0, ra = 0;
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        if (b[j] != 0) {
+            if (c.crc_cb) rb ^= pow_b[n - 1 - j];
+            if (j < L) ra ^= pw[L - 1 - j];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rb ^= __shfl_xor(rb, off);
+        ra ^= __shfl_xor(ra, off);
+    }
+    __shared__ uint32_t part[2][4];
+    if ((threadIdx.x & 63u) == 0) {
+        part[0][threadIdx.x / 64] = rb;
+        part[1][threadIdx.x / 64] = ra;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        rb = part[0][0] ^ part[0][1] ^ part[0][2] ^ part[0][3];
+        ra = part[1][0] ^ part[1][1] ^ part[1][2] ^ part[1][3];
+        cb_ok[blockIdx.x] = c.crc_cb ? static_cast<uint8_t>(rb == 0) : static_cast<uint8_t>(1);
+        tb_part[blockIdx.x] = airan::CrcMulMod(ra, c.xpow, c.poly_tb, c.len_tb);
+    }
+}
+
+// tb_ok[t] = 1 when the parts of TB t's code blocks (first[t] .. first[t] +
+// count[t] - 1) XOR to zero.
+__global__ void __launch_bounds__(64) RaggedCrcFinishKernel(const uint32_t *__restrict__ tb_part,
+                                                            const uint32_t *__restrict__ first,
+                                                            const uint32_t *__restrict__ count,
+                                                            uint8_t *__restrict__ tb_ok, uint32_t num_tbs)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tbs) return;
+    uint32_t x = 0;
+    for (uint32_t r = 0; r < count[t]; ++r) x ^= tb_part[first[t] + r];
+    tb_ok[t] = x == 0 ? 1 : 0;
+}
+
 }  // namespace
 
 namespace airan {
@@ -395,6 +451,23 @@ uint32_t CrcXpowHost(const AiranTbPlanV1 &p, uint32_t r)
         e >>= 1;
     }
     return x;
+}
+
+uint32_t CrcTbPoly(uint32_t crc_tb) { return crc_tb == 24 ? kPolyA : kPoly16; }
+
+int32_t LaunchCrcRagged(const RaggedCb *cbs, uint32_t num_cb, const int8_t *bits, uint8_t *cb_ok, uint32_t *tb_part,
+                        const uint32_t *tb_first, const uint32_t *tb_count, uint8_t *tb_ok, uint32_t num_tbs,
+                        hipStream_t s)
+{
+    CrcTables *t = nullptr;
+    const int32_t st = EnsureTables(&t);
+    if (st != AIRAN_OK) return st;
+    if (num_cb == 0 || num_tbs == 0) return AIRAN_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(RaggedCrcKernel, dim3(num_cb), dim3(256), 0, s, bits, cbs, t->pow_b, t->pow_a, t->pow_16,
+                       cb_ok, tb_part);
+    hipLaunchKernelGGL(RaggedCrcFinishKernel, dim3(CeilDiv(num_tbs, 64)), dim3(64), 0, s, tb_part, tb_first,
+                       tb_count, tb_ok, num_tbs);
+    return LaunchStatus();
 }
 
 int32_t CrcJobSetup(const AiranTbPlanV1 &p, uint8_t *cb_ok, uint32_t *tb_part, CrcJob *out)

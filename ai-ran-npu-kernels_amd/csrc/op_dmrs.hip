@@ -3,6 +3,7 @@
 #include "common.h"
 #include "gold.h"
 #include "launch.h"
+#include "ragged.h"
 
 using airan::DevGeom;
 using airan::Geom;
@@ -44,25 +45,22 @@ __device__ inline uint32_t DevDmrsCinit(uint32_t slot_number, uint32_t symbol, u
 // Grid (m blocks, D, slots * L).  slot_no == nullptr: one reference from
 // d.cinit; else one reference per slot ([slots][L][D][ref_pad]), its c_init
 // from the slot's number slot_no[slot].
-__global__ void __launch_bounds__(256) DmrsGenKernel(uint16_t *__restrict__ re, uint16_t *__restrict__ im,
-                                                     const uint16_t *__restrict__ slot_no, DmrsGenDev d)
+// 16 consecutive pilots m0 .. m0 + 15 of one reference [ref_pad] at `base`:
+// pilot m of DMRS c_init `cinit` (Gold offset `off`), zero past n_dmrs_re.
+__device__ inline void DmrsGen16(uint16_t *__restrict__ re, uint16_t *__restrict__ im, size_t base, uint32_t m0,
+                                 uint32_t ref_pad, uint32_t n_dmrs_re, uint32_t cinit, uint32_t off, bool wf_odd)
 {
-    const uint32_t m0 = (blockIdx.x * blockDim.x + threadIdx.x) * 16;
-    if (m0 >= d.ref_pad) return;
-    const uint32_t dd = blockIdx.y, slot = blockIdx.z / d.nl, l = blockIdx.z - slot * d.nl;
     const uint16_t pos = 0x39a8, neg = 0xb9a8;  // fp16(+-0.70710678)
-    const uint32_t cinit = slot_no == nullptr ? d.cinit[dd] : DevDmrsCinit(slot_no[slot], d.sym[dd], d.nid, d.nscid);
-    const uint32_t bits = airan::GoldWordAt(cinit, 2ull * (m0 + d.off));
-    const size_t base = ((static_cast<size_t>(slot) * d.nl + l) * d.ndmrs + dd) * d.ref_pad;
+    const uint32_t bits = airan::GoldWordAt(cinit, 2ull * (m0 + off));
 #pragma unroll
     for (uint32_t j = 0; j < 16; ++j) {
         const uint32_t m = m0 + j;
-        if (m >= d.ref_pad) break;
+        if (m >= ref_pad) break;
         uint16_t vr = 0, vi = 0;
-        if (m < d.n_dmrs_re) {
+        if (m < n_dmrs_re) {
             vr = ((bits >> (2 * j)) & 1u) ? neg : pos;
             vi = ((bits >> (2 * j + 1)) & 1u) ? neg : pos;
-            if (d.wf_odd[l] && (m & 1u)) {
+            if (wf_odd && (m & 1u)) {
                 vr ^= 0x8000u;
                 vi ^= 0x8000u;
             }
@@ -72,11 +70,65 @@ __global__ void __launch_bounds__(256) DmrsGenKernel(uint16_t *__restrict__ re, 
     }
 }
 
+__global__ void __launch_bounds__(256) DmrsGenKernel(uint16_t *__restrict__ re, uint16_t *__restrict__ im,
+                                                     const uint16_t *__restrict__ slot_no, DmrsGenDev d)
+{
+    const uint32_t m0 = (blockIdx.x * blockDim.x + threadIdx.x) * 16;
+    if (m0 >= d.ref_pad) return;
+    const uint32_t dd = blockIdx.y, slot = blockIdx.z / d.nl, l = blockIdx.z - slot * d.nl;
+    const uint32_t cinit = slot_no == nullptr ? d.cinit[dd] : DevDmrsCinit(slot_no[slot], d.sym[dd], d.nid, d.nscid);
+    const size_t base = ((static_cast<size_t>(slot) * d.nl + l) * d.ndmrs + dd) * d.ref_pad;
+    DmrsGen16(re, im, base, m0, d.ref_pad, d.n_dmrs_re, cinit, d.off, d.wf_odd[l] != 0);
+}
+
+// Ragged batch: item = (grant, layer); grid (m blocks of the largest grant,
+// D max, items).
+__global__ void __launch_bounds__(256) RaggedDmrsGenKernel(uint16_t *__restrict__ re, uint16_t *__restrict__ im,
+                                                           const airan::RaggedGrant *__restrict__ grants,
+                                                           const airan::RaggedItem *__restrict__ items)
+{
+    const airan::RaggedItem it = items[blockIdx.z];
+    const airan::RaggedGrant &gr = grants[it.grant];
+    const uint32_t m0 = (blockIdx.x * blockDim.x + threadIdx.x) * 16;
+    const uint32_t dd = blockIdx.y;
+    if (m0 >= gr.g.ref_pad || dd >= gr.g.ndmrs) return;
+    const size_t base = gr.ref_off + (static_cast<size_t>(it.a) * gr.g.ndmrs + dd) * gr.g.ref_pad;
+    DmrsGen16(re, im, base, m0, gr.g.ref_pad, gr.g.n_dmrs_re, gr.dmrs_cinit[dd], gr.dmrs_off, gr.wf_odd[it.a] != 0);
+}
+
 // channel_est/mimo_dmrs_ls/mimo_dmrs_ls_kernel.cpp:146-177,301-331: h = y x*
 // in the reference fp16 operation order, FD-OCC2 pair mean when two layers
 // share the comb.  Grid (pilot blocks, D, slots * NR * L).  The DMRS
 // references are [L][D][ref_pad], one for every slot (ref_per_slot, their
 // c_init depends on the slot number, mimo_dmrs_gen.cpp:38-45) or one shared.
+// Observation p of layer l (comb `comb`, FD-OCC2 pair mean when occ) from
+// the grid row y and the layer's reference x.
+__device__ inline void DmrsLsPilot(const __half *__restrict__ y_r, const __half *__restrict__ y_i,
+                                   const __half *__restrict__ x_r, const __half *__restrict__ x_i, uint32_t comb,
+                                   bool occ, uint32_t n_occ, uint32_t n_dmrs_re, uint32_t p, __half *vr, __half *vi)
+{
+    auto ls = [&](uint32_t m, __half *or_, __half *oi) {
+        const uint32_t sc = comb + 2 * m;
+        const __half a = y_r[sc], b = y_i[sc], c = x_r[m], e = x_i[m];
+        *or_ = HAdd(HMul(a, c), HMul(b, e));
+        *oi = HSub(HMul(b, c), HMul(a, e));
+    };
+    *vr = __ushort_as_half(0);
+    *vi = __ushort_as_half(0);
+    if (occ) {
+        if (p < n_occ) {
+            __half r0, i0, r1, i1;
+            ls(2 * p, &r0, &i0);
+            ls(2 * p + 1, &r1, &i1);
+            const __half half_one = __float2half_rn(0.5f);
+            *vr = HMul(HAdd(r0, r1), half_one);
+            *vi = HMul(HAdd(i0, i1), half_one);
+        }
+    } else if (p < n_dmrs_re) {
+        ls(p, vr, vi);
+    }
+}
+
 __global__ void __launch_bounds__(256) DmrsLsKernel(const __half *__restrict__ yr,
                                                     const __half *__restrict__ yi,
                                                     const __half *__restrict__ xr,
@@ -94,28 +146,35 @@ __global__ void __launch_bounds__(256) DmrsLsKernel(const __half *__restrict__ y
     const __half *y_i = yi + (static_cast<size_t>(srx) * g.nsym + g.dmrs_sym[d]) * g.pad;
     const uint32_t ref_slot = ref_per_slot ? srx / g.nr : 0u;
     const size_t ref = (static_cast<size_t>(ref_slot) * g.nl + l) * g.ndmrs + d;
-    const __half *x_r = xr + ref * g.ref_pad;
-    const __half *x_i = xi + ref * g.ref_pad;
-    const uint32_t comb = g.comb[l];
-    auto ls = [&](uint32_t m, __half *or_, __half *oi) {
-        const uint32_t sc = comb + 2 * m;
-        const __half a = y_r[sc], b = y_i[sc], c = x_r[m], e = x_i[m];
-        *or_ = HAdd(HMul(a, c), HMul(b, e));
-        *oi = HSub(HMul(b, c), HMul(a, e));
-    };
-    __half vr = __ushort_as_half(0), vi = __ushort_as_half(0);
-    if (g.occ[l]) {
-        if (p < g.n_occ) {
-            __half r0, i0, r1, i1;
-            ls(2 * p, &r0, &i0);
-            ls(2 * p + 1, &r1, &i1);
-            const __half half_one = __float2half_rn(0.5f);
-            vr = HMul(HAdd(r0, r1), half_one);
-            vi = HMul(HAdd(i0, i1), half_one);
-        }
-    } else if (p < g.n_dmrs_re) {
-        ls(p, &vr, &vi);
-    }
+    __half vr, vi;
+    DmrsLsPilot(y_r, y_i, xr + ref * g.ref_pad, xi + ref * g.ref_pad, g.comb[l], g.occ[l] != 0, g.n_occ,
+                g.n_dmrs_re, p, &vr, &vi);
+    hr[out] = vr;
+    hi[out] = vi;
+}
+
+// Ragged batch: item = (grant, rx, layer); grid (pilot blocks of the largest
+// grant, D max, items).
+__global__ void __launch_bounds__(256) RaggedDmrsLsKernel(const __half *__restrict__ yr,
+                                                          const __half *__restrict__ yi,
+                                                          const __half *__restrict__ xr,
+                                                          const __half *__restrict__ xi, __half *__restrict__ hr,
+                                                          __half *__restrict__ hi,
+                                                          const airan::RaggedGrant *__restrict__ grants,
+                                                          const airan::RaggedItem *__restrict__ items)
+{
+    const airan::RaggedItem it = items[blockIdx.z];
+    const airan::RaggedGrant &gr = grants[it.grant];
+    const DevGeom &g = gr.g;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x, d = blockIdx.y;
+    if (p >= g.pilot_pad || d >= g.ndmrs) return;
+    const uint32_t rx = it.a, l = it.b;
+    const size_t out = gr.hls_off + ((static_cast<size_t>(rx) * g.nl + l) * g.ndmrs + d) * g.pilot_pad + p;
+    const size_t row = gr.rx_off + (static_cast<size_t>(rx) * g.nsym + g.dmrs_sym[d]) * g.pad;
+    const size_t ref = gr.ref_off + (static_cast<size_t>(l) * g.ndmrs + d) * g.ref_pad;
+    __half vr, vi;
+    DmrsLsPilot(yr + row, yi + row, xr + ref, xi + ref, g.comb[l], g.occ[l] != 0, g.n_occ, g.n_dmrs_re, p, &vr,
+                &vi);
     hr[out] = vr;
     hi[out] = vi;
 }
@@ -124,15 +183,14 @@ __global__ void __launch_bounds__(256) DmrsLsKernel(const __half *__restrict__ y
 // sum_d sum_p |h_{p+1} - h_p|^2 / (D (cnt - 1)), halved for the comb model.
 // One workgroup per (slot, Rx) = blockIdx.x, fp32 accumulation, fixed
 // reduction tree.
-__global__ void __launch_bounds__(256) DmrsNoiseKernel(const __half *__restrict__ hr,
-                                                       const __half *__restrict__ hi,
-                                                       __half *__restrict__ noise, DevGeom g)
+// The noise of one Rx: `h` at its layer-0 observations [D][pilot_pad].
+__device__ inline void DmrsNoise(const __half *__restrict__ hr, const __half *__restrict__ hi, size_t rx_base,
+                                 const DevGeom &g, __half *out)
 {
-    const uint32_t rx = blockIdx.x;
     const uint32_t cnt = g.occ[0] ? g.n_occ : g.n_dmrs_re;
     float acc = 0.0f;
     for (uint32_t d = 0; d < g.ndmrs; ++d) {
-        const size_t base = (static_cast<size_t>(rx) * g.nl * g.ndmrs + d) * g.pilot_pad;
+        const size_t base = rx_base + static_cast<size_t>(d) * g.pilot_pad;
         for (uint32_t p = threadIdx.x; p + 1 < cnt; p += blockDim.x) {
             const float dr = __half2float(hr[base + p + 1]) - __half2float(hr[base + p]);
             const float di = __half2float(hi[base + p + 1]) - __half2float(hi[base + p]);
@@ -147,8 +205,33 @@ __global__ void __launch_bounds__(256) DmrsNoiseKernel(const __half *__restrict_
         const float s = (part[0] + part[1]) + (part[2] + part[3]);
         float v = s / (static_cast<float>(g.ndmrs) * static_cast<float>(cnt - 1));
         if (!g.occ[0]) v *= 0.5f;
-        noise[rx] = __float2half_rn(v);
+        *out = __float2half_rn(v);
     }
+}
+
+// Noise per Rx from layer 0 (mimo_dmrs_ls_kernel.cpp:179-196,334-337):
+// sum_d sum_p |h_{p+1} - h_p|^2 / (D (cnt - 1)), halved for the comb model.
+// One workgroup per (slot, Rx) = blockIdx.x, fp32 accumulation, fixed
+// reduction tree.
+__global__ void __launch_bounds__(256) DmrsNoiseKernel(const __half *__restrict__ hr,
+                                                       const __half *__restrict__ hi,
+                                                       __half *__restrict__ noise, DevGeom g)
+{
+    const uint32_t rx = blockIdx.x;
+    DmrsNoise(hr, hi, static_cast<size_t>(rx) * g.nl * g.ndmrs * g.pilot_pad, g, &noise[rx]);
+}
+
+// Ragged batch: one workgroup per item (grant, rx).
+__global__ void __launch_bounds__(256) RaggedDmrsNoiseKernel(const __half *__restrict__ hr,
+                                                             const __half *__restrict__ hi,
+                                                             __half *__restrict__ noise,
+                                                             const airan::RaggedGrant *__restrict__ grants,
+                                                             const airan::RaggedItem *__restrict__ items)
+{
+    const airan::RaggedItem it = items[blockIdx.x];
+    const airan::RaggedGrant &gr = grants[it.grant];
+    DmrsNoise(hr, hi, gr.hls_off + static_cast<size_t>(it.a) * gr.g.nl * gr.g.ndmrs * gr.g.pilot_pad, gr.g,
+              &noise[gr.noise_off + it.a]);
 }
 
 // Pilot description (mimo_dmrs_ls_kernel.cpp:209-240): observation p of layer
@@ -210,6 +293,25 @@ DmrsGenDev GenDev(const PuschMimoConfig &cfg, const Geom &g)
 }
 
 }  // namespace
+
+uint32_t HostDmrsCinit(const PuschMimoConfig &cfg, uint32_t symbol) { return DmrsCinit(cfg, symbol); }
+
+int32_t LaunchDmrsRagged(const RaggedGrant *grants, const RaggedItem *gen_items, uint32_t n_gen,
+                         const RaggedItem *ls_items, uint32_t n_ls, const RaggedItem *noise_items, uint32_t n_noise,
+                         uint32_t max_ref_pad, uint32_t max_pilot_pad, uint32_t max_dmrs, const __half *rx_re,
+                         const __half *rx_im, __half *ref_re, __half *ref_im, __half *h_re, __half *h_im,
+                         __half *noise, hipStream_t s)
+{
+    if (n_gen == 0 || n_ls == 0 || n_noise == 0) return AIRAN_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(RaggedDmrsGenKernel, dim3(CeilDiv(CeilDiv(max_ref_pad, 16), 256), max_dmrs, n_gen), dim3(256),
+                       0, s, reinterpret_cast<uint16_t *>(ref_re), reinterpret_cast<uint16_t *>(ref_im), grants,
+                       gen_items);
+    hipLaunchKernelGGL(RaggedDmrsLsKernel, dim3(CeilDiv(max_pilot_pad, 256), max_dmrs, n_ls), dim3(256), 0, s, rx_re,
+                       rx_im, ref_re, ref_im, h_re, h_im, grants, ls_items);
+    hipLaunchKernelGGL(RaggedDmrsNoiseKernel, dim3(n_noise), dim3(256), 0, s, h_re, h_im, noise, grants,
+                       noise_items);
+    return LaunchStatus();
+}
 
 int32_t LaunchDmrsGenSlots(const PuschMimoConfig &cfg, const Geom &g, const uint16_t *slot_no, uint32_t slots,
                            __half *ref_re, __half *ref_im, hipStream_t s)

@@ -8,6 +8,7 @@
 #include "gold.h"
 #include "launch.h"
 #include "qam.h"
+#include "ragged.h"
 #include "ratematch.h"
 
 using airan::DevGeom;
@@ -520,6 +521,22 @@ __global__ void __launch_bounds__(256) RateDematchKernel(const int16_t *__restri
     out[static_cast<size_t>(cb) * n + p] = airan::RateDematchValue(p, c, r, in);
 }
 
+// Ragged batch: code block blockIdx.y of any grant, its geometry from the
+// grant's record, its decoder input at its own offset.
+__global__ void __launch_bounds__(256) RaggedRateDematchKernel(const int16_t *__restrict__ cw,
+                                                               const airan::RaggedGrant *__restrict__ grants,
+                                                               const airan::RaggedCb *__restrict__ cbs,
+                                                               int16_t *__restrict__ out)
+{
+    const airan::RaggedCb c = cbs[blockIdx.y];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= 68u * c.z) return;
+    const airan::RaggedGrant &gr = grants[c.grant];
+    const airan::RateDev r{gr.g.qm, gr.g.cw_symbols, gr.g.cw_stride, c.z, c.kprime, c.filler, c.ncb, 1};
+    const RateMatchDescriptor desc{c.e, c.k0, c.ncb, c.cw_symbol_offset};
+    out[c.lam_off + p] = airan::RateDematchValue(p, airan::RateCbSetup(desc, r), r, cw + gr.cw_off);
+}
+
 int32_t BuildRateProfile(const PuschMimoConfig *config, uint32_t num_slots, const AiranTbPlanV1 &p,
                          PuschMimoLayout *layout, RateDematchKernelMetadata *m,
                          RateMatchDescriptor *desc, size_t desc_count)
@@ -574,6 +591,15 @@ int32_t LaunchRateDematch(const Geom &g, const AiranTbPlanV1 &p, const RateMatch
     if (num_cb == 0 || num_cb > 65535 || p.Z == 0 || p.Ncb <= p.filler) return AIRAN_INVALID_ARGUMENT;
     const RateDev r{g.qm, g.cw_symbols, g.cw_stride, p.Z, p.Kprime, p.filler, p.Ncb, num_cb};
     hipLaunchKernelGGL(RateDematchKernel, dim3(CeilDiv(68u * p.Z, 256), num_cb), dim3(256), 0, s, in, desc, out, r);
+    return LaunchStatus();
+}
+
+int32_t LaunchRateDematchRagged(const RaggedGrant *grants, const RaggedCb *cbs, uint32_t num_cb, const int16_t *cw,
+                                int16_t *out, hipStream_t s)
+{
+    if (num_cb == 0 || num_cb > 65535) return AIRAN_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(RaggedRateDematchKernel, dim3(CeilDiv(68u * 384u, 256), num_cb), dim3(256), 0, s, cw, grants,
+                       cbs, out);
     return LaunchStatus();
 }
 

@@ -19,6 +19,7 @@
 #include "common.h"
 #include "airan_ldpc_bg1.h"
 #include "launch.h"
+#include "ragged.h"
 #include "ratematch.h"
 
 // Posterior of the code block owned by the workgroup ([68][Z] x 16 bit).
@@ -170,18 +171,37 @@ struct LdpcDev {
     uint32_t z, k, iters, rows, early_stop;
 };
 
-template <int MODE, int REG_ROWS>
+// RAGGED: block b decodes code block list[b] of a ragged batch, with that
+// block's Z, K, rows and buffer offsets (no lam_out).
+template <int MODE, int REG_ROWS, bool RAGGED = false>
 __global__ void __launch_bounds__(384) LdpcKernel(const int16_t *__restrict__ lam_in,
                                                   int8_t *__restrict__ bits,
                                                   int16_t *__restrict__ lam_out,
                                                   uint8_t *__restrict__ iters_out,
-                                                  uint2 *__restrict__ msg_ws, LdpcDev d)
+                                                  uint2 *__restrict__ msg_ws, LdpcDev d,
+                                                  const airan::RaggedCb *__restrict__ cbs = nullptr,
+                                                  const uint32_t *__restrict__ list = nullptr)
 {
     uint16_t *lam = airan_ldpc_lam;
+    uint32_t cb = blockIdx.x;
+    size_t lam_off, bits_off, ws_off;
+    if constexpr (RAGGED) {
+        cb = list[blockIdx.x];
+        const airan::RaggedCb &c = cbs[cb];
+        d.z = c.z;
+        d.k = c.k;
+        d.rows = c.rows;
+        lam_off = c.lam_off;
+        bits_off = c.bits_off;
+        ws_off = c.ws_off;
+    } else {
+        lam_off = static_cast<size_t>(cb) * 68u * d.z;
+        bits_off = static_cast<size_t>(cb) * d.k;
+        ws_off = static_cast<size_t>(cb) * d.rows * d.z;
+    }
     const uint32_t Z = d.z, N = 68u * Z;
-    const uint32_t cb = blockIdx.x;
     const uint32_t i = threadIdx.x;
-    const int16_t *src = lam_in + static_cast<size_t>(cb) * N;
+    const int16_t *src = lam_in + lam_off;
     for (uint32_t t = i; t < N; t += blockDim.x) {
         const float scale = MODE == AIRAN_LDPC_MODE_REF ? 1.0f : 1.0f / 256.0f;
         lam[t] = __half_as_ushort(__float2half_rn(static_cast<float>(src[t]) * scale));
@@ -192,7 +212,7 @@ __global__ void __launch_bounds__(384) LdpcKernel(const int16_t *__restrict__ la
         rw0[r] = 0;
         rw1[r] = 0;
     }
-    uint2 *ws = msg_ws + static_cast<size_t>(cb) * d.rows * Z;
+    uint2 *ws = msg_ws + ws_off;
     if (REG_ROWS == 0 && i < Z)
         for (uint32_t r = 0; r < d.rows; ++r) ws[r * Z + i] = make_uint2(0u, 0u);
     __syncthreads();
@@ -226,10 +246,10 @@ __global__ void __launch_bounds__(384) LdpcKernel(const int16_t *__restrict__ la
         }
     }
     if (iters_out != nullptr && i == 0) iters_out[cb] = static_cast<uint8_t>(it);
-    int8_t *bo = bits + static_cast<size_t>(cb) * d.k;
+    int8_t *bo = bits + bits_off;
     for (uint32_t t = i; t < d.k; t += blockDim.x) bo[t] = HardBit<MODE>(lam, t) ? 1 : 0;
-    if (lam_out != nullptr) {
-        int16_t *lo = lam_out + static_cast<size_t>(cb) * N;
+    if (!RAGGED && lam_out != nullptr) {
+        int16_t *lo = lam_out + lam_off;
         const float clip = MODE == AIRAN_LDPC_MODE_REF ? 5120.0f : 20.0f;
         const float out_scale = MODE == AIRAN_LDPC_MODE_REF ? 1.0f : 256.0f;
         for (uint32_t t = i; t < N; t += blockDim.x) {
@@ -895,6 +915,23 @@ int32_t airan_ldpc_decode_Enqueue(const LdpcDecodeOpArgsV1 *a, void *workspace, 
 }  // extern "C"
 
 namespace airan {
+
+int32_t LaunchLdpcRagged(const RaggedCb *cbs, const uint32_t *reg_list, uint32_t n_reg, const uint32_t *ws_list,
+                         uint32_t n_ws, uint32_t iters, bool early_stop, const int16_t *lam, int8_t *bits,
+                         uint8_t *iters_out, uint2 *ws, hipStream_t s)
+{
+    int32_t st = LoadTable();
+    if (st != AIRAN_OK) return st;
+    const LdpcDev d{0, 0, iters, 0, early_stop ? 1u : 0u};
+    const size_t lds = static_cast<size_t>(68) * 384 * sizeof(uint16_t);
+    if (n_reg > 0)
+        hipLaunchKernelGGL((LdpcKernel<AIRAN_LDPC_MODE_FP16, kRegRows, true>), dim3(n_reg), dim3(384), lds, s, lam,
+                           bits, nullptr, iters_out, nullptr, d, cbs, reg_list);
+    if (n_ws > 0)
+        hipLaunchKernelGGL((LdpcKernel<AIRAN_LDPC_MODE_FP16, 0, true>), dim3(n_ws), dim3(384), lds, s, lam, bits,
+                           nullptr, iters_out, ws, d, cbs, ws_list);
+    return LaunchStatus();
+}
 
 int32_t InitLdpcScratch()
 {

@@ -18,6 +18,7 @@
 
 #include "common.h"
 #include "launch.h"
+#include "ragged.h"
 
 using airan::Geom;
 
@@ -209,9 +210,12 @@ __device__ inline void LdsPass(float *lre, float *lim, uint32_t j, uint32_t ns, 
 }
 
 // OUT_MODE 0: stage-4 order [14, N] per stream; 1: used-subcarrier grid [14, pad].
-template <int LOGN, bool IQ_FLOAT, int OUT_MODE>
+// RAGGED (with OUT_MODE 1): stream i is streams[i] (its own IQ and grid
+// offsets and used / pad), all of this FFT size.
+template <int LOGN, bool IQ_FLOAT, int OUT_MODE, bool RAGGED = false>
 __global__ void __launch_bounds__(kThreads) OfdmKernel(const void *__restrict__ iq, __half *__restrict__ out_re,
-                                                       __half *__restrict__ out_im, OfdmArgs a)
+                                                       __half *__restrict__ out_im, OfdmArgs a,
+                                                       const airan::RaggedStream *__restrict__ streams = nullptr)
 {
     constexpr uint32_t N = 1u << LOGN;
     constexpr uint32_t TPS = N / 16;
@@ -224,10 +228,22 @@ __global__ void __launch_bounds__(kThreads) OfdmKernel(const void *__restrict__ 
     float *lre = lds_re + sub * (N + N / 16);
     float *lim = lds_im + sub * (N + N / 16);
 
+    uint64_t iq_base = static_cast<uint64_t>(stream) * a.samples;
+    uint64_t out_base = static_cast<uint64_t>(gsym) * a.pad;
+    uint32_t used = a.used, pad = a.pad;
+    if constexpr (RAGGED) {
+        if (active) {
+            const airan::RaggedStream st = streams[stream];
+            iq_base = st.iq_off;
+            used = st.used;
+            pad = st.pad;
+            out_base = st.out_off + static_cast<uint64_t>(sym) * pad;
+        }
+    }
     // First radix-16 pass straight from HBM (ns = 1: no twiddles).
     if (active) {
         float2 v[16];
-        const size_t s0 = static_cast<size_t>(stream) * a.samples + a.sym_start[sym];
+        const size_t s0 = iq_base + a.sym_start[sym];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const size_t n = s0 + j + r * TPS;
@@ -276,18 +292,18 @@ __global__ void __launch_bounds__(kThreads) OfdmKernel(const void *__restrict__ 
             oim[q] = __floats2half2_rn(lim[Pad(k0)], lim[Pad(k1)]);
         }
     } else {
-        const uint32_t half_used = a.used / 2;
-        __half2 *ore = reinterpret_cast<__half2 *>(out_re + static_cast<size_t>(gsym) * a.pad);
-        __half2 *oim = reinterpret_cast<__half2 *>(out_im + static_cast<size_t>(gsym) * a.pad);
-        for (uint32_t q = j; q < a.pad / 2; q += TPS) {
+        const uint32_t half_used = used / 2;
+        __half2 *ore = reinterpret_cast<__half2 *>(out_re + out_base);
+        __half2 *oim = reinterpret_cast<__half2 *>(out_im + out_base);
+        for (uint32_t q = j; q < pad / 2; q += TPS) {
             const uint32_t u = 2 * q;
             float r0 = 0.0f, i0 = 0.0f, r1 = 0.0f, i1 = 0.0f;
-            if (u < a.used) {
+            if (u < used) {
                 const uint32_t k = u < half_used ? N - half_used + u : u - half_used;
                 r0 = lre[Pad(k)];
                 i0 = lim[Pad(k)];
             }
-            if (u + 1 < a.used) {
+            if (u + 1 < used) {
                 const uint32_t k = u + 1 < half_used ? N - half_used + u + 1 : u + 1 - half_used;
                 r1 = lre[Pad(k)];
                 i1 = lim[Pad(k)];
@@ -346,6 +362,54 @@ int32_t Launch(const OfdmDemodBatchOpArgsV1 *args, int out_mode, void *ore, void
 }
 
 }  // namespace
+
+namespace airan {
+
+int32_t LaunchOfdmRagged(uint32_t fft, uint32_t iq_format, const RaggedStream *streams, uint32_t num_streams,
+                         const void *iq, __half *out_re, __half *out_im, hipStream_t s)
+{
+    if (num_streams == 0) return AIRAN_OK;
+    if (InitOfdmTables() != AIRAN_OK) return AIRAN_LAUNCH_FAILED;
+    uint32_t logn = 0;
+    while ((1u << logn) < fft) ++logn;
+    if ((1u << logn) != fft || logn < 7 || logn > 12) return AIRAN_UNSUPPORTED_PROFILE;
+    OfdmArgs a{};
+    a.total_symbols = num_streams * kSymbols;
+    a.out_scale = static_cast<float>((iq_format == AIRAN_IQ_INT16_Q8_8 ? 1.0 / 256.0 : 1.0) /
+                                     std::sqrt(static_cast<double>(fft)));
+    // The NR cyclic prefixes of this FFT size (CP 9N/128, plus N/64 on symbol 0).
+    const uint32_t cs = fft * 9 / 128, cl = cs + fft / 64;
+    uint32_t pos = 0;
+    for (uint32_t sym = 0; sym < kSymbols; ++sym) {
+        pos += sym == 0 ? cl : cs;
+        a.sym_start[sym] = pos;
+        pos += fft;
+    }
+    const dim3 grid(CeilDiv(a.total_symbols, kThreads / (fft / 16)));
+    const bool f = iq_format == AIRAN_IQ_COMPLEX64;
+#define AIRAN_RAGGED_OFDM(L)                                                                                      \
+    case L:                                                                                                      \
+        if (f)                                                                                                   \
+            hipLaunchKernelGGL((OfdmKernel<L, true, 1, true>), grid, dim3(kThreads), 0, s, iq, out_re, out_im, a, \
+                               streams);                                                                         \
+        else                                                                                                     \
+            hipLaunchKernelGGL((OfdmKernel<L, false, 1, true>), grid, dim3(kThreads), 0, s, iq, out_re, out_im,  \
+                               a, streams);                                                                      \
+        break;
+    switch (logn) {
+        AIRAN_RAGGED_OFDM(7)
+        AIRAN_RAGGED_OFDM(8)
+        AIRAN_RAGGED_OFDM(9)
+        AIRAN_RAGGED_OFDM(10)
+        AIRAN_RAGGED_OFDM(11)
+        AIRAN_RAGGED_OFDM(12)
+    default: return AIRAN_UNSUPPORTED_PROFILE;
+    }
+#undef AIRAN_RAGGED_OFDM
+    return LaunchStatus();
+}
+
+}  // namespace airan
 
 extern "C" {
 

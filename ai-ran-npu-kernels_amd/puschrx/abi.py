@@ -284,6 +284,22 @@ GRAM_BF16 = 1
 GRAM_FP16 = 2
 
 
+class AiranGrantBatchOptionsV1(C.Structure):
+    _fields_ = _args(("iq_format", u32), ("ldpc_iterations", u32), ("early_stop", u32), ("ce_max_rank", u32),
+                     ("use_graph", u32), ("tau_s", C.c_double), ("fd_hz", C.c_double), ("scs_hz", C.c_double),
+                     ("detect_tile", u32), ("reserved", u32 * 7))
+
+
+class AiranGrantLayoutV1(C.Structure):
+    _fields_ = [("iq_offset", C.c_uint64), ("iq_samples", C.c_uint64), ("bits_offset", C.c_uint64),
+                ("cw_offset", C.c_uint64), ("first_cb", u32), ("num_cb", u32), ("K", u32), ("fft", u32)]
+
+
+class AiranGrantBatchOutputsV1(C.Structure):
+    _fields_ = _args(("num_grants", u32), ("bits", vp), ("tb_ok", vp), ("cb_ok", vp), ("iterations", vp),
+                     ("cw_llr", vp), ("num_cb", C.c_uint64))
+
+
 class AiranPuschChainOutputsV1(C.Structure):
     _fields_ = _args(("num_slots", u32), ("bits", vp), ("tb_ok", vp), ("cb_ok", vp), ("iterations", vp),
                      ("rx_grid_re", vp), ("rx_grid_im", vp), ("h_ls_re", vp), ("h_ls_im", vp),
@@ -416,6 +432,17 @@ PROTOTYPES = {
     "airan_pusch_chain_SetTiming": (i32, [vp, u32]),
     "airan_pusch_chain_StageTimes": (i32, [vp, P(C.c_float), u32]),
     "airan_pusch_chain_SetRefiner": (i32, [vp, vp, C.c_float, C.c_float]),
+    # ragged multi-grant batches (include/airan_pusch_grants.h)
+    "airan_pusch_grants_Create": (i32, [P(PuschMimoConfig), P(AiranTbPlanV1), u32, P(AiranGrantBatchOptionsV1), vp,
+                                        P(vp)]),
+    "airan_pusch_grants_Destroy": (None, [vp]),
+    "airan_pusch_grants_Layout": (i32, [vp, P(AiranGrantLayoutV1), u32, P(C.c_uint64)]),
+    "airan_pusch_grants_Enqueue": (i32, [vp, vp, vp]),
+    "airan_pusch_grants_Outputs": (i32, [vp, P(AiranGrantBatchOutputsV1)]),
+    "airan_pusch_grants_SetDetectTile": (i32, [vp, u32]),
+    "airan_pusch_grants_NumShapes": (u32, [vp]),
+    "airan_pusch_grants_FilterCacheSize": (u32, []),
+    "airan_pusch_grants_ClearFilterCache": (None, []),
     # learned CE refiner (include/airan_ce_refiner.h)
     "airan_ce_refiner_Create": (i32, [P(PuschMimoConfig), u32, vp, C.c_uint64, vp, P(vp)]),
     "airan_ce_refiner_Destroy": (None, [vp]),
