@@ -194,11 +194,17 @@ struct AiranGrantBatch {
         uint64_t hls_off = 0, noise_off = 0, h_off = 0;
     };
     std::vector<CeGroup> groups;
+    // The CE groups are independent: they run as kCeBranches parallel
+    // branches (side streams forked from and joined to the caller's stream,
+    // parallel branches of the captured graph), each with its own workspace.
+    static constexpr int kCeBranches = 4;
+    hipStream_t side[kCeBranches] = {};
+    hipEvent_t fork = nullptr, join[kCeBranches] = {};
+    uint8_t *ce_ws_branch[kCeBranches] = {};
     // Buffers.
     __half *rx_re = nullptr, *rx_im = nullptr, *ref_re = nullptr, *ref_im = nullptr;
     __half *hls_re = nullptr, *hls_im = nullptr, *noise_ls = nullptr, *noise_ce = nullptr;
     __half *h_re = nullptr, *h_im = nullptr;
-    uint8_t *ce_ws = nullptr;
     size_t ce_ws_bytes = 0;
     uint32_t *gold = nullptr;
     int16_t *cw = nullptr, *lam = nullptr;
@@ -250,12 +256,22 @@ int32_t EnqueueStages(AiranGrantBatch *b, const void *iq, hipStream_t s)
                                       b->n_noise, b->max_ref_pad, b->max_pilot_pad, b->max_dmrs, b->rx_re, b->rx_im,
                                       b->ref_re, b->ref_im, b->hls_re, b->hls_im, b->noise_ls, s)) != AIRAN_OK)
         return st;
-    for (const auto &gp : b->groups)
+    constexpr int K = AiranGrantBatch::kCeBranches;
+    if (hipEventRecord(b->fork, s) != hipSuccess) return AIRAN_LAUNCH_FAILED;
+    for (int k = 0; k < K; ++k)
+        if (hipStreamWaitEvent(b->side[k], b->fork, 0) != hipSuccess) return AIRAN_LAUNCH_FAILED;
+    for (size_t j = 0; j < b->groups.size(); ++j) {
+        const auto &gp = b->groups[j];
+        const int k = static_cast<int>(j % K);
         if ((st = airan::LaunchCeApply(gp.filter, b->hls_re + gp.hls_off, b->hls_im + gp.hls_off,
-                                       b->noise_ls + gp.noise_off, b->noise_ce + gp.noise_off, b->ce_ws,
+                                       b->noise_ls + gp.noise_off, b->noise_ce + gp.noise_off, b->ce_ws_branch[k],
                                        b->ce_ws_bytes, b->h_re + gp.h_off, b->h_im + gp.h_off, gp.nl, gp.count,
-                                       s)) != AIRAN_OK)
+                                       b->side[k])) != AIRAN_OK)
             return st;
+    }
+    for (int k = 0; k < K; ++k)
+        if (hipEventRecord(b->join[k], b->side[k]) != hipSuccess || hipStreamWaitEvent(s, b->join[k], 0) != hipSuccess)
+            return AIRAN_LAUNCH_FAILED;
     hipLaunchKernelGGL(RaggedDetectKernel, dim3(b->n_tiles), dim3(b->tile), 0, s, b->rx_re, b->rx_im, b->h_re,
                        b->h_im, b->noise_ce, b->gold, b->cw, b->grants, b->tiles);
     if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
@@ -478,12 +494,18 @@ int32_t airan_pusch_grants_Create(const PuschMimoConfig *configs, const AiranTbP
         !DevAlloc(&b->ref_re, ref, &by) || !DevAlloc(&b->ref_im, ref, &by) || !DevAlloc(&b->hls_re, hls, &by) ||
         !DevAlloc(&b->hls_im, hls, &by) || !DevAlloc(&b->noise_ls, noise, &by) ||
         !DevAlloc(&b->noise_ce, noise, &by) || !DevAlloc(&b->h_re, h, &by) || !DevAlloc(&b->h_im, h, &by) ||
-        !DevAlloc(&b->ce_ws, b->ce_ws_bytes, &by) || !DevAlloc(&b->gold, gold, &by) ||
+        !DevAlloc(&b->gold, gold, &by) ||
         !DevAlloc(&b->cw, cwo, &by) || !DevAlloc(&b->lam, lam, &by) || !DevAlloc(&b->ldpc_ws, ws, &by) ||
         !DevAlloc(&b->bits, bits, &by) || !DevAlloc(&b->iters, b->num_cb, &by) ||
         !DevAlloc(&b->cb_ok, b->num_cb, &by) || !DevAlloc(&b->tb_ok, n, &by) ||
         !DevAlloc(&b->tb_part, b->num_cb, &by))
         return fail(AIRAN_RESOURCE_TOO_SMALL);
+    for (int k = 0; k < AiranGrantBatch::kCeBranches; ++k)
+        if (!DevAlloc(&b->ce_ws_branch[k], b->ce_ws_bytes, &by) ||
+            hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) != hipSuccess)
+            return fail(AIRAN_RESOURCE_TOO_SMALL);
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
     int32_t st = BuildTiles(b, options->detect_tile);
     if (st != AIRAN_OK) return fail(st);
     // Gold bits of every grant's data scrambling sequence.
@@ -528,9 +550,16 @@ void airan_pusch_grants_Destroy(AiranGrantBatch *b)
     }
     for (auto &f : b->ffts)
         if (f.streams) (void)hipFree(f.streams);
+    for (int k = 0; k < AiranGrantBatch::kCeBranches; ++k) {
+        if (b->side[k]) (void)hipStreamSynchronize(b->side[k]);
+        if (b->side[k]) (void)hipStreamDestroy(b->side[k]);
+        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+        if (b->ce_ws_branch[k]) (void)hipFree(b->ce_ws_branch[k]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     void *bufs[] = {b->grants, b->gen_items, b->ls_items, b->noise_items, b->tiles, b->cbs, b->reg_list, b->ws_list,
                     b->tb_first, b->tb_count, b->rx_re, b->rx_im, b->ref_re, b->ref_im, b->hls_re, b->hls_im,
-                    b->noise_ls, b->noise_ce, b->h_re, b->h_im, b->ce_ws, b->gold, b->cw, b->lam, b->ldpc_ws,
+                    b->noise_ls, b->noise_ce, b->h_re, b->h_im, b->gold, b->cw, b->lam, b->ldpc_ws,
                     b->bits, b->iters, b->cb_ok, b->tb_ok, b->tb_part};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
