@@ -145,3 +145,90 @@ def gold(oracle, cinit: int, start: int, count: int) -> np.ndarray:
     c = np.zeros(count, np.uint8)
     oracle.or_gold(cinit, start, count, ptr(c))
     return c
+
+
+# ---------------------------------------------------------------- CPU chain
+# oracle/lib/libcpuchain_{v3,v4}.so: the FP32 C++ chain timed as the CPU
+# baseline (one build per x86-64 level; the highest the host supports runs).
+_V4_FLAGS = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+_V3_FLAGS = {"avx2", "fma", "bmi2", "movbe"}
+
+
+def host_cpu() -> dict:
+    """CPU model, feature level and logical CPU count of this host."""
+    model, flags = "unknown", set()
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("flags") and not flags:
+                flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    import os
+    level = "v4" if _V4_FLAGS <= flags else ("v3" if _V3_FLAGS <= flags else None)
+    return {"model": model, "isa": level, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+class CpuChain:
+    """One grant of the FP32 CPU chain: the CE eigenpairs come from LAPACK
+    (numpy.linalg.eigh, FP64) once, as the GPU chain takes them from
+    rocSOLVER; the top `rank` eigenpairs above 1e-12 of the largest are kept."""
+
+    def __init__(self, cfg, op: OrTbPlan, prm: OrCeParams, ldpc_iters: int = 8, rank: int = 384, isa=None):
+        info = host_cpu()
+        isa = isa or info["isa"]
+        if isa is None:
+            raise RuntimeError("host CPU lacks x86-64-v3 (AVX2 / FMA)")
+        path = ROOT / "oracle" / "lib" / f"libcpuchain_{isa}.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: run __graft_entry__.build()")
+        lib = C.CDLL(str(path))
+        vp = C.c_void_p
+        lib.cpu_chain_create.argtypes = [vp, vp, vp, C.c_int, C.POINTER(vp)]
+        lib.cpu_chain_num_models.argtypes = [vp]
+        lib.cpu_chain_model_rpp.argtypes = [vp, C.c_int, vp]
+        lib.cpu_chain_set_eigen.argtypes = [vp, C.c_int, C.c_int, vp, vp]
+        lib.cpu_chain_run.argtypes = [vp, vp, C.c_int, vp, C.c_int, vp, vp]
+        lib.cpu_chain_destroy.argtypes = [vp]
+        lib.cpu_chain_max_threads.restype = C.c_int
+        self.lib, self.isa, self.op = lib, isa, op
+        self._keep = (cfg, op, prm)
+        self.h = vp()
+        if lib.cpu_chain_create(C.byref(cfg), C.byref(op), C.byref(prm), ldpc_iters, C.byref(self.h)) != 0:
+            raise RuntimeError("cpu_chain_create failed")
+        cache = {}
+        for q in range(lib.cpu_chain_num_models(self.h)):
+            n = lib.cpu_chain_model_rpp(self.h, q, None)
+            r = np.zeros((n, n, 2))
+            lib.cpu_chain_model_rpp(self.h, q, r.ctypes.data)
+            key = r.tobytes()
+            if key not in cache:
+                lam, v = np.linalg.eigh(r[..., 0] + 1j * r[..., 1])
+                lam, v = lam[::-1], v[:, ::-1]
+                kept = max(1, int(min(rank, np.count_nonzero(lam >= 1e-12 * lam[0]))))
+                cache[key] = (np.ascontiguousarray(lam[:kept]),
+                              np.ascontiguousarray(v[:, :kept].T).view(np.float64).copy())
+            lam, vec = cache[key]
+            if lib.cpu_chain_set_eigen(self.h, q, len(lam), lam.ctypes.data, vec.ctypes.data) != 0:
+                raise RuntimeError("cpu_chain_set_eigen failed")
+
+    def max_threads(self) -> int:
+        return int(self.lib.cpu_chain_max_threads())
+
+    def run(self, iq: np.ndarray, threads: int):
+        """iq complex64 as float32 [slots, NR, samples, 2] -> (bits int8
+        [slots, C, K], tb_ok uint8 [slots])."""
+        iq = np.ascontiguousarray(iq, dtype=np.float32)
+        n = iq.shape[0]
+        bits = np.zeros((n, self.op.C, self.op.K), np.int8)
+        ok = np.zeros(n, np.uint8)
+        if self.lib.cpu_chain_run(self.h, iq.ctypes.data, n, None, threads, bits.ctypes.data, ok.ctypes.data) != 0:
+            raise RuntimeError("cpu_chain_run failed")
+        return bits, ok
+
+    def close(self):
+        if self.h:
+            self.lib.cpu_chain_destroy(self.h)
+            self.h = C.c_void_p()
