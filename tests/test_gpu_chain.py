@@ -453,3 +453,49 @@ def test_native_chain_mixed_precision(oracle, gpu_stream, dft, gram):
         assert err <= {abi.DFT_FP16: -60.0, abi.DFT_BF16: -40.0}[dft]
     assert b["tb_ok"].tolist() == [1] * S
     assert np.array_equal(a["bits"], b["bits"])
+
+
+def test_config2_stated_point(gpu_stream):
+    """The mid-size plain MIMO workload at its stated point: 4 Rx, rank 2 on
+    CDM ports 0/1, 106 PRB, 64QAM R ~ 0.6, 12-tap 300 ns channel with 50 Hz
+    Jakes Doppler and Rx correlation 0.5, AWGN 18 dB (per Rx and layer), 32
+    complex64 slots through the batched chain with a matched CE prior.  The
+    BLER is printed beside the FP32 CPU chain's on the same slots; every
+    passing TB carries the transmitted payload, and the GPU chain loses no
+    more than one TB against the CPU chain."""
+    from oracle_lib import CpuChain, OrCeParams, host_cpu, load_oracle, or_tb_plan
+    from puschrx.chain import NativeChain
+    prof = P.MID_4RX_RANK2
+    cfg, g, plan = plan_for(prof)
+    S = 32
+    spec = synth.ChannelSpec(taps=12, delay_spread_s=300e-9, doppler_hz=50.0, rx_corr=0.5, snr_db=18.0)
+    rng = np.random.default_rng(0xC0F2)
+    payload = synth.random_payload(rng, S, plan.tbs)
+    grid = synth.tx_grid(cfg, plan, payload)
+    iq = synth.receive(rng, cfg, grid, synth.channel(rng, cfg, spec, S), spec.snr_db, iq_format=abi.IQ_COMPLEX64)
+    opts = ChainOptions(tau_s=spec.delay_spread_s, fd_hz=spec.doppler_hz, iq_format=abi.IQ_COMPLEX64)
+    print(f"config 2: {S} slots at {spec.snr_db} dB, TBS {plan.tbs}, C = {plan.C}", flush=True)
+    chain = NativeChain(cfg, plan, max_slots=S, opts=opts, use_graph=False)
+    chain.enqueue(torch.from_numpy(iq).cuda(), S)
+    tb_ok = chain.fetch("tb_ok", torch.uint8, S)
+    bits = chain.fetch("bits", torch.int8, S * plan.C * plan.K)
+    torch.cuda.synchronize()
+    chain.close()
+    ok = tb_ok.cpu().numpy()
+    b = bits.cpu().numpy().reshape(S, plan.C, plan.K).astype(np.uint8)
+    per_cb = plan.Kprime - plan.crc_cb
+    for s in range(S):
+        if ok[s]:
+            assert np.array_equal(b[s, :, :per_cb].reshape(-1)[: plan.tbs], payload[s])
+    gpu_bler = 1.0 - float(ok.mean())
+    msg = f"config 2 at 18 dB: GPU chain BLER {gpu_bler:.3f} over {S} slots"
+    if host_cpu()["isa"] is not None:
+        op = or_tb_plan(load_oracle(), plan.tbs, plan.G, g.qm, g.nl)
+        cc = CpuChain(cfg, op, OrCeParams(spec.delay_spread_s, spec.doppler_hz, 30e3), 8)
+        try:
+            _, cpu_ok = cc.run(iq, 1)
+        finally:
+            cc.close()
+        msg += f", FP32 CPU chain BLER {1.0 - float(cpu_ok.mean()):.3f}"
+        assert int(ok.sum()) >= int(cpu_ok.sum()) - 1
+    print(msg, flush=True)
