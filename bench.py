@@ -16,16 +16,20 @@ decode to the same bits as the single-stream run, whose time is reported
 beside the throughput.
 
 The operating SNR is the lowest point of a 20..46 dB sweep (step 2) at which
-the chain's BLER is <= 10 %, measured first on a smaller batch (4 Rx carrying
-4 layers leaves the LMMSE detector no receive diversity, so plain LMMSE needs
-an SNR above the 20..32 dB range quoted for the workload).  Reported
-beside the throughput: BLER and goodput at that SNR, the per-stage times
-(HIP events on the chain's stream), the roofline of the dominant kernel and
-the CPU chain (the C oracle, OpenMP) on a bounded sample of the same slots.
+the chain's BLER is <= 10 %, measured on 64 other slots per point (4 Rx
+carrying 4 layers leaves the LMMSE detector no receive diversity, so plain
+LMMSE needs an SNR above the 20..32 dB range quoted for the workload; the
+SNR is per receive antenna and layer).  Reported beside the throughput:
+BLER and goodput at that SNR, the per-stage times (HIP events on the
+chain's stream), the BF16 / FP16 precision variants of the chain on the same
+slots, the learned estimator's sweep, the roofline of the dominant kernel
+and the FP32 C++ CPU chain on the host cores (single-thread and all
+threads, p50 of 20 runs).
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import ctypes as C
 import json
 import os
@@ -225,7 +229,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--slots", type=int, default=64, help="slots per step")
     ap.add_argument("--snr", type=float, default=None, help="skip the sweep and use this SNR (dB)")
-    ap.add_argument("--sweep-slots", type=int, default=16)
+    ap.add_argument("--sweep-slots", type=int, default=64)
+    ap.add_argument("--no-variants", action="store_true", help="skip the BF16/FP16 precision variants")
     ap.add_argument("--inflight", type=int, default=2,
                     help="slot batches in flight: chain instances (own workspace, own stream) that take the "
                          "steps in turn, so one batch's decoder tail overlaps the next batch's front end")
@@ -436,6 +441,31 @@ def main() -> int:
         stage_ms = {k: v / args.steps for k, v in acc.items()}
         log("stage ms: " + ", ".join(f"{k} {v:.3f}" for k, v in stage_ms.items()))
 
+        # 1b. The precision variants of the same chain (OFDM DFT and the
+        # detector's Gram on the matrix cores) on the measured slots.
+        variants = {}
+        if not args.no_variants:
+            iq_dev_p = plain["iq_dev"]
+            for vname, dft, gram in (("fp16", abi.DFT_FP16, abi.GRAM_FP16), ("bf16", abi.DFT_BF16, abi.GRAM_BF16)):
+                vopts = dataclasses.replace(opts, dft_precision=dft, gram_precision=gram)
+                vc = NativeChain(cfg, plan, max_slots=S, opts=vopts, use_graph=True)
+                for _ in range(args.warmup):
+                    vc.enqueue(iq_dev_p, S)
+                torch.cuda.synchronize()
+                ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                ev0.record(stream)
+                for _ in range(args.steps):
+                    vc.enqueue(iq_dev_p, S)
+                ev1.record(stream)
+                vok = vc.fetch("tb_ok", torch.uint8, S)
+                torch.cuda.synchronize()
+                vms = ev0.elapsed_time(ev1) / args.steps
+                variants[vname] = {"dft_precision": dft, "gram_precision": gram, "ms_per_step": round(vms, 4),
+                                   "gbps": round(S * plan.tbs / (vms * 1e-3) / 1e9, 4),
+                                   "bler": round(1.0 - float(vok.float().mean()), 4)}
+                log(f"variant {vname}: {vms:.3f} ms/step, BLER {variants[vname]['bler']:.3f}")
+                vc.close()
+
         # 2. The same chain with the learned estimator (LMMSE fallback outside
         # its training SNR range).
         learned = None
@@ -509,6 +539,7 @@ def main() -> int:
             "sweep_bler": {str(k): round(v, 4) for k, v in sweep.items()},
             "estimator": "lmmse",
             "learned": learned,
+            "precision_variants": variants,
             "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
             "chain_hbm_gbps": round(chain_bytes / (ms_per_step * 1e-3) / 1e9, 2),
         },
