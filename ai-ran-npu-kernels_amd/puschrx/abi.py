@@ -418,6 +418,10 @@ PROTOTYPES = {
     # synthetic transmitter (include/airan_pusch_tx.h)
     "airan_pusch_tx_EncodeTbs": (i32, [P(PuschMimoConfig), P(AiranTbPlanV1), u32, u32, vp, vp, vp]),
     "airan_pusch_tx_EncodeTbsSlots": (i32, [P(PuschMimoConfig), P(AiranTbPlanV1), u32, u32, vp, vp, vp, vp]),
+    "airan_pusch_txgpu_Create": (i32, [P(PuschMimoConfig), P(AiranTbPlanV1), u32, u32, P(vp)]),
+    "airan_pusch_txgpu_Destroy": (None, [vp]),
+    "airan_pusch_txgpu_EncodeTbs": (i32, [vp, vp, u32, vp, vp, vp, vp]),
+    "airan_pusch_txgpu_Channel": (i32, [vp, vp, u32, vp, vp, u32, C.c_float, C.c_float, C.c_uint64, u32, vp, vp]),
     "airan_ldpc_bg1_Encode": (i32, [u32, vp, vp]),
     "airan_crc_Compute": (u32, [u32, vp, C.c_uint64]),
     # batched chain (include/airan_pusch_chain.h)
@@ -450,6 +454,8 @@ PROTOTYPES = {
     "airan_ce_refiner_SetParams": (i32, [vp, vp]),
     "airan_ce_refiner_Enqueue": (i32, [vp, vp, vp, u32, vp, vp, vp, vp]),
     "airan_ce_refiner_SetInput": (i32, [vp, u32, vp]),
+    "airan_ce_refiner_SetDilation": (i32, [vp, vp, u32]),
+    "airan_ce_refiner_GetDilation": (i32, [vp, vp, u32]),
     "airan_ce_refiner_SynthBatch": (i32, [vp, vp, vp, u32, vp, u32, C.c_float, C.c_uint64, vp]),
     "airan_ce_refiner_Batch": (i32, [vp, P(AiranCeRefinerBatchV1)]),
     "airan_ce_refiner_TrainStep": (i32, [vp, u32, C.c_float, vp, vp]),
