@@ -10,10 +10,13 @@
 // input channels per instruction.  A workgroup (4 waves) owns a tile of one
 // symbol row: 64 pixels x 64 channels (or 128 x 32); each wave keeps the B
 // fragments of its 32 output channels for all 9 taps in registers for the
-// whole kernel (persistent grid), stages the input tile with its 1-pixel
-// halo (3 rows x (tile + 2) pixels) in LDS, and runs 9 * CIN / 16 MFMAs per
-// tile.  Weight gradients are fp32 VALU outer products over LDS tiles,
-// reduced across workgroups in a fixed order.
+// whole kernel (persistent grid), stages the input tile with its halo
+// (3 rows x (tile + 2 dil) pixels) in LDS, and runs 9 * CIN / 16 MFMAs per
+// tile.  Each layer may dilate its taps along the subcarrier axis (taps at
+// subcarrier offsets -dil, 0, +dil; symbol offsets stay -1, 0, +1), which
+// widens the network's view of the band: dilations 1, 2, 4, 8, 16, 1 see
+// 65 subcarriers.  Weight gradients are fp32 VALU outer products over LDS
+// tiles, reduced across workgroups in a fixed order.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -263,7 +266,10 @@ struct ConvArgs {
     bf16 *out;           // kEpiRelu, kEpiMask: [S][14][W][COUT]
     float *out_f32;      // kEpiResid: [S][14][W][COUT]
     uint32_t S, W;
+    int dil;             // subcarrier dilation of the taps, 1..HALO
 };
+
+constexpr int kMaxDil = 16;
 
 // out[p][co] = epilogue(sum_{tap, ci} in[p + off(tap)][ci] wt[tap][co][ci]):
 //   kEpiRelu   bf16(max(acc + bias, 0))
@@ -273,7 +279,8 @@ struct ConvArgs {
 // (channels) 8 (lane / 32) .. + 7; B column (output channel) = lane % 32,
 // same K; C column = lane % 32, row of accumulator i = 8 (i / 4) +
 // 4 (lane / 32) + i % 4.
-template <int CIN, int COUT, int EPI>
+// HALO: the largest dilation the LDS tile holds (1, or kMaxDil).
+template <int CIN, int COUT, int EPI, int HALO>
 __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
 {
     constexpr int CO_BLOCKS = COUT / 32;
@@ -281,7 +288,9 @@ __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
     constexpr int TILE = 32 * PX_WAVES;
     constexpr int KS = CIN / 16;
     constexpr int VPP = CIN / 8 + 1;  // bf16x8 vectors per LDS pixel (one of padding)
-    __shared__ bf16x8 tile[3][TILE + 2][VPP];
+    __shared__ bf16x8 tile[3][TILE + 2 * HALO][VPP];
+    const int dil = HALO == 1 ? 1 : a.dil;
+    const int tw = TILE + 2 * dil;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wco = wave % CO_BLOCKS, wpx = wave / CO_BLOCKS;
     const int co = wco * 32 + (lane & 31);
@@ -302,9 +311,9 @@ __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
         const size_t s = sh / kSym;
         const int w0 = static_cast<int>(wt_i) * TILE;
         constexpr int CH = CIN / 8;
-        for (int e = threadIdx.x; e < 3 * (TILE + 2) * CH; e += 256) {
-            const int c = e % CH, rp = e / CH, p = rp % (TILE + 2), r = rp / (TILE + 2);
-            const int hh = static_cast<int>(h) + r - 1, ww = w0 + p - 1;
+        for (int e = threadIdx.x; e < 3 * tw * CH; e += 256) {
+            const int c = e % CH, rp = e / CH, p = rp % tw, r = rp / tw;
+            const int hh = static_cast<int>(h) + r - 1, ww = w0 + p - dil;
             bf16x8 v;
 #pragma unroll
             for (int j = 0; j < 8; ++j) v[j] = static_cast<bf16>(0.0f);
@@ -323,7 +332,7 @@ __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
             for (int kw = 0; kw < 3; ++kw)
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(tile[kh][pl + kw][2 * ks + kh8],
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(tile[kh][pl + kw * dil][2 * ks + kh8],
                                                                   wf[kh * 3 + kw][ks], acc, 0, 0, 0);
         __syncthreads();
 #pragma unroll
@@ -349,14 +358,16 @@ __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
 // row): part[tap][co][ci] = sum_p dz[p][co] a[p + off(tap)][ci] and
 // part[9 COUT CIN + co] = sum_p dz[p][co].  Thread: co = t % COUT and the
 // channel group g = t / COUT of CIN / (256 / COUT) input channels.
-template <int CIN, int COUT>
+template <int CIN, int COUT, int HALO>
 __global__ void __launch_bounds__(256) WgradKernel(const bf16 *__restrict__ act, const bf16 *__restrict__ dz,
-                                                   float *__restrict__ part, uint32_t S, uint32_t W)
+                                                   float *__restrict__ part, uint32_t S, uint32_t W, int dil_arg)
 {
     constexpr int TILE = 64;
     constexpr int G = 256 / COUT;
     constexpr int CG = CIN / G;
-    __shared__ float sa[3][TILE + 2][CIN];
+    __shared__ float sa[3][TILE + 2 * HALO][CIN];
+    const int dil = HALO == 1 ? 1 : dil_arg;
+    const int tw = TILE + 2 * dil;
     __shared__ float sd[TILE][COUT];
     const int t = threadIdx.x;
     const int co = t % COUT, g = t / COUT;
@@ -373,9 +384,9 @@ __global__ void __launch_bounds__(256) WgradKernel(const bf16 *__restrict__ act,
         const uint32_t h = sh % kSym;
         const size_t s = sh / kSym;
         const int w0 = static_cast<int>(wt_i) * TILE;
-        for (int e = t; e < 3 * (TILE + 2) * CIN; e += 256) {
-            const int c = e % CIN, rp = e / CIN, p = rp % (TILE + 2), r = rp / (TILE + 2);
-            const int hh = static_cast<int>(h) + r - 1, ww = w0 + p - 1;
+        for (int e = t; e < 3 * tw * CIN; e += 256) {
+            const int c = e % CIN, rp = e / CIN, p = rp % tw, r = rp / tw;
+            const int hh = static_cast<int>(h) + r - 1, ww = w0 + p - dil;
             float v = 0.0f;
             if (hh >= 0 && hh < kSym && ww >= 0 && ww < static_cast<int>(W))
                 v = static_cast<float>(act[((s * kSym + hh) * W + ww) * CIN + c]);
@@ -394,7 +405,7 @@ __global__ void __launch_bounds__(256) WgradKernel(const bf16 *__restrict__ act,
             for (int kh = 0; kh < 3; ++kh)
 #pragma unroll
                 for (int kw = 0; kw < 3; ++kw) {
-                    const float *ar = &sa[kh][p + kw][g * CG];
+                    const float *ar = &sa[kh][p + kw * dil][g * CG];
 #pragma unroll
                     for (int c = 0; c < CG; ++c) acc[kh * 3 + kw][c] += d * ar[c];
                 }
@@ -544,6 +555,7 @@ struct AiranCeRefiner {
     void *ce_ws = nullptr;
     size_t ce_ws_bytes = 0;
     __half *est_re = nullptr, *est_im = nullptr;  // [slots][NR][L][14][padded]
+    int dil[kLayers] = {1, 1, 1, 1, 1, 1};        // subcarrier dilation per layer (SetDilation)
 };
 
 namespace {
@@ -556,7 +568,10 @@ void LaunchConv(const AiranCeRefiner *r, const ConvArgs &a, hipStream_t s)
     constexpr int TILE = 32 * (4 / (COUT / 32));
     const uint32_t ntiles = a.S * kSym * ((a.W + TILE - 1) / TILE);
     const uint32_t grid = std::min<uint32_t>(ntiles, static_cast<uint32_t>(r->cus) * 2);
-    hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI>), dim3(grid), dim3(256), 0, s, a);
+    if (a.dil == 1)
+        hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI, 1>), dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI, kMaxDil>), dim3(grid), dim3(256), 0, s, a);
 }
 
 // Forward of the network on x [S][14][W][32] -> out [S][14][W][32] fp32,
@@ -569,6 +584,7 @@ void Forward(const AiranCeRefiner *r, const bf16 *x, uint32_t S, uint32_t W, hip
     const bf16 *in = x;
     for (int i = 0; i < kLayers; ++i) {
         a.in = in;
+        a.dil = r->dil[i];
         a.wt = r->wf + ParamOffset(i);
         a.bias = r->params + ParamOffset(i) + WeightCount(i);
         if (i < kLayers - 1) {
@@ -592,7 +608,12 @@ void LaunchWgrad(const AiranCeRefiner *r, const bf16 *act, const bf16 *dz, uint3
 {
     const uint32_t ntiles = S * kSym * ((W + 63) / 64);
     const uint32_t nb = std::min<uint32_t>(ntiles, static_cast<uint32_t>(r->cus) * 2);
-    hipLaunchKernelGGL((WgradKernel<CIN, COUT>), dim3(nb), dim3(256), 0, s, act, dz, r->part, S, W);
+    const int dil = r->dil[layer];
+    if (dil == 1)
+        hipLaunchKernelGGL((WgradKernel<CIN, COUT, 1>), dim3(nb), dim3(256), 0, s, act, dz, r->part, S, W, 1);
+    else
+        hipLaunchKernelGGL((WgradKernel<CIN, COUT, kMaxDil>), dim3(nb), dim3(256), 0, s, act, dz, r->part, S, W,
+                           dil);
     const uint32_t n = static_cast<uint32_t>(WeightCount(layer) + kCout[layer]);
     hipLaunchKernelGGL(ReduceKernel, dim3(airan::CeilDiv(n, 256)), dim3(256), 0, s, r->part, nb, n,
                        r->grad + ParamOffset(layer));
@@ -621,6 +642,7 @@ void Backward(const AiranCeRefiner *r, const bf16 *x, const float *target, const
         ConvArgs a{};
         a.S = S;
         a.W = W;
+        a.dil = r->dil[i];
         a.in = dzi;
         a.wt = r->wb + ParamOffset(i);
         a.aux = r->act[i];
@@ -843,6 +865,22 @@ int32_t airan_ce_refiner_SetInput(AiranCeRefiner *r, uint32_t mode, const struct
     }
     r->input_mode = mode;
     r->filter = mode == 1 ? filter : nullptr;
+    return AIRAN_OK;
+}
+
+int32_t airan_ce_refiner_SetDilation(AiranCeRefiner *r, const uint32_t *dilation, uint32_t count)
+{
+    if (r == nullptr || dilation == nullptr || count != kLayers) return AIRAN_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < count; ++i)
+        if (dilation[i] < 1 || dilation[i] > static_cast<uint32_t>(kMaxDil)) return AIRAN_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < count; ++i) r->dil[i] = static_cast<int>(dilation[i]);
+    return AIRAN_OK;
+}
+
+int32_t airan_ce_refiner_GetDilation(const AiranCeRefiner *r, uint32_t *dilation, uint32_t count)
+{
+    if (r == nullptr || dilation == nullptr || count != kLayers) return AIRAN_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < count; ++i) dilation[i] = static_cast<uint32_t>(r->dil[i]);
     return AIRAN_OK;
 }
 

@@ -61,9 +61,53 @@ def tdl_gains(rng: np.random.Generator, nr: int, nl: int, taps: int, delay_sprea
     return out.astype(np.complex64), delays.astype(np.float32)
 
 
+def tdl_gains_device(gen: torch.Generator, nr: int, nl: int, taps: int, delay_spread: np.ndarray,
+                     doppler: np.ndarray, rx_corr: np.ndarray, scs_hz: float, sinusoids: int = 16):
+    """tdl_gains computed on the device with torch (the same model; a
+    different random stream), returned on the host: a 64-slot batch is
+    ~4 M complex exponentials, which numpy takes ~0.2 s for."""
+    dev = gen.device
+    ds = torch.as_tensor(delay_spread, dtype=torch.float64, device=dev)
+    fdmax = torch.as_tensor(doppler, dtype=torch.float64, device=dev)
+    rho = torch.as_tensor(rx_corr, dtype=torch.float64, device=dev)
+    S = ds.numel()
+    delays = torch.arange(taps, dtype=torch.float64, device=dev)[None, :] * (4.0 * ds[:, None] / taps)
+    pdp = torch.exp(-delays / ds[:, None])
+    pdp = pdp / pdp.sum(dim=1, keepdim=True)
+    t = torch.arange(14, dtype=torch.float64, device=dev) * ((1.0 + 9.0 / 128.0) / scs_hz)
+    M = sinusoids
+    two_pi = 2.0 * np.pi
+    alpha = torch.rand((S, nr, nl, taps, M), generator=gen, dtype=torch.float64, device=dev) * two_pi
+    phi = torch.rand((S, nr, nl, taps, M), generator=gen, dtype=torch.float64, device=dev) * two_pi
+    fd = fdmax[:, None, None, None, None] * torch.cos(alpha)
+    ph = two_pi * fd[..., None] * t + phi[..., None]
+    g = torch.polar(torch.ones_like(ph), ph).sum(dim=4) / np.sqrt(M)                # [S, r, l, T, 14]
+    g = g * torch.sqrt(pdp)[:, None, None, :, None]
+    idx = torch.arange(nr, dtype=torch.float64, device=dev)
+    R = rho[:, None, None] ** (idx[:, None] - idx[None, :]).abs()                    # [S, r, r]
+    w, v = torch.linalg.eigh(R)
+    rsq = (v * torch.sqrt(torch.clamp_min(w, 0.0))[:, None, :]) @ v.transpose(1, 2)
+    g = torch.einsum("sij,sjltk->siltk", rsq.to(g.dtype), g)
+    return g.to(torch.complex64).cpu().numpy(), delays.to(torch.float32).cpu().numpy()
+
+
 def nmse_db(est: torch.Tensor, ref: torch.Tensor) -> float:
     e = (est.double() - ref.double()).pow(2).sum()
     return float(10.0 * torch.log10(e / ref.double().pow(2).sum()).item())
+
+
+CIN = (32, 64, 64, 64, 64, 64)
+COUT = (64, 64, 64, 64, 64, 32)
+
+
+def bias_slices():
+    """Slices of each layer's biases in the parameter vector."""
+    out, off = [], 0
+    for ci, co in zip(CIN, COUT):
+        off += 9 * co * ci
+        out.append(slice(off, off + co))
+        off += co
+    return out
 
 
 # First parameter of the last convolution (32 <- 64): layer i holds
@@ -79,7 +123,7 @@ class Refiner:
     mode-1 weights start with a zero last layer, i.e. at the LMMSE estimate."""
 
     def __init__(self, cfg, max_slots: int, params: np.ndarray | None = None, seed: int = 1,
-                 input_mode: int = 0, estimator: ops.LmmseFilter | None = None):
+                 input_mode: int = 0, estimator: ops.LmmseFilter | None = None, dilation=None):
         self.lib = abi.load_library()
         self.cfg = cfg
         self.g = ops.Geometry(cfg)
@@ -91,6 +135,9 @@ class Refiner:
             C.byref(cfg), max_slots, None if p is None else p.ctypes.data_as(C.c_void_p), seed,
             C.c_void_p(ops.stream_ptr()), C.byref(self.handle)))
         self._loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        self.dilation = [1] * 6 if dilation is None else [int(d) for d in dilation]
+        d = (C.c_uint32 * 6)(*self.dilation)
+        ops._check("refiner SetDilation", self.lib.airan_ce_refiner_SetDilation(self.handle, d, 6))
         if input_mode:
             ops._check("refiner SetInput", self.lib.airan_ce_refiner_SetInput(
                 self.handle, input_mode, estimator.handle if estimator is not None else None))
@@ -112,16 +159,28 @@ class Refiner:
         ops._check("refiner SetParams", self.lib.airan_ce_refiner_SetParams(self.handle,
                                                                           p.ctypes.data_as(C.c_void_p)))
 
-    def save(self, path: Path = WEIGHTS) -> None:
-        """Weights (fp32 raw) and a JSON sidecar with the input mode."""
+    def save(self, path: Path = WEIGHTS, extra: dict | None = None) -> None:
+        """Weights (fp32 raw) and a JSON sidecar with the input mode, the
+        dilations and `extra` (training record)."""
         path.parent.mkdir(parents=True, exist_ok=True)
         self.params().tofile(path)
-        Path(str(path) + ".json").write_text(json.dumps({"input_mode": self.input_mode}) + "\n")
+        meta = {"input_mode": self.input_mode, "dilation": self.dilation, **(extra or {})}
+        Path(str(path) + ".json").write_text(json.dumps(meta) + "\n")
 
     @staticmethod
     def load_meta(path: Path = WEIGHTS) -> dict:
         meta = Path(str(path) + ".json")
-        return json.loads(meta.read_text()) if meta.exists() else {"input_mode": 0}
+        m = json.loads(meta.read_text()) if meta.exists() else {}
+        m.setdefault("input_mode", 0)
+        m.setdefault("dilation", [1] * 6)
+        return m
+
+    @classmethod
+    def from_file(cls, cfg, max_slots: int, path: Path = WEIGHTS, estimator: ops.LmmseFilter | None = None):
+        """A refiner with the weights, input mode and dilations of `path`."""
+        meta = cls.load_meta(path)
+        return cls(cfg, max_slots=max_slots, params=cls.load_params(path), input_mode=int(meta["input_mode"]),
+                   estimator=estimator, dilation=meta["dilation"])
 
     @staticmethod
     def load_params(path: Path = WEIGHTS) -> np.ndarray:
@@ -133,14 +192,18 @@ class Refiner:
     # ----------------------------------------------------------------- batches
     def synth(self, rng: np.random.Generator, slots: int, spec: TrainSpec, seed: int,
               snr_db: np.ndarray | None = None, delay_spread: np.ndarray | None = None,
-              doppler: np.ndarray | None = None, rx_corr: np.ndarray | None = None) -> None:
+              doppler: np.ndarray | None = None, rx_corr: np.ndarray | None = None,
+              gen: torch.Generator | None = None) -> None:
+        """A batch from the TrainSpec ranges (or the given values); `gen`: draw
+        the fading gains on the device (tdl_gains_device) instead of numpy."""
         u = lambda r: rng.uniform(r[0], r[1], slots)  # noqa: E731
         snr = u(spec.snr_db) if snr_db is None else np.broadcast_to(snr_db, slots)
         ds = u(spec.delay_spread_s) if delay_spread is None else np.broadcast_to(delay_spread, slots)
         fd = u(spec.doppler_hz) if doppler is None else np.broadcast_to(doppler, slots)
         rc = u(spec.rx_corr) if rx_corr is None else np.broadcast_to(rx_corr, slots)
-        gains, tau = tdl_gains(rng, self.g.nr, self.g.nl, spec.taps, np.asarray(ds, np.float64),
-                               np.asarray(fd, np.float64), np.asarray(rc, np.float64), spec.scs_hz, spec.sinusoids)
+        make = tdl_gains if gen is None else tdl_gains_device
+        gains, tau = make(rng if gen is None else gen, self.g.nr, self.g.nl, spec.taps, np.asarray(ds, np.float64),
+                          np.asarray(fd, np.float64), np.asarray(rc, np.float64), spec.scs_hz, spec.sinusoids)
         snr = np.ascontiguousarray(snr, np.float32)
         self._keep = (gains, tau, snr)  # host buffers stay alive until the copies complete
         ops._check("refiner SynthBatch", self.lib.airan_ce_refiner_SynthBatch(
@@ -181,12 +244,17 @@ class Refiner:
         return grad, loss
 
     def train(self, rng: np.random.Generator, steps: int, slots: int, spec: TrainSpec, lr: float = 1e-3,
-              log_every: int = 0, seed0: int = 1000, lr_final: float | None = None):
-        """Adam on freshly synthesised batches; returns the loss history."""
+              log_every: int = 0, seed0: int = 1000, lr_final: float | None = None, schedule=None,
+              gen: torch.Generator | None = None):
+        """Adam on freshly synthesised batches; returns the loss history.
+        schedule(i) -> learning rate of step i (overrides lr / lr_final)."""
         hist = []
         for i in range(steps):
-            self.synth(rng, slots, spec, seed0 + i)
-            a = lr if lr_final is None else lr * (lr_final / lr) ** (i / max(steps - 1, 1))
+            self.synth(rng, slots, spec, seed0 + i, gen=gen)
+            if schedule is not None:
+                a = schedule(i)
+            else:
+                a = lr if lr_final is None else lr * (lr_final / lr) ** (i / max(steps - 1, 1))
             loss = self.train_step(slots, a)
             if log_every and (i % log_every == 0 or i == steps - 1):
                 hist.append((i, float(loss.item())))

@@ -472,8 +472,7 @@ def main() -> int:
         wpath = Path(args.refiner) if args.refiner else None
         if wpath is not None and wpath.exists():
             from puschrx.refiner import Refiner
-            ref = Refiner(cfg, max_slots=max(S, args.sweep_slots), params=Refiner.load_params(wpath),
-                          input_mode=int(Refiner.load_meta(wpath).get("input_mode", 0)))
+            ref = Refiner.from_file(cfg, max(S, args.sweep_slots), wpath)
             chain.set_refiner(ref, args.refiner_snr_min, args.refiner_snr_max)
             l_snr, l_sweep = (operating_point("learned") if args.snr is None else (args.snr, {}))
             learned = {"weights": str(wpath.relative_to(ROOT)) if wpath.is_relative_to(ROOT) else str(wpath),

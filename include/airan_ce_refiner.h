@@ -6,7 +6,8 @@
  *
  * Network (4 Rx x 4 layers): input X [slots][14][used][32] bf16, channel
  * c = 2 (rx * 4 + layer) + {0: Re, 1: Im}; six 3x3 convolutions (stride 1,
- * zero padding) 32 -> 64 -> 64 -> 64 -> 64 -> 64 -> 32 with ReLU after the
+ * zero padding, optionally dilated along the subcarriers:
+ * airan_ce_refiner_SetDilation) 32 -> 64 -> 64 -> 64 -> 64 -> 64 -> 32 with ReLU after the
  * first five, and a residual: H = X + conv6(...).  Weights and activations
  * bf16, accumulation fp32 (MFMA), master weights fp32 trained with Adam on
  * the device.  Parameters, in order per layer: weights [9][cout][cin]
@@ -64,6 +65,14 @@ int32_t airan_ce_refiner_Enqueue(AiranCeRefiner *refiner, const void *h_ls_re, c
  * grant shape; required for SynthBatch in mode 1, not owned).  Weights are
  * trained for one mode. */
 int32_t airan_ce_refiner_SetInput(AiranCeRefiner *refiner, uint32_t mode, const struct LmmseFilter *filter);
+
+/* Subcarrier dilation of each of the six layers' taps (count = 6, each
+ * 1..16; default 1): layer i reads subcarrier offsets -d_i, 0, +d_i (symbol
+ * offsets stay -1, 0, +1).  The dilations are part of the architecture: the
+ * weights are trained for one setting.  Dilations 1, 2, 4, 8, 16, 1 give the
+ * network a 65-subcarrier view of the band instead of 13. */
+int32_t airan_ce_refiner_SetDilation(AiranCeRefiner *refiner, const uint32_t *dilation, uint32_t count);
+int32_t airan_ce_refiner_GetDilation(const AiranCeRefiner *refiner, uint32_t *dilation, uint32_t count);
 
 /* Synthesises a training or evaluation batch on the device into the
  * refiner's batch buffers: gains host complex64 [slots][NR][L][taps][14]

@@ -13,7 +13,10 @@
  * differences of the loss.
  *
  * Notation: layer i (0..5) maps a_i (a_0 = X, 32 channels) to
- * z_i = conv3x3(a_i, W_i) + b_i; a_{i+1} = relu(z_i) for i < 5;
+ * z_i = conv3x3(a_i, W_i) + b_i; a_{i+1} = relu(z_i) for i < 5; the taps
+ * of layer i sit at symbol offsets -1, 0, +1 and subcarrier offsets
+ * -d_i, 0, +d_i (d_i its dilation, 1 by default): off(tap) = (kh - 1,
+ * (kw - 1) d_i);
  * out = X + z_5; loss = mean((out - T)^2) over all N values.
  * Backward: dz_5 = 2 (out - T) / N; gW_i[tap][co][ci] = sum_p dz_i[p][co]
  * a_i[p + off(tap)][ci]; gb_i = sum_p dz_i; dz_{i-1} = convT(dz_i, W_i)
@@ -52,7 +55,7 @@ static double rbf16(double v)
 double or_bf16_round(double v) { return rbf16(v); }
 
 /* out[p][co] = b[co] + sum_{tap,ci} in[p + off(tap)][ci] w[tap][co][ci] (zero padding). */
-static void conv_fwd(const double *in, int cin, const double *w, const double *b, int cout, int S, int W,
+static void conv_fwd(const double *in, int cin, const double *w, const double *b, int cout, int S, int W, int d,
                      double *out)
 {
 #pragma omp parallel for collapse(2) schedule(static)
@@ -65,7 +68,7 @@ static void conv_fwd(const double *in, int cin, const double *w, const double *b
                         const int hh = h + kh - 1;
                         if (hh < 0 || hh >= 14) continue;
                         for (int kw = 0; kw < 3; ++kw) {
-                            const int ww = x + kw - 1;
+                            const int ww = x + (kw - 1) * d;
                             if (ww < 0 || ww >= W) continue;
                             const double *ip = in + (((size_t)s * 14 + hh) * W + ww) * cin;
                             const double *wp = w + ((size_t)(kh * 3 + kw) * cout + co) * cin;
@@ -77,7 +80,7 @@ static void conv_fwd(const double *in, int cin, const double *w, const double *b
 }
 
 /* da[q][ci] = sum_{tap,co} dz[q - off(tap)][co] w[tap][co][ci]. */
-static void conv_bwd_data(const double *dz, int cout, const double *w, int cin, int S, int W, double *da)
+static void conv_bwd_data(const double *dz, int cout, const double *w, int cin, int S, int W, int d, double *da)
 {
 #pragma omp parallel for collapse(2) schedule(static)
     for (int s = 0; s < S; ++s)
@@ -89,7 +92,7 @@ static void conv_bwd_data(const double *dz, int cout, const double *w, int cin, 
                         const int hh = h - (kh - 1);
                         if (hh < 0 || hh >= 14) continue;
                         for (int kw = 0; kw < 3; ++kw) {
-                            const int ww = x - (kw - 1);
+                            const int ww = x - (kw - 1) * d;
                             if (ww < 0 || ww >= W) continue;
                             const double *dp = dz + (((size_t)s * 14 + hh) * W + ww) * cout;
                             for (int co = 0; co < cout; ++co)
@@ -101,7 +104,7 @@ static void conv_bwd_data(const double *dz, int cout, const double *w, int cin, 
 }
 
 /* gw[tap][co][ci] = sum_p dz[p][co] a[p + off(tap)][ci]; gb[co] = sum_p dz[p][co]. */
-static void conv_bwd_weight(const double *dz, int cout, const double *a, int cin, int S, int W, double *gw,
+static void conv_bwd_weight(const double *dz, int cout, const double *a, int cin, int S, int W, int d, double *gw,
                             double *gb)
 {
 #pragma omp parallel for collapse(2) schedule(static)
@@ -115,7 +118,7 @@ static void conv_bwd_weight(const double *dz, int cout, const double *a, int cin
                         const int hh = h + kh - 1;
                         if (hh < 0 || hh >= 14) continue;
                         for (int x = 0; x < W; ++x) {
-                            const int ww = x + kw - 1;
+                            const int ww = x + (kw - 1) * d;
                             if (ww < 0 || ww >= W) continue;
                             acc += dz[(((size_t)s * 14 + h) * W + x) * cout + co] *
                                    a[(((size_t)s * 14 + hh) * W + ww) * cin + ci];
@@ -133,6 +136,13 @@ static void conv_bwd_weight(const double *dz, int cout, const double *a, int cin
 
 int or_refiner_grad(const float *params, const float *x, const float *target, int S, int W, int device_rounding,
                     double *grad, double *loss, float *out)
+{
+    static const int ones[OR_RF_LAYERS] = {1, 1, 1, 1, 1, 1};
+    return or_refiner_grad_dil(params, x, target, S, W, device_rounding, ones, grad, loss, out);
+}
+
+int or_refiner_grad_dil(const float *params, const float *x, const float *target, int S, int W, int device_rounding,
+                        const int *dil, double *grad, double *loss, float *out)
 {
     const size_t P = (size_t)S * 14 * W;
     const size_t np = or_refiner_param_offset(OR_RF_LAYERS);
@@ -155,13 +165,13 @@ int or_refiner_grad(const float *params, const float *x, const float *target, in
         const double *b = w + (size_t)9 * kRfCout[i] * kRfCin[i];
         if (i < OR_RF_LAYERS - 1) {
             act[i + 1] = malloc(P * kRfCout[i] * sizeof(double));
-            conv_fwd(act[i], kRfCin[i], w, b, kRfCout[i], S, W, act[i + 1]);
+            conv_fwd(act[i], kRfCin[i], w, b, kRfCout[i], S, W, dil[i], act[i + 1]);
             for (size_t k = 0; k < P * kRfCout[i]; ++k) {
                 double v = act[i + 1][k] > 0.0 ? act[i + 1][k] : 0.0;
                 act[i + 1][k] = device_rounding ? rbf16(v) : v;
             }
         } else {
-            conv_fwd(act[i], kRfCin[i], w, b, kRfCout[i], S, W, z5);
+            conv_fwd(act[i], kRfCin[i], w, b, kRfCout[i], S, W, dil[i], z5);
         }
     }
     const double n = (double)P * 32;
@@ -178,9 +188,9 @@ int or_refiner_grad(const float *params, const float *x, const float *target, in
     for (int i = OR_RF_LAYERS - 1; i >= 0; --i) {
         const double *w = wq + or_refiner_param_offset(i);
         double *gw = grad + or_refiner_param_offset(i);
-        conv_bwd_weight(dz, kRfCout[i], act[i], kRfCin[i], S, W, gw, gw + (size_t)9 * kRfCout[i] * kRfCin[i]);
+        conv_bwd_weight(dz, kRfCout[i], act[i], kRfCin[i], S, W, dil[i], gw, gw + (size_t)9 * kRfCout[i] * kRfCin[i]);
         if (i == 0) break;
-        conv_bwd_data(dz, kRfCout[i], w, kRfCin[i], S, W, da);
+        conv_bwd_data(dz, kRfCout[i], w, kRfCin[i], S, W, dil[i], da);
         for (size_t k = 0; k < P * kRfCin[i]; ++k) {
             const double v = act[i][k] > 0.0 ? da[k] : 0.0;
             dz[k] = device_rounding ? rbf16(v) : v;

@@ -198,6 +198,9 @@ int or_num_threads(void);
 size_t or_refiner_param_offset(int layer);
 int or_refiner_grad(const float *params, const float *x, const float *target, int S, int W, int device_rounding,
                     double *grad, double *loss, float *out);
+/* The same with a subcarrier dilation per layer (dil [6], each >= 1). */
+int or_refiner_grad_dil(const float *params, const float *x, const float *target, int S, int W, int device_rounding,
+                        const int *dil, double *grad, double *loss, float *out);
 double or_bf16_round(double v);
 
 #ifdef __cplusplus

@@ -20,15 +20,19 @@ def oracle():
     return load_oracle()
 
 
-@pytest.mark.parametrize("S,W", [(1, 40), (2, 70)])
-def test_refiner_gradients_match_oracle(oracle, gpu_stream, S, W):
+GRAD_CASES = [(1, 40, None), (2, 70, None), (1, 70, [1, 2, 4, 8, 16, 1]), (2, 150, [2, 1, 16, 3, 5, 16])]
+
+
+@pytest.mark.parametrize("S,W,dil", GRAD_CASES, ids=["w40", "w70", "w70_dil1-16", "w150_dil_mixed"])
+def test_refiner_gradients_match_oracle(oracle, gpu_stream, S, W, dil):
     rng = np.random.default_rng(11 + W)
     cfg = P.FULL_CARRIER_4RX.config()
     params = refiner_params(oracle, rng)
     x = torch.from_numpy(rng.normal(0, 1, (S, 14, W, 32)).astype(np.float32)).to(torch.bfloat16)
     xf = x.float().numpy()
     t = (xf + rng.normal(0, 0.3, xf.shape)).astype(np.float32)
-    ref = Refiner(cfg, max_slots=S, params=params)
+    ref = Refiner(cfg, max_slots=S, params=params, dilation=dil)
+    print(f"refiner gradients S={S} W={W} dilation {ref.dilation}", flush=True)
     grad, loss = ref.gradients(x.cuda(), torch.from_numpy(t).cuda(), S, W)
     n = S * 14 * W * 32
     out = ref.fetch("out", torch.float32, n)
@@ -37,8 +41,9 @@ def test_refiner_gradients_match_oracle(oracle, gpu_stream, S, W):
     want_g = np.zeros(params.size)
     want_l = np.zeros(1)
     want_o = np.zeros(n, np.float32)
-    assert oracle.or_refiner_grad(ptr(params), ptr(np.ascontiguousarray(xf)), ptr(t), S, W, 1, ptr(want_g),
-                                  ptr(want_l), ptr(want_o)) == 0
+    d = np.asarray(ref.dilation, np.int32)
+    assert oracle.or_refiner_grad_dil(ptr(params), ptr(np.ascontiguousarray(xf)), ptr(t), S, W, 1, ptr(d),
+                                      ptr(want_g), ptr(want_l), ptr(want_o)) == 0
     # fp32 accumulation on the device against double in the oracle flips an
     # occasional bf16 rounding of an activation (one bf16 ulp, 2^-8), which
     # the later layers carry: agreement is to ~1e-4 in norm, not bit-exact.

@@ -102,3 +102,66 @@ def test_bf16_rounding(oracle):
     assert np.array_equal(got, r)
     assert oracle.or_bf16_round(1.0 + 2 ** -8) == 1.0          # tie to even
     assert oracle.or_bf16_round(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6
+
+
+def oracle_loss_grad_dil(oracle, params, x, t, S, W, rounding, dil):
+    grad = np.zeros(params.size, np.float64)
+    loss = np.zeros(1, np.float64)
+    out = np.zeros(x.size, np.float32)
+    d = np.asarray(dil, np.int32)
+    assert oracle.or_refiner_grad_dil(ptr(params), ptr(x), ptr(t), S, W, rounding, ptr(d), ptr(grad), ptr(loss),
+                                      ptr(out)) == 0
+    return float(loss[0]), grad, out.reshape(x.shape)
+
+
+def test_dilated_gradients_match_finite_differences(oracle):
+    """Per-layer subcarrier dilations (taps at -d, 0, +d): analytic gradients
+    against central differences of the loss, with dilations that reach past
+    the edges of a 9-subcarrier input."""
+    rng = np.random.default_rng(8)
+    S, W = 1, 9
+    dil = [1, 2, 3, 1, 4, 2]
+    params = refiner_params(oracle, rng)
+    x = rng.normal(0, 1, (S, 14, W, 32)).astype(np.float32)
+    t = (x + rng.normal(0, 0.3, x.shape)).astype(np.float32)
+    _, grad, _ = oracle_loss_grad_dil(oracle, params, x, t, S, W, 0, dil)
+    picks = []
+    for i in range(6):
+        o = oracle.or_refiner_param_offset(i)
+        nw = 9 * COUT[i] * CIN[i]
+        picks += list(rng.integers(o, o + nw, 6)) + list(rng.integers(o + nw, o + nw + COUT[i], 2))
+    ok = 0
+    for k in picks:
+        eps = 1e-4
+        pp, pm = params.copy(), params.copy()
+        pp[k] = np.float32(params[k] + eps)
+        pm[k] = np.float32(params[k] - eps)
+        lp, _, _ = oracle_loss_grad_dil(oracle, pp, x, t, S, W, 0, dil)
+        lm, _, _ = oracle_loss_grad_dil(oracle, pm, x, t, S, W, 0, dil)
+        fd = (lp - lm) / (float(pp[k]) - float(pm[k]))
+        ok += abs(fd - grad[k]) <= 1e-5 * max(abs(fd), abs(grad[k])) + 1e-10
+    assert ok >= len(picks) - 3, (ok, len(picks))
+
+
+def test_uniform_dilation_splits_into_phases(oracle):
+    """Every layer dilated by d: the subcarriers of each phase (w mod d) only
+    see each other, so the network on the full width equals the undilated
+    network on each phase's subsequence; the loss is their size-weighted
+    mean and the gradients their sum with the same weights."""
+    rng = np.random.default_rng(9)
+    S, W, d = 2, 11, 3
+    params = refiner_params(oracle, rng)
+    x = rng.normal(0, 1, (S, 14, W, 32)).astype(np.float32)
+    t = (x + rng.normal(0, 0.3, x.shape)).astype(np.float32)
+    loss, grad, out = oracle_loss_grad_dil(oracle, params, x, t, S, W, 0, [d] * 6)
+    want_loss, want_grad = 0.0, np.zeros_like(grad)
+    for r in range(d):
+        xs = np.ascontiguousarray(x[:, :, r::d, :])
+        ts = np.ascontiguousarray(t[:, :, r::d, :])
+        w = xs.shape[2]
+        l_r, g_r, o_r = oracle_loss_grad_dil(oracle, params, xs, ts, S, w, 0, [1] * 6)
+        np.testing.assert_allclose(out[:, :, r::d, :], o_r, rtol=1e-6, atol=1e-6)
+        want_loss += l_r * w / W
+        want_grad += g_r * w / W
+    assert loss == pytest.approx(want_loss, rel=1e-10)
+    np.testing.assert_allclose(grad, want_grad, rtol=1e-9, atol=1e-12)
