@@ -385,7 +385,8 @@ int32_t EnqueueStages(AiranPuschChain *c, const void *iq, const uint16_t *slot_n
     if (c->refiner != nullptr) {
         hipLaunchKernelGGL(RefinerKeepKernel, dim3(airan::CeilDiv(slots, 64)), dim3(64), 0, s, c->noise_ce,
                            c->refiner_keep, g.nr, slots, c->refiner_snr_min, c->refiner_snr_max);
-        st = airan_ce_refiner_Enqueue(c->refiner, c->hls_re, c->hls_im, slots, c->refiner_keep, c->h_re, c->h_im, s);
+        st = airan_ce_refiner_EnqueueWithNoise(c->refiner, c->hls_re, c->hls_im, c->noise_ce, slots, c->refiner_keep,
+                                               c->h_re, c->h_im, s);
         if (st != AIRAN_OK) return st;
     }
     mark();

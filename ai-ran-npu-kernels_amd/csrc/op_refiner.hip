@@ -147,6 +147,7 @@ struct InputArgs {
     const __half *h_re;  // [slots][NR][L][D][pilot_pad]
     const __half *h_im;
     bf16 *x;             // [slots][14][used][32]
+    float *x32;          // the same in fp32 (the residual the network adds to)
     uint32_t used, nr, nl, ndmrs, pilot_pad;
     uint32_t dmrs_sym[4], comb[4], count[4];
     int32_t partner[4];  // FD-OCC partner layer, -1 for a layer alone on its comb
@@ -183,6 +184,7 @@ __global__ void __launch_bounds__(256) InputKernel(InputArgs a)
     const uint32_t s = blockIdx.y;
     if (u >= a.used) return;
     bf16 *xo = a.x + (static_cast<size_t>(s) * kSym * a.used + u) * kCh;
+    float *x32o = a.x32 + (static_cast<size_t>(s) * kSym * a.used + u) * kCh;
     for (uint32_t r = 0; r < a.nr; ++r)
         for (uint32_t l = 0; l < a.nl; ++l) {
             float2 hd[4];
@@ -227,32 +229,65 @@ __global__ void __launch_bounds__(256) InputKernel(InputArgs a)
                     const float2 v1 = hd[a.ndmrs - 1];
                     v = make_float2(hd[0].x + wt * (v1.x - hd[0].x), hd[0].y + wt * (v1.y - hd[0].y));
                 }
-                xo[static_cast<size_t>(k) * a.used * kCh + c] = static_cast<bf16>(v.x);
-                xo[static_cast<size_t>(k) * a.used * kCh + c + 1] = static_cast<bf16>(v.y);
+                const size_t o = static_cast<size_t>(k) * a.used * kCh + c;
+                xo[o] = static_cast<bf16>(v.x);
+                xo[o + 1] = static_cast<bf16>(v.y);
+                x32o[o] = v.x;
+                x32o[o + 1] = v.y;
             }
         }
 }
 
 // Network input from a channel estimate grid h fp16 [slots][NR][L][14][pad]
 // (the LMMSE estimator's output): x[s][k][u][2 (r L + l) + {re, im}] =
-// bf16(h).  Grid (used / 256, slots).
+// bf16(h), and x32 = h exactly (fp16 -> fp32).  Grid (used / 256, slots).
 __global__ void __launch_bounds__(256) EstimateInputKernel(const __half *__restrict__ h_re,
                                                            const __half *__restrict__ h_im, bf16 *__restrict__ x,
-                                                           uint32_t used, uint32_t pad, uint32_t nr, uint32_t nl)
+                                                           float *__restrict__ x32, uint32_t used, uint32_t pad,
+                                                           uint32_t nr, uint32_t nl)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t s = blockIdx.y;
     if (u >= used) return;
     bf16 *xo = x + (static_cast<size_t>(s) * kSym * used + u) * kCh;
+    float *x32o = x32 + (static_cast<size_t>(s) * kSym * used + u) * kCh;
     for (uint32_t r = 0; r < nr; ++r)
         for (uint32_t l = 0; l < nl; ++l) {
             const size_t base = ((static_cast<size_t>(s) * nr + r) * nl + l) * kSym * pad + u;
             const uint32_t c = 2 * (r * nl + l);
             for (uint32_t k = 0; k < kSym; ++k) {
-                xo[static_cast<size_t>(k) * used * kCh + c] = static_cast<bf16>(__half2float(h_re[base + k * pad]));
-                xo[static_cast<size_t>(k) * used * kCh + c + 1] = static_cast<bf16>(__half2float(h_im[base + k * pad]));
+                const size_t o = static_cast<size_t>(k) * used * kCh + c;
+                const float vr = __half2float(h_re[base + k * pad]), vi = __half2float(h_im[base + k * pad]);
+                xo[o] = static_cast<bf16>(vr);
+                xo[o + 1] = static_cast<bf16>(vi);
+                x32o[o] = vr;
+                x32o[o + 1] = vi;
             }
         }
+}
+
+// x32 = float(x) for an input given in bf16 (airan_ce_refiner_Gradients).
+__global__ void __launch_bounds__(256) WidenKernel(const bf16 *__restrict__ x, float *__restrict__ x32, size_t n)
+{
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        x32[i] = static_cast<float>(x[i]);
+}
+
+// Residual scale per slot: sqrt of the mean over Rx of the slot's noise
+// variance (fp16 [slots][NR]), or 1 without a noise input.
+__global__ void __launch_bounds__(64) NoiseScaleKernel(const __half *__restrict__ noise, uint32_t nr, uint32_t slots,
+                                                       float *__restrict__ scale)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slots) return;
+    if (noise == nullptr) {
+        scale[s] = 1.0f;
+        return;
+    }
+    float acc = 0.0f;
+    for (uint32_t r = 0; r < nr; ++r) acc += __half2float(noise[static_cast<size_t>(s) * nr + r]);
+    scale[s] = sqrtf(fmaxf(acc / static_cast<float>(nr), 0.0f));
 }
 
 // ----------------------------------------------------------------- conv
@@ -262,7 +297,9 @@ struct ConvArgs {
     const bf16 *in;      // [S][14][W][CIN]
     const bf16 *wt;      // [9][COUT][CIN]
     const float *bias;   // [COUT] (kEpiRelu, kEpiResid)
-    const bf16 *aux;     // kEpiResid: X [S][14][W][COUT]; kEpiMask: activation [S][14][W][COUT]
+    const bf16 *aux;     // kEpiMask: activation [S][14][W][COUT]
+    const float *aux32;  // kEpiResid: X in fp32 [S][14][W][COUT]
+    const float *scale;  // kEpiResid: residual scale per slot [S]
     bf16 *out;           // kEpiRelu, kEpiMask: [S][14][W][COUT]
     float *out_f32;      // kEpiResid: [S][14][W][COUT]
     uint32_t S, W;
@@ -273,7 +310,7 @@ constexpr int kMaxDil = 16;
 
 // out[p][co] = epilogue(sum_{tap, ci} in[p + off(tap)][ci] wt[tap][co][ci]):
 //   kEpiRelu   bf16(max(acc + bias, 0))
-//   kEpiResid  acc + bias + aux          (fp32)
+//   kEpiResid  scale[slot] (acc + bias) + aux32   (fp32)
 //   kEpiMask   bf16(aux > 0 ? acc : 0)   (gradient through the ReLU)
 // MFMA 32x32x16 bf16 lane layout: A row (pixel) = lane % 32, its 8 K values
 // (channels) 8 (lane / 32) .. + 7; B column (output channel) = lane % 32,
@@ -303,6 +340,7 @@ __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
             wf[tap][ks] = *reinterpret_cast<const bf16x8 *>(a.wt + (static_cast<size_t>(tap) * COUT + co) * CIN +
                                                             ks * 16 + 8 * kh8);
     const float bias = EPI == kEpiMask ? 0.0f : a.bias[co];
+    (void)bias;
     const uint32_t wtiles = (a.W + TILE - 1) / TILE;
     const uint32_t ntiles = a.S * kSym * wtiles;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -345,7 +383,7 @@ __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
             if (EPI == kEpiRelu) {
                 a.out[o] = static_cast<bf16>(fmaxf(v + bias, 0.0f));
             } else if (EPI == kEpiResid) {
-                a.out_f32[o] = v + bias + static_cast<float>(a.aux[o]);
+                a.out_f32[o] = a.scale[s] * (v + bias) + a.aux32[o];
             } else {
                 a.out[o] = static_cast<float>(a.aux[o]) > 0.0f ? static_cast<bf16>(v) : static_cast<bf16>(0.0f);
             }
@@ -436,7 +474,8 @@ __global__ void __launch_bounds__(256) ReduceKernel(const float *__restrict__ pa
 // w_s the weight of the element's slot (1 without weights; the training
 // weights have mean 1 over the batch).
 __global__ void __launch_bounds__(256) LossKernel(const float *__restrict__ out, const float *__restrict__ tgt,
-                                                  const float *__restrict__ slot_w, size_t per_slot,
+                                                  const float *__restrict__ slot_w,
+                                                  const float *__restrict__ res_scale, size_t per_slot,
                                                   bf16 *__restrict__ dz, double *__restrict__ part, size_t n)
 {
     __shared__ double red[4];
@@ -447,7 +486,8 @@ __global__ void __launch_bounds__(256) LossKernel(const float *__restrict__ out,
         const float w = slot_w != nullptr ? slot_w[i / per_slot] : 1.0f;
         const float e = out[i] - tgt[i];
         acc += static_cast<double>(w) * e * e;
-        dz[i] = static_cast<bf16>(w * e * scale);
+        // out = aux + scale_s z: the gradient of the last pre-activation z.
+        dz[i] = static_cast<bf16>(w * e * scale * res_scale[i / per_slot]);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -556,6 +596,9 @@ struct AiranCeRefiner {
     size_t ce_ws_bytes = 0;
     __half *est_re = nullptr, *est_im = nullptr;  // [slots][NR][L][14][padded]
     int dil[kLayers] = {1, 1, 1, 1, 1, 1};        // subcarrier dilation per layer (SetDilation)
+    float *x32 = nullptr;                         // the input in fp32 (the residual's base)
+    float *res_scale = nullptr;                   // residual scale per slot
+    uint32_t res_mode = 0;                        // 0: scale 1; 1: the slot's noise standard deviation
 };
 
 namespace {
@@ -595,7 +638,8 @@ void Forward(const AiranCeRefiner *r, const bf16 *x, uint32_t S, uint32_t W, hip
                 LaunchConv<64, 64, kEpiRelu>(r, a, s);
             in = r->act[i + 1];
         } else {
-            a.aux = x;
+            a.aux32 = r->x32;
+            a.scale = r->res_scale;
             a.out_f32 = r->out;
             LaunchConv<64, 32, kEpiResid>(r, a, s);
         }
@@ -624,7 +668,7 @@ void Backward(const AiranCeRefiner *r, const bf16 *x, const float *target, const
               uint32_t W, float *loss, hipStream_t s)
 {
     const size_t n = static_cast<size_t>(S) * kSym * W * kCh;
-    hipLaunchKernelGGL(LossKernel, dim3(kLossBlocks), dim3(256), 0, s, r->out, target, slot_w,
+    hipLaunchKernelGGL(LossKernel, dim3(kLossBlocks), dim3(256), 0, s, r->out, target, slot_w, r->res_scale,
                        static_cast<size_t>(kSym) * W * kCh, r->dz[0], r->loss_part, n);
     hipLaunchKernelGGL(LossFinishKernel, dim3(1), dim3(64), 0, s, r->loss_part, kLossBlocks, n,
                        loss != nullptr ? loss : r->loss_dev);
@@ -676,6 +720,7 @@ InputArgs MakeInput(const AiranCeRefiner *r, const __half *hr, const __half *hi,
     a.h_re = hr;
     a.h_im = hi;
     a.x = x;
+    a.x32 = r->x32;
     a.used = g.used;
     a.nr = g.nr;
     a.nl = g.nl;
@@ -744,7 +789,10 @@ int32_t airan_ce_refiner_Create(const PuschMimoConfig *config, uint32_t max_slot
               alloc(&r->hls_im, static_cast<size_t>(max_slots) * g.nr * g.nl * g.ndmrs * g.pilot_pad * 2) &&
               alloc(&r->noise, static_cast<size_t>(max_slots) * g.nr * 2);
     for (int i = 1; i < kLayers && ok; ++i) ok = alloc(&r->act[i], px * kHid * 2);
+    ok = ok && alloc(&r->x32, px * kCh * 4) && alloc(&r->res_scale, static_cast<size_t>(max_slots) * 4);
     if (!ok) return fail(AIRAN_RESOURCE_TOO_SMALL);
+    hipLaunchKernelGGL(NoiseScaleKernel, dim3(airan::CeilDiv(max_slots, 64)), dim3(64), 0, s,
+                       static_cast<const __half *>(nullptr), g.nr, max_slots, r->res_scale);
     // Initial weights: given, or He-normal (std sqrt(2 / (9 cin))) with zero
     // biases from a host generator seeded by `seed`.
     std::vector<float> p(kParams, 0.0f);
@@ -806,7 +854,8 @@ void airan_ce_refiner_Destroy(AiranCeRefiner *r)
     void *bufs[] = {r->params, r->adam_m, r->adam_v, r->grad, r->wf, r->wb, r->x, r->target, r->out, r->dz[0],
                     r->dz[1], r->part, r->loss_part, r->loss_dev, r->gains, r->tau, r->snr, r->slot_w, r->dmrs_re,
                     r->dmrs_im,
-                    r->rx_re, r->rx_im, r->hls_re, r->hls_im, r->noise, r->ce_ws, r->est_re, r->est_im};
+                    r->rx_re, r->rx_im, r->hls_re, r->hls_im, r->noise, r->ce_ws, r->est_re, r->est_im,
+                    r->x32, r->res_scale};
     for (void *b : bufs) FreeDev(b);
     for (bf16 *a : r->act) FreeDev(a);
     delete r;
@@ -830,15 +879,24 @@ int32_t airan_ce_refiner_SetParams(AiranCeRefiner *r, const float *params)
 int32_t airan_ce_refiner_Enqueue(AiranCeRefiner *r, const void *h_ls_re, const void *h_ls_im, uint32_t slots,
                                  const uint8_t *keep, void *h_re, void *h_im, void *stream)
 {
+    return airan_ce_refiner_EnqueueWithNoise(r, h_ls_re, h_ls_im, nullptr, slots, keep, h_re, h_im, stream);
+}
+
+int32_t airan_ce_refiner_EnqueueWithNoise(AiranCeRefiner *r, const void *h_ls_re, const void *h_ls_im,
+                                          const void *noise, uint32_t slots, const uint8_t *keep, void *h_re,
+                                          void *h_im, void *stream)
+{
     if (r == nullptr || h_ls_re == nullptr || h_ls_im == nullptr || h_re == nullptr || h_im == nullptr ||
-        stream == nullptr || slots == 0 || slots > r->max_slots)
+        stream == nullptr || slots == 0 || slots > r->max_slots || (r->res_mode == 1 && noise == nullptr))
         return AIRAN_INVALID_ARGUMENT;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Geom &g = r->g;
+    hipLaunchKernelGGL(NoiseScaleKernel, dim3(airan::CeilDiv(slots, 64)), dim3(64), 0, s,
+                       r->res_mode == 1 ? static_cast<const __half *>(noise) : nullptr, g.nr, slots, r->res_scale);
     if (r->input_mode == 1) {
         hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s,
-                           static_cast<const __half *>(h_re), static_cast<const __half *>(h_im), r->x, g.used, g.pad,
-                           g.nr, g.nl);
+                           static_cast<const __half *>(h_re), static_cast<const __half *>(h_im), r->x, r->x32, g.used,
+                           g.pad, g.nr, g.nl);
     } else {
         const InputArgs ia =
             MakeInput(r, static_cast<const __half *>(h_ls_re), static_cast<const __half *>(h_ls_im), r->x);
@@ -866,6 +924,20 @@ int32_t airan_ce_refiner_SetInput(AiranCeRefiner *r, uint32_t mode, const struct
     r->input_mode = mode;
     r->filter = mode == 1 ? filter : nullptr;
     return AIRAN_OK;
+}
+
+int32_t airan_ce_refiner_SetResidualScale(AiranCeRefiner *r, uint32_t mode)
+{
+    if (r == nullptr || mode > 1) return AIRAN_INVALID_ARGUMENT;
+    r->res_mode = mode;
+    return AIRAN_OK;
+}
+
+int32_t airan_ce_refiner_SetSlotScale(AiranCeRefiner *r, const float *scale, uint32_t slots)
+{
+    if (r == nullptr || scale == nullptr || slots == 0 || slots > r->max_slots) return AIRAN_INVALID_ARGUMENT;
+    return hipMemcpy(r->res_scale, scale, slots * 4, hipMemcpyHostToDevice) == hipSuccess ? AIRAN_OK
+                                                                                          : AIRAN_LAUNCH_FAILED;
 }
 
 int32_t airan_ce_refiner_SetDilation(AiranCeRefiner *r, const uint32_t *dilation, uint32_t count)
@@ -899,11 +971,15 @@ int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *r, const float *gains, const
     // the batch counts alike instead of the noisiest slots dominating.
     double wsum = 0.0;
     for (uint32_t i = 0; i < slots; ++i) wsum += std::pow(10.0, 0.1 * snr_db[i]);
-    r->host_w.resize(slots);
-    for (uint32_t i = 0; i < slots; ++i)
+    r->host_w.resize(2 * static_cast<size_t>(slots));
+    for (uint32_t i = 0; i < slots; ++i) {
         r->host_w[i] = static_cast<float>(std::pow(10.0, 0.1 * snr_db[i]) * slots / wsum);
+        // Residual scale: the noise standard deviation of the slot.
+        r->host_w[slots + i] = r->res_mode == 1 ? static_cast<float>(std::pow(10.0, -0.05 * snr_db[i])) : 1.0f;
+    }
     if (hipMemcpyAsync(r->gains, gains, ng * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(r->slot_w, r->host_w.data(), slots * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(r->res_scale, r->host_w.data() + slots, slots * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(r->tau, tap_delay_s, static_cast<size_t>(slots) * taps * 4, hipMemcpyHostToDevice, s) !=
             hipSuccess ||
         hipMemcpyAsync(r->snr, snr_db, slots * 4, hipMemcpyHostToDevice, s) != hipSuccess)
@@ -937,7 +1013,7 @@ int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *r, const float *gains, const
                                   r->est_re, r->est_im, g.nl, slots, s);
         if (st != AIRAN_OK) return st;
         hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, r->est_re,
-                           r->est_im, r->x, g.used, g.pad, g.nr, g.nl);
+                           r->est_im, r->x, r->x32, g.used, g.pad, g.nr, g.nl);
     } else {
         const InputArgs ia = MakeInput(r, r->hls_re, r->hls_im, r->x);
         hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
@@ -991,6 +1067,8 @@ int32_t airan_ce_refiner_Gradients(AiranCeRefiner *r, const void *x, const void 
         return AIRAN_INVALID_ARGUMENT;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bf16 *xb = static_cast<const bf16 *>(x);
+    hipLaunchKernelGGL(WidenKernel, dim3(256), dim3(256), 0, s, xb, r->x32,
+                       static_cast<size_t>(slots) * kSym * width * kCh);
     Forward(r, xb, slots, width, s);
     Backward(r, xb, static_cast<const float *>(target), nullptr, slots, width, loss, s);
     if (hipMemcpyAsync(grad, r->grad, kParams * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)

@@ -455,6 +455,9 @@ PROTOTYPES = {
     "airan_ce_refiner_Enqueue": (i32, [vp, vp, vp, u32, vp, vp, vp, vp]),
     "airan_ce_refiner_SetInput": (i32, [vp, u32, vp]),
     "airan_ce_refiner_SetDilation": (i32, [vp, vp, u32]),
+    "airan_ce_refiner_EnqueueWithNoise": (i32, [vp, vp, vp, vp, u32, vp, vp, vp, vp]),
+    "airan_ce_refiner_SetResidualScale": (i32, [vp, u32]),
+    "airan_ce_refiner_SetSlotScale": (i32, [vp, vp, u32]),
     "airan_ce_refiner_GetDilation": (i32, [vp, vp, u32]),
     "airan_ce_refiner_SynthBatch": (i32, [vp, vp, vp, u32, vp, u32, C.c_float, C.c_uint64, vp]),
     "airan_ce_refiner_Batch": (i32, [vp, P(AiranCeRefinerBatchV1)]),

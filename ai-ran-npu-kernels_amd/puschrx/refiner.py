@@ -123,7 +123,8 @@ class Refiner:
     mode-1 weights start with a zero last layer, i.e. at the LMMSE estimate."""
 
     def __init__(self, cfg, max_slots: int, params: np.ndarray | None = None, seed: int = 1,
-                 input_mode: int = 0, estimator: ops.LmmseFilter | None = None, dilation=None):
+                 input_mode: int = 0, estimator: ops.LmmseFilter | None = None, dilation=None,
+                 residual_scale: int = 0):
         self.lib = abi.load_library()
         self.cfg = cfg
         self.g = ops.Geometry(cfg)
@@ -138,6 +139,9 @@ class Refiner:
         self.dilation = [1] * 6 if dilation is None else [int(d) for d in dilation]
         d = (C.c_uint32 * 6)(*self.dilation)
         ops._check("refiner SetDilation", self.lib.airan_ce_refiner_SetDilation(self.handle, d, 6))
+        self.residual_scale = int(residual_scale)
+        ops._check("refiner SetResidualScale", self.lib.airan_ce_refiner_SetResidualScale(self.handle,
+                                                                                          self.residual_scale))
         if input_mode:
             ops._check("refiner SetInput", self.lib.airan_ce_refiner_SetInput(
                 self.handle, input_mode, estimator.handle if estimator is not None else None))
@@ -164,7 +168,8 @@ class Refiner:
         dilations and `extra` (training record)."""
         path.parent.mkdir(parents=True, exist_ok=True)
         self.params().tofile(path)
-        meta = {"input_mode": self.input_mode, "dilation": self.dilation, **(extra or {})}
+        meta = {"input_mode": self.input_mode, "dilation": self.dilation, "residual_scale": self.residual_scale,
+                **(extra or {})}
         Path(str(path) + ".json").write_text(json.dumps(meta) + "\n")
 
     @staticmethod
@@ -173,6 +178,7 @@ class Refiner:
         m = json.loads(meta.read_text()) if meta.exists() else {}
         m.setdefault("input_mode", 0)
         m.setdefault("dilation", [1] * 6)
+        m.setdefault("residual_scale", 0)
         return m
 
     @classmethod
@@ -180,7 +186,7 @@ class Refiner:
         """A refiner with the weights, input mode and dilations of `path`."""
         meta = cls.load_meta(path)
         return cls(cfg, max_slots=max_slots, params=cls.load_params(path), input_mode=int(meta["input_mode"]),
-                   estimator=estimator, dilation=meta["dilation"])
+                   estimator=estimator, dilation=meta["dilation"], residual_scale=int(meta["residual_scale"]))
 
     @staticmethod
     def load_params(path: Path = WEIGHTS) -> np.ndarray:
@@ -234,6 +240,12 @@ class Refiner:
     def forward(self, slots: int) -> None:
         ops._check("refiner Forward", self.lib.airan_ce_refiner_Forward(self.handle, slots,
                                                                       C.c_void_p(ops.stream_ptr())))
+
+    def set_slot_scale(self, scale) -> None:
+        """Residual scale per slot for `gradients` (host float [slots])."""
+        sc = np.ascontiguousarray(scale, np.float32)
+        ops._check("refiner SetSlotScale", self.lib.airan_ce_refiner_SetSlotScale(
+            self.handle, sc.ctypes.data_as(C.c_void_p), sc.size))
 
     def gradients(self, x: torch.Tensor, target: torch.Tensor, slots: int, width: int):
         grad = torch.zeros(abi.REFINER_PARAMS, dtype=torch.float32, device="cuda")

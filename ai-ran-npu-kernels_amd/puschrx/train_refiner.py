@@ -18,7 +18,7 @@ follows a cosine to `--lr-final`.
 
 Every `--chunk` steps the network is evaluated on held-out batches (fixed
 seeds): the workload channel (300 ns, 100 Hz, Rx correlation 0.5) at 34, 38
-and 42 dB and mixed draws of the training family at 10, 20 and 30 dB, each
+and 42 dB and mixed draws of the training family at 20, 30, 38 and 44 dB, each
 against the chain's LMMSE filter (300 ns, 100 Hz prior, rank 384) on the same
 slots.  The weights with the lowest mean held-out NMSE are kept.
 """
@@ -35,7 +35,7 @@ import numpy as np
 import torch
 
 HELD_OUT_WORKLOAD = (34.0, 38.0, 42.0)
-HELD_OUT_MIXED = (10.0, 20.0, 30.0)
+HELD_OUT_MIXED = (20.0, 30.0, 38.0, 44.0)
 
 
 def held_out(ref, spec, filt, slots: int = 4) -> dict:
@@ -61,7 +61,7 @@ def main(argv=None) -> int:
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-final", type=float, default=1e-5)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--chunk", type=int, default=200, help="steps between held-out evaluations")
     ap.add_argument("--snr-min", type=float, default=0.0)
     ap.add_argument("--snr-max", type=float, default=46.0)
@@ -71,9 +71,11 @@ def main(argv=None) -> int:
     ap.add_argument("--dilation", type=int, nargs=6, default=(1, 2, 4, 8, 16, 1))
     ap.add_argument("--bias-init", type=float, default=0.0,
                     help="initial bias of the hidden layers (> 0 starts the ReLUs in their linear range)")
+    ap.add_argument("--residual-scale", type=int, choices=(0, 1), default=1,
+                    help="1: the network's residual in units of the slot's noise standard deviation")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--init", type=Path, default=None, help="start from these weights")
-    ap.add_argument("--input", choices=("lmmse", "ls"), default="ls",
+    ap.add_argument("--input", choices=("lmmse", "ls"), default="lmmse",
                     help="network input: the LS interpolation, or a residual correction of the LMMSE estimate")
     ap.add_argument("--out", type=Path, default=None)
     args = ap.parse_args(argv)
@@ -94,7 +96,7 @@ def main(argv=None) -> int:
         filt = ops.LmmseFilter(cfg, tau_s=300e-9, fd_hz=100.0, scs_hz=30e3, max_rank=384)
         mode = 1 if args.input == "lmmse" else 0
         ref = Refiner(cfg, max_slots=max(args.slots, 4), params=init, seed=args.seed, input_mode=mode,
-                      estimator=filt if mode else None, dilation=args.dilation)
+                      estimator=filt if mode else None, dilation=args.dilation, residual_scale=args.residual_scale)
         if init is None and args.bias_init != 0.0:
             from .refiner import bias_slices
             p = ref.params()
@@ -127,8 +129,9 @@ def main(argv=None) -> int:
             sc = score(h)
             print(f"[train] {done} steps ({time.perf_counter() - t0:.0f} s): held-out mean NMSE {sc:.2f} dB "
                   f"(best {best_score:.2f}); workload 38 dB refiner {h['workload_38dB']['refiner']:.2f} / "
-                  f"LMMSE {h['workload_38dB']['lmmse']:.2f}, mixed 20 dB refiner {h['mixed_20dB']['refiner']:.2f} / "
-                  f"LMMSE {h['mixed_20dB']['lmmse']:.2f}", flush=True)
+                  f"LMMSE {h['workload_38dB']['lmmse']:.2f}, mixed 38 dB refiner {h['mixed_38dB']['refiner']:.2f} / "
+                  f"LMMSE {h['mixed_38dB']['lmmse']:.2f}, mixed 44 dB refiner {h['mixed_44dB']['refiner']:.2f} / "
+                  f"LMMSE {h['mixed_44dB']['lmmse']:.2f}", flush=True)
             if sc < best_score:
                 best, best_score, best_h, best_step = ref.params(), sc, h, done
         torch.cuda.synchronize()

@@ -45,6 +45,35 @@ def device_channel(gen: torch.Generator, cfg, spec: ChannelSpec, slots: int):
     return gains.to(torch.complex64).contiguous(), delays.to(torch.float32).expand(slots, taps).contiguous()
 
 
+def device_channel_family(gen: torch.Generator, cfg, slots: int, delay_spread_s, doppler_hz, rx_corr,
+                          taps: int = 16, sinusoids: int = 16, scs_hz: float = 30e3):
+    """device_channel with per-slot parameters (arrays of `slots` values):
+    gains complex64 [slots, NR, L, taps, 14], delays float32 [slots, taps]."""
+    g = ops.Geometry(cfg)
+    dev = gen.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    ds = torch.as_tensor(np.asarray(delay_spread_s, np.float64), **f64)
+    fdmax = torch.as_tensor(np.asarray(doppler_hz, np.float64), **f64)
+    rho = torch.as_tensor(np.asarray(rx_corr, np.float64), **f64)
+    delays = torch.arange(taps, **f64)[None, :] * (4.0 * ds[:, None] / taps)
+    pdp = torch.exp(-delays / ds[:, None])
+    pdp = pdp / pdp.sum(dim=1, keepdim=True)
+    t = torch.arange(14, **f64) * ((1.0 + 9.0 / 128.0) / scs_hz)
+    two_pi = 2.0 * np.pi
+    alpha = torch.rand((slots, g.nr, g.nl, taps, sinusoids), generator=gen, **f64) * two_pi
+    phi = torch.rand((slots, g.nr, g.nl, taps, sinusoids), generator=gen, **f64) * two_pi
+    fd = fdmax[:, None, None, None, None] * torch.cos(alpha)
+    ph = two_pi * fd[..., None] * t + phi[..., None]
+    gains = torch.polar(torch.ones_like(ph), ph).sum(dim=4) / np.sqrt(sinusoids)
+    gains = gains * torch.sqrt(pdp)[:, None, None, :, None]
+    idx = torch.arange(g.nr, **f64)
+    R = rho[:, None, None] ** (idx[:, None] - idx[None, :]).abs()
+    w, v = torch.linalg.eigh(R)
+    rsq = (v * torch.sqrt(torch.clamp_min(w, 0.0))[:, None, :]) @ v.transpose(1, 2)
+    gains = torch.einsum("sij,sjltk->siltk", rsq.to(gains.dtype), gains)
+    return gains.to(torch.complex64).contiguous(), delays.to(torch.float32).contiguous()
+
+
 class GpuTransmitter:
     """Encoder + channel for one grant shape and up to max_tbs TBs per call."""
 

@@ -8,7 +8,9 @@
  * c = 2 (rx * 4 + layer) + {0: Re, 1: Im}; six 3x3 convolutions (stride 1,
  * zero padding, optionally dilated along the subcarriers:
  * airan_ce_refiner_SetDilation) 32 -> 64 -> 64 -> 64 -> 64 -> 64 -> 32 with ReLU after the
- * first five, and a residual: H = X + conv6(...).  Weights and activations
+ * first five, and a residual: H = X + s conv6(...) (s = 1, or the slot's
+ * noise standard deviation: airan_ce_refiner_SetResidualScale), X added
+ * in fp32.  Weights and activations
  * bf16, accumulation fp32 (MFMA), master weights fp32 trained with Adam on
  * the device.  Parameters, in order per layer: weights [9][cout][cin]
  * (tap = 3 kh + kw over symbol offset kh - 1 and subcarrier offset kw - 1),
@@ -56,6 +58,25 @@ int32_t airan_ce_refiner_SetParams(AiranCeRefiner *refiner, const float *params)
  * slot whose flag keep[slot] is non-zero (keep may be NULL: all slots). */
 int32_t airan_ce_refiner_Enqueue(AiranCeRefiner *refiner, const void *h_ls_re, const void *h_ls_im,
                                  uint32_t slots, const uint8_t *keep, void *h_re, void *h_im, void *stream);
+
+/* Enqueue with each slot's noise variance (fp16 [slots][NR], the CE's
+ * estimate): needed when the residual is scaled by the noise
+ * (airan_ce_refiner_SetResidualScale mode 1); Enqueue is this call with
+ * noise NULL. */
+int32_t airan_ce_refiner_EnqueueWithNoise(AiranCeRefiner *refiner, const void *h_ls_re, const void *h_ls_im,
+                                          const void *noise, uint32_t slots, const uint8_t *keep, void *h_re,
+                                          void *h_im, void *stream);
+
+/* Residual scale: the network's output is X + s * conv6(...), s per slot.
+ * mode 0 (the default): s = 1.  mode 1: s = the slot's noise standard
+ * deviation (training: 10^(-snr/20) of the synthesised slot; inference: the
+ * square root of the mean noise variance given to EnqueueWithNoise), so the
+ * last layer works in units of the noise and an optimiser step moves the
+ * output by an amount proportional to the error it corrects.  Part of the
+ * architecture: weights are trained for one mode.  SetSlotScale sets the
+ * scales explicitly (for airan_ce_refiner_Gradients: host float [slots]). */
+int32_t airan_ce_refiner_SetResidualScale(AiranCeRefiner *refiner, uint32_t mode);
+int32_t airan_ce_refiner_SetSlotScale(AiranCeRefiner *refiner, const float *scale, uint32_t slots);
 
 /* Network input.  mode 0 (the default): the LS interpolation of h_ls.
  * mode 1: a residual correction of the LMMSE estimate -- Enqueue reads the

@@ -17,8 +17,9 @@
  * of layer i sit at symbol offsets -1, 0, +1 and subcarrier offsets
  * -d_i, 0, +d_i (d_i its dilation, 1 by default): off(tap) = (kh - 1,
  * (kw - 1) d_i);
- * out = X + z_5; loss = mean((out - T)^2) over all N values.
- * Backward: dz_5 = 2 (out - T) / N; gW_i[tap][co][ci] = sum_p dz_i[p][co]
+ * out = X + s_slot z_5 (s_slot the slot's residual scale, 1 by default);
+ * loss = mean((out - T)^2) over all N values.
+ * Backward: dz_5 = 2 s_slot (out - T) / N; gW_i[tap][co][ci] = sum_p dz_i[p][co]
  * a_i[p + off(tap)][ci]; gb_i = sum_p dz_i; dz_{i-1} = convT(dz_i, W_i)
  * masked by a_i > 0.  Tensors are NHWC [S][14][W][C].
  */
@@ -144,6 +145,12 @@ int or_refiner_grad(const float *params, const float *x, const float *target, in
 int or_refiner_grad_dil(const float *params, const float *x, const float *target, int S, int W, int device_rounding,
                         const int *dil, double *grad, double *loss, float *out)
 {
+    return or_refiner_grad_full(params, x, target, S, W, device_rounding, dil, NULL, grad, loss, out);
+}
+
+int or_refiner_grad_full(const float *params, const float *x, const float *target, int S, int W, int device_rounding,
+                         const int *dil, const float *slot_scale, double *grad, double *loss, float *out)
+{
     const size_t P = (size_t)S * 14 * W;
     const size_t np = or_refiner_param_offset(OR_RF_LAYERS);
     double *wq = malloc(np * sizeof(double));
@@ -176,12 +183,14 @@ int or_refiner_grad_dil(const float *params, const float *x, const float *target
     }
     const double n = (double)P * 32;
     double l = 0.0;
+    const size_t per_slot = (size_t)14 * W * 32;
     for (size_t k = 0; k < P * 32; ++k) {
-        const double o = act[0][k] + z5[k];
+        const double sc = slot_scale ? (double)slot_scale[k / per_slot] : 1.0;
+        const double o = act[0][k] + sc * z5[k];
         if (out) out[k] = (float)o;
         const double e = o - target[k];
         l += e * e;
-        const double g = 2.0 * e / n;
+        const double g = 2.0 * sc * e / n;
         dz[k] = device_rounding ? rbf16(g) : g;
     }
     *loss = l / n;

@@ -201,6 +201,10 @@ int or_refiner_grad(const float *params, const float *x, const float *target, in
 /* The same with a subcarrier dilation per layer (dil [6], each >= 1). */
 int or_refiner_grad_dil(const float *params, const float *x, const float *target, int S, int W, int device_rounding,
                         const int *dil, double *grad, double *loss, float *out);
+/* ... and a residual scale per slot (slot_scale [S] or NULL for 1):
+ * out = x + slot_scale[s] z_5. */
+int or_refiner_grad_full(const float *params, const float *x, const float *target, int S, int W, int device_rounding,
+                         const int *dil, const float *slot_scale, double *grad, double *loss, float *out);
 double or_bf16_round(double v);
 
 #ifdef __cplusplus

@@ -76,6 +76,9 @@ def load_oracle():
     lib.or_refiner_grad_dil.restype = C.c_int
     lib.or_refiner_grad_dil.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.or_refiner_grad_full.restype = C.c_int
+    lib.or_refiner_grad_full.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.or_bf16_round.restype = C.c_double
     lib.or_bf16_round.argtypes = [C.c_double]
     _ORACLE = lib

@@ -20,11 +20,13 @@ def oracle():
     return load_oracle()
 
 
-GRAD_CASES = [(1, 40, None), (2, 70, None), (1, 70, [1, 2, 4, 8, 16, 1]), (2, 150, [2, 1, 16, 3, 5, 16])]
+GRAD_CASES = [(1, 40, None, None), (2, 70, None, None), (1, 70, [1, 2, 4, 8, 16, 1], None),
+              (2, 150, [2, 1, 16, 3, 5, 16], None), (2, 90, [1, 2, 4, 8, 16, 1], [0.3, 0.02])]
 
 
-@pytest.mark.parametrize("S,W,dil", GRAD_CASES, ids=["w40", "w70", "w70_dil1-16", "w150_dil_mixed"])
-def test_refiner_gradients_match_oracle(oracle, gpu_stream, S, W, dil):
+@pytest.mark.parametrize("S,W,dil,scale", GRAD_CASES,
+                         ids=["w40", "w70", "w70_dil1-16", "w150_dil_mixed", "w90_dil_scaled"])
+def test_refiner_gradients_match_oracle(oracle, gpu_stream, S, W, dil, scale):
     rng = np.random.default_rng(11 + W)
     cfg = P.FULL_CARRIER_4RX.config()
     params = refiner_params(oracle, rng)
@@ -32,7 +34,10 @@ def test_refiner_gradients_match_oracle(oracle, gpu_stream, S, W, dil):
     xf = x.float().numpy()
     t = (xf + rng.normal(0, 0.3, xf.shape)).astype(np.float32)
     ref = Refiner(cfg, max_slots=S, params=params, dilation=dil)
-    print(f"refiner gradients S={S} W={W} dilation {ref.dilation}", flush=True)
+    sc = None if scale is None else np.asarray(scale, np.float32)
+    if sc is not None:
+        ref.set_slot_scale(sc)
+    print(f"refiner gradients S={S} W={W} dilation {ref.dilation} residual scale {scale}", flush=True)
     grad, loss = ref.gradients(x.cuda(), torch.from_numpy(t).cuda(), S, W)
     n = S * 14 * W * 32
     out = ref.fetch("out", torch.float32, n)
@@ -42,8 +47,8 @@ def test_refiner_gradients_match_oracle(oracle, gpu_stream, S, W, dil):
     want_l = np.zeros(1)
     want_o = np.zeros(n, np.float32)
     d = np.asarray(ref.dilation, np.int32)
-    assert oracle.or_refiner_grad_dil(ptr(params), ptr(np.ascontiguousarray(xf)), ptr(t), S, W, 1, ptr(d),
-                                      ptr(want_g), ptr(want_l), ptr(want_o)) == 0
+    assert oracle.or_refiner_grad_full(ptr(params), ptr(np.ascontiguousarray(xf)), ptr(t), S, W, 1, ptr(d),
+                                       None if sc is None else ptr(sc), ptr(want_g), ptr(want_l), ptr(want_o)) == 0
     # fp32 accumulation on the device against double in the oracle flips an
     # occasional bf16 rounding of an activation (one bf16 ulp, 2^-8), which
     # the later layers carry: agreement is to ~1e-4 in norm, not bit-exact.
@@ -86,7 +91,8 @@ def test_refiner_training_beats_ls_interpolation(gpu_stream):
 def test_refiner_lmmse_input_mode(gpu_stream):
     """Input mode 1: the batch input is the LMMSE estimate of the synthesised
     slots (the chain's filter: 300 ns, 100 Hz, rank 384) rounded to bf16,
-    and a fresh network (zero last layer) returns it unchanged."""
+    and a fresh network (zero last layer) returns the estimate itself (the
+    residual is added to it in fp32, not to its bf16 copy)."""
     from puschrx import ops
 
     cfg = P.FULL_CARRIER_4RX.config()
@@ -98,9 +104,8 @@ def test_refiner_lmmse_input_mode(gpu_stream):
     res = ref.evaluate(4, lmmse=filt)
     print(res, flush=True)
     n = 4 * 14 * ref.g.used * 32
-    x = ref.fetch("x", torch.bfloat16, n).float()
     out = ref.fetch("out", torch.float32, n)
-    assert torch.equal(out, x)
+    assert torch.equal(out, ref.lmmse_grid(4, filt))
     # bf16 rounding of the estimate costs well under a dB at 34 dB.
     assert res["lmmse"] - 0.05 <= res["input"] <= res["lmmse"] + 1.0
     assert res["input"] < -38.0

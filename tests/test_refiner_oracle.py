@@ -165,3 +165,52 @@ def test_uniform_dilation_splits_into_phases(oracle):
         want_grad += g_r * w / W
     assert loss == pytest.approx(want_loss, rel=1e-10)
     np.testing.assert_allclose(grad, want_grad, rtol=1e-9, atol=1e-12)
+
+
+def test_residual_scale_gradients_match_finite_differences(oracle):
+    """out = x + s_slot z_5 with a scale per slot: analytic gradients against
+    central differences, and the scaled network equals the unscaled one with
+    the last layer's weights and bias multiplied by s (one slot)."""
+    rng = np.random.default_rng(10)
+    S, W = 2, 6
+    params = refiner_params(oracle, rng)
+    x = rng.normal(0, 1, (S, 14, W, 32)).astype(np.float32)
+    t = (x + rng.normal(0, 0.3, x.shape)).astype(np.float32)
+    sc = np.array([0.25, 0.05], np.float32)
+    dil = np.array([1, 2, 1, 1, 3, 1], np.int32)
+
+    def run(p, scale):
+        grad = np.zeros(p.size, np.float64)
+        loss = np.zeros(1, np.float64)
+        out = np.zeros(x.size, np.float32)
+        assert oracle.or_refiner_grad_full(ptr(p), ptr(x), ptr(t), S, W, 0, ptr(dil),
+                                           None if scale is None else ptr(scale), ptr(grad), ptr(loss), ptr(out)) == 0
+        return float(loss[0]), grad, out.reshape(x.shape)
+
+    _, grad, out = run(params, sc)
+    picks = []
+    for i in range(6):
+        o = oracle.or_refiner_param_offset(i)
+        nw = 9 * COUT[i] * CIN[i]
+        picks += list(rng.integers(o, o + nw, 5)) + list(rng.integers(o + nw, o + nw + COUT[i], 2))
+    ok = 0
+    for k in picks:
+        eps = 1e-4
+        pp, pm = params.copy(), params.copy()
+        pp[k] = np.float32(params[k] + eps)
+        pm[k] = np.float32(params[k] - eps)
+        fd = (run(pp, sc)[0] - run(pm, sc)[0]) / (float(pp[k]) - float(pm[k]))
+        ok += abs(fd - grad[k]) <= 1e-5 * max(abs(fd), abs(grad[k])) + 1e-10
+    assert ok >= len(picks) - 3, (ok, len(picks))
+    # Slot 0 alone: scaling the last layer by s is the same network.
+    o5 = oracle.or_refiner_param_offset(5)
+    p2 = params.copy()
+    p2[o5:] = (p2[o5:].astype(np.float64) * float(sc[0])).astype(np.float32)
+    x0 = np.ascontiguousarray(x[:1])
+    t0 = np.ascontiguousarray(t[:1])
+    grad2 = np.zeros(params.size, np.float64)
+    loss2 = np.zeros(1, np.float64)
+    out2 = np.zeros(x0.size, np.float32)
+    assert oracle.or_refiner_grad_full(ptr(p2), ptr(x0), ptr(t0), 1, W, 0, ptr(dil), None, ptr(grad2), ptr(loss2),
+                                       ptr(out2)) == 0
+    np.testing.assert_allclose(out[0], out2.reshape(x0.shape)[0], rtol=1e-5, atol=1e-6)
