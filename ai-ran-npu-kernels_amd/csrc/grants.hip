@@ -6,8 +6,8 @@
 // single-grant chain: the same device functions, only the indexing differs.
 // The CE filter of each distinct grant shape is built once per process and
 // reused (keyed by the shape and the CE prior); grants of one shape are
-// laid out next to each other in the channel buffers so that one batched CE
-// application serves them.
+// laid out next to each other in the channel buffers so that one CE job
+// serves them, and the jobs of all shapes run in one launch per CE step.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -194,13 +194,11 @@ struct AiranGrantBatch {
         uint64_t hls_off = 0, noise_off = 0, h_off = 0;
     };
     std::vector<CeGroup> groups;
-    // The CE groups are independent: they run as kCeBranches parallel
-    // branches (side streams forked from and joined to the caller's stream,
-    // parallel branches of the captured graph), each with its own workspace.
-    static constexpr int kCeBranches = 4;
-    hipStream_t side[kCeBranches] = {};
-    hipEvent_t fork = nullptr, join[kCeBranches] = {};
-    uint8_t *ce_ws_branch[kCeBranches] = {};
+    // The CE groups run together, one launch per CE step over all of them
+    // (airan::LaunchCeGrouped), each group with its own workspace.
+    airan::CeGrouped *ce_grouped = nullptr;
+    uint8_t *ce_ws_all = nullptr;
+    std::vector<size_t> ce_ws_off;
     // Buffers.
     __half *rx_re = nullptr, *rx_im = nullptr, *ref_re = nullptr, *ref_im = nullptr;
     __half *hls_re = nullptr, *hls_im = nullptr, *noise_ls = nullptr, *noise_ce = nullptr;
@@ -256,22 +254,7 @@ int32_t EnqueueStages(AiranGrantBatch *b, const void *iq, hipStream_t s)
                                       b->n_noise, b->max_ref_pad, b->max_pilot_pad, b->max_dmrs, b->rx_re, b->rx_im,
                                       b->ref_re, b->ref_im, b->hls_re, b->hls_im, b->noise_ls, s)) != AIRAN_OK)
         return st;
-    constexpr int K = AiranGrantBatch::kCeBranches;
-    if (hipEventRecord(b->fork, s) != hipSuccess) return AIRAN_LAUNCH_FAILED;
-    for (int k = 0; k < K; ++k)
-        if (hipStreamWaitEvent(b->side[k], b->fork, 0) != hipSuccess) return AIRAN_LAUNCH_FAILED;
-    for (size_t j = 0; j < b->groups.size(); ++j) {
-        const auto &gp = b->groups[j];
-        const int k = static_cast<int>(j % K);
-        if ((st = airan::LaunchCeApply(gp.filter, b->hls_re + gp.hls_off, b->hls_im + gp.hls_off,
-                                       b->noise_ls + gp.noise_off, b->noise_ce + gp.noise_off, b->ce_ws_branch[k],
-                                       b->ce_ws_bytes, b->h_re + gp.h_off, b->h_im + gp.h_off, gp.nl, gp.count,
-                                       b->side[k])) != AIRAN_OK)
-            return st;
-    }
-    for (int k = 0; k < K; ++k)
-        if (hipEventRecord(b->join[k], b->side[k]) != hipSuccess || hipStreamWaitEvent(s, b->join[k], 0) != hipSuccess)
-            return AIRAN_LAUNCH_FAILED;
+    if ((st = airan::LaunchCeGrouped(b->ce_grouped, s)) != AIRAN_OK) return st;
     hipLaunchKernelGGL(RaggedDetectKernel, dim3(b->n_tiles), dim3(b->tile), 0, s, b->rx_re, b->rx_im, b->h_re,
                        b->h_im, b->noise_ce, b->gold, b->cw, b->grants, b->tiles);
     if ((st = airan::LaunchStatus()) != AIRAN_OK) return st;
@@ -418,7 +401,8 @@ int32_t airan_pusch_grants_Create(const PuschMimoConfig *configs, const AiranTbP
     size_t ce_ws = 0;
     for (uint32_t j = 0, k = 0; k < b->groups.size(); ++k) {
         const Geom &g = geom[order[j]];
-        ce_ws = std::max(ce_ws, airan::CeWorkspaceBytes(g, b->groups[k].count));
+        b->ce_ws_off.push_back(ce_ws);
+        ce_ws += (airan::CeWorkspaceBytes(g, b->groups[k].count) + 255) / 256 * 256;
         j += b->groups[k].count;
     }
     b->ce_ws_bytes = ce_ws;
@@ -500,12 +484,20 @@ int32_t airan_pusch_grants_Create(const PuschMimoConfig *configs, const AiranTbP
         !DevAlloc(&b->cb_ok, b->num_cb, &by) || !DevAlloc(&b->tb_ok, n, &by) ||
         !DevAlloc(&b->tb_part, b->num_cb, &by))
         return fail(AIRAN_RESOURCE_TOO_SMALL);
-    for (int k = 0; k < AiranGrantBatch::kCeBranches; ++k)
-        if (!DevAlloc(&b->ce_ws_branch[k], b->ce_ws_bytes, &by) ||
-            hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) != hipSuccess)
-            return fail(AIRAN_RESOURCE_TOO_SMALL);
-    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return fail(AIRAN_LAUNCH_FAILED);
+    if (!DevAlloc(&b->ce_ws_all, b->ce_ws_bytes, &by)) return fail(AIRAN_RESOURCE_TOO_SMALL);
+    {
+        std::vector<airan::CeGroupJob> jobs;
+        for (size_t k = 0; k < b->groups.size(); ++k) {
+            const auto &gp = b->groups[k];
+            const size_t end = k + 1 < b->groups.size() ? b->ce_ws_off[k + 1] : b->ce_ws_bytes;
+            jobs.push_back(airan::CeGroupJob{gp.filter, b->hls_re + gp.hls_off, b->hls_im + gp.hls_off,
+                                             b->noise_ls + gp.noise_off, b->noise_ce + gp.noise_off,
+                                             b->h_re + gp.h_off, b->h_im + gp.h_off, b->ce_ws_all + b->ce_ws_off[k],
+                                             end - b->ce_ws_off[k], gp.nl, gp.count});
+        }
+        const int32_t cst = airan::CreateCeGrouped(jobs.data(), static_cast<uint32_t>(jobs.size()), &b->ce_grouped);
+        if (cst != AIRAN_OK) return fail(cst);
+    }
     int32_t st = BuildTiles(b, options->detect_tile);
     if (st != AIRAN_OK) return fail(st);
     // Gold bits of every grant's data scrambling sequence.
@@ -550,13 +542,8 @@ void airan_pusch_grants_Destroy(AiranGrantBatch *b)
     }
     for (auto &f : b->ffts)
         if (f.streams) (void)hipFree(f.streams);
-    for (int k = 0; k < AiranGrantBatch::kCeBranches; ++k) {
-        if (b->side[k]) (void)hipStreamSynchronize(b->side[k]);
-        if (b->side[k]) (void)hipStreamDestroy(b->side[k]);
-        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
-        if (b->ce_ws_branch[k]) (void)hipFree(b->ce_ws_branch[k]);
-    }
-    if (b->fork) (void)hipEventDestroy(b->fork);
+    airan::DestroyCeGrouped(b->ce_grouped);
+    if (b->ce_ws_all) (void)hipFree(b->ce_ws_all);
     void *bufs[] = {b->grants, b->gen_items, b->ls_items, b->noise_items, b->tiles, b->cbs, b->reg_list, b->ws_list,
                     b->tb_first, b->tb_count, b->rx_re, b->rx_im, b->ref_re, b->ref_im, b->hls_re, b->hls_im,
                     b->noise_ls, b->noise_ce, b->h_re, b->h_im, b->gold, b->cw, b->lam, b->ldpc_ws,

@@ -46,6 +46,22 @@ size_t CeWorkspaceBytes(const Geom &g, uint32_t slots);
 int32_t LaunchCeApply(const LmmseFilter *f, const __half *h_re, const __half *h_im, const __half *noise_in,
                       __half *noise_out, void *ws, size_t ws_bytes, __half *out_re, __half *out_im,
                       uint32_t out_layers, uint32_t slots, hipStream_t s);
+
+// Several CE applications launched together, one launch per CE step (the
+// grant shapes of a ragged batch): each job computes exactly what
+// LaunchCeApply computes for it.  Jobs must not share a workspace.
+struct CeGroupJob {
+    const LmmseFilter *filter;
+    const __half *hr, *hi, *noise_in;
+    __half *noise_out, *out_re, *out_im;
+    void *ws;
+    size_t ws_bytes;
+    uint32_t out_layers, slots;
+};
+struct CeGrouped;
+int32_t CreateCeGrouped(const CeGroupJob *jobs, uint32_t n, CeGrouped **out);
+int32_t LaunchCeGrouped(const CeGrouped *g, hipStream_t s);
+void DestroyCeGrouped(CeGrouped *g);
 // The filter's geometry must match `g` (same grant shape).
 bool CeFilterMatches(const LmmseFilter *f, const Geom &g);
 

@@ -310,12 +310,12 @@ __device__ inline void ProjRange(uint32_t n, uint32_t part, uint32_t *p0, uint32
 // zb[q] = z[slot, col][p] in 3-part fragment planes (rows n = slot * NR*D +
 // col, zero for n >= slots * NR*D and p >= n_q), one thread per row and 8
 // observations.  Grid (npad * AlignUp(max n, 16) / 8 / 256, models).
-__global__ void __launch_bounds__(256) CeZSplitKernel(const __half *__restrict__ hr, const __half *__restrict__ hi,
-                                                      CeWs w, uint32_t slots, CeApply c)
+__device__ __forceinline__ void CeZSplitKernelBody(const __half *__restrict__ hr, const __half *__restrict__ hi,
+                                                      const CeWs &w, uint32_t slots, const CeApply &c, uint32_t bx, uint32_t by)
 {
-    const uint32_t q = blockIdx.y;
+    const uint32_t q = by;
     const CeModelDev &md = c.m[q];
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t e = bx * blockDim.x + threadIdx.x;
     const uint32_t n = e % w.npad, p0 = (e / w.npad) * 8u;
     const uint32_t pk = airan::AlignUp(md.n, 16u);
     if (p0 >= pk) return;
@@ -357,6 +357,11 @@ __global__ void __launch_bounds__(256) CeZSplitKernel(const __half *__restrict__
     }
     airan::StoreFrag8<3>(w.zb + q * w.zb_stride, n, p0, pk / 16u, v);
 }
+__global__ void __launch_bounds__(256) CeZSplitKernel(const __half *__restrict__ hr, const __half *__restrict__ hi,
+                                                      CeWs w, uint32_t slots, CeApply c)
+{
+    CeZSplitKernelBody(hr, hi, w, slots, c, blockIdx.x, blockIdx.y);
+}
 
 // tp[part][slot, q][col][k] = sum_{p in range(part)} z[slot, col][p] conj(U[p][k])
 // on split bf16 MFMA with 3 parts (fp32-accurate products, which the
@@ -366,10 +371,10 @@ __global__ void __launch_bounds__(256) CeZSplitKernel(const __half *__restrict__
 // 8 * per_xcd workgroups; each XCD takes a contiguous range of tiles ordered
 // (model, split) major, so the operands of one (model, split) -- about 2 MB
 // at 273 PRB and 64 slots -- are read through one L2.
-__global__ void __launch_bounds__(256, 2) CeProjectMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t k_tiles,
-                                                              uint32_t per_xcd)
+__device__ __forceinline__ void CeProjectMfmaKernelBody(const CeWs &w, uint32_t slots, const CeApply &c, uint32_t k_tiles,
+                                                              uint32_t per_xcd, uint32_t bx, uint32_t by)
 {
-    const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    const uint32_t tile = (bx & 7u) * per_xcd + (bx >> 3);
     const uint32_t per_split = (w.npad / 64u) * k_tiles;
     if (tile >= c.nm * kProjSplits * per_split) return;
     const uint32_t qs = tile / per_split, rest = tile - qs * per_split;
@@ -420,18 +425,23 @@ __global__ void __launch_bounds__(256, 2) CeProjectMfmaKernel(CeWs w, uint32_t s
         }
     }
 }
+__global__ void __launch_bounds__(256, 2) CeProjectMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t k_tiles,
+                                                              uint32_t per_xcd)
+{
+    CeProjectMfmaKernelBody(w, slots, c, k_tiles, per_xcd, blockIdx.x, blockIdx.y);
+}
 
 // Column energies: |z|^2 over the observations and |u|^2 over the kept
 // eigenvectors, fp64 with a fixed reduction tree.  The projection's split-K
 // partials are summed here first, t = ((tp[0] + tp[1]) + tp[2]) + tp[3] ...
 // in split order, and t is stored for the kernels after.  Grid
 // slots * models * NR*D.
-__global__ void __launch_bounds__(256) CeEnergyKernel(const __half *__restrict__ hr, const __half *__restrict__ hi,
-                                                      CeWs w, CeApply c, uint32_t slots)
+__device__ __forceinline__ void CeEnergyKernelBody(const __half *__restrict__ hr, const __half *__restrict__ hi,
+                                                      const CeWs &w, const CeApply &c, uint32_t slots, uint32_t bx, uint32_t by)
 {
     __shared__ double red[2][4];
     const uint32_t ncol = c.nr * c.ndmrs;
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = bx;
     const uint32_t col = b % ncol, sq = b / ncol;
     const uint32_t q = sq % c.nm, slot = sq / c.nm;
     const CeModelDev &md = c.m[q];
@@ -467,6 +477,11 @@ __global__ void __launch_bounds__(256) CeEnergyKernel(const __half *__restrict__
         e[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     }
 }
+__global__ void __launch_bounds__(256) CeEnergyKernel(const __half *__restrict__ hr, const __half *__restrict__ hi,
+                                                      CeWs w, CeApply c, uint32_t slots)
+{
+    CeEnergyKernelBody(hr, hi, w, c, slots, blockIdx.x, blockIdx.y);
+}
 
 // Noise and per-model statistics of one slot (thread 0, fp64, fixed order),
 // then inv[slot, q][k] = 1 / (lam_k + n_q) by all threads.
@@ -478,15 +493,15 @@ __global__ void __launch_bounds__(256) CeEnergyKernel(const __half *__restrict__
 // kMinNoiseDims dimensions.  P_q = mean |z|^2 - sigma^2 and the filter's
 // relative noise n_q = sigma^2 / P_q -> stats[4 slot + q];
 // noise_out[slot][rx] = fp16(sigma^2_rx) when non-null.
-__global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ noise_in,
-                                                     __half *__restrict__ noise_out, CeWs w, CeApply c)
+__device__ __forceinline__ void CeStatsKernelBody(const __half *__restrict__ noise_in,
+                                                     __half *__restrict__ noise_out, const CeWs &w, const CeApply &c, uint32_t bx, uint32_t by)
 {
     __shared__ float nq[kMaxModels];
     // The slot's energies staged in LDS by all threads, so thread 0's serial
     // sums read LDS instead of waiting on one global load at a time.
     constexpr uint32_t kStage = 2 * kMaxModels * 32;
     __shared__ double stage[kStage];
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = bx;
     const uint32_t ncol = c.nr * c.ndmrs;
     const uint32_t n_e = 2 * c.nm * ncol;
     const double *eg = w.energy + 2 * static_cast<size_t>(slot) * c.nm * ncol;
@@ -531,6 +546,11 @@ __global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ 
         for (uint32_t k = threadIdx.x; k < c.m[q].rank; k += blockDim.x)
             w.inv[(static_cast<size_t>(slot) * c.nm + q) * kMaxRank + k] = 1.0f / (c.m[q].lam[k] + nq[q]);
 }
+__global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ noise_in,
+                                                     __half *__restrict__ noise_out, CeWs w, CeApply c)
+{
+    CeStatsKernelBody(noise_in, noise_out, w, c, blockIdx.x, blockIdx.y);
+}
 
 // hhat_var[slot, q][sc] = sum_k |A[k][sc]|^2 / (lam_k + n_q) (ref.py:109-111)
 // on fp32 MFMA (v_mfma_f32_32x32x2f32, exact fp32 products): a workgroup per
@@ -542,11 +562,11 @@ __global__ void __launch_bounds__(256) CeStatsKernel(const __half *__restrict__ 
 // `used` read clamped (valid) addresses and are not stored.  Grid: slot
 // tiles fastest (they share the |A|^2 tile), then subcarrier tiles, then
 // models.
-__global__ void __launch_bounds__(256) CeHvMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t slot_tiles,
-                                                      uint32_t sc_tiles)
+__device__ __forceinline__ void CeHvMfmaKernelBody(const CeWs &w, uint32_t slots, const CeApply &c, uint32_t slot_tiles,
+                                                      uint32_t sc_tiles, uint32_t bx, uint32_t by)
 {
     __shared__ float part[3][16][64];
-    const uint32_t st = blockIdx.x % slot_tiles, rest = blockIdx.x / slot_tiles;
+    const uint32_t st = bx % slot_tiles, rest = bx / slot_tiles;
     const uint32_t ct = rest % sc_tiles, q = rest / sc_tiles;
     const CeModelDev &md = c.m[q];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, r = lane & 31u, kh = lane >> 5;
@@ -583,15 +603,20 @@ __global__ void __launch_bounds__(256) CeHvMfmaKernel(CeWs w, uint32_t slots, Ce
         if (s < slots) w.hv[(static_cast<size_t>(s) * c.nm + q) * c.used + col] = acc[i];
     }
 }
+__global__ void __launch_bounds__(256) CeHvMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t slot_tiles,
+                                                      uint32_t sc_tiles)
+{
+    CeHvMfmaKernelBody(w, slots, c, slot_tiles, sc_tiles, blockIdx.x, blockIdx.y);
+}
 
 // tb[q] = t[slot, q][col][k] / (lam_k + n_q) in 2-part fragment planes, row
 // n = slot * NR*D + col (zero for n >= slots * NR*D), one thread per row and
 // 8 eigenvectors.  Grid (npad * kMaxRank / 8 / 256, models).
-__global__ void __launch_bounds__(256) CeScaleSplitKernel(CeWs w, uint32_t slots, CeApply c)
+__device__ __forceinline__ void CeScaleSplitKernelBody(const CeWs &w, uint32_t slots, const CeApply &c, uint32_t bx, uint32_t by)
 {
-    const uint32_t q = blockIdx.y;
+    const uint32_t q = by;
     const CeModelDev &md = c.m[q];
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t e = bx * blockDim.x + threadIdx.x;
     const uint32_t n = e % w.npad, k0 = (e / w.npad) * 8u;
     if (k0 >= md.rank) return;
     const uint32_t ncol = c.nr * c.ndmrs;
@@ -611,6 +636,10 @@ __global__ void __launch_bounds__(256) CeScaleSplitKernel(CeWs w, uint32_t slots
     }
     airan::StoreFrag8<2>(w.tb + q * TbModelElems(w.npad), n, k0, md.rank / 16u, v);
 }
+__global__ void __launch_bounds__(256) CeScaleSplitKernel(CeWs w, uint32_t slots, CeApply c)
+{
+    CeScaleSplitKernelBody(w, slots, c, blockIdx.x, blockIdx.y);
+}
 
 // hf[slot, q][col][sc] = sum_k (t[slot, q][col][k] / (lam_k + n_q)) A[k][sc]
 // on split bf16 MFMA with 2 parts: C = TB A^T with rows n (slot, col) and
@@ -624,11 +653,11 @@ __global__ void __launch_bounds__(256) CeScaleSplitKernel(CeWs w, uint32_t slots
 // range of tiles with the row tiles of one subcarrier tile adjacent: the
 // subcarrier-side operand is read once per XCD, the small row-side operand
 // stays in every L2.  Linear grid of 8 * per_xcd.
-__global__ void __launch_bounds__(256, 2) CeInterpMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t sc_tiles,
-                                                          uint32_t per_xcd)
+__device__ __forceinline__ void CeInterpMfmaKernelBody(const CeWs &w, uint32_t slots, const CeApply &c, uint32_t sc_tiles,
+                                                          uint32_t per_xcd, uint32_t bx, uint32_t by)
 {
     __shared__ uint4 stage[2][8][4][64];  // [buffer][block: 4 row, 4 subcarrier][plane][lane]
-    const uint32_t xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
+    const uint32_t xcd = bx & 7u, j = bx >> 3;
     const uint32_t n_tiles = w.npad / kInterpTile;
     const uint32_t tile = xcd * per_xcd + j;
     if (tile >= c.nm * sc_tiles * n_tiles) return;
@@ -703,6 +732,11 @@ __global__ void __launch_bounds__(256, 2) CeInterpMfmaKernel(CeWs w, uint32_t sl
             }
         }
 }
+__global__ void __launch_bounds__(256, 2) CeInterpMfmaKernel(CeWs w, uint32_t slots, CeApply c, uint32_t sc_tiles,
+                                                          uint32_t per_xcd)
+{
+    CeInterpMfmaKernelBody(w, slots, c, sc_tiles, per_xcd, blockIdx.x, blockIdx.y);
+}
 
 // (R_tt + e I) for D DMRS symbols, eliminated once (no pivoting; the matrix
 // is symmetric positive definite) and applied to each symbol's right-hand
@@ -766,11 +800,11 @@ __device__ inline void Rescale(float hv, float *scale, float *err2)
 // symbols.  One thread per subcarrier of one (slot, layer) = blockIdx.y.
 // Writes H[slot][rx][l][sym][sc] fp16 for sc < pad (zero beyond used).
 template <int D>
-__global__ void __launch_bounds__(256) CePostKernel(CeWs w, __half *__restrict__ out_r, __half *__restrict__ out_i,
-                                                    uint32_t out_layers, CeApply c)
+__device__ __forceinline__ void CePostKernelBody(const CeWs &w, __half *__restrict__ out_r, __half *__restrict__ out_i,
+                                                    uint32_t out_layers, const CeApply &c, uint32_t bx, uint32_t by)
 {
-    const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t slot = blockIdx.y / c.nl, l = blockIdx.y - slot * c.nl;
+    const uint32_t sc = bx * blockDim.x + threadIdx.x;
+    const uint32_t slot = by / c.nl, l = by - slot * c.nl;
     if (sc >= c.pad) return;
     const size_t plane = static_cast<size_t>(c.nsym) * c.pad;
     out_r += static_cast<size_t>(slot) * c.nr * out_layers * plane;
@@ -832,6 +866,12 @@ __global__ void __launch_bounds__(256) CePostKernel(CeWs w, __half *__restrict__
             out_i[o] = __float2half_rn(vi);
         }
     }
+}
+template <int D>
+__global__ void __launch_bounds__(256) CePostKernel(CeWs w, __half *__restrict__ out_r, __half *__restrict__ out_i,
+                                                    uint32_t out_layers, CeApply c)
+{
+    CePostKernelBody<D>(w, out_r, out_i, out_layers, c, blockIdx.x, blockIdx.y);
 }
 
 bool CheckRocblas(rocblas_status s) { return s == rocblas_status_success; }
@@ -1023,6 +1063,145 @@ uint64_t PilotHash(const uint16_t *sc, uint32_t count)
 
 }  // namespace
 
+namespace {
+
+// ------------------------------------------------------ grouped launches
+// Several CE applications (grant shapes of a ragged batch, each with its own
+// filter, slots and workspace) in one launch per step: step k's grid is the
+// concatenation of every job's grid for that step; a workgroup finds its job
+// by binary search over the block prefix sums and runs the step's body with
+// the job's arguments and its block coordinates within the job's grid, so
+// each job computes exactly what its own LaunchCeApply would.
+constexpr int kCeSteps = 8;  // ZSplit, Project, Energy, Stats, Hv, ScaleSplit, Interp, Post
+
+struct CeJobDev {
+    CeApply c;
+    CeWs w;
+    const __half *hr, *hi, *noise_in;
+    __half *noise_out, *out_re, *out_im;
+    uint32_t slots, out_layers;
+    uint32_t k_tiles, proj_per_xcd, hv_slot_tiles, hv_sc_tiles, sc_tiles, interp_per_xcd;
+    uint32_t gx[kCeSteps], gy[kCeSteps];
+};
+
+struct CeJobBlock {
+    const CeJobDev *job;
+    uint32_t bx, by;
+};
+
+// starts: [kCeSteps][njobs + 1] block prefix sums (starts[step][njobs] = total).
+__device__ inline CeJobBlock LocateJob(const CeJobDev *__restrict__ jobs, const uint32_t *__restrict__ starts,
+                                       uint32_t njobs, int step)
+{
+    const uint32_t *st = starts + static_cast<size_t>(step) * (njobs + 1);
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (st[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const CeJobDev *j = jobs + lo;
+    const uint32_t li = b - st[lo];
+    return CeJobBlock{j, li % j->gx[step], li / j->gx[step]};
+}
+
+__global__ void __launch_bounds__(256) CeZSplitGrouped(const CeJobDev *jobs, const uint32_t *starts, uint32_t n)
+{
+    const CeJobBlock jb = LocateJob(jobs, starts, n, 0);
+    const CeJobDev &J = *jb.job;
+    CeZSplitKernelBody(J.hr, J.hi, J.w, J.slots, J.c, jb.bx, jb.by);
+}
+
+__global__ void __launch_bounds__(256, 2) CeProjectGrouped(const CeJobDev *jobs, const uint32_t *starts, uint32_t n)
+{
+    const CeJobBlock jb = LocateJob(jobs, starts, n, 1);
+    const CeJobDev &J = *jb.job;
+    CeProjectMfmaKernelBody(J.w, J.slots, J.c, J.k_tiles, J.proj_per_xcd, jb.bx, jb.by);
+}
+
+__global__ void __launch_bounds__(256) CeEnergyGrouped(const CeJobDev *jobs, const uint32_t *starts, uint32_t n)
+{
+    const CeJobBlock jb = LocateJob(jobs, starts, n, 2);
+    const CeJobDev &J = *jb.job;
+    CeEnergyKernelBody(J.hr, J.hi, J.w, J.c, J.slots, jb.bx, jb.by);
+}
+
+__global__ void __launch_bounds__(256) CeStatsGrouped(const CeJobDev *jobs, const uint32_t *starts, uint32_t n)
+{
+    const CeJobBlock jb = LocateJob(jobs, starts, n, 3);
+    const CeJobDev &J = *jb.job;
+    CeStatsKernelBody(J.noise_in, J.noise_out, J.w, J.c, jb.bx, jb.by);
+}
+
+__global__ void __launch_bounds__(256) CeHvGrouped(const CeJobDev *jobs, const uint32_t *starts, uint32_t n)
+{
+    const CeJobBlock jb = LocateJob(jobs, starts, n, 4);
+    const CeJobDev &J = *jb.job;
+    CeHvMfmaKernelBody(J.w, J.slots, J.c, J.hv_slot_tiles, J.hv_sc_tiles, jb.bx, jb.by);
+}
+
+__global__ void __launch_bounds__(256) CeScaleSplitGrouped(const CeJobDev *jobs, const uint32_t *starts, uint32_t n)
+{
+    const CeJobBlock jb = LocateJob(jobs, starts, n, 5);
+    const CeJobDev &J = *jb.job;
+    CeScaleSplitKernelBody(J.w, J.slots, J.c, jb.bx, jb.by);
+}
+
+__global__ void __launch_bounds__(256, 2) CeInterpGrouped(const CeJobDev *jobs, const uint32_t *starts, uint32_t n)
+{
+    const CeJobBlock jb = LocateJob(jobs, starts, n, 6);
+    const CeJobDev &J = *jb.job;
+    CeInterpMfmaKernelBody(J.w, J.slots, J.c, J.sc_tiles, J.interp_per_xcd, jb.bx, jb.by);
+}
+
+__global__ void __launch_bounds__(256) CePostGrouped(const CeJobDev *jobs, const uint32_t *starts, uint32_t n)
+{
+    const CeJobBlock jb = LocateJob(jobs, starts, n, 7);
+    const CeJobDev &J = *jb.job;
+    switch (J.c.ndmrs) {
+    case 1: CePostKernelBody<1>(J.w, J.out_re, J.out_im, J.out_layers, J.c, jb.bx, jb.by); break;
+    case 2: CePostKernelBody<2>(J.w, J.out_re, J.out_im, J.out_layers, J.c, jb.bx, jb.by); break;
+    case 3: CePostKernelBody<3>(J.w, J.out_re, J.out_im, J.out_layers, J.c, jb.bx, jb.by); break;
+    default: CePostKernelBody<4>(J.w, J.out_re, J.out_im, J.out_layers, J.c, jb.bx, jb.by); break;
+    }
+}
+
+// The grids LaunchCeApply uses for one application.
+void StepGrids(const LmmseFilter &f, const CeWs &w, uint32_t slots, CeJobDev *j)
+{
+    const Geom &g = f.g;
+    const uint32_t ncol = g.nr * g.ndmrs;
+    uint32_t max_rank = 0, max_n = 0;
+    for (uint32_t q = 0; q < f.nm; ++q) {
+        max_rank = std::max(max_rank, f.model[q].rank);
+        max_n = std::max(max_n, f.model[q].n);
+    }
+    j->k_tiles = airan::AlignUp(max_rank, kProjKTile) / kProjKTile;
+    j->proj_per_xcd = airan::CeilDiv(f.nm * kProjSplits * (w.npad / 64) * j->k_tiles, 8);
+    j->hv_slot_tiles = airan::CeilDiv(slots, 32);
+    j->hv_sc_tiles = airan::CeilDiv(g.used, 32);
+    j->sc_tiles = airan::AlignUp(g.used, kInterpTile) / kInterpTile;
+    j->interp_per_xcd = airan::CeilDiv(f.nm * j->sc_tiles * (w.npad / kInterpTile), 8);
+    const uint32_t gx[kCeSteps] = {airan::CeilDiv(static_cast<uint64_t>(w.npad) * airan::AlignUp(max_n, 16u) / 8, 256),
+                                   8 * j->proj_per_xcd,
+                                   slots * f.nm * ncol,
+                                   slots,
+                                   j->hv_slot_tiles * j->hv_sc_tiles * f.nm,
+                                   airan::CeilDiv(static_cast<uint64_t>(w.npad) * kMaxRank / 8, 256),
+                                   8 * j->interp_per_xcd,
+                                   airan::CeilDiv(g.pad, 256)};
+    const uint32_t gy[kCeSteps] = {f.nm, 1, 1, 1, 1, f.nm, 1, slots * g.nl};
+    for (int k = 0; k < kCeSteps; ++k) {
+        j->gx[k] = gx[k];
+        j->gy[k] = gy[k];
+    }
+}
+
+}  // namespace
+
 namespace airan {
 
 size_t CeWorkspaceBytes(const Geom &g, uint32_t slots) { return CeWsLayout(g, slots, nullptr, nullptr); }
@@ -1074,6 +1253,88 @@ int32_t LaunchCeApply(const LmmseFilter *f, const __half *hr, const __half *hi, 
     case 3: hipLaunchKernelGGL(CePostKernel<3>, pgrid, dim3(256), 0, s, w, out_re, out_im, out_layers, c); break;
     default: hipLaunchKernelGGL(CePostKernel<4>, pgrid, dim3(256), 0, s, w, out_re, out_im, out_layers, c); break;
     }
+    return LaunchStatus();
+}
+
+struct CeGrouped {
+    CeJobDev *jobs = nullptr;
+    uint32_t *starts = nullptr;
+    uint32_t njobs = 0;
+    uint32_t total[kCeSteps] = {};
+};
+
+void DestroyCeGrouped(CeGrouped *g)
+{
+    if (g == nullptr) return;
+    FreeDev(g->jobs);
+    FreeDev(g->starts);
+    delete g;
+}
+
+int32_t CreateCeGrouped(const CeGroupJob *jobs, uint32_t n, CeGrouped **out)
+{
+    if (jobs == nullptr || n == 0 || out == nullptr) return AIRAN_INVALID_ARGUMENT;
+    *out = nullptr;
+    std::vector<CeJobDev> dev(n);
+    std::vector<uint32_t> starts(static_cast<size_t>(kCeSteps) * (n + 1), 0);
+    uint64_t acc[kCeSteps] = {};
+    for (uint32_t i = 0; i < n; ++i) {
+        const CeGroupJob &jb = jobs[i];
+        if (jb.filter == nullptr) return AIRAN_INVALID_ARGUMENT;
+        const Geom &g = jb.filter->g;
+        if (jb.slots == 0 || jb.slots > 65535 || jb.out_layers < g.nl) return AIRAN_INVALID_ARGUMENT;
+        if (jb.ws_bytes < CeWorkspaceBytes(g, jb.slots)) return AIRAN_RESOURCE_TOO_SMALL;
+        CeJobDev &d = dev[i];
+        d.c = MakeApply(*jb.filter);
+        CeWsLayout(g, jb.slots, static_cast<uint8_t *>(jb.ws), &d.w);
+        d.hr = jb.hr;
+        d.hi = jb.hi;
+        d.noise_in = jb.noise_in;
+        d.noise_out = jb.noise_out;
+        d.out_re = jb.out_re;
+        d.out_im = jb.out_im;
+        d.slots = jb.slots;
+        d.out_layers = jb.out_layers;
+        StepGrids(*jb.filter, d.w, jb.slots, &d);
+        for (int k = 0; k < kCeSteps; ++k) {
+            starts[static_cast<size_t>(k) * (n + 1) + i] = static_cast<uint32_t>(acc[k]);
+            acc[k] += static_cast<uint64_t>(d.gx[k]) * d.gy[k];
+        }
+    }
+    auto *g = new CeGrouped{};
+    g->njobs = n;
+    for (int k = 0; k < kCeSteps; ++k) {
+        if (acc[k] > 0x7fffffffull) {
+            delete g;
+            return AIRAN_UNSUPPORTED_PROFILE;
+        }
+        starts[static_cast<size_t>(k) * (n + 1) + n] = static_cast<uint32_t>(acc[k]);
+        g->total[k] = static_cast<uint32_t>(acc[k]);
+    }
+    if (hipMalloc(reinterpret_cast<void **>(&g->jobs), sizeof(CeJobDev) * n) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&g->starts), sizeof(uint32_t) * starts.size()) != hipSuccess ||
+        hipMemcpy(g->jobs, dev.data(), sizeof(CeJobDev) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->starts, starts.data(), sizeof(uint32_t) * starts.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        DestroyCeGrouped(g);
+        return AIRAN_RESOURCE_TOO_SMALL;
+    }
+    *out = g;
+    return AIRAN_OK;
+}
+
+int32_t LaunchCeGrouped(const CeGrouped *g, hipStream_t s)
+{
+    if (g == nullptr) return AIRAN_INVALID_ARGUMENT;
+    const CeJobDev *j = g->jobs;
+    const uint32_t *st = g->starts, n = g->njobs;
+    if (g->total[0]) hipLaunchKernelGGL(CeZSplitGrouped, dim3(g->total[0]), dim3(256), 0, s, j, st, n);
+    if (g->total[1]) hipLaunchKernelGGL(CeProjectGrouped, dim3(g->total[1]), dim3(256), 0, s, j, st, n);
+    if (g->total[2]) hipLaunchKernelGGL(CeEnergyGrouped, dim3(g->total[2]), dim3(256), 0, s, j, st, n);
+    if (g->total[3]) hipLaunchKernelGGL(CeStatsGrouped, dim3(g->total[3]), dim3(256), 0, s, j, st, n);
+    if (g->total[4]) hipLaunchKernelGGL(CeHvGrouped, dim3(g->total[4]), dim3(256), 0, s, j, st, n);
+    if (g->total[5]) hipLaunchKernelGGL(CeScaleSplitGrouped, dim3(g->total[5]), dim3(256), 0, s, j, st, n);
+    if (g->total[6]) hipLaunchKernelGGL(CeInterpGrouped, dim3(g->total[6]), dim3(256), 0, s, j, st, n);
+    if (g->total[7]) hipLaunchKernelGGL(CePostGrouped, dim3(g->total[7]), dim3(256), 0, s, j, st, n);
     return LaunchStatus();
 }
 

@@ -13,8 +13,8 @@
  *   DMRS gen, LS, noise     one launch each over (grant, layer / Rx) items
  *   LMMSE CE                one filter per distinct grant shape, built once
  *                           and cached for the process (keyed by the shape
- *                           and the CE prior), applied to the grants of that
- *                           shape together
+ *                           and the CE prior); one launch per CE step over
+ *                           every shape, each shape's grants together
  *   detect + demap + descramble   one launch over (grant, symbol, tile) items
  *   rate dematch            one launch over all code blocks
  *   LDPC (FP16 messages)    one launch over the code blocks with <= 8 active
