@@ -611,8 +611,17 @@ void LaunchConv(const AiranCeRefiner *r, const ConvArgs &a, hipStream_t s)
     constexpr int TILE = 32 * (4 / (COUT / 32));
     const uint32_t ntiles = a.S * kSym * ((a.W + TILE - 1) / TILE);
     const uint32_t grid = std::min<uint32_t>(ntiles, static_cast<uint32_t>(r->cus) * 2);
+    // The LDS tile holds the halo of the layer's dilation rounded up to a
+    // power of two, so a small dilation keeps a small tile (more workgroups
+    // per CU).
     if (a.dil == 1)
         hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI, 1>), dim3(grid), dim3(256), 0, s, a);
+    else if (a.dil <= 2)
+        hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI, 2>), dim3(grid), dim3(256), 0, s, a);
+    else if (a.dil <= 4)
+        hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI, 4>), dim3(grid), dim3(256), 0, s, a);
+    else if (a.dil <= 8)
+        hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI, 8>), dim3(grid), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI, kMaxDil>), dim3(grid), dim3(256), 0, s, a);
 }
@@ -655,6 +664,12 @@ void LaunchWgrad(const AiranCeRefiner *r, const bf16 *act, const bf16 *dz, uint3
     const int dil = r->dil[layer];
     if (dil == 1)
         hipLaunchKernelGGL((WgradKernel<CIN, COUT, 1>), dim3(nb), dim3(256), 0, s, act, dz, r->part, S, W, 1);
+    else if (dil <= 2)
+        hipLaunchKernelGGL((WgradKernel<CIN, COUT, 2>), dim3(nb), dim3(256), 0, s, act, dz, r->part, S, W, dil);
+    else if (dil <= 4)
+        hipLaunchKernelGGL((WgradKernel<CIN, COUT, 4>), dim3(nb), dim3(256), 0, s, act, dz, r->part, S, W, dil);
+    else if (dil <= 8)
+        hipLaunchKernelGGL((WgradKernel<CIN, COUT, 8>), dim3(nb), dim3(256), 0, s, act, dz, r->part, S, W, dil);
     else
         hipLaunchKernelGGL((WgradKernel<CIN, COUT, kMaxDil>), dim3(nb), dim3(256), 0, s, act, dz, r->part, S, W,
                            dil);
