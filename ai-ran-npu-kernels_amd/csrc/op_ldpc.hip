@@ -402,17 +402,22 @@ __device__ __forceinline__ uint32_t NegBit2(uint32_t x)
 }
 
 // Byte offset in LDS of edge e's posterior word for check i: column base +
-// 4 ((i + shift) mod Z), i4 = 4 i.  With ZT != 0, Z = ZT and the edge's
-// column and shift are compile-time constants (callers unroll over e): the
-// wrap is min(4 (i + s), 4 (i + s - Z)) (unsigned) in three VALU
-// instructions and the column base goes to the LDS instruction's offset;
-// otherwise the edge word comes from the per-lifting-size table.
+// 4 ((i + shift) mod Z), i4 = 4 i, i4m = 4 i - 4 Z (mod 2^32).  With ZT != 0,
+// Z = ZT and the edge's column and shift are compile-time constants (callers
+// unroll over e): the lanes past the wrap (4 i >= 4 (Z - s)) take i4m, the
+// others i4 (a compare and a select), and the constant column base + 4 s
+// goes to the LDS instruction's offset field, which the hardware adds to the
+// address register modulo 2^32 (so i4m + offset lands on 4 (i + s - Z));
+// the posterior must start at LDS address 0 for the offset to hold all of
+// it.  Otherwise the edge word comes from the per-lifting-size table.
 template <uint32_t ZT>
-__device__ __forceinline__ uint32_t EdgeByte(const uint32_t *lw, int e, uint32_t i, uint32_t i4, uint32_t Z)
+__device__ __forceinline__ uint32_t EdgeByte(const uint32_t *lw, int e, uint32_t i, uint32_t i4, uint32_t i4m,
+                                             uint32_t Z)
 {
     if constexpr (ZT != 0) {
         const uint32_t s4 = 4u * (static_cast<uint32_t>(AIRAN_BG1_SHIFT[e]) % ZT);
-        return 4u * static_cast<uint32_t>(AIRAN_BG1_COL[e]) * ZT + min(i4 + s4, i4 + s4 - 4u * ZT);
+        const uint32_t base = i4 < 4u * ZT - s4 ? i4 : i4m;
+        return base + (4u * static_cast<uint32_t>(AIRAN_BG1_COL[e]) * ZT + s4);
     } else {
         return 4u * LiftAddr(lw[e], i, Z);
     }
@@ -426,20 +431,22 @@ __device__ __forceinline__ uint32_t &LdsWord(uint32_t *lam2, uint32_t byte)
 // Check i of row R for both blocks.  min1 / min2 are tracked on the fp16
 // magnitudes (with duplicates: two edges at the minimum give min2 == min1),
 // starting from two virtual edges of magnitude 0x7c00, which is what the
-// keys (|x| << 5 | k) from (0x7c00 << 5) give for the magnitudes.
+// keys (|x| << 5 | k) from (0x7c00 << 5) give for the magnitudes.  A row
+// with an infinite or NaN extrinsic sets *inf_flag.
 template <int R, uint32_t ZT>
-__device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i, uint32_t Z, uint32_t *msg)
+__device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i, uint32_t Z, uint32_t *msg,
+                                        uint32_t *inf_flag)
 {
     constexpr int DEG = kFastDeg[R], E0 = FastRowStart(R);
     const uint32_t *lw = kLift.w[zi];
     // Opaque per row: with compile-time edges the addresses depend on i
     // alone and would otherwise be hoisted out of the iteration loop, every
     // row's at once, in registers.
-    uint32_t i4 = 4u * i;
-    asm volatile("" : "+v"(i4));
+    uint32_t i4 = 4u * i, i4m = 4u * i - 4u * Z;
+    asm volatile("" : "+v"(i4), "+v"(i4m));
     uint32_t m1 = 0x7c007c00u, m2 = 0x7c007c00u;
     // XOR of the extrinsics (bits 15 / 31: the sign parity of each half) and
-    // the largest magnitude per half (a NaN shows as one above 0x7c00).
+    // the largest magnitude per half (above 0x7c00 a NaN, 0x7c00 an infinity).
     uint32_t sx = 0, mx = 0;
     // The row's extrinsics and addresses are not held: the message pass
     // reads each posterior again (no other thread writes it within the row)
@@ -450,7 +457,7 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
         if (k % kChunk == 0 && k > 0) __builtin_amdgcn_sched_barrier(0);
-        const uint32_t x = WH(H2(LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, Z))) - H2(msg[E0 + k]));
+        const uint32_t x = WH(H2(LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, i4m, Z))) - H2(msg[E0 + k]));
         const uint32_t mg = x & 0x7fff7fffu;
         m2 = PkMinU16(m2, PkMaxU16(m1, mg));
         m1 = PkMinU16(m1, mg);
@@ -460,14 +467,16 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
     const h2_t q = {static_cast<_Float16>(0.75f), static_cast<_Float16>(0.75f)};
     // The parity counts the edges whose bits exceed 0x7c00 (negative, or a
     // NaN of either sign).  Without a NaN in the row that is the sign bit,
-    // so the parity is sx's; a row with a NaN (rare, and sticky once a
-    // posterior is NaN) counts the extrinsics again.
+    // so the parity is sx's; a row with an infinite or NaN extrinsic (rare,
+    // and sticky once a posterior is NaN) counts the extrinsics again and
+    // sets the flag.
     uint32_t par = sx & 0x80008000u;
-    if (PkSubSatU16(mx, 0x7c007c00u) != 0) {
+    if (PkSubSatU16(mx, 0x7bff7bffu) != 0) {
+        *inf_flag = 1u;
         uint32_t cnt = 0;
 #pragma unroll
         for (int k = 0; k < DEG; ++k)
-            cnt += AboveInf2(WH(H2(LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, Z))) - H2(msg[E0 + k])));
+            cnt += AboveInf2(WH(H2(LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, i4m, Z))) - H2(msg[E0 + k])));
         par = (cnt & 1u) << 15 | ((cnt >> 16) & 1u) << 31;
     }
     const uint32_t s1p = WH(q * H2(m1)) | par, s2p = WH(q * H2(m2)) | par;
@@ -475,7 +484,7 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
 #pragma unroll
     for (int k = 0; k < DEG; ++k) {
         if (k % kChunk == 0 && k > 0) __builtin_amdgcn_sched_barrier(0);
-        const uint32_t a = EdgeByte<ZT>(lw, E0 + k, i, i4, Z);
+        const uint32_t a = EdgeByte<ZT>(lw, E0 + k, i, i4, i4m, Z);
         const uint32_t x = WH(H2(LdsWord(lam2, a)) - H2(msg[E0 + k]));
         // 1 per half where |x_k| == min1, then s1p + that * (s2p - s1p).
         const uint32_t e = PkSubSatU16(0x00010001u, (x & 0x7fff7fffu) ^ m1);
@@ -486,43 +495,69 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
 }
 
 template <int R, int ROWS, uint32_t ZT>
-__device__ __forceinline__ void PairIteration(uint32_t *lam2, uint32_t zi, uint32_t i, uint32_t Z, uint32_t *msg)
+__device__ __forceinline__ void PairIteration(uint32_t *lam2, uint32_t zi, uint32_t i, uint32_t Z, uint32_t *msg,
+                                              uint32_t *inf_flag)
 {
     if constexpr (R < ROWS) {
-        if (i < Z) PairRow<R, ZT>(lam2, zi, i, Z, msg);
+        if (i < Z) PairRow<R, ZT>(lam2, zi, i, Z, msg, inf_flag);
         __syncthreads();
-        PairIteration<R + 1, ROWS, ZT>(lam2, zi, i, Z, msg);
+        PairIteration<R + 1, ROWS, ZT>(lam2, zi, i, Z, msg, inf_flag);
     }
 }
 
 // Bit 0: A's check i fails a row's parity; bit 16: B's.  Rows are taken
 // one at a time within the 128-VGPR budget of up to 6 active rows, three at
-// a time above (their loads in flight together).
-template <int R, int ROWS, uint32_t ZT>
+// a time above (their loads in flight together).  A hard decision is
+// NegBit (negative, non-zero, not NaN).  No posterior is ever -0 (the
+// start values are +0 or non-zero, and round-to-nearest sums and
+// differences give -0 only from -0 operands), so without a NaN it is the
+// sign bit and a row's parity is the XOR of its posteriors' words (!EXACT);
+// EXACT evaluates NegBit per posterior.
+template <int R, int ROWS, uint32_t ZT, bool EXACT>
 __device__ __forceinline__ uint32_t PairSyndrome(uint32_t *lam2, const uint32_t *lw, uint32_t i, uint32_t Z)
 {
     constexpr int G = ROWS <= 6 ? 1 : 3;
     if constexpr (R < ROWS) {
         constexpr int RE = R + G < ROWS ? R + G : ROWS;
         constexpr int E0 = FastRowStart(R), N = FastRowStart(RE) - E0;
-        uint32_t i4 = 4u * i;
-        asm volatile("" : "+v"(i4));
+        uint32_t i4 = 4u * i, i4m = 4u * i - 4u * Z;
+        asm volatile("" : "+v"(i4), "+v"(i4m));
         uint32_t v[N];
 #pragma unroll
-        for (int k = 0; k < N; ++k) v[k] = LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, Z));
+        for (int k = 0; k < N; ++k) v[k] = LdsWord(lam2, EdgeByte<ZT>(lw, E0 + k, i, i4, i4m, Z));
         __builtin_amdgcn_sched_barrier(0);
         uint32_t par = 0;
 #pragma unroll
         for (int r = R; r < RE; ++r) {
             uint32_t pr = 0;
 #pragma unroll
-            for (int k = FastRowStart(r) - E0; k < FastRowStart(r + 1) - E0; ++k) pr ^= NegBit2(v[k]);
-            par |= pr;
+            for (int k = FastRowStart(r) - E0; k < FastRowStart(r + 1) - E0; ++k) pr ^= EXACT ? NegBit2(v[k]) : v[k];
+            par |= EXACT ? pr : (pr >> 15) & 0x00010001u;
         }
         asm volatile("" : "+v"(par) : : "memory");
-        return par | PairSyndrome<RE, ROWS, ZT>(lam2, lw, i, Z);
+        return par | PairSyndrome<RE, ROWS, ZT, EXACT>(lam2, lw, i, Z);
     }
     return 0;
+}
+
+// Shared words of the pair decoder after its posterior: taken_cb[2],
+// syn_bad[8], syn_exact[8], inf_flag[2].
+constexpr uint32_t kPairTailWords = 20;
+
+// The pair decoder's LDS: posterior then kPairTailWords.  With Z = ZT known
+// at compile time a static array, whose address is a compile-time constant
+// (0: the kernel has no other shared variable), so that every edge's column
+// base and shift fold into the LDS instructions' offsets; otherwise the
+// dynamic allocation.
+template <int ROWS, uint32_t ZT>
+__device__ __forceinline__ uint32_t *PairLds()
+{
+    if constexpr (ZT != 0) {
+        __shared__ uint32_t buf[(22 + ROWS) * ZT + kPairTailWords];
+        return buf;
+    } else {
+        return reinterpret_cast<uint32_t *>(airan_ldpc_lam);
+    }
 }
 
 struct LdpcFastDev {
@@ -701,13 +736,18 @@ __global__ void __launch_bounds__(384)
                    int8_t *__restrict__ bits, int16_t *__restrict__ lam_out, uint8_t *__restrict__ iters_out,
                    LdpcFastDev d, uint32_t *__restrict__ queue)
 {
-    // One slot per half: a wave may still be reading half 0's take when
-    // thread 0 takes for half 1 (PairLoad need not end with a barrier).
-    __shared__ uint32_t taken_cb[2];
-    __shared__ uint32_t syn_bad[8];  // per wave: bit h set when half h fails a check
-    uint32_t *lam2 = reinterpret_cast<uint32_t *>(airan_ldpc_lam);
+    uint32_t *lam2 = PairLds<ROWS, ZT>();
     const uint32_t Z = d.z, i = threadIdx.x;
     const uint32_t nl = (22u + ROWS) * Z;
+    // The few shared words live after the posterior (kPairTailWords).  taken_cb: one slot per half (a wave may
+    // still be reading half 0's take when thread 0 takes for half 1:
+    // PairLoad need not end with a barrier).  syn_bad / syn_exact: per wave,
+    // bit h set when half h fails a check.  inf_flag[it0 & 1] (it0 before
+    // the pass's increment): some row of the pass saw an infinite or NaN
+    // extrinsic, or (harmless: it only selects the exact syndrome) an
+    // earlier pass that used the same word did.
+    uint32_t *taken_cb = lam2 + nl, *syn_bad = taken_cb + 2, *syn_exact = syn_bad + 8, *inf_flag = syn_exact + 8;
+    if (i < 2) inf_flag[i] = 0;  // before the first barrier
     constexpr int kEdges = FastRowStart(ROWS);
     uint32_t msg[kEdges];
 #pragma unroll
@@ -733,7 +773,7 @@ __global__ void __launch_bounds__(384)
                     next = __builtin_amdgcn_readfirstlane(taken_cb[h]);
                 }
                 ++taken;
-                PairLoad<FUSED>(airan_ldpc_lam, h, nl, src, desc, d, next);
+                PairLoad<FUSED>(reinterpret_cast<uint16_t *>(lam2), h, nl, src, desc, d, next);
                 if (h) {
                     cur1 = next;
                     it1 = 0;
@@ -748,19 +788,44 @@ __global__ void __launch_bounds__(384)
             __syncthreads();
             if (cur0 >= d.num_cb && cur1 >= d.num_cb) break;
         }
-        PairIteration<0, ROWS, ZT>(lam2, d.zi, i, Z, msg);
+        const uint32_t slot = it0 & 1u;
+        PairIteration<0, ROWS, ZT>(lam2, d.zi, i, Z, msg, inf_flag + slot);
         ++it0;
         ++it1;
         uint32_t ok = 0;  // bit h: half h's syndrome is clear
         if (d.early_stop) {
             // Both halves' flags in one barrier: each wave's ballots into
-            // syn_bad[wave] (rewritten only after further barriers).
-            const uint32_t bad = i < Z ? PairSyndrome<0, ROWS, ZT>(lam2, lw, i, Z) : 0u;
+            // syn_bad[wave] (rewritten only after further barriers).  The
+            // parity is the sign-bit XOR unless some thread saw an infinite
+            // or NaN extrinsic in the iteration: a NaN posterior is created
+            // only from an infinite operand, which the row reading it would
+            // have seen, and every posterior the syndrome reads was read and
+            // rewritten in the iteration.  Then the syndrome is evaluated
+            // again with NegBit (rare: saturated blocks only).
+            //
+            // inf_flag[slot] was set (if at all) before the last row's
+            // barrier and is read below; the other word is cleared here for
+            // a later pass (this pass writes and reads only inf_flag[slot]):
+            // its last reads were in an earlier pass, before this pass's
+            // first row barrier, and its next writes follow this pass's
+            // syndrome barrier.
+            const uint32_t bad = i < Z ? PairSyndrome<0, ROWS, ZT, false>(lam2, lw, i, Z) : 0u;
             const uint32_t wb = (__ballot((bad & 1u) != 0) != 0 ? 1u : 0u) | (__ballot((bad >> 16) != 0) != 0 ? 2u : 0u);
             if ((i & 63u) == 0) syn_bad[i >> 6] = wb;
+            if (i == 0) inf_flag[slot ^ 1u] = 0;
+            const bool exact = inf_flag[slot] != 0;
             __syncthreads();
             uint32_t any = 0;
             for (uint32_t w = 0; w < blockDim.x / 64u; ++w) any |= syn_bad[w];
+            if (exact) {
+                const uint32_t bx = i < Z ? PairSyndrome<0, ROWS, ZT, true>(lam2, lw, i, Z) : 0u;
+                const uint32_t wx =
+                    (__ballot((bx & 1u) != 0) != 0 ? 1u : 0u) | (__ballot((bx >> 16) != 0) != 0 ? 2u : 0u);
+                if ((i & 63u) == 0) syn_exact[i >> 6] = wx;
+                __syncthreads();
+                any = 0;
+                for (uint32_t w = 0; w < blockDim.x / 64u; ++w) any |= syn_exact[w];
+            }
             ok = ~any & 3u;
         }
         refill = 0;
@@ -790,7 +855,9 @@ uint32_t PairCapacity(const void *kernel, uint32_t threads, size_t lds)
     return static_cast<uint32_t>(std::max(cus, 0)) * static_cast<uint32_t>(std::max(per_cu, 0));
 }
 
-// Launches LdpcPairKernel<FUSED, rows, ZT> (rows 4 .. kFastRows).
+// Launches LdpcPairKernel<FUSED, rows, ZT> (rows 4 .. kFastRows); lds: the
+// posterior's bytes (the shared words after it are added here; at Z = 384
+// the kernel's LDS is static and none is allocated at launch).
 template <bool FUSED>
 void LaunchPair(uint32_t rows, uint32_t threads, size_t lds, hipStream_t s, const int16_t *src,
                 const RateMatchDescriptor *desc, int8_t *bits, int16_t *lam_out, uint8_t *iters_out,
@@ -805,6 +872,7 @@ void LaunchPair(uint32_t rows, uint32_t threads, size_t lds, hipStream_t s, cons
            : rows == 6 ? (z384 ? LdpcPairKernel<FUSED, 6, 384> : LdpcPairKernel<FUSED, 6, 0>)
            : rows == 7 ? (z384 ? LdpcPairKernel<FUSED, 7, 384> : LdpcPairKernel<FUSED, 7, 0>)
                        : (z384 ? LdpcPairKernel<FUSED, 8, 384> : LdpcPairKernel<FUSED, 8, 0>);
+    lds = z384 ? 0 : lds + kPairTailWords * sizeof(uint32_t);
     // One resident wave of workgroups, each walking its list of blocks.
     uint32_t grid = (d.num_cb + 1u) / 2u;
     const uint32_t cap = PairCapacity(reinterpret_cast<const void *>(k), threads, lds);
