@@ -481,16 +481,33 @@ __device__ __forceinline__ void PairRow(uint32_t *lam2, uint32_t zi, uint32_t i,
     }
     const uint32_t s1p = WH(q * H2(m1)) | par, s2p = WH(q * H2(m2)) | par;
     const uint32_t d12 = PkSubU16(s2p, s1p);
+    // Message pass, kUpd edges at a time: their posteriors are all read
+    // before any is written, so the reads of a group are in flight together
+    // (within a row every posterior word belongs to one edge of one thread,
+    // so no read of the row aliases another edge's write).
+    constexpr int kUpd = 6;
 #pragma unroll
-    for (int k = 0; k < DEG; ++k) {
-        if (k % kChunk == 0 && k > 0) __builtin_amdgcn_sched_barrier(0);
-        const uint32_t a = EdgeByte<ZT>(lw, E0 + k, i, i4, i4m, Z);
-        const uint32_t x = WH(H2(LdsWord(lam2, a)) - H2(msg[E0 + k]));
-        // 1 per half where |x_k| == min1, then s1p + that * (s2p - s1p).
-        const uint32_t e = PkSubSatU16(0x00010001u, (x & 0x7fff7fffu) ^ m1);
-        const uint32_t m = PkMadU16(e, d12, s1p) ^ (x & 0x80008000u);
-        msg[E0 + k] = m;
-        LdsWord(lam2, a) = WH(H2(x) + H2(m));
+    for (int k0 = 0; k0 < DEG; k0 += kUpd) {
+        constexpr int kMax = kUpd;
+        const int n = DEG - k0 < kMax ? DEG - k0 : kMax;
+        if (k0 > 0) __builtin_amdgcn_sched_barrier(0);
+        uint32_t a[kMax], v[kMax];
+#pragma unroll
+        for (int k = 0; k < kMax; ++k)
+            if (k < n) {
+                a[k] = EdgeByte<ZT>(lw, E0 + k0 + k, i, i4, i4m, Z);
+                v[k] = LdsWord(lam2, a[k]);
+            }
+#pragma unroll
+        for (int k = 0; k < kMax; ++k)
+            if (k < n) {
+                const uint32_t x = WH(H2(v[k]) - H2(msg[E0 + k0 + k]));
+                // 1 per half where |x_k| == min1, then s1p + that * (s2p - s1p).
+                const uint32_t e = PkSubSatU16(0x00010001u, (x & 0x7fff7fffu) ^ m1);
+                const uint32_t m = PkMadU16(e, d12, s1p) ^ (x & 0x80008000u);
+                msg[E0 + k0 + k] = m;
+                LdsWord(lam2, a[k]) = WH(H2(x) + H2(m));
+            }
     }
 }
 
