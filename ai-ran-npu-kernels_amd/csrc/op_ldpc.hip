@@ -392,6 +392,21 @@ __device__ __forceinline__ uint32_t PkMadU16(uint32_t a, uint32_t b, uint32_t c)
     return r;
 }
 
+// Signed 16-bit max / min per half.
+__device__ __forceinline__ uint32_t PkMaxI16(uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_pk_max_i16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ uint32_t PkMinI16(uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_pk_min_i16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
 // Per half: 1 if the fp16 bits exceed 0x7c00 (negative, or a NaN), else 0.
 __device__ __forceinline__ uint32_t AboveInf2(uint32_t x) { return PkMinU16(PkSubSatU16(x, 0x7c007c00u), 0x00010001u); }
 
@@ -608,26 +623,51 @@ __device__ inline void PairLoad(uint16_t *lam16, uint32_t h, uint32_t nl, const 
             // inverse of RateSingleIndex): q = (q0 + kk) mod (Ncb - filler),
             // position 2Z + (q < fstart ? q : q + filler).  So the block's
             // codeword LLRs -- Qm planes of E/Qm consecutive symbols -- are
-            // read in aligned 16-byte chunks and scattered into LDS, after the
-            // other positions are set (zero, or +5120 for filler bits).
+            // scattered into LDS after the other positions are set (zero, or
+            // +5120 for filler bits).  The symbols of a plane are read in
+            // aligned 16-byte chunks of 8 (cw_stride is a multiple of 8) where
+            // a chunk lies inside the plane's [rem0, rem0 + eq), and one by one
+            // at the plane's two ends (at most 7 each).  A chunk whose 8
+            // positions are consecutive (no wrap of q, no filler boundary, all
+            // below nl: every chunk but about one per block) is converted with
+            // packed int16 clamps and an exact int16 -> fp16 conversion:
+            // InputH(clamp(8 x, +-5120)) = fp16(clamp(x, +-640)) / 32, exact.
             const uint32_t two_z = 2u * d.z, fstart = d.rate.kprime - two_z;
             const uint32_t nf = d.rate.ncb - d.rate.filler;
-            // Symbols rem0 .. rem0 + eq - 1 of each plane of the slot, in
-            // chunks of 8 aligned to the plane start (cw_stride is a multiple
-            // of 8).
             const int16_t *slot = src + static_cast<size_t>(rc.slot0) * d.rate.qm * d.rate.cw_stride;
-            const uint32_t c0 = rc.rem0 / 8u, chunks = (rc.rem0 + rc.eq + 7u) / 8u - c0;
-            const uint32_t total = chunks * d.rate.qm;
-            constexpr uint32_t kBatch = 4;  // chunk loads of a thread in flight together
+            const uint32_t cfirst = (rc.rem0 + 7u) / 8u, clast = (rc.rem0 + rc.eq) / 8u;
+            const uint32_t ni = clast > cfirst ? clast - cfirst : 0u;  // whole chunks per plane
+            const uint32_t nh = ni ? 8u * cfirst - rc.rem0 : rc.eq;     // symbols before them
+            const uint32_t nt = ni ? rc.rem0 + rc.eq - 8u * clast : 0u; // and after them
+            const uint32_t total = ni * d.rate.qm;
+            const float rcp_ni = 1.0f / static_cast<float>(ni ? ni : 1u);
+            // Symbol j (< eq) of plane b, raw codeword LLR v: its position and value.
+            auto put = [&](uint32_t b, uint32_t j, int32_t v) {
+                uint32_t q = rc.q0 + b * rc.eq + j;
+                q = q >= nf ? q - nf : q;
+                const uint32_t t = two_z + (q < fstart ? q : q + d.rate.filler);
+                if (t >= nl) return;
+                v *= 8;
+                v = v > 5120 ? 5120 : (v < -5120 ? -5120 : v);
+                lam16[2 * t + h] = static_cast<uint16_t>(InputH(v));
+            };
+            auto chunk_of = [&](uint32_t e, uint32_t &b, uint32_t &cc) {
+                b = static_cast<uint32_t>(static_cast<float>(e) * rcp_ni);
+                b = b * ni > e ? b - 1u : b;
+                b = (b + 1u) * ni <= e ? b + 1u : b;
+                cc = cfirst + (e - b * ni);
+            };
+            constexpr uint32_t kBatch = 2;  // chunk loads of a thread in flight together
             uint4 raw[kBatch];
             auto fetch = [&](uint32_t e0) {
 #pragma unroll
                 for (uint32_t u = 0; u < kBatch; ++u) {
                     const uint32_t e = e0 + u * blockDim.x;
                     if (e < total) {
-                        const uint32_t b = e / chunks, c = c0 + (e - b * chunks);
+                        uint32_t b, cc;
+                        chunk_of(e, b, cc);
                         raw[u] = *reinterpret_cast<const uint4 *>(slot + static_cast<size_t>(b) * d.rate.cw_stride +
-                                                                  8u * c);
+                                                                  8u * cc);
                     }
                 }
             };
@@ -639,30 +679,43 @@ __device__ inline void PairLoad(uint16_t *lam16, uint32_t h, uint32_t nl, const 
                 lam16[2 * t + h] = static_cast<uint16_t>(
                     t >= two_z + fstart && t < two_z + fstart + d.rate.filler ? fill : 0u);
             __syncthreads();
-            for (;;) {
+            // The plane ends, one symbol per thread (at most 14 Qm of them).
+            const uint32_t ne = nh + nt;
+            for (uint32_t f = i; f < ne * d.rate.qm; f += blockDim.x) {
+                const uint32_t b = f / ne, r = f - b * ne;
+                const uint32_t j = r < nh ? r : 8u * clast - rc.rem0 + (r - nh);
+                put(b, j, slot[static_cast<size_t>(b) * d.rate.cw_stride + rc.rem0 + j]);
+            }
+            const h2_t scale = {static_cast<_Float16>(1.0f / 32.0f), static_cast<_Float16>(1.0f / 32.0f)};
+            while (e0 < total) {
 #pragma unroll
                 for (uint32_t u = 0; u < kBatch; ++u) {
                     const uint32_t e = e0 + u * blockDim.x;
                     if (e >= total) continue;
-                    const uint32_t b = e / chunks, c = c0 + (e - b * chunks);
+                    uint32_t b, cc;
+                    chunk_of(e, b, cc);
+                    const uint32_t j0 = 8u * cc - rc.rem0;
+                    uint32_t q = rc.q0 + b * rc.eq + j0;
+                    q = q >= nf ? q - nf : q;
+                    const uint32_t t0 = two_z + (q < fstart ? q : q + d.rate.filler);
                     const uint32_t w[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
+                    if (q + 8u <= nf && (q >= fstart || q + 8u <= fstart) && t0 + 8u <= nl) {
 #pragma unroll
-                    for (uint32_t x = 0; x < 8; ++x) {
-                        const int32_t j = static_cast<int32_t>(8u * c + x) - static_cast<int32_t>(rc.rem0);
-                        if (j < 0 || j >= static_cast<int32_t>(rc.eq)) continue;
-                        const uint32_t kk = b * rc.eq + static_cast<uint32_t>(j);
-                        uint32_t q = rc.q0 + kk;
-                        q = q >= nf ? q - nf : q;
-                        const uint32_t t = two_z + (q < fstart ? q : q + d.rate.filler);
-                        if (t >= nl) continue;
-                        int32_t v = static_cast<int16_t>(w[x >> 1] >> (16u * (x & 1u))) * 8;
-                        v = v > 5120 ? 5120 : (v < -5120 ? -5120 : v);
-                        lam16[2 * t + h] = static_cast<uint16_t>(InputH(v));
+                        for (uint32_t k = 0; k < 4; ++k) {
+                            const uint32_t c = PkMinI16(PkMaxI16(w[k], 0xfd80fd80u), 0x02800280u);
+                            const h2_t v = __builtin_convertvector(__builtin_bit_cast(s16x2, c), h2_t) * scale;
+                            const uint32_t vb = WH(v);
+                            lam16[2 * (t0 + 2u * k) + h] = static_cast<uint16_t>(vb);
+                            lam16[2 * (t0 + 2u * k + 1u) + h] = static_cast<uint16_t>(vb >> 16);
+                        }
+                    } else {
+#pragma unroll
+                        for (uint32_t x = 0; x < 8; ++x)
+                            put(b, j0 + x, static_cast<int16_t>(w[x >> 1] >> (16u * (x & 1u))));
                     }
                 }
                 e0 += kBatch * blockDim.x;
-                if (e0 >= total) break;
-                fetch(e0);
+                if (e0 < total) fetch(e0);
             }
         } else {
             for (uint32_t t = i; t < nl; t += blockDim.x)
