@@ -266,8 +266,11 @@ def test_ldpc_small_lifting_sizes(oracle, gpu_stream, rows, early):
 
 def test_ldpc_fp16_saturated_posteriors(oracle, gpu_stream):
     """Clean high-SNR blocks decoded for many iterations without early stop:
-    the layered posteriors grow past the fp16 range (inf, then NaN extrinsics)
-    and the decoder must still follow the oracle's fp16 arithmetic."""
+    the layered posteriors grow to their largest values (about 480 at 6 rows
+    and 2,250 at 46 rows in the oracle; the parity columns of degree one or
+    two cap every row's messages, so they stay far below the fp16 range) and
+    the decoder must still follow the oracle's fp16 arithmetic there, where
+    the rounding of the large messages matters most."""
     Z, ncb = 384, 4
     _, lam = _encoded_llrs(oracle, Z, ncb, 12.0, 77)
     for rows in (6, 46):
