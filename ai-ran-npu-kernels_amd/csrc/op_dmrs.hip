@@ -70,15 +70,52 @@ __device__ inline void DmrsGen16(uint16_t *__restrict__ re, uint16_t *__restrict
     }
 }
 
+// Grid (m blocks, D, slots): the layers of a (slot, DMRS symbol) share one
+// Gold sequence (they differ in w_f only), so a thread derives the word of
+// its 16 pilots once and writes every layer's 16 values; vec: the outputs
+// are 16-byte aligned (ref_pad is a multiple of 128, so are all the rows),
+// which allows 16-byte stores.
 __global__ void __launch_bounds__(256) DmrsGenKernel(uint16_t *__restrict__ re, uint16_t *__restrict__ im,
-                                                     const uint16_t *__restrict__ slot_no, DmrsGenDev d)
+                                                     const uint16_t *__restrict__ slot_no, DmrsGenDev d,
+                                                     uint32_t vec)
 {
     const uint32_t m0 = (blockIdx.x * blockDim.x + threadIdx.x) * 16;
     if (m0 >= d.ref_pad) return;
-    const uint32_t dd = blockIdx.y, slot = blockIdx.z / d.nl, l = blockIdx.z - slot * d.nl;
+    const uint32_t dd = blockIdx.y, slot = blockIdx.z;
     const uint32_t cinit = slot_no == nullptr ? d.cinit[dd] : DevDmrsCinit(slot_no[slot], d.sym[dd], d.nid, d.nscid);
-    const size_t base = ((static_cast<size_t>(slot) * d.nl + l) * d.ndmrs + dd) * d.ref_pad;
-    DmrsGen16(re, im, base, m0, d.ref_pad, d.n_dmrs_re, cinit, d.off, d.wf_odd[l] != 0);
+    if (!vec) {
+        for (uint32_t l = 0; l < d.nl; ++l) {
+            const size_t base = ((static_cast<size_t>(slot) * d.nl + l) * d.ndmrs + dd) * d.ref_pad;
+            DmrsGen16(re, im, base, m0, d.ref_pad, d.n_dmrs_re, cinit, d.off, d.wf_odd[l] != 0);
+        }
+        return;
+    }
+    // Pilots m0 .. m0 + 15 (m0 is a multiple of 16 and ref_pad of 128: all
+    // below ref_pad) as fp16 pairs, w_f = +1; pilots past n_dmrs_re are +0.
+    const uint32_t pos = 0x39a8u, neg = 0xb9a8u;  // fp16(+-0.70710678)
+    const uint32_t bits = m0 < d.n_dmrs_re ? airan::GoldWordAt(cinit, 2ull * (m0 + d.off)) : 0u;
+    uint32_t wr[8], wi[8], odd[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t m = m0 + 2u * k;
+        const bool v0 = m < d.n_dmrs_re, v1 = m + 1u < d.n_dmrs_re;
+        const uint32_t r0 = v0 ? ((bits >> (4u * k)) & 1u ? neg : pos) : 0u;
+        const uint32_t i0 = v0 ? ((bits >> (4u * k + 1u)) & 1u ? neg : pos) : 0u;
+        const uint32_t r1 = v1 ? ((bits >> (4u * k + 2u)) & 1u ? neg : pos) : 0u;
+        const uint32_t i1 = v1 ? ((bits >> (4u * k + 3u)) & 1u ? neg : pos) : 0u;
+        wr[k] = r0 | r1 << 16;
+        wi[k] = i0 | i1 << 16;
+        odd[k] = v1 ? 0x80000000u : 0u;  // w_f = -1 negates the odd pilot m + 1
+    }
+    for (uint32_t l = 0; l < d.nl; ++l) {
+        const uint32_t f = d.wf_odd[l] != 0 ? ~0u : 0u;
+        const size_t base = ((static_cast<size_t>(slot) * d.nl + l) * d.ndmrs + dd) * d.ref_pad + m0;
+        uint4 *pr = reinterpret_cast<uint4 *>(re + base), *pi = reinterpret_cast<uint4 *>(im + base);
+        pr[0] = uint4{wr[0] ^ (odd[0] & f), wr[1] ^ (odd[1] & f), wr[2] ^ (odd[2] & f), wr[3] ^ (odd[3] & f)};
+        pr[1] = uint4{wr[4] ^ (odd[4] & f), wr[5] ^ (odd[5] & f), wr[6] ^ (odd[6] & f), wr[7] ^ (odd[7] & f)};
+        pi[0] = uint4{wi[0] ^ (odd[0] & f), wi[1] ^ (odd[1] & f), wi[2] ^ (odd[2] & f), wi[3] ^ (odd[3] & f)};
+        pi[1] = uint4{wi[4] ^ (odd[4] & f), wi[5] ^ (odd[5] & f), wi[6] ^ (odd[6] & f), wi[7] ^ (odd[7] & f)};
+    }
 }
 
 // Ragged batch: item = (grant, layer); grid (m blocks of the largest grant,
@@ -318,9 +355,10 @@ int32_t LaunchDmrsGenSlots(const PuschMimoConfig &cfg, const Geom &g, const uint
 {
     if (slot_no == nullptr || slots == 0) return AIRAN_INVALID_ARGUMENT;
     const DmrsGenDev d = GenDev(cfg, g);
-    const dim3 grid(CeilDiv(CeilDiv(g.ref_pad, 16), 256), g.ndmrs, slots * g.nl);
+    const dim3 grid(CeilDiv(CeilDiv(g.ref_pad, 16), 256), g.ndmrs, slots);
+    const uint32_t vec = ((reinterpret_cast<uintptr_t>(ref_re) | reinterpret_cast<uintptr_t>(ref_im)) & 15u) == 0;
     hipLaunchKernelGGL(DmrsGenKernel, grid, dim3(256), 0, s, reinterpret_cast<uint16_t *>(ref_re),
-                       reinterpret_cast<uint16_t *>(ref_im), slot_no, d);
+                       reinterpret_cast<uint16_t *>(ref_im), slot_no, d, vec);
     return LaunchStatus();
 }
 
@@ -408,9 +446,10 @@ int32_t airan_mimo_dmrs_gen_Enqueue(MimoDmrsGenRuntimeV1 *runtime, const MimoDmr
     Geom g;
     airan::DeriveGeom(*a->config, &g);
     const DmrsGenDev d = airan::GenDev(*a->config, g);
-    dim3 grid(airan::CeilDiv(airan::CeilDiv(g.ref_pad, 16), 256), g.ndmrs, g.nl);
+    dim3 grid(airan::CeilDiv(airan::CeilDiv(g.ref_pad, 16), 256), g.ndmrs, 1);
+    const uint32_t vec = ((reinterpret_cast<uintptr_t>(a->dmrs_re) | reinterpret_cast<uintptr_t>(a->dmrs_im)) & 15u) == 0;
     hipLaunchKernelGGL(DmrsGenKernel, grid, dim3(256), 0, static_cast<hipStream_t>(a->stream),
-                       static_cast<uint16_t *>(a->dmrs_re), static_cast<uint16_t *>(a->dmrs_im), nullptr, d);
+                       static_cast<uint16_t *>(a->dmrs_re), static_cast<uint16_t *>(a->dmrs_im), nullptr, d, vec);
     return airan::LaunchStatus();
 }
 
