@@ -168,16 +168,19 @@ __global__ void __launch_bounds__(256) DetectDemapKernel(const __half *__restric
     for (uint32_t r = 0; r < nr; ++r) s += __half2float(noise[static_cast<size_t>(slot) * nr + r]);
     const float no = __half2float(__float2half_rn(s * (1.0f / static_cast<float>(nr))));
     const size_t plane = static_cast<size_t>(a.nsym) * a.pad;
-    const size_t off = static_cast<size_t>(sym) * a.pad + sc;
     const size_t rx_slot = static_cast<size_t>(slot) * nr * plane;
     const size_t h_slot = rx_slot * NL;
+    // The slot's planes from uniform base pointers with 32-bit element
+    // offsets (NR NL nsym pad < 2^32 for every geometry).
+    const __half *hr_s = h_re + h_slot, *hi_s = h_im + h_slot, *yr_s = rx_re + rx_slot, *yi_s = rx_im + rx_slot;
+    const uint32_t plane32 = a.nsym * a.pad, off32 = sym * a.pad + sc;
     auto loadh = [&](int r, int l) {
-        const size_t i = h_slot + (static_cast<size_t>(r) * NL + l) * plane + off;
-        return Cf{__half2float(h_re[i]), __half2float(h_im[i])};
+        const uint32_t i = (static_cast<uint32_t>(r) * NL + static_cast<uint32_t>(l)) * plane32 + off32;
+        return Cf{__half2float(hr_s[i]), __half2float(hi_s[i])};
     };
     auto loady = [&](int r) {
-        const size_t i = rx_slot + static_cast<size_t>(r) * plane + off;
-        return Cf{__half2float(rx_re[i]), __half2float(rx_im[i])};
+        const uint32_t i = static_cast<uint32_t>(r) * plane32 + off32;
+        return Cf{__half2float(yr_s[i]), __half2float(yi_s[i])};
     };
     Cf xh[4];
     float ne[4];

@@ -21,6 +21,7 @@ __device__ inline void DemapStore(const Cf (&xh)[4], const float (&ne)[4], uint3
                                   uint64_t gw, int16_t *out, uint32_t cw_stride)
 {
     const uint32_t qm = QM > 0 ? static_cast<uint32_t>(QM) : qm_rt, m = qm / 2;
+    const uint32_t g32 = static_cast<uint32_t>(gw);
     // LLRs of all layers first, then one contiguous NL-element store per bit
     // plane (the thread's REs are adjacent within a plane).
     int16_t v[8][NL];
@@ -35,7 +36,8 @@ __device__ inline void DemapStore(const Cf (&xh)[4], const float (&ne)[4], uint3
         for (uint32_t b = 0; b < 8; ++b) {
             const uint32_t stream = (b & 1u) ? m + b / 2 : b / 2;
             const int x = b < qm ? llr[stream] : 0;
-            const bool flip = (gw >> (static_cast<uint32_t>(l) * qm + b)) & 1u;
+            // l qm + b < NL qm <= 32: the RE's bits are the low word's.
+            const bool flip = (g32 >> (static_cast<uint32_t>(l) * qm + b)) & 1u;
             v[b][l] = flip ? SatNeg16(x) : static_cast<int16_t>(x);
         }
     }

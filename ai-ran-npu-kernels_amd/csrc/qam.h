@@ -46,14 +46,16 @@ __device__ inline void Axis256(int u, int t, int *out)
 {
     const int a = u < 0 ? -u : u;
     int cl[8];
-    int ss = 0;
+    int sa = 0;
 #pragma unroll
     for (int i = 1; i <= 7; ++i) {
         const int kt = 2 * i * t;
         cl[i] = a < kt ? a : kt;
-        ss += u < -kt ? -kt : (u > kt ? kt : u);
+        sa += cl[i];
     }
-    // cl[i] = min(a, 2i t) for k = 2i.
+    // cl[i] = min(a, 2i t) for k = 2i, and clamp(u, -2i t, 2i t) =
+    // sign(u) min(|u|, 2i t) (t >= 0), so the sum of the clamps is +-sa.
+    const int ss = u < 0 ? -sa : sa;
     const int v0 = 32 * u - 4 * ss;
     const int v1 = 80 * t - 16 * a - 4 * (cl[1] + cl[2] + cl[3]) + 4 * (cl[5] + cl[6] + cl[7]);
     const int v2 = -24 * t - 8 * a + 4 * cl[1] - 4 * cl[3] + 16 * cl[4] - 4 * cl[5] + 4 * cl[7];
