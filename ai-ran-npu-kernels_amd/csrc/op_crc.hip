@@ -248,8 +248,8 @@ __global__ void __launch_bounds__(kCrcThreads) CbCrcKernel(const int8_t *__restr
 // pair.
 typedef int i32x16_t __attribute__((ext_vector_type(16)));
 constexpr uint32_t kCrcCols = 32;   // blocks per workgroup (MFMA columns)
-constexpr uint32_t kCrcWaves = 8;   // waves splitting the positions
-constexpr uint32_t kCrcUnroll = 11; // position pairs of a wave in flight together
+constexpr uint32_t kCrcWaves = 4;   // waves splitting the positions
+constexpr uint32_t kCrcUnroll = 6;  // position pairs of a wave in flight together
 
 __global__ void __launch_bounds__(256) CrcFragKernel(airan::CrcJob c, uint32_t P, uint4 *__restrict__ frag)
 {
