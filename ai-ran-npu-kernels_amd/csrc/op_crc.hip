@@ -275,7 +275,7 @@ __device__ inline long Hi8(uint4 v) { return static_cast<long>(static_cast<uint6
 
 // bits: int8 0/1, [cb][K], K a multiple of 32 and bits 16-byte aligned.
 // Workgroup b checks blocks 32 b .. 32 b + 31; wave w takes a contiguous
-// eighth of the position pairs (consecutive loads of a lane share cache
+// quarter of the position pairs (consecutive loads of a lane share cache
 // lines); the parities of the waves' partial sums are XOR-combined.
 __global__ void __launch_bounds__(kCrcWaves * 64) CrcMfmaKernel(const int8_t *__restrict__ bits, airan::CrcJob c,
                                                                uint32_t K, uint32_t num_cb,
