@@ -858,7 +858,9 @@ __global__ void __launch_bounds__(384)
             __syncthreads();
             if (cur0 >= d.num_cb && cur1 >= d.num_cb) break;
         }
-        const uint32_t slot = it0 & 1u;
+        // Uniform (it0 is the same in every lane): a scalar, so that the
+        // flag's address takes no vector register across the pass.
+        const uint32_t slot = __builtin_amdgcn_readfirstlane(it0) & 1u;
         PairIteration<0, ROWS, ZT>(lam2, d.zi, i, Z, msg, inf_flag + slot);
         ++it0;
         ++it1;
