@@ -833,15 +833,16 @@ __global__ void __launch_bounds__(384)
             for (uint32_t h = 0; h < 2; ++h) {
                 if (((refill >> h) & 1u) == 0) continue;
                 const uint32_t cb = h ? cur1 : cur0;
-                if (cb < d.num_cb) PairEmit<FUSED>(lam2, h, cb, h ? it1 : it0, nl, src, desc, bits, lam_out, iters_out, d);
-                uint32_t next;
-                if (queue == nullptr) {
-                    next = blockIdx.x + taken * gridDim.x;
-                } else {
-                    if (i == 0) taken_cb[h] = atomicAdd(queue, 1u);
+                // The next block is taken before this one's outputs are
+                // written, so the queue atomic overlaps the emission, whose
+                // closing barrier also publishes the taken index.
+                if (queue != nullptr && i == 0) taken_cb[h] = atomicAdd(queue, 1u);
+                if (cb < d.num_cb)
+                    PairEmit<FUSED>(lam2, h, cb, h ? it1 : it0, nl, src, desc, bits, lam_out, iters_out, d);
+                else
                     __syncthreads();
-                    next = __builtin_amdgcn_readfirstlane(taken_cb[h]);
-                }
+                const uint32_t next =
+                    queue == nullptr ? blockIdx.x + taken * gridDim.x : __builtin_amdgcn_readfirstlane(taken_cb[h]);
                 ++taken;
                 PairLoad<FUSED>(reinterpret_cast<uint16_t *>(lam2), h, nl, src, desc, d, next);
                 if (h) {
