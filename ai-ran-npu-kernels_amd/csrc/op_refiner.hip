@@ -238,32 +238,62 @@ __global__ void __launch_bounds__(256) InputKernel(InputArgs a)
         }
 }
 
-// Network input from a channel estimate grid h fp16 [slots][NR][L][14][pad]
-// (the LMMSE estimator's output): x[s][k][u][2 (r L + l) + {re, im}] =
-// bf16(h), and x32 = h exactly (fp16 -> fp32).  Grid (used / 256, slots).
+// Network input from a channel estimate grid h fp16 [slots][16 links][14][pad]
+// (the LMMSE estimator's output; link r L + l): x[s][k][u][2 link + {re, im}]
+// = bf16(h) and, when x32 is not null, x32 = h exactly (fp16 -> fp32).
+// Thread: 4 consecutive subcarriers of one symbol, so that the 32 planes'
+// loads are coalesced along the subcarriers and every pixel's 32 channels
+// leave as whole 16-byte vectors.  Grid (ceil(used / 1024), 14, slots).
 __global__ void __launch_bounds__(256) EstimateInputKernel(const __half *__restrict__ h_re,
                                                            const __half *__restrict__ h_im, bf16 *__restrict__ x,
-                                                           float *__restrict__ x32, uint32_t used, uint32_t pad,
-                                                           uint32_t nr, uint32_t nl)
+                                                           float *__restrict__ x32, uint32_t used, uint32_t pad)
 {
-    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t s = blockIdx.y;
-    if (u >= used) return;
-    bf16 *xo = x + (static_cast<size_t>(s) * kSym * used + u) * kCh;
-    float *x32o = x32 + (static_cast<size_t>(s) * kSym * used + u) * kCh;
-    for (uint32_t r = 0; r < nr; ++r)
-        for (uint32_t l = 0; l < nl; ++l) {
-            const size_t base = ((static_cast<size_t>(s) * nr + r) * nl + l) * kSym * pad + u;
-            const uint32_t c = 2 * (r * nl + l);
-            for (uint32_t k = 0; k < kSym; ++k) {
-                const size_t o = static_cast<size_t>(k) * used * kCh + c;
-                const float vr = __half2float(h_re[base + k * pad]), vi = __half2float(h_im[base + k * pad]);
-                xo[o] = static_cast<bf16>(vr);
-                xo[o + 1] = static_cast<bf16>(vi);
-                x32o[o] = vr;
-                x32o[o + 1] = vi;
+    constexpr uint32_t kPx = 4, kLinks = kCh / 2;
+    const uint32_t u0 = kPx * (blockIdx.x * blockDim.x + threadIdx.x);
+    const uint32_t k = blockIdx.y, s = blockIdx.z;
+    if (u0 >= used) return;
+    const uint32_t n = min(kPx, used - u0);
+    const bool vec = n == kPx && pad % kPx == 0;
+    float v[kCh][kPx];
+#pragma unroll
+    for (uint32_t p = 0; p < kLinks; ++p) {
+        const size_t b = ((static_cast<size_t>(s) * kLinks + p) * kSym + k) * pad + u0;
+        if (vec) {
+            const uint2 wr = *reinterpret_cast<const uint2 *>(h_re + b);
+            const uint2 wi = *reinterpret_cast<const uint2 *>(h_im + b);
+            const uint32_t r4[4] = {wr.x & 0xffffu, wr.x >> 16, wr.y & 0xffffu, wr.y >> 16};
+            const uint32_t i4[4] = {wi.x & 0xffffu, wi.x >> 16, wi.y & 0xffffu, wi.y >> 16};
+#pragma unroll
+            for (uint32_t j = 0; j < kPx; ++j) {
+                v[2 * p][j] = __half2float(__ushort_as_half(static_cast<unsigned short>(r4[j])));
+                v[2 * p + 1][j] = __half2float(__ushort_as_half(static_cast<unsigned short>(i4[j])));
+            }
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kPx; ++j) {
+                v[2 * p][j] = j < n ? __half2float(h_re[b + j]) : 0.0f;
+                v[2 * p + 1][j] = j < n ? __half2float(h_im[b + j]) : 0.0f;
             }
         }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPx; ++j) {
+        if (j >= n) break;
+        const size_t o = ((static_cast<size_t>(s) * kSym + k) * used + u0 + j) * kCh;
+#pragma unroll
+        for (uint32_t q = 0; q < kCh / 8; ++q) {
+            bf16x8 b;
+#pragma unroll
+            for (uint32_t e = 0; e < 8; ++e) b[e] = static_cast<bf16>(v[8 * q + e][j]);
+            *reinterpret_cast<bf16x8 *>(x + o + 8 * q) = b;
+        }
+        if (x32 != nullptr) {
+#pragma unroll
+            for (uint32_t q = 0; q < kCh / 4; ++q)
+                *reinterpret_cast<float4 *>(x32 + o + 4 * q) =
+                    make_float4(v[4 * q][j], v[4 * q + 1][j], v[4 * q + 2][j], v[4 * q + 3][j]);
+        }
+    }
 }
 
 // x32 = float(x) for an input given in bf16 (airan_ce_refiner_Gradients).
@@ -291,17 +321,21 @@ __global__ void __launch_bounds__(64) NoiseScaleKernel(const __half *__restrict_
 }
 
 // ----------------------------------------------------------------- conv
-enum { kEpiRelu = 0, kEpiResid = 1, kEpiMask = 2 };
+enum { kEpiRelu = 0, kEpiResid = 1, kEpiMask = 2, kEpiGrid = 3 };
 
 struct ConvArgs {
     const bf16 *in;      // [S][14][W][CIN]
     const bf16 *wt;      // [9][COUT][CIN]
-    const float *bias;   // [COUT] (kEpiRelu, kEpiResid)
+    const float *bias;   // [COUT] (kEpiRelu, kEpiResid, kEpiGrid)
     const bf16 *aux;     // kEpiMask: activation [S][14][W][COUT]
     const float *aux32;  // kEpiResid: X in fp32 [S][14][W][COUT]
-    const float *scale;  // kEpiResid: residual scale per slot [S]
+    const float *scale;  // kEpiResid, kEpiGrid: residual scale per slot [S]
     bf16 *out;           // kEpiRelu, kEpiMask: [S][14][W][COUT]
     float *out_f32;      // kEpiResid: [S][14][W][COUT]
+    __half *g_re;        // kEpiGrid: residual base and output, fp16 [S][COUT / 2][14][pad]
+    __half *g_im;
+    const uint8_t *keep; // kEpiGrid: slots written (null: all)
+    uint32_t pad;        // kEpiGrid: subcarriers per grid row (>= W; zero beyond W)
     uint32_t S, W;
     int dil;             // subcarrier dilation of the taps, 1..HALO
 };
@@ -312,6 +346,13 @@ constexpr int kMaxDil = 16;
 //   kEpiRelu   bf16(max(acc + bias, 0))
 //   kEpiResid  scale[slot] (acc + bias) + aux32   (fp32)
 //   kEpiMask   bf16(aux > 0 ? acc : 0)   (gradient through the ReLU)
+//   kEpiGrid   the residual output straight into the channel grid:
+//              g[co / 2] = fp16(scale[slot] (acc + bias) + float(g[co / 2]))
+//              (re for even co, im for odd; in place; the same value as
+//              kEpiResid with aux32 = float(g) rounded to fp16), zero for
+//              subcarriers W .. pad - 1, slots with keep[slot] == 0 untouched.
+//              The tile is transposed through LDS so that each thread reads
+//              and writes 16 consecutive subcarriers of one grid row.
 // MFMA 32x32x16 bf16 lane layout: A row (pixel) = lane % 32, its 8 K values
 // (channels) 8 (lane / 32) .. + 7; B column (output channel) = lane % 32,
 // same K; C column = lane % 32, row of accumulator i = 8 (i / 4) +
@@ -348,6 +389,7 @@ __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
         const uint32_t h = sh % kSym;
         const size_t s = sh / kSym;
         const int w0 = static_cast<int>(wt_i) * TILE;
+        if (EPI == kEpiGrid && a.keep != nullptr && a.keep[s] == 0) continue;
         constexpr int CH = CIN / 8;
         for (int e = threadIdx.x; e < 3 * tw * CH; e += 256) {
             const int c = e % CH, rp = e / CH, p = rp % tw, r = rp / tw;
@@ -373,6 +415,52 @@ __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
                     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(tile[kh][pl + kw * dil][2 * ks + kh8],
                                                                   wf[kh * 3 + kw][ks], acc, 0, 0, 0);
         __syncthreads();
+        if constexpr (EPI == kEpiGrid) {
+            static_assert(COUT == 32 && TILE == 128, "grid epilogue: one 32-channel wave column, 128 pixels");
+            constexpr int EPS = COUT + 1;  // LDS row stride (floats) of the transposed tile
+            static_assert(sizeof(tile) >= TILE * EPS * sizeof(float), "epilogue tile fits the input tile");
+            float *ep = reinterpret_cast<float *>(&tile[0][0][0]);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) ep[(wpx * 32 + 8 * (i >> 2) + 4 * kh8 + (i & 3)) * EPS + co] = acc[i];
+            __syncthreads();
+            // Thread: channel c (grid row c / 2, re or im), 16-subcarrier
+            // chunks q, q + 8, ... of the tile (and, in the last tile, of
+            // the zero tail up to pad).
+            const int c = threadIdx.x >> 3, q = threadIdx.x & 7;
+            __half *row = (c & 1 ? a.g_im : a.g_re) + ((s * (COUT / 2) + (c >> 1)) * kSym + h) * a.pad;
+            const float cb = a.bias[c], sc = a.scale[s];
+            // The sum is rounded to fp32 first, then to fp16 (as the fp32
+            // output route does): the empty asm keeps the compiler from
+            // fusing the two roundings into one mixed-precision fma.
+            auto resid = [&](float acc_v, float base) {
+                float o = fmaf(sc, acc_v + cb, base);
+                asm volatile("" : "+v"(o));
+                return __float2half_rn(o);
+            };
+            const int W = static_cast<int>(a.W);
+            const int end = wt_i + 1 == wtiles ? static_cast<int>(a.pad) : w0 + TILE;
+            for (int b = w0 + 16 * q; b < end; b += 128) {
+                const int pl0 = b - w0;
+                if (b + 16 <= W && a.pad % 8 == 0) {
+                    uint4 hv[2];
+                    hv[0] = *reinterpret_cast<const uint4 *>(row + b);
+                    hv[1] = *reinterpret_cast<const uint4 *>(row + b + 8);
+                    __half *hh = reinterpret_cast<__half *>(hv);
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        hh[j] = resid(ep[(pl0 + j) * EPS + c], __half2float(hh[j]));
+                    *reinterpret_cast<uint4 *>(row + b) = hv[0];
+                    *reinterpret_cast<uint4 *>(row + b + 8) = hv[1];
+                } else {
+                    for (int j = 0; j < 16 && b + j < end; ++j) {
+                        const int w = b + j;
+                        row[w] = w < W ? resid(ep[(pl0 + j) * EPS + c], __half2float(row[w])) : __float2half_rn(0.0f);
+                    }
+                }
+            }
+            __syncthreads();
+            continue;
+        }
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int m = 8 * (i >> 2) + 4 * kh8 + (i & 3);
@@ -383,7 +471,7 @@ __global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
             if (EPI == kEpiRelu) {
                 a.out[o] = static_cast<bf16>(fmaxf(v + bias, 0.0f));
             } else if (EPI == kEpiResid) {
-                a.out_f32[o] = a.scale[s] * (v + bias) + a.aux32[o];
+                a.out_f32[o] = fmaf(a.scale[s], v + bias, a.aux32[o]);
             } else {
                 a.out[o] = static_cast<float>(a.aux[o]) > 0.0f ? static_cast<bf16>(v) : static_cast<bf16>(0.0f);
             }
@@ -627,8 +715,11 @@ void LaunchConv(const AiranCeRefiner *r, const ConvArgs &a, hipStream_t s)
 }
 
 // Forward of the network on x [S][14][W][32] -> out [S][14][W][32] fp32,
-// keeping the hidden activations.
-void Forward(const AiranCeRefiner *r, const bf16 *x, uint32_t S, uint32_t W, hipStream_t s)
+// keeping the hidden activations; with grid_re / grid_im the last layer
+// instead adds its output to the fp16 channel grid [S][16][14][pad] in place
+// (the residual base is the grid itself; slots with keep[s] == 0 untouched).
+void Forward(const AiranCeRefiner *r, const bf16 *x, uint32_t S, uint32_t W, hipStream_t s,
+             __half *grid_re = nullptr, __half *grid_im = nullptr, const uint8_t *keep = nullptr)
 {
     ConvArgs a{};
     a.S = S;
@@ -646,6 +737,13 @@ void Forward(const AiranCeRefiner *r, const bf16 *x, uint32_t S, uint32_t W, hip
             else
                 LaunchConv<64, 64, kEpiRelu>(r, a, s);
             in = r->act[i + 1];
+        } else if (grid_re != nullptr) {
+            a.scale = r->res_scale;
+            a.g_re = grid_re;
+            a.g_im = grid_im;
+            a.keep = keep;
+            a.pad = r->g.pad;
+            LaunchConv<64, 32, kEpiGrid>(r, a, s);
         } else {
             a.aux32 = r->x32;
             a.scale = r->res_scale;
@@ -909,14 +1007,17 @@ int32_t airan_ce_refiner_EnqueueWithNoise(AiranCeRefiner *r, const void *h_ls_re
     hipLaunchKernelGGL(NoiseScaleKernel, dim3(airan::CeilDiv(slots, 64)), dim3(64), 0, s,
                        r->res_mode == 1 ? static_cast<const __half *>(noise) : nullptr, g.nr, slots, r->res_scale);
     if (r->input_mode == 1) {
-        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s,
-                           static_cast<const __half *>(h_re), static_cast<const __half *>(h_im), r->x, r->x32, g.used,
-                           g.pad, g.nr, g.nl);
-    } else {
-        const InputArgs ia =
-            MakeInput(r, static_cast<const __half *>(h_ls_re), static_cast<const __half *>(h_ls_im), r->x);
-        hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
+        // The residual base is the estimate grid itself: the last layer
+        // writes the refined grid in place, no fp32 copies.
+        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 1024), kSym, slots), dim3(256), 0, s,
+                           static_cast<const __half *>(h_re), static_cast<const __half *>(h_im), r->x, nullptr,
+                           g.used, g.pad);
+        Forward(r, r->x, slots, g.used, s, static_cast<__half *>(h_re), static_cast<__half *>(h_im), keep);
+        return airan::LaunchStatus();
     }
+    const InputArgs ia =
+        MakeInput(r, static_cast<const __half *>(h_ls_re), static_cast<const __half *>(h_ls_im), r->x);
+    hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
     Forward(r, r->x, slots, g.used, s);
     hipLaunchKernelGGL(OutToGridKernel, dim3(airan::CeilDiv(g.pad, 256), kSym, slots), dim3(256), 0, s, r->out, keep,
                        static_cast<__half *>(h_re), static_cast<__half *>(h_im), g.used, g.pad, g.nr, g.nl);
@@ -1027,8 +1128,8 @@ int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *r, const float *gains, const
         st = airan::LaunchCeApply(r->filter, r->hls_re, r->hls_im, r->noise, nullptr, r->ce_ws, r->ce_ws_bytes,
                                   r->est_re, r->est_im, g.nl, slots, s);
         if (st != AIRAN_OK) return st;
-        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, r->est_re,
-                           r->est_im, r->x, r->x32, g.used, g.pad, g.nr, g.nl);
+        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 1024), kSym, slots), dim3(256), 0, s,
+                           r->est_re, r->est_im, r->x, r->x32, g.used, g.pad);
     } else {
         const InputArgs ia = MakeInput(r, r->hls_re, r->hls_im, r->x);
         hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);

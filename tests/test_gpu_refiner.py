@@ -179,3 +179,76 @@ def test_refiner_against_lmmse(gpu_stream):
     tx.close()
     ref.close()
     filt.close()
+
+
+ENQUEUE_CASES = [("full_273prb", None, 3, [1, 0, 1]), ("narrow_20prb_pad512", 20, 2, None)]
+
+
+@pytest.mark.parametrize("name,num_rb,S,keep", ENQUEUE_CASES, ids=[c[0] for c in ENQUEUE_CASES])
+def test_refiner_enqueue_grid_matches_forward(oracle, gpu_stream, name, num_rb, S, keep):
+    """Inference in input mode 1 (airan_ce_refiner_EnqueueWithNoise: the
+    estimate grid in, the refined grid written in place by the last layer)
+    against the fp32 route whose output the oracle pins (the forward of
+    airan_ce_refiner_Gradients on the same input, rounded to fp16 into the
+    grid): bit-exact for the kept slots, zero beyond the used subcarriers,
+    untouched for the others.  The input is bf16-exact so that both routes'
+    residual bases are the same; the noise variances have exact square
+    roots (the residual scales 0.5, 0.25, 0.125)."""
+    import dataclasses
+
+    from puschrx import abi, ops
+
+    prof = P.FULL_CARRIER_4RX
+    if num_rb is not None:
+        prof = dataclasses.replace(prof, num_rb=num_rb, fft=1024, padded_subcarriers=512)
+    cfg = prof.config()
+    rng = np.random.default_rng(77 + S)
+    params = refiner_params(oracle, rng)
+    ref = Refiner(cfg, max_slots=S, params=params, input_mode=1, dilation=[1, 2, 4, 8, 16, 1], residual_scale=1)
+    g = ref.g
+    used, pad = g.used, g.pad
+    print(f"refiner enqueue {name}: S={S} used={used} pad={pad} keep={keep}", flush=True)
+    def exact_grid():
+        # [S][16 links][14][pad], values exact in bf16 and in fp16.
+        v = rng.normal(0, 0.5, (S, 16, 14, pad)).astype(np.float32)
+        v[np.abs(v) < 2.0 ** -10] = 0.0
+        return torch.from_numpy(v).to(torch.bfloat16).float()
+
+    grid_re, grid_im = exact_grid(), exact_grid()
+    in_re, in_im = grid_re.half().cuda(), grid_im.half().cuda()
+    h_re, h_im = in_re.clone(), in_im.clone()
+    scales = np.array([0.5, 0.25, 0.125][:S], np.float32)
+    noise = torch.from_numpy(np.repeat((scales ** 2)[:, None], 4, 1)).half().cuda()
+    keep_t = None if keep is None else torch.tensor(keep, dtype=torch.uint8, device="cuda")
+    ops._check("refiner EnqueueWithNoise", ref.lib.airan_ce_refiner_EnqueueWithNoise(
+        ref.handle, C_vp(h_re), C_vp(h_im), C_vp(noise), S, None if keep_t is None else C_vp(keep_t),
+        C_vp(h_re), C_vp(h_im), C_vp_int(ops.stream_ptr())))
+
+    # The fp32 route on the same input: x [S][14][used][32], channel 2 link + {re, im}.
+    x = torch.stack([grid_re[..., :used], grid_im[..., :used]], -1)      # [S][16][14][used][2]
+    x = x.permute(0, 2, 3, 1, 4).reshape(S, 14, used, 32).to(torch.bfloat16).contiguous().cuda()
+    ref.set_slot_scale(scales)
+    ref.gradients(x, torch.zeros(x.shape, dtype=torch.float32, device="cuda"), S, used)
+    out = ref.fetch("out", torch.float32, S * 14 * used * 32).reshape(S, 14, used, 16, 2)
+    want = torch.zeros((2, S, 16, 14, pad), dtype=torch.float16, device="cuda")
+    want[..., :used] = out.permute(4, 0, 3, 1, 2).half()
+    torch.cuda.synchronize()
+    for s in range(S):
+        if keep is not None and keep[s] == 0:
+            assert torch.equal(h_re[s], in_re[s]) and torch.equal(h_im[s], in_im[s]), f"slot {s} not kept but written"
+            continue
+        assert torch.equal(h_re[s], want[0, s]), (s, (h_re[s].float() - want[0, s].float()).abs().max().item())
+        assert torch.equal(h_im[s], want[1, s]), (s, (h_im[s].float() - want[1, s].float()).abs().max().item())
+    # The network changed the estimate (the check above is not the identity).
+    assert not torch.equal(h_re[..., :used], in_re[..., :used])
+    ref.close()
+
+
+def C_vp(t):
+    import ctypes as C
+    return C.c_void_p(t.data_ptr())
+
+
+def C_vp_int(p):
+    import ctypes as C
+    return C.c_void_p(p)
