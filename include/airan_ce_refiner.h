@@ -111,7 +111,9 @@ typedef struct AiranCeRefinerBatchV1 {
     uint16_t struct_size;
     const void *x;              /* bf16 [slots][14][used][32] network input */
     const void *target;         /* fp32 [slots][14][used][32] true channel */
-    const void *out;            /* fp32 [slots][14][used][32] last forward output */
+    const void *out;            /* fp32 [slots][14][used][32] output of the last Forward,
+                                   TrainStep or Gradients (Enqueue in input mode 1 writes
+                                   its grid directly and leaves this unchanged) */
     const void *h_ls_re;        /* fp16 [slots][NR][L][D][pilot_pad] */
     const void *h_ls_im;
     const void *noise_ls;       /* fp16 [slots][NR] */
