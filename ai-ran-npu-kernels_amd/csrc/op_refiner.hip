@@ -241,16 +241,30 @@ __global__ void __launch_bounds__(256) InputKernel(InputArgs a)
 // Network input from a channel estimate grid h fp16 [slots][16 links][14][pad]
 // (the LMMSE estimator's output; link r L + l): x[s][k][u][2 link + {re, im}]
 // = bf16(h) and, when x32 is not null, x32 = h exactly (fp16 -> fp32).
-// Thread: 4 consecutive subcarriers of one symbol, so that the 32 planes'
-// loads are coalesced along the subcarriers and every pixel's 32 channels
-// leave as whole 16-byte vectors.  Grid (ceil(used / 1024), 14, slots).
-__global__ void __launch_bounds__(256) EstimateInputKernel(const __half *__restrict__ h_re,
-                                                           const __half *__restrict__ h_im, bf16 *__restrict__ x,
-                                                           float *__restrict__ x32, uint32_t used, uint32_t pad)
+// With zero_tail (inference, where the network's last layer then writes the
+// refined grid in place over subcarriers 0 .. used - 1), it also zeroes the
+// grid's subcarriers used .. pad - 1 of the slots it will write (keep[s] != 0,
+// or all with keep null).  Thread: 4 consecutive subcarriers of one symbol,
+// so that the 32 planes' loads are coalesced along the subcarriers and every
+// pixel's 32 channels leave as whole 16-byte vectors.  Grid
+// (ceil(pad / 1024), 14, slots).
+__global__ void __launch_bounds__(256) EstimateInputKernel(__half *__restrict__ h_re, __half *__restrict__ h_im,
+                                                           bf16 *__restrict__ x, float *__restrict__ x32,
+                                                           uint32_t used, uint32_t pad, bool zero_tail,
+                                                           const uint8_t *__restrict__ keep)
 {
     constexpr uint32_t kPx = 4, kLinks = kCh / 2;
     const uint32_t u0 = kPx * (blockIdx.x * blockDim.x + threadIdx.x);
     const uint32_t k = blockIdx.y, s = blockIdx.z;
+    if (zero_tail && u0 + kPx > used && u0 < pad && (keep == nullptr || keep[s] != 0)) {
+        for (uint32_t p = 0; p < kLinks; ++p) {
+            const size_t b = ((static_cast<size_t>(s) * kLinks + p) * kSym + k) * pad;
+            for (uint32_t u = max(u0, used); u < min(u0 + kPx, pad); ++u) {
+                h_re[b + u] = __float2half_rn(0.0f);
+                h_im[b + u] = __float2half_rn(0.0f);
+            }
+        }
+    }
     if (u0 >= used) return;
     const uint32_t n = min(kPx, used - u0);
     const bool vec = n == kPx && pad % kPx == 0;
@@ -342,138 +356,238 @@ struct ConvArgs {
 
 constexpr int kMaxDil = 16;
 
-// out[p][co] = epilogue(sum_{tap, ci} in[p + off(tap)][ci] wt[tap][co][ci]):
-//   kEpiRelu   bf16(max(acc + bias, 0))
-//   kEpiResid  scale[slot] (acc + bias) + aux32   (fp32)
+// Position of 16-byte chunk c of pixel p in its LDS row (XOR swizzle): any
+// 8 consecutive pixels put a given chunk in 8 different 16-byte bank groups,
+// so the MFMA operand reads (32 consecutive pixels, one chunk) are free of
+// bank conflicts without padding.
+template <int CH>
+__device__ inline int ChunkPos(int p, int c)
+{
+    return CH == 8 ? c ^ (p & 7) : c ^ ((p >> 1) & 3);
+}
+
+// One 16-byte-per-lane buffer load of [base, base + bytes) at byte offset
+// voff straight into LDS at lds + 16 lane (lds wave-uniform); an offset at or
+// beyond `bytes` loads zero (the range check; kOutside with bytes < 2^31
+// stays outside for every dword of the 16 bytes, no wrap).
+constexpr uint32_t kOutside = 0x80000000u;
+__device__ inline void BufferToLds16(const void *base, uint32_t bytes, uint32_t voff, void *lds)
+{
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, static_cast<int>(bytes), 0x00020000);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (__attribute__((address_space(3))) void *)lds, 16, voff, 0, 0, 0);
+}
+
+// out[p][co] = epilogue(acc), acc = bias[co] + sum_{tap, ci} in[p + off(tap)][ci]
+// wt[tap][co][ci] (the bias is the accumulator's initial value; none for kEpiMask):
+//   kEpiRelu   bf16(max(acc, 0))
+//   kEpiResid  scale[slot] acc + aux32   (fp32)
 //   kEpiMask   bf16(aux > 0 ? acc : 0)   (gradient through the ReLU)
 //   kEpiGrid   the residual output straight into the channel grid:
-//              g[co / 2] = fp16(scale[slot] (acc + bias) + float(g[co / 2]))
+//              g[co / 2] = fp16(scale[slot] acc + float(g[co / 2]))
 //              (re for even co, im for odd; in place; the same value as
-//              kEpiResid with aux32 = float(g) rounded to fp16), zero for
-//              subcarriers W .. pad - 1, slots with keep[slot] == 0 untouched.
-//              The tile is transposed through LDS so that each thread reads
-//              and writes 16 consecutive subcarriers of one grid row.
-// MFMA 32x32x16 bf16 lane layout: A row (pixel) = lane % 32, its 8 K values
-// (channels) 8 (lane / 32) .. + 7; B column (output channel) = lane % 32,
-// same K; C column = lane % 32, row of accumulator i = 8 (i / 4) +
-// 4 (lane / 32) + i % 4.
-// HALO: the largest dilation the LDS tile holds (1, or kMaxDil).
+//              kEpiResid with aux32 = float(g) rounded to fp16) for
+//              subcarriers 0 .. W - 1, slots with keep[slot] == 0 untouched.
+//
+// Work item: one TILE-pixel column of half a slot's rows (7 output symbol
+// rows; TILE = 64 for 64 output channels, 128 for 32).  The item's 9 input
+// rows pass through a ring of 4 LDS rows: step j computes output row h0 + j
+// from input rows j .. j + 2 (of h0 - 1 ..) and then starts row j + 3 by
+// direct global -> LDS copies (no registers; they complete behind the
+// epilogue and the other waves' MFMAs), so each input row is read from
+// memory once per item.  The copies are buffer loads over the item's slot
+// (descriptor in scalar registers, one 32-bit offset per copy); positions
+// outside the grid get an offset beyond the buffer's end, which the
+// hardware's range check returns as zero.  The copies are issued after the
+// step's operand reads, so the compiler's wait for pending LDS copies never
+// falls between them.
+//
+// MFMA v_mfma_f32_32x32x16_bf16 with the weights as A (32 output channels x
+// 16 input channels, in registers for the whole kernel) and the pixels as B
+// (16 input channels x 32 pixels, from LDS).  Lane layout: A row = lane % 32,
+// its 8 K values 8 (lane / 32) .. + 7; B column = lane % 32, the same K;
+// C column (pixel) = lane % 32, row (output channel) of accumulator i =
+// 8 (i / 4) + 4 (lane / 32) + i % 4 -- each lane holds 4 consecutive output
+// channels of one pixel per group of 4 accumulators.
+// HALO: the largest dilation the LDS rows hold (1, 2, 4, 8 or kMaxDil).
 template <int CIN, int COUT, int EPI, int HALO>
-__global__ void __launch_bounds__(256) ConvKernel(ConvArgs a)
+__global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
 {
     constexpr int CO_BLOCKS = COUT / 32;
     constexpr int PX_WAVES = 4 / CO_BLOCKS;
     constexpr int TILE = 32 * PX_WAVES;
     constexpr int KS = CIN / 16;
-    constexpr int VPP = CIN / 8 + 1;  // bf16x8 vectors per LDS pixel (one of padding)
-    __shared__ bf16x8 tile[3][TILE + 2 * HALO][VPP];
+    constexpr int CH = CIN / 8;                            // 16-byte chunks per pixel
+    constexpr int ROW = ((TILE + 2 * HALO) * CH + 63) / 64 * 64;  // chunks per LDS row (whole waves)
+    constexpr int NK = (ROW + 255) / 256;                  // copies per thread per row
+    constexpr int HROWS = kSym / 2;                        // output rows per item
+    static_assert(CH == 4 || CH == 8, "32 or 64 input channels");
+    __shared__ uint4 ring[4 * ROW];
+    __shared__ float4 bias4[COUT / 4];
     const int dil = HALO == 1 ? 1 : a.dil;
     const int tw = TILE + 2 * dil;
+    const int W = static_cast<int>(a.W);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wco = wave % CO_BLOCKS, wpx = wave / CO_BLOCKS;
-    const int co = wco * 32 + (lane & 31);
     const int kh8 = lane >> 5;
+    const int pl = wpx * 32 + (lane & 31);                 // the lane's pixel in the tile
+    const int co0 = wco * 32 + 4 * kh8;                    // + 8 g + e: accumulator 4 g + e
     bf16x8 wf[9][KS];
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
-            wf[tap][ks] = *reinterpret_cast<const bf16x8 *>(a.wt + (static_cast<size_t>(tap) * COUT + co) * CIN +
-                                                            ks * 16 + 8 * kh8);
-    const float bias = EPI == kEpiMask ? 0.0f : a.bias[co];
-    (void)bias;
+            wf[tap][ks] = *reinterpret_cast<const bf16x8 *>(
+                a.wt + (static_cast<size_t>(tap) * COUT + wco * 32 + (lane & 31)) * CIN + ks * 16 + 8 * kh8);
+    if (threadIdx.x < COUT / 4)
+        bias4[threadIdx.x] = EPI == kEpiMask ? make_float4(0, 0, 0, 0)
+                                             : reinterpret_cast<const float4 *>(a.bias)[threadIdx.x];
+    // LDS byte offsets of the lane's B fragments within a ring row.
+    uint32_t off[3][KS];
+#pragma unroll
+    for (int kw = 0; kw < 3; ++kw) {
+        const int q = pl + kw * dil;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) off[kw][ks] = 16u * (q * CH + ChunkPos<CH>(q, 2 * ks + kh8));
+    }
     const uint32_t wtiles = (a.W + TILE - 1) / TILE;
-    const uint32_t ntiles = a.S * kSym * wtiles;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t items = a.S * 2 * wtiles;
+    for (uint32_t t = blockIdx.x; t < items; t += gridDim.x) {
         const uint32_t wt_i = t % wtiles, sh = t / wtiles;
-        const uint32_t h = sh % kSym;
-        const size_t s = sh / kSym;
+        const int h0 = static_cast<int>(sh % 2) * HROWS;
+        const size_t s = sh / 2;
         const int w0 = static_cast<int>(wt_i) * TILE;
         if (EPI == kEpiGrid && a.keep != nullptr && a.keep[s] == 0) continue;
-        constexpr int CH = CIN / 8;
-        for (int e = threadIdx.x; e < 3 * tw * CH; e += 256) {
-            const int c = e % CH, rp = e / CH, p = rp % tw, r = rp / tw;
-            const int hh = static_cast<int>(h) + r - 1, ww = w0 + p - dil;
-            bf16x8 v;
+        const uint32_t slot_bytes = kSym * a.W * CIN * 2;
+        const bf16 *in_s = a.in + s * kSym * a.W * CIN;
+        // Byte offset of copy k's chunk in its input row, or kOutside (outside
+        // the grid's subcarriers).
+        uint32_t col[NK];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = static_cast<bf16>(0.0f);
-            if (hh >= 0 && hh < kSym && ww >= 0 && ww < static_cast<int>(a.W))
-                v = *reinterpret_cast<const bf16x8 *>(a.in + ((s * kSym + hh) * a.W + ww) * CIN + 8 * c);
-            tile[r][p][c] = v;
+        for (int k = 0; k < NK; ++k) {
+            const int e = k * 256 + static_cast<int>(threadIdx.x);
+            const int px = e / CH;
+            const int ww = w0 - dil + px;
+            col[k] = px < tw && ww >= 0 && ww < W ? 16u * (static_cast<uint32_t>(ww) * CH + ChunkPos<CH>(px, e % CH))
+                                                  : kOutside;
         }
-        __syncthreads();
-        f32x16 acc;
+        // Input row h0 - 1 + m into ring row m % 4 (chunk e of the row:
+        // pixel e / CH, position e % CH holding chunk ChunkPos(pixel, e % CH)).
+        auto copy_row = [&](int m, uint4 *dst) {
+            const int hh = h0 - 1 + m;
+            const uint32_t row_off = hh >= 0 && hh < kSym ? static_cast<uint32_t>(hh) * a.W * CIN * 2 : kOutside;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-        const int pl = wpx * 32 + (lane & 31);
-#pragma unroll
-        for (int kh = 0; kh < 3; ++kh)
-#pragma unroll
-            for (int kw = 0; kw < 3; ++kw)
-#pragma unroll
-                for (int ks = 0; ks < KS; ++ks)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(tile[kh][pl + kw * dil][2 * ks + kh8],
-                                                                  wf[kh * 3 + kw][ks], acc, 0, 0, 0);
-        __syncthreads();
-        if constexpr (EPI == kEpiGrid) {
-            static_assert(COUT == 32 && TILE == 128, "grid epilogue: one 32-channel wave column, 128 pixels");
-            constexpr int EPS = COUT + 1;  // LDS row stride (floats) of the transposed tile
-            static_assert(sizeof(tile) >= TILE * EPS * sizeof(float), "epilogue tile fits the input tile");
-            float *ep = reinterpret_cast<float *>(&tile[0][0][0]);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) ep[(wpx * 32 + 8 * (i >> 2) + 4 * kh8 + (i & 3)) * EPS + co] = acc[i];
+            for (int k = 0; k < NK; ++k)
+                if (k * 256 + wave * 64 < ROW)  // wave-uniform: the row's last waves
+                    BufferToLds16(in_s, slot_bytes,
+                                  col[k] == kOutside || row_off == kOutside ? kOutside : col[k] + row_off,
+                                  dst + k * 256 + wave * 64);
+        };
+        __syncthreads();  // the previous item's last step has read its rows
+        copy_row(0, ring);
+        copy_row(1, ring + ROW);
+        copy_row(2, ring + 2 * ROW);
+#pragma unroll 1
+        for (int j = 0; j < HROWS; ++j) {
+            const int h = h0 + j;
+            const int w = w0 + pl;
+            const size_t px_out = (s * kSym + h) * a.W + w;
+            // Rows j .. j + 1 were waited for at earlier steps; row j + 2
+            // (this wave's part, and the previous step's stores) now.  The
+            // barrier then makes every wave's part visible and frees ring
+            // row (j + 3) % 4 (read at step j - 1) for this step's copy.
+            __builtin_amdgcn_s_waitcnt(0xF70);
             __syncthreads();
-            // Thread: channel c (grid row c / 2, re or im), 16-subcarrier
-            // chunks q, q + 8, ... of the tile (and, in the last tile, of
-            // the zero tail up to pad).
-            const int c = threadIdx.x >> 3, q = threadIdx.x & 7;
-            __half *row = (c & 1 ? a.g_im : a.g_re) + ((s * (COUT / 2) + (c >> 1)) * kSym + h) * a.pad;
-            const float cb = a.bias[c], sc = a.scale[s];
-            // The sum is rounded to fp32 first, then to fp16 (as the fp32
-            // output route does): the empty asm keeps the compiler from
-            // fusing the two roundings into one mixed-precision fma.
-            auto resid = [&](float acc_v, float base) {
-                float o = fmaf(sc, acc_v + cb, base);
-                asm volatile("" : "+v"(o));
-                return __float2half_rn(o);
-            };
-            const int W = static_cast<int>(a.W);
-            const int end = wt_i + 1 == wtiles ? static_cast<int>(a.pad) : w0 + TILE;
-            for (int b = w0 + 16 * q; b < end; b += 128) {
-                const int pl0 = b - w0;
-                if (b + 16 <= W && a.pad % 8 == 0) {
-                    uint4 hv[2];
-                    hv[0] = *reinterpret_cast<const uint4 *>(row + b);
-                    hv[1] = *reinterpret_cast<const uint4 *>(row + b + 8);
-                    __half *hh = reinterpret_cast<__half *>(hv);
+            // Epilogue inputs, loaded before this step's copy is issued so
+            // that waiting for them does not wait for the copy.
+            uint2 mask[4];
+            float4 res32[4];
+            if constexpr (EPI == kEpiMask) {
 #pragma unroll
-                    for (int j = 0; j < 16; ++j)
-                        hh[j] = resid(ep[(pl0 + j) * EPS + c], __half2float(hh[j]));
-                    *reinterpret_cast<uint4 *>(row + b) = hv[0];
-                    *reinterpret_cast<uint4 *>(row + b + 8) = hv[1];
-                } else {
-                    for (int j = 0; j < 16 && b + j < end; ++j) {
-                        const int w = b + j;
-                        row[w] = w < W ? resid(ep[(pl0 + j) * EPS + c], __half2float(row[w])) : __float2half_rn(0.0f);
-                    }
+                for (int g = 0; g < 4; ++g)
+                    mask[g] = w < W ? *reinterpret_cast<const uint2 *>(a.aux + px_out * COUT + co0 + 8 * g)
+                                    : make_uint2(0, 0);
+            }
+            if constexpr (EPI == kEpiResid) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    res32[g] = w < W ? *reinterpret_cast<const float4 *>(a.aux32 + px_out * COUT + co0 + 8 * g)
+                                     : make_float4(0, 0, 0, 0);
+            }
+            // Grid element of accumulator i = 4 g + e: channel co0 + 8 g + e,
+            // i.e. link co0 / 2 + 4 g + e / 2 (re for even e, im for odd).
+            const uint32_t gstride = kSym * a.pad;
+            const uint32_t gbase = ((static_cast<uint32_t>(s) * (COUT / 2) + (co0 >> 1)) * kSym + h) * a.pad;
+            __half2 gin[8];  // (re, im) of link co0 / 2 + 4 g + k: gin[2 g + k]
+            if constexpr (EPI == kEpiGrid) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint32_t o = gbase + (4 * (q >> 1) + (q & 1)) * gstride + w;
+                    gin[q] = w < W ? __halves2half2(a.g_re[o], a.g_im[o]) : __float2half2_rn(0.0f);
                 }
             }
-            __syncthreads();
-            continue;
-        }
+            f32x16 acc;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int m = 8 * (i >> 2) + 4 * kh8 + (i & 3);
-            const int w = w0 + wpx * 32 + m;
-            if (w >= static_cast<int>(a.W)) continue;
-            const size_t o = ((s * kSym + h) * a.W + w) * COUT + co;
-            const float v = acc[i];
-            if (EPI == kEpiRelu) {
-                a.out[o] = static_cast<bf16>(fmaxf(v + bias, 0.0f));
-            } else if (EPI == kEpiResid) {
-                a.out_f32[o] = fmaf(a.scale[s], v + bias, a.aux32[o]);
-            } else {
-                a.out[o] = static_cast<float>(a.aux[o]) > 0.0f ? static_cast<bf16>(v) : static_cast<bf16>(0.0f);
+            for (int g = 0; g < 4; ++g) {
+                const float4 b = bias4[(co0 >> 2) + 2 * g];
+                acc[4 * g] = b.x;
+                acc[4 * g + 1] = b.y;
+                acc[4 * g + 2] = b.z;
+                acc[4 * g + 3] = b.w;
+            }
+#pragma unroll
+            for (int kh = 0; kh < 3; ++kh) {
+                const char *rb = reinterpret_cast<const char *>(ring + ((j + kh) & 3) * ROW);
+#pragma unroll
+                for (int kw = 0; kw < 3; ++kw)
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks)
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            wf[kh * 3 + kw][ks], *reinterpret_cast<const bf16x8 *>(rb + off[kw][ks]), acc, 0, 0, 0);
+            }
+            if (j + 3 <= HROWS + 1) copy_row(j + 3, ring + ((j + 3) & 3) * ROW);
+            if constexpr (EPI == kEpiGrid) {
+                const float sc = a.scale[s];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t o = gbase + (4 * (i >> 2) + ((i & 3) >> 1)) * gstride + w;
+                    // Rounded to fp32 first, then to fp16 (as the fp32 route
+                    // does): the empty asm keeps the compiler from fusing
+                    // the two roundings into one mixed-precision fma.
+                    const __half2 gp = gin[2 * (i >> 2) + ((i & 3) >> 1)];
+                    float v = fmaf(sc, acc[i], __half2float(i & 1 ? __high2half(gp) : __low2half(gp)));
+                    asm volatile("" : "+v"(v));
+                    if (w < W) (i & 1 ? a.g_im : a.g_re)[o] = __float2half_rn(v);
+                }
+                continue;
+            }
+            if (w >= W) continue;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const size_t o = px_out * COUT + co0 + 8 * g;
+                if constexpr (EPI == kEpiResid) {
+                    const float sc = a.scale[s];
+                    *reinterpret_cast<float4 *>(a.out_f32 + o) =
+                        make_float4(fmaf(sc, acc[4 * g], res32[g].x), fmaf(sc, acc[4 * g + 1], res32[g].y),
+                                    fmaf(sc, acc[4 * g + 2], res32[g].z), fmaf(sc, acc[4 * g + 3], res32[g].w));
+                } else {
+                    uint32_t v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float x = acc[4 * g + e];
+                        bf16 r;
+                        if constexpr (EPI == kEpiRelu) {
+                            r = static_cast<bf16>(fmaxf(x, 0.0f));
+                        } else {
+                            const uint32_t mw = e < 2 ? mask[g].x : mask[g].y;
+                            const uint16_t mb = static_cast<uint16_t>(e & 1 ? mw >> 16 : mw & 0xffffu);
+                            r = static_cast<float>(__builtin_bit_cast(bf16, mb)) > 0.0f ? static_cast<bf16>(x)
+                                                                                      : static_cast<bf16>(0.0f);
+                        }
+                        v[e] = __builtin_bit_cast(uint16_t, r);
+                    }
+                    *reinterpret_cast<uint2 *>(a.out + o) = make_uint2(v[0] | v[1] << 16, v[2] | v[3] << 16);
+                }
             }
         }
     }
@@ -697,11 +811,11 @@ template <int CIN, int COUT, int EPI>
 void LaunchConv(const AiranCeRefiner *r, const ConvArgs &a, hipStream_t s)
 {
     constexpr int TILE = 32 * (4 / (COUT / 32));
-    const uint32_t ntiles = a.S * kSym * ((a.W + TILE - 1) / TILE);
-    const uint32_t grid = std::min<uint32_t>(ntiles, static_cast<uint32_t>(r->cus) * 2);
-    // The LDS tile holds the halo of the layer's dilation rounded up to a
-    // power of two, so a small dilation keeps a small tile (more workgroups
-    // per CU).
+    // Work items: half a slot's symbol rows x one TILE-pixel column.
+    const uint32_t items = a.S * 2 * ((a.W + TILE - 1) / TILE);
+    const uint32_t grid = std::min<uint32_t>(items, static_cast<uint32_t>(r->cus) * 2);
+    // The LDS rows hold the halo of the layer's dilation rounded up to a
+    // power of two, so a small dilation keeps the rows short.
     if (a.dil == 1)
         hipLaunchKernelGGL((ConvKernel<CIN, COUT, EPI, 1>), dim3(grid), dim3(256), 0, s, a);
     else if (a.dil <= 2)
@@ -1009,9 +1123,9 @@ int32_t airan_ce_refiner_EnqueueWithNoise(AiranCeRefiner *r, const void *h_ls_re
     if (r->input_mode == 1) {
         // The residual base is the estimate grid itself: the last layer
         // writes the refined grid in place, no fp32 copies.
-        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 1024), kSym, slots), dim3(256), 0, s,
-                           static_cast<const __half *>(h_re), static_cast<const __half *>(h_im), r->x, nullptr,
-                           g.used, g.pad);
+        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.pad, 1024), kSym, slots), dim3(256), 0, s,
+                           static_cast<__half *>(h_re), static_cast<__half *>(h_im), r->x, nullptr, g.used, g.pad,
+                           true, keep);
         Forward(r, r->x, slots, g.used, s, static_cast<__half *>(h_re), static_cast<__half *>(h_im), keep);
         return airan::LaunchStatus();
     }
@@ -1129,7 +1243,7 @@ int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *r, const float *gains, const
                                   r->est_re, r->est_im, g.nl, slots, s);
         if (st != AIRAN_OK) return st;
         hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 1024), kSym, slots), dim3(256), 0, s,
-                           r->est_re, r->est_im, r->x, r->x32, g.used, g.pad);
+                           r->est_re, r->est_im, r->x, r->x32, g.used, g.pad, false, nullptr);
     } else {
         const InputArgs ia = MakeInput(r, r->hls_re, r->hls_im, r->x);
         hipLaunchKernelGGL(InputKernel, dim3(airan::CeilDiv(g.used, 256), slots), dim3(256), 0, s, ia);
