@@ -240,73 +240,59 @@ __global__ void __launch_bounds__(256) InputKernel(InputArgs a)
 
 // Network input from a channel estimate grid h fp16 [slots][16 links][14][pad]
 // (the LMMSE estimator's output; link r L + l): x[s][k][u][2 link + {re, im}]
-// = bf16(h) and, when x32 is not null, x32 = h exactly (fp16 -> fp32).
-// With zero_tail (inference, where the network's last layer then writes the
-// refined grid in place over subcarriers 0 .. used - 1), it also zeroes the
-// grid's subcarriers used .. pad - 1 of the slots it will write (keep[s] != 0,
-// or all with keep null).  Thread: 4 consecutive subcarriers of one symbol,
-// so that the 32 planes' loads are coalesced along the subcarriers and every
-// pixel's 32 channels leave as whole 16-byte vectors.  Grid
-// (ceil(pad / 1024), 14, slots).
+// = bf16(h) and, with X32, x32 = h exactly (fp16 -> fp32).  With zero_tail
+// (inference, where the network's last layer then writes the refined grid in
+// place over subcarriers 0 .. used - 1) it also zeroes the grid's subcarriers
+// used .. pad - 1 of the slots it will write (keep[s] != 0, or all with keep
+// null).  Block: 64 subcarriers of one symbol row, transposed through LDS:
+// thread t first reads 8 subcarriers of channel t / 8 (one 16-byte load),
+// then writes 8 channels of pixel t / 4 (16 bytes; four threads per 64-byte
+// pixel, so each wave writes 1 KB contiguous).  Grid (ceil(pad / 64), 14,
+// slots).
+template <bool X32>
 __global__ void __launch_bounds__(256) EstimateInputKernel(__half *__restrict__ h_re, __half *__restrict__ h_im,
                                                            bf16 *__restrict__ x, float *__restrict__ x32,
                                                            uint32_t used, uint32_t pad, bool zero_tail,
                                                            const uint8_t *__restrict__ keep)
 {
-    constexpr uint32_t kPx = 4, kLinks = kCh / 2;
-    const uint32_t u0 = kPx * (blockIdx.x * blockDim.x + threadIdx.x);
+    constexpr int kPx = 64, kStride = kCh + 1;  // LDS pixel row: 32 channels + 1 (fp32 words)
+    __shared__ float tile[kPx * kStride];
+    const uint32_t u0 = blockIdx.x * kPx;
     const uint32_t k = blockIdx.y, s = blockIdx.z;
-    if (zero_tail && u0 + kPx > used && u0 < pad && (keep == nullptr || keep[s] != 0)) {
-        for (uint32_t p = 0; p < kLinks; ++p) {
-            const size_t b = ((static_cast<size_t>(s) * kLinks + p) * kSym + k) * pad;
-            for (uint32_t u = max(u0, used); u < min(u0 + kPx, pad); ++u) {
-                h_re[b + u] = __float2half_rn(0.0f);
-                h_im[b + u] = __float2half_rn(0.0f);
-            }
-        }
-    }
-    if (u0 >= used) return;
-    const uint32_t n = min(kPx, used - u0);
-    const bool vec = n == kPx && pad % kPx == 0;
-    float v[kCh][kPx];
+    const int t = static_cast<int>(threadIdx.x);
+    {
+        const int c = t >> 3, q = (t & 7) * 8;
+        __half *plane = (c & 1 ? h_im : h_re) + ((static_cast<size_t>(s) * (kCh / 2) + (c >> 1)) * kSym + k) * pad;
+        const uint32_t u = u0 + q;
+        float v[8];
+        if (u + 8 <= used && pad % 8 == 0) {
+            const uint4 w = *reinterpret_cast<const uint4 *>(plane + u);
+            const __half *hv = reinterpret_cast<const __half *>(&w);
 #pragma unroll
-    for (uint32_t p = 0; p < kLinks; ++p) {
-        const size_t b = ((static_cast<size_t>(s) * kLinks + p) * kSym + k) * pad + u0;
-        if (vec) {
-            const uint2 wr = *reinterpret_cast<const uint2 *>(h_re + b);
-            const uint2 wi = *reinterpret_cast<const uint2 *>(h_im + b);
-            const uint32_t r4[4] = {wr.x & 0xffffu, wr.x >> 16, wr.y & 0xffffu, wr.y >> 16};
-            const uint32_t i4[4] = {wi.x & 0xffffu, wi.x >> 16, wi.y & 0xffffu, wi.y >> 16};
-#pragma unroll
-            for (uint32_t j = 0; j < kPx; ++j) {
-                v[2 * p][j] = __half2float(__ushort_as_half(static_cast<unsigned short>(r4[j])));
-                v[2 * p + 1][j] = __half2float(__ushort_as_half(static_cast<unsigned short>(i4[j])));
-            }
+            for (int j = 0; j < 8; ++j) v[j] = __half2float(hv[j]);
         } else {
 #pragma unroll
-            for (uint32_t j = 0; j < kPx; ++j) {
-                v[2 * p][j] = j < n ? __half2float(h_re[b + j]) : 0.0f;
-                v[2 * p + 1][j] = j < n ? __half2float(h_im[b + j]) : 0.0f;
-            }
+            for (int j = 0; j < 8; ++j) v[j] = u + j < used ? __half2float(plane[u + j]) : 0.0f;
+            if (zero_tail && (keep == nullptr || keep[s] != 0))
+                for (uint32_t j = max(u, used); j < min(u + 8, pad); ++j) plane[j] = __float2half_rn(0.0f);
         }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) tile[(q + j) * kStride + c] = v[j];
     }
+    __syncthreads();
+    const int px = t >> 2, ch = (t & 3) * 8;
+    if (u0 + px >= used) return;
+    float v[8];
 #pragma unroll
-    for (uint32_t j = 0; j < kPx; ++j) {
-        if (j >= n) break;
-        const size_t o = ((static_cast<size_t>(s) * kSym + k) * used + u0 + j) * kCh;
+    for (int j = 0; j < 8; ++j) v[j] = tile[px * kStride + ch + j];
+    const size_t o = ((static_cast<size_t>(s) * kSym + k) * used + u0 + px) * kCh + ch;
+    bf16x8 b;
 #pragma unroll
-        for (uint32_t q = 0; q < kCh / 8; ++q) {
-            bf16x8 b;
-#pragma unroll
-            for (uint32_t e = 0; e < 8; ++e) b[e] = static_cast<bf16>(v[8 * q + e][j]);
-            *reinterpret_cast<bf16x8 *>(x + o + 8 * q) = b;
-        }
-        if (x32 != nullptr) {
-#pragma unroll
-            for (uint32_t q = 0; q < kCh / 4; ++q)
-                *reinterpret_cast<float4 *>(x32 + o + 4 * q) =
-                    make_float4(v[4 * q][j], v[4 * q + 1][j], v[4 * q + 2][j], v[4 * q + 3][j]);
-        }
+    for (int j = 0; j < 8; ++j) b[j] = static_cast<bf16>(v[j]);
+    *reinterpret_cast<bf16x8 *>(x + o) = b;
+    if constexpr (X32) {
+        *reinterpret_cast<float4 *>(x32 + o) = make_float4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<float4 *>(x32 + o + 4) = make_float4(v[4], v[5], v[6], v[7]);
     }
 }
 
@@ -518,14 +504,6 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
             // i.e. link co0 / 2 + 4 g + e / 2 (re for even e, im for odd).
             const uint32_t gstride = kSym * a.pad;
             const uint32_t gbase = ((static_cast<uint32_t>(s) * (COUT / 2) + (co0 >> 1)) * kSym + h) * a.pad;
-            __half2 gin[8];  // (re, im) of link co0 / 2 + 4 g + k: gin[2 g + k]
-            if constexpr (EPI == kEpiGrid) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const uint32_t o = gbase + (4 * (q >> 1) + (q & 1)) * gstride + w;
-                    gin[q] = w < W ? __halves2half2(a.g_re[o], a.g_im[o]) : __float2half2_rn(0.0f);
-                }
-            }
             f32x16 acc;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
@@ -547,17 +525,27 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
             }
             if (j + 3 <= HROWS + 1) copy_row(j + 3, ring + ((j + 3) & 3) * ROW);
             if constexpr (EPI == kEpiGrid) {
+                if (w >= W) continue;
                 const float sc = a.scale[s];
+                // The residual loads in two groups of 8 (one wait each), after
+                // the MFMAs: their operand registers are free by now.
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const uint32_t o = gbase + (4 * (i >> 2) + ((i & 3) >> 1)) * gstride + w;
-                    // Rounded to fp32 first, then to fp16 (as the fp32 route
-                    // does): the empty asm keeps the compiler from fusing
-                    // the two roundings into one mixed-precision fma.
-                    const __half2 gp = gin[2 * (i >> 2) + ((i & 3) >> 1)];
-                    float v = fmaf(sc, acc[i], __half2float(i & 1 ? __high2half(gp) : __low2half(gp)));
-                    asm volatile("" : "+v"(v));
-                    if (w < W) (i & 1 ? a.g_im : a.g_re)[o] = __float2half_rn(v);
+                for (int h8 = 0; h8 < 16; h8 += 8) {
+                    __half gin[8];
+#pragma unroll
+                    for (int i = h8; i < h8 + 8; ++i)
+                        gin[i - h8] =
+                            (i & 1 ? a.g_im : a.g_re)[gbase + (4 * (i >> 2) + ((i & 3) >> 1)) * gstride + w];
+#pragma unroll
+                    for (int i = h8; i < h8 + 8; ++i) {
+                        // Rounded to fp32 first, then to fp16 (as the fp32
+                        // route does): the empty asm keeps the compiler from
+                        // fusing the two roundings into one mixed-precision fma.
+                        float v = fmaf(sc, acc[i], __half2float(gin[i - h8]));
+                        asm volatile("" : "+v"(v));
+                        (i & 1 ? a.g_im : a.g_re)[gbase + (4 * (i >> 2) + ((i & 3) >> 1)) * gstride + w] =
+                            __float2half_rn(v);
+                    }
                 }
                 continue;
             }
@@ -1123,8 +1111,8 @@ int32_t airan_ce_refiner_EnqueueWithNoise(AiranCeRefiner *r, const void *h_ls_re
     if (r->input_mode == 1) {
         // The residual base is the estimate grid itself: the last layer
         // writes the refined grid in place, no fp32 copies.
-        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.pad, 1024), kSym, slots), dim3(256), 0, s,
-                           static_cast<__half *>(h_re), static_cast<__half *>(h_im), r->x, nullptr, g.used, g.pad,
+        hipLaunchKernelGGL(EstimateInputKernel<false>, dim3(airan::CeilDiv(g.pad, 64), kSym, slots), dim3(256), 0,
+                           s, static_cast<__half *>(h_re), static_cast<__half *>(h_im), r->x, nullptr, g.used, g.pad,
                            true, keep);
         Forward(r, r->x, slots, g.used, s, static_cast<__half *>(h_re), static_cast<__half *>(h_im), keep);
         return airan::LaunchStatus();
@@ -1242,7 +1230,7 @@ int32_t airan_ce_refiner_SynthBatch(AiranCeRefiner *r, const float *gains, const
         st = airan::LaunchCeApply(r->filter, r->hls_re, r->hls_im, r->noise, nullptr, r->ce_ws, r->ce_ws_bytes,
                                   r->est_re, r->est_im, g.nl, slots, s);
         if (st != AIRAN_OK) return st;
-        hipLaunchKernelGGL(EstimateInputKernel, dim3(airan::CeilDiv(g.used, 1024), kSym, slots), dim3(256), 0, s,
+        hipLaunchKernelGGL(EstimateInputKernel<true>, dim3(airan::CeilDiv(g.used, 64), kSym, slots), dim3(256), 0, s,
                            r->est_re, r->est_im, r->x, r->x32, g.used, g.pad, false, nullptr);
     } else {
         const InputArgs ia = MakeInput(r, r->hls_re, r->hls_im, r->x);
