@@ -410,6 +410,11 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
     static_assert(CH == 4 || CH == 8, "32 or 64 input channels");
     __shared__ uint4 ring[4 * ROW];
     __shared__ float4 bias4[COUT / 4];
+    // bf16 epilogues: per wave a 32-pixel x 32-channel tile, chunk g (channels
+    // 8 g .. + 7) of pixel p at 34 g + p (the 2-chunk pad makes both the
+    // pixel-major writes and the chunk-major reads conflict-free).
+    constexpr bool kBf16Out = EPI == kEpiRelu || EPI == kEpiMask;
+    __shared__ uint4 escr[kBf16Out ? 4 * 4 * 34 : 1];
     const int dil = HALO == 1 ? 1 : a.dil;
     const int tw = TILE + 2 * dil;
     const int W = static_cast<int>(a.W);
@@ -486,13 +491,19 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
             __syncthreads();
             // Epilogue inputs, loaded before this step's copy is issued so
             // that waiting for them does not wait for the copy.
-            uint2 mask[4];
+            // bf16 outputs leave transposed: lane -> pixel (lane >> 2) + 16 r of
+            // the wave's 32, channels wco 32 + 8 (lane & 3) .. + 7 (16 bytes).
+            const int tg = lane & 3;
+            uint4 mask[2];
             float4 res32[4];
             if constexpr (EPI == kEpiMask) {
 #pragma unroll
-                for (int g = 0; g < 4; ++g)
-                    mask[g] = w < W ? *reinterpret_cast<const uint2 *>(a.aux + px_out * COUT + co0 + 8 * g)
-                                    : make_uint2(0, 0);
+                for (int r = 0; r < 2; ++r) {
+                    const int wt = w0 + wpx * 32 + (lane >> 2) + 16 * r;
+                    mask[r] = wt < W ? *reinterpret_cast<const uint4 *>(
+                                           a.aux + ((s * kSym + h) * a.W + wt) * COUT + wco * 32 + 8 * tg)
+                                     : make_uint4(0, 0, 0, 0);
+                }
             }
             if constexpr (EPI == kEpiResid) {
 #pragma unroll
@@ -549,33 +560,52 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
                 }
                 continue;
             }
-            if (w >= W) continue;
+            if constexpr (kBf16Out) {
+                uint4 *scr = escr + wave * (4 * 34);
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const size_t o = px_out * COUT + co0 + 8 * g;
-                if constexpr (EPI == kEpiResid) {
-                    const float sc = a.scale[s];
-                    *reinterpret_cast<float4 *>(a.out_f32 + o) =
-                        make_float4(fmaf(sc, acc[4 * g], res32[g].x), fmaf(sc, acc[4 * g + 1], res32[g].y),
-                                    fmaf(sc, acc[4 * g + 2], res32[g].z), fmaf(sc, acc[4 * g + 3], res32[g].w));
-                } else {
+                for (int g = 0; g < 4; ++g) {
                     uint32_t v[4];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         const float x = acc[4 * g + e];
-                        bf16 r;
-                        if constexpr (EPI == kEpiRelu) {
-                            r = static_cast<bf16>(fmaxf(x, 0.0f));
-                        } else {
-                            const uint32_t mw = e < 2 ? mask[g].x : mask[g].y;
-                            const uint16_t mb = static_cast<uint16_t>(e & 1 ? mw >> 16 : mw & 0xffffu);
-                            r = static_cast<float>(__builtin_bit_cast(bf16, mb)) > 0.0f ? static_cast<bf16>(x)
-                                                                                      : static_cast<bf16>(0.0f);
-                        }
-                        v[e] = __builtin_bit_cast(uint16_t, r);
+                        v[e] = __builtin_bit_cast(uint16_t, static_cast<bf16>(EPI == kEpiRelu ? fmaxf(x, 0.0f) : x));
                     }
-                    *reinterpret_cast<uint2 *>(a.out + o) = make_uint2(v[0] | v[1] << 16, v[2] | v[3] << 16);
+                    *reinterpret_cast<uint2 *>(reinterpret_cast<char *>(scr + 34 * g + (lane & 31)) + 8 * kh8) =
+                        make_uint2(v[0] | v[1] << 16, v[2] | v[3] << 16);
                 }
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int pxw = (lane >> 2) + 16 * r;
+                    uint4 v = scr[34 * tg + pxw];
+                    if constexpr (EPI == kEpiMask) {
+                        // Gradient through the ReLU: zero where the activation is not positive.
+                        uint32_t *vw = reinterpret_cast<uint32_t *>(&v);
+                        const uint32_t *mw = reinterpret_cast<const uint32_t *>(&mask[r]);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            uint32_t keepw = 0;
+#pragma unroll
+                            for (int hb = 0; hb < 2; ++hb) {
+                                const uint16_t mb = static_cast<uint16_t>(mw[q] >> (16 * hb));
+                                if (static_cast<float>(__builtin_bit_cast(bf16, mb)) > 0.0f) keepw |= 0xffffu << (16 * hb);
+                            }
+                            vw[q] &= keepw;
+                        }
+                    }
+                    const int wt = w0 + wpx * 32 + pxw;
+                    if (wt < W)
+                        *reinterpret_cast<uint4 *>(a.out + ((s * kSym + h) * a.W + wt) * COUT + wco * 32 + 8 * tg) = v;
+                }
+                continue;
+            }
+            if (w >= W) continue;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const size_t o = px_out * COUT + co0 + 8 * g;
+                const float sc = a.scale[s];
+                *reinterpret_cast<float4 *>(a.out_f32 + o) =
+                    make_float4(fmaf(sc, acc[4 * g], res32[g].x), fmaf(sc, acc[4 * g + 1], res32[g].y),
+                                fmaf(sc, acc[4 * g + 2], res32[g].z), fmaf(sc, acc[4 * g + 3], res32[g].w));
             }
         }
     }
