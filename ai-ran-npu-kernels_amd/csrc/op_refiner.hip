@@ -342,14 +342,16 @@ struct ConvArgs {
 
 constexpr int kMaxDil = 16;
 
-// Position of 16-byte chunk c of pixel p in its LDS row (XOR swizzle): any
-// 8 consecutive pixels put a given chunk in 8 different 16-byte bank groups,
-// so the MFMA operand reads (32 consecutive pixels, one chunk) are free of
-// bank conflicts without padding.
+// Position of 16-byte chunk c of pixel p in its LDS row (XOR swizzle).  LDS
+// has 64 four-byte banks and serves a 16-byte read 16 lanes at a time, so a
+// read of one chunk of 16 consecutive pixels is conflict-free when their
+// 16-byte units (pixel p: CH units) are distinct modulo 16; the unit of
+// chunk c of pixel p depends on p only through p mod 16, injectively:
+// CH = 8: 8 (p & 1) + (c ^ ((p >> 1) & 7)); CH = 4: 4 (p & 3) + (c ^ ((p >> 2) & 3)).
 template <int CH>
 __device__ inline int ChunkPos(int p, int c)
 {
-    return CH == 8 ? c ^ (p & 7) : c ^ ((p >> 1) & 3);
+    return CH == 8 ? c ^ ((p >> 1) & 7) : c ^ ((p >> 2) & 3);
 }
 
 // One 16-byte-per-lane buffer load of [base, base + bytes) at byte offset
@@ -362,6 +364,14 @@ __device__ inline void BufferToLds16(const void *base, uint32_t bytes, uint32_t 
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, static_cast<int>(bytes), 0x00020000);
     __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (__attribute__((address_space(3))) void *)lds, 16, voff, 0, 0, 0);
+}
+
+// The same load into registers (zero beyond `bytes`).
+__device__ inline uint4 BufferLoad16(const void *base, uint32_t bytes, uint32_t voff)
+{
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, static_cast<int>(bytes), 0x00020000);
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, 0, 0));
 }
 
 // out[p][co] = epilogue(acc), acc = bias[co] + sum_{tap, ci} in[p + off(tap)][ci]
@@ -378,15 +388,17 @@ __device__ inline void BufferToLds16(const void *base, uint32_t bytes, uint32_t 
 // Work item: one TILE-pixel column of half a slot's rows (7 output symbol
 // rows; TILE = 64 for 64 output channels, 128 for 32).  The item's 9 input
 // rows pass through a ring of 4 LDS rows: step j computes output row h0 + j
-// from input rows j .. j + 2 (of h0 - 1 ..) and then starts row j + 3 by
-// direct global -> LDS copies (no registers; they complete behind the
-// epilogue and the other waves' MFMAs), so each input row is read from
-// memory once per item.  The copies are buffer loads over the item's slot
-// (descriptor in scalar registers, one 32-bit offset per copy); positions
-// outside the grid get an offset beyond the buffer's end, which the
-// hardware's range check returns as zero.  The copies are issued after the
-// step's operand reads, so the compiler's wait for pending LDS copies never
-// falls between them.
+// from input rows j .. j + 2 (of h0 - 1 ..) while row j + 3 is on its way,
+// so each input row is read from memory once per item.  The row copies are
+// buffer loads over the item's slot (descriptor in scalar registers, one
+// 32-bit offset per copy); positions outside the grid get an offset beyond
+// the buffer's end, which the hardware's range check returns as zero.  With
+// 64 output channels a step loads row j + 3 into registers before its MFMAs
+// and writes it to LDS after them (the loads' latency hides behind the
+// MFMAs, and only the LDS writes need waiting for at the next barrier).  The
+// 128-pixel tiles (32 output channels) have no registers to spare for that:
+// they copy global -> LDS directly after the step's operand reads, waited
+// for at the next step.
 //
 // MFMA v_mfma_f32_32x32x16_bf16 with the weights as A (32 output channels x
 // 16 input channels, in registers for the whole kernel) and the pixels as B
@@ -407,8 +419,10 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
     constexpr int ROW = ((TILE + 2 * HALO) * CH + 63) / 64 * 64;  // chunks per LDS row (whole waves)
     constexpr int NK = (ROW + 255) / 256;                  // copies per thread per row
     constexpr int HROWS = kSym / 2;                        // output rows per item
+    constexpr int RING = 4;                                // LDS input rows
+    constexpr bool kStage = COUT == 64;                    // row copies through registers
     static_assert(CH == 4 || CH == 8, "32 or 64 input channels");
-    __shared__ uint4 ring[4 * ROW];
+    __shared__ uint4 ring[RING * ROW];
     __shared__ float4 bias4[COUT / 4];
     // bf16 epilogues: per wave a 32-pixel x 32-channel tile, chunk g (channels
     // 8 g .. + 7) of pixel p at 34 g + p (the 2-chunk pad makes both the
@@ -464,31 +478,52 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
         }
         // Input row h0 - 1 + m into ring row m % 4 (chunk e of the row:
         // pixel e / CH, position e % CH holding chunk ChunkPos(pixel, e % CH)).
-        auto copy_row = [&](int m, uint4 *dst) {
+        auto voff = [&](int m, int k) {
             const int hh = h0 - 1 + m;
             const uint32_t row_off = hh >= 0 && hh < kSym ? static_cast<uint32_t>(hh) * a.W * CIN * 2 : kOutside;
+            return col[k] == kOutside || row_off == kOutside ? kOutside : col[k] + row_off;
+        };
+        auto copy_row = [&](int m, uint4 *dst) {
 #pragma unroll
             for (int k = 0; k < NK; ++k)
                 if (k * 256 + wave * 64 < ROW)  // wave-uniform: the row's last waves
-                    BufferToLds16(in_s, slot_bytes,
-                                  col[k] == kOutside || row_off == kOutside ? kOutside : col[k] + row_off,
-                                  dst + k * 256 + wave * 64);
+                    BufferToLds16(in_s, slot_bytes, voff(m, k), dst + k * 256 + wave * 64);
+        };
+        uint4 stage[kStage ? NK : 1];
+        auto load_row = [&](int m) {
+#pragma unroll
+            for (int k = 0; k < NK; ++k)
+                if (k * 256 + wave * 64 < ROW) stage[k] = BufferLoad16(in_s, slot_bytes, voff(m, k));
+        };
+        auto store_row = [&](uint4 *dst) {
+#pragma unroll
+            for (int k = 0; k < NK; ++k)
+                if (k * 256 + wave * 64 < ROW) dst[k * 256 + threadIdx.x] = stage[k];
         };
         __syncthreads();  // the previous item's last step has read its rows
-        copy_row(0, ring);
-        copy_row(1, ring + ROW);
-        copy_row(2, ring + 2 * ROW);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) copy_row(m, ring + m * ROW);
+        // Staged: no direct copies are issued inside the row loop, so the
+        // compiler must know these are done (it would otherwise wait for
+        // them before every operand read).
+        if constexpr (kStage) __builtin_amdgcn_s_waitcnt(0xF70);
 #pragma unroll 1
         for (int j = 0; j < HROWS; ++j) {
             const int h = h0 + j;
             const int w = w0 + pl;
             const size_t px_out = (s * kSym + h) * a.W + w;
             // Rows j .. j + 1 were waited for at earlier steps; row j + 2
-            // (this wave's part, and the previous step's stores) now.  The
-            // barrier then makes every wave's part visible and frees ring
-            // row (j + 3) % 4 (read at step j - 1) for this step's copy.
-            __builtin_amdgcn_s_waitcnt(0xF70);
-            __syncthreads();
+            // (this wave's part) now: its LDS writes (staged) or its direct
+            // copies and, with them, the previous epilogue's stores.  The
+            // barrier then makes every wave's part visible and frees ring row
+            // (j + 3) % 4 (read at step j - 1).  A bare barrier: a fence would
+            // also wait for the epilogue's global stores.
+            if constexpr (kStage)
+                __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+            else
+                __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0)
+            __builtin_amdgcn_s_barrier();
+            if (kStage && j + 3 <= HROWS + 1) load_row(j + 3);
             // Epilogue inputs, loaded before this step's copy is issued so
             // that waiting for them does not wait for the copy.
             // bf16 outputs leave transposed: lane -> pixel (lane >> 2) + 16 r of
@@ -524,17 +559,34 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
                 acc[4 * g + 2] = b.z;
                 acc[4 * g + 3] = b.w;
             }
+            // 9 KS operand reads, kept kDepth ahead of the MFMA that uses them
+            // (an LDS read takes several MFMA durations to return).
+            {
+                constexpr int kN = 9 * KS, kDepth = 4;
+                auto operand = [&](int n) {
+                    const int kh = n / (3 * KS), kw = (n / KS) % 3, ks = n % KS;
+                    return *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(ring + ((j + kh) & 3) * ROW) +
+                                                             off[kw][ks]);
+                };
+                bf16x8 bq[kDepth];
 #pragma unroll
-            for (int kh = 0; kh < 3; ++kh) {
-                const char *rb = reinterpret_cast<const char *>(ring + ((j + kh) & 3) * ROW);
+                for (int n = 0; n < kDepth; ++n) bq[n] = operand(n);
 #pragma unroll
-                for (int kw = 0; kw < 3; ++kw)
-#pragma unroll
-                    for (int ks = 0; ks < KS; ++ks)
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            wf[kh * 3 + kw][ks], *reinterpret_cast<const bf16x8 *>(rb + off[kw][ks]), acc, 0, 0, 0);
+                for (int n = 0; n < kN; ++n) {
+                    const bf16x8 b = bq[n % kDepth];
+                    if (n + kDepth < kN) bq[n % kDepth] = operand(n + kDepth);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[n / KS][n % KS], b, acc, 0, 0, 0);
+                    // Keep the order as written (the scheduler would sink
+                    // each read next to its MFMA).
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             }
-            if (j + 3 <= HROWS + 1) copy_row(j + 3, ring + ((j + 3) & 3) * ROW);
+            if (j + 3 <= HROWS + 1) {
+                if constexpr (kStage)
+                    store_row(ring + ((j + 3) & 3) * ROW);
+                else
+                    copy_row(j + 3, ring + ((j + 3) & 3) * ROW);
+            }
             if constexpr (EPI == kEpiGrid) {
                 if (w >= W) continue;
                 const float sc = a.scale[s];
