@@ -6,16 +6,17 @@
 //
 // Convolution (forward, and the data gradient as a convolution with the
 // flipped, transposed weights): implicit GEMM on MFMA
-// v_mfma_f32_32x32x16_bf16, M = 32 pixels, N = 32 output channels, K = 16
-// input channels per instruction.  A workgroup (4 waves) owns a tile of one
-// symbol row: 64 pixels x 64 channels (or 128 x 32); each wave keeps the B
-// fragments of its 32 output channels for all 9 taps in registers for the
-// whole kernel (persistent grid), stages the input tile with its halo
-// (3 rows x (tile + 2 dil) pixels) in LDS, and runs 9 * CIN / 16 MFMAs per
-// tile.  Each layer may dilate its taps along the subcarrier axis (taps at
-// subcarrier offsets -dil, 0, +dil; symbol offsets stay -1, 0, +1), which
-// widens the network's view of the band: dilations 1, 2, 4, 8, 16, 1 see
-// 65 subcarriers.  Weight gradients are fp32 VALU outer products over LDS
+// v_mfma_f32_32x32x16_bf16 with the weights as A (32 output channels x 16
+// input channels, in registers for the whole persistent kernel) and 32
+// pixels as B.  A workgroup (4 waves) takes work items of 7 output symbol
+// rows x 64 pixels (x 64 channels; 128 x 32 for the last layer) and walks
+// the rows through a 4-row LDS ring of input rows with their halo, so each
+// input row is read once per item; rows are XOR-swizzled in 16-byte chunks
+// so the operand reads are free of bank conflicts.  Each layer may dilate
+// its taps along the subcarrier axis (taps at subcarrier offsets -dil, 0,
+// +dil; symbol offsets stay -1, 0, +1), which widens the network's view of
+// the band: dilations 1, 2, 4, 8, 16, 1 see 65 subcarriers.  Weight
+// gradients are fp32 VALU outer products over LDS
 // tiles, reduced across workgroups in a fixed order.
 #include <algorithm>
 #include <cmath>
