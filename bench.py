@@ -22,7 +22,8 @@ LMMSE needs an SNR above the 20..32 dB range quoted for the workload; the
 SNR is per receive antenna and layer).  Reported beside the throughput:
 BLER and goodput at that SNR, the per-stage times (HIP events on the
 chain's stream), the BF16 / FP16 precision variants of the chain on the same
-slots, the learned estimator's sweep, the roofline of the dominant kernel
+slots, the learned estimator's sweep and throughput (the same batches in
+flight, each chain with its own refiner), the roofline of the dominant kernel
 and the FP32 C++ CPU chain on the host cores (single-thread and all
 threads, p50 of 20 runs).
 """
@@ -374,20 +375,23 @@ def main() -> int:
         ms_per_step, tb_ok, mean_iters = plain["ms_per_step"], plain["tb_ok"], plain["mean_iters"]
         iq_dev, iq = plain["iq_dev"], plain["iq"]
 
-        # Throughput with `inflight` batches in flight: chain j (its own
-        # workspace and stream) takes steps j, j + n, ...; batch 0 is the
-        # input just measured and must decode to the same bits.
-        pipe = None
-        if args.inflight > 1:
+        def pipelined(first, label, refiners=None):
+            """Throughput with `inflight` batches in flight: chain j (its own
+            workspace and stream; with refiners[j] attached for the learned
+            estimator) takes steps j, j + n, ...; batch 0 is `first`'s input
+            and must decode to the same bits."""
             n_in = args.inflight
             streams = [stream] + [torch.cuda.Stream() for _ in range(n_in - 1)]
             chains = [chain]
             for j in range(1, n_in):
                 with torch.cuda.stream(streams[j]):
-                    chains.append(NativeChain(cfg, plan, max_slots=S, opts=opts, use_graph=True))
-            batches = [(plain["payload"], plain["iq_dev"])]
+                    cj = NativeChain(cfg, plan, max_slots=S, opts=opts, use_graph=True)
+                    if refiners is not None:
+                        cj.set_refiner(refiners[j], args.refiner_snr_min, args.refiner_snr_max)
+                    chains.append(cj)
+            batches = [(first["payload"], first["iq_dev"])]
             for j in range(1, n_in):
-                pl, _, _, iqj = make_slots(S, snr_op)
+                pl, _, _, iqj = make_slots(S, first["snr_db"])
                 batches.append((pl, torch.from_numpy(iqj).cuda()))
             torch.cuda.synchronize()
             for _ in range(max(args.warmup, 1)):
@@ -418,16 +422,19 @@ def main() -> int:
                     bj = chains[j].fetch("bits", torch.int8, S * plan.C * plan.K)
                 torch.cuda.synchronize()
                 bj = bj.cpu().numpy().reshape(S, plan.C, plan.K).astype(np.uint8)
-                if j == 0 and not np.array_equal(bj, plain["bits"]):
-                    raise RuntimeError("pipelined batch 0 decoded differently from the single-stream run")
+                if j == 0 and not np.array_equal(bj, first["bits"]):
+                    raise RuntimeError(f"{label}: pipelined batch 0 decoded differently from the single-stream run")
                 pipe_ok += int(okj.cpu().numpy().sum())
                 pipe_mism += sum(int(not np.array_equal(bj[s, :, :per_cb].reshape(-1)[: plan.tbs],
                                                         batches[j][0][s])) for s in range(S))
             for j in range(1, n_in):
+                chains[j].set_refiner(None)
                 chains[j].close()
-            pipe = {"ms_per_step": pipe_ms, "tb_ok": pipe_ok, "slots": n_in * S, "mismatches": pipe_mism}
-            log(f"lmmse, {n_in} batches in flight: {pipe_ms:.3f} ms/step, tb_ok {pipe_ok}/{n_in * S}, "
+            log(f"{label}, {n_in} batches in flight: {pipe_ms:.3f} ms/step, tb_ok {pipe_ok}/{n_in * S}, "
                 f"payload mismatches {pipe_mism}, batch 0 bit-identical to the single-stream run")
+            return {"ms_per_step": pipe_ms, "tb_ok": pipe_ok, "slots": n_in * S, "mismatches": pipe_mism}
+
+        pipe = pipelined(plain, "lmmse") if args.inflight > 1 else None
 
         # Per-stage times (HIP events on the chain's stream, direct launches).
         chain.set_timing(True)
@@ -487,11 +494,22 @@ def main() -> int:
                     chain.enqueue(m["iq_dev"], S)
                     lacc += chain.stage_times()["ce_lmmse"]
                 chain.set_timing(False)
+                l_ms = m["ms_per_step"]
                 learned.update({
                     "snr_db": l_snr, "bler": round(1.0 - float(m["tb_ok"].mean()), 4),
-                    "value_gbps": round(S * plan.tbs / (m["ms_per_step"] * 1e-3) / 1e9, 4),
-                    "ms_per_step": round(m["ms_per_step"], 4),
+                    "single_stream": {"ms_per_step": round(l_ms, 4),
+                                      "gbps": round(S * plan.tbs / (l_ms * 1e-3) / 1e9, 4)},
                     "ce_stage_ms": round(lacc / args.steps, 4)})
+                if args.inflight > 1:
+                    # A refiner's activations live in its handle: one per chain in flight.
+                    extra = [Refiner.from_file(cfg, S, wpath) for _ in range(args.inflight - 1)]
+                    lp = pipelined(m, "learned", refiners=[ref] + extra)
+                    for r_ in extra:
+                        r_.close()
+                    l_ms = lp["ms_per_step"]
+                    learned["inflight"] = args.inflight
+                learned.update({"value_gbps": round(S * plan.tbs / (l_ms * 1e-3) / 1e9, 4),
+                                "ms_per_step": round(l_ms, 4)})
             else:
                 learned.update({"snr_db": None, "value_gbps": 0.0})
             chain.set_refiner(None)
