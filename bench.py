@@ -50,7 +50,8 @@ HBM_PEAK_GBS = 8000.0
 # (scaled by a 256 MiB device copy: x2.0 / x1.0), from
 # profiles/chain_kernels_4rx_273prb_64slots.md; reported as roofline.traffic
 # when the dominant stage and batch match that profile.
-MEASURED_TRAFFIC_64 = {"ldpc_decode": (176.2 + 74.1) * 1e6, "ce_lmmse": None, "detect_demap": (205.0 + 161.0) * 1e6}
+MEASURED_TRAFFIC_64 = {"ldpc_decode": round((176.2 + 74.1) * 1e6), "ce_lmmse": None,
+                       "detect_demap": round((205.0 + 161.0) * 1e6)}
 
 
 def log(msg: str) -> None:
