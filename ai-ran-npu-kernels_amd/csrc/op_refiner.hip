@@ -393,13 +393,11 @@ __device__ inline uint4 BufferLoad16(const void *base, uint32_t bytes, uint32_t 
 // so each input row is read from memory once per item.  The row copies are
 // buffer loads over the item's slot (descriptor in scalar registers, one
 // 32-bit offset per copy); positions outside the grid get an offset beyond
-// the buffer's end, which the hardware's range check returns as zero.  With
-// 64 output channels a step loads row j + 3 into registers before its MFMAs
-// and writes it to LDS after them (the loads' latency hides behind the
-// MFMAs, and only the LDS writes need waiting for at the next barrier).  The
-// 128-pixel tiles (32 output channels) have no registers to spare for that:
-// they copy global -> LDS directly after the step's operand reads, waited
-// for at the next step.
+// the buffer's end, which the hardware's range check returns as zero.  A
+// step loads row j + 3 into registers before its MFMAs and writes it to LDS
+// after them, so the loads' latency hides behind the MFMAs and only the LDS
+// writes need waiting for at the next barrier.  The item's first three rows
+// are copied global -> LDS directly and waited for before the first step.
 //
 // MFMA v_mfma_f32_32x32x16_bf16 with the weights as A (32 output channels x
 // 16 input channels, in registers for the whole kernel) and the pixels as B
@@ -421,7 +419,6 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
     constexpr int NK = (ROW + 255) / 256;                  // copies per thread per row
     constexpr int HROWS = kSym / 2;                        // output rows per item
     constexpr int RING = 4;                                // LDS input rows
-    constexpr bool kStage = COUT == 64;                    // row copies through registers
     static_assert(CH == 4 || CH == 8, "32 or 64 input channels");
     __shared__ uint4 ring[RING * ROW];
     __shared__ float4 bias4[COUT / 4];
@@ -490,7 +487,7 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
                 if (k * 256 + wave * 64 < ROW)  // wave-uniform: the row's last waves
                     BufferToLds16(in_s, slot_bytes, voff(m, k), dst + k * 256 + wave * 64);
         };
-        uint4 stage[kStage ? NK : 1];
+        uint4 stage[NK];
         auto load_row = [&](int m) {
 #pragma unroll
             for (int k = 0; k < NK; ++k)
@@ -504,27 +501,23 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
         __syncthreads();  // the previous item's last step has read its rows
 #pragma unroll
         for (int m = 0; m < 3; ++m) copy_row(m, ring + m * ROW);
-        // Staged: no direct copies are issued inside the row loop, so the
-        // compiler must know these are done (it would otherwise wait for
-        // them before every operand read).
-        if constexpr (kStage) __builtin_amdgcn_s_waitcnt(0xF70);
+        // No direct copies are issued inside the row loop, so the compiler
+        // must know these are done (it would otherwise wait for them before
+        // every operand read).
+        __builtin_amdgcn_s_waitcnt(0xF70);
 #pragma unroll 1
         for (int j = 0; j < HROWS; ++j) {
             const int h = h0 + j;
             const int w = w0 + pl;
             const size_t px_out = (s * kSym + h) * a.W + w;
             // Rows j .. j + 1 were waited for at earlier steps; row j + 2
-            // (this wave's part) now: its LDS writes (staged) or its direct
-            // copies and, with them, the previous epilogue's stores.  The
-            // barrier then makes every wave's part visible and frees ring row
-            // (j + 3) % 4 (read at step j - 1).  A bare barrier: a fence would
-            // also wait for the epilogue's global stores.
-            if constexpr (kStage)
-                __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-            else
-                __builtin_amdgcn_s_waitcnt(0xF70);   // vmcnt(0)
+            // (this wave's LDS writes of it) now.  The barrier then makes
+            // every wave's part visible and frees ring row (j + 3) % 4 (read
+            // at step j - 1).  A bare barrier: a fence would also wait for the
+            // epilogue's global stores.
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
             __builtin_amdgcn_s_barrier();
-            if (kStage && j + 3 <= HROWS + 1) load_row(j + 3);
+            if (j + 3 <= HROWS + 1) load_row(j + 3);
             // Epilogue inputs, loaded before this step's copy is issued so
             // that waiting for them does not wait for the copy.
             // bf16 outputs leave transposed: lane -> pixel (lane >> 2) + 16 r of
@@ -561,9 +554,10 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
                 acc[4 * g + 3] = b.w;
             }
             // 9 KS operand reads, kept kDepth ahead of the MFMA that uses them
-            // (an LDS read takes several MFMA durations to return).
+            // (an LDS read takes several MFMA durations to return; 2 for the
+            // 32-channel layer, whose registers also hold a 5-copy staged row).
             {
-                constexpr int kN = 9 * KS, kDepth = 4;
+                constexpr int kN = 9 * KS, kDepth = COUT == 64 ? 4 : 2;
                 auto operand = [&](int n) {
                     const int kh = n / (3 * KS), kw = (n / KS) % 3, ks = n % KS;
                     return *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(ring + ((j + kh) & 3) * ROW) +
@@ -582,12 +576,7 @@ __global__ void __launch_bounds__(256, 2) ConvKernel(ConvArgs a)
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }
-            if (j + 3 <= HROWS + 1) {
-                if constexpr (kStage)
-                    store_row(ring + ((j + 3) & 3) * ROW);
-                else
-                    copy_row(j + 3, ring + ((j + 3) & 3) * ROW);
-            }
+            if (j + 3 <= HROWS + 1) store_row(ring + ((j + 3) & 3) * ROW);
             if constexpr (EPI == kEpiGrid) {
                 if (w >= W) continue;
                 const float sc = a.scale[s];
